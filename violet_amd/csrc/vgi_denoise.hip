@@ -1,0 +1,683 @@
+/*
+ * vgi_denoise.hip — ReLAX/SVGF denoiser passes for gfx950.
+ *
+ * One wave64 per 8x8 pixel tile (lanes grouped in 2x2 quads, see
+ * tile_pixel()).  All passes are HBM-bandwidth-bound gathers; neighbour
+ * taps of the stencil passes hit L2/MALL because a tile's footprint is
+ * shared with its neighbours' waves.
+ *
+ * Reference passes (violet, shader/denoise/):
+ *   k_temporal        svgf_temporal_filter.hlsl:96-362
+ *   k_disocclusion    relax_disocclusion_fix.hlsl:27-122
+ *   k_fast_clamp      relax_fast_history_clamp.hlsl:12-75
+ *   k_fallback        svgf_spatial_fallback_filter.hlsl:146-246
+ *   k_atrous          svgf_atrous_filter.hlsl:24-88 (+ svgf.inc.hlsl:6-84)
+ *
+ * Pixels without geometry (depth == 0): the reference leaves the colour
+ * outputs of the temporal, disocclusion and fallback passes unwritten
+ * (undefined pooled-texture contents); these kernels write zeros there.
+ */
+#include "vgi_runtime.h"
+#include "vgi_storage.h"
+
+namespace vgi {
+
+// ---------------------------------------------------------------------------
+// Shared helpers
+// ---------------------------------------------------------------------------
+
+// pow(x, 2^k) by k squarings (x in [0,1]); sigma_n is 128 or 32 here.
+__device__ __forceinline__ float pow_pow2(float x, int k) {
+    for (int i = 0; i < k; ++i) x = x * x;
+    return x;
+}
+
+// svgf.inc.hlsl:6-84 EdgeStoppingFunc
+struct EdgeStop {
+    float depth_center;
+    float fwidth_z;
+    f3 normal_center;
+    f2 lumi_center;
+    f2 lumi_std;
+    bool use_lumi;
+    float sigma_z;
+    int sigma_n_log2;  // sigma_n = 2^sigma_n_log2
+
+    __device__ __forceinline__ static EdgeStop create(float depth, float fwidth_z, f3 normal) {
+        EdgeStop e;
+        e.depth_center = depth;
+        e.fwidth_z = fwidth_z;
+        e.normal_center = normal;
+        e.lumi_center = mk2(0.0f, 0.0f);
+        e.lumi_std = mk2(0.0f, 0.0f);
+        e.use_lumi = false;
+        e.sigma_z = 1.0f;
+        e.sigma_n_log2 = 7;  // 128
+        return e;
+    }
+    __device__ __forceinline__ f2 weight(float tap_dist, float depth, f3 normal, f2 lumi) const {
+        const float sigma_l = 4.0f;
+        const float epsilon = 1e-8f;
+        float dist_z = fabsf(depth - depth_center);
+        float weight_z_log = -dist_z / ((sigma_z * fwidth_z) * tap_dist + epsilon);
+        float n_dot = saturate(dot3(normal_center, normal));
+        float weight_n = pow_pow2(n_dot, sigma_n_log2);
+        float wlx = 0.0f, wly = 0.0f;
+        if (use_lumi) {
+            wlx = -fabsf(lumi.x - lumi_center.x) * rcp(sigma_l * lumi_std.x + epsilon);
+            wly = -fabsf(lumi.y - lumi_center.y) * rcp(sigma_l * lumi_std.y + epsilon);
+        }
+        return mk2(weight_n * expf(wlx + weight_z_log), weight_n * expf(wly + weight_z_log));
+    }
+};
+
+// Bilinear fetch with clamp-to-edge addressing (sampler_linear_clamp), in
+// exact f32: texel t = uv*size - 0.5, weights from t - floor(t).
+template <class S>
+__device__ __forceinline__ f3 sample_bilinear3(const void* tex, uint32_t w, uint32_t h, float u, float v) {
+    float tx = u * (float)w - 0.5f;
+    float ty = v * (float)h - 0.5f;
+    float x0f = floorf(tx), y0f = floorf(ty);
+    float fx = tx - x0f, fy = ty - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int xa = clampi(x0, 0, (int)w - 1), xb = clampi(x0 + 1, 0, (int)w - 1);
+    int ya = clampi(y0, 0, (int)h - 1), yb = clampi(y0 + 1, 0, (int)h - 1);
+    f3 c00 = S::ld3(tex, (size_t)ya * w + xa);
+    f3 c10 = S::ld3(tex, (size_t)ya * w + xb);
+    f3 c01 = S::ld3(tex, (size_t)yb * w + xa);
+    f3 c11 = S::ld3(tex, (size_t)yb * w + xb);
+    f3 top = c00 * (1.0f - fx) + c10 * fx;
+    f3 bot = c01 * (1.0f - fx) + c11 * fx;
+    return top * (1.0f - fy) + bot * fy;
+}
+
+__device__ __forceinline__ f3 load_normal(const uint32_t* gb, size_t plane, size_t i) {
+    return decode_gbuffer_normal(gb[plane + i]).normal;
+}
+
+// ---------------------------------------------------------------------------
+// Temporal filter (svgf_temporal_filter.hlsl)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dsr_weight(f3 color, f3 mean, f3 deviation) {
+    float dist = luminance(abs3(color - mean) / deviation);
+    return exp2f(-10.0f * dist);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_temporal_io io,
+                                                 vgi_pc_denoise_temporal pc) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+
+    float depth = io.depth_texture[i];
+    if (depth == 0.0f) {
+        io.rw_history_len_texture[i] = 0;
+        S::st3(io.rw_diff_texture, i, mk3s(0.0f));
+        S::st3(io.rw_spec_texture, i, mk3s(0.0f));
+        S::st2(io.rw_moments_texture, i, mk2(0.0f, 0.0f));
+        return;
+    }
+
+    const float bsx = pc.buffer_size[0], bsy = pc.buffer_size[1];
+    const int bix = (int)bsx, biy = (int)bsy;
+    const float suv_x = ((float)px + 0.5f) / bsx;
+    const float suv_y = ((float)py + 0.5f) / bsy;
+
+    GBufferNormal gbn = decode_gbuffer_normal(io.gbuffer_texture[plane + i]);
+    f2 fwidth = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
+    const bool has_history = pc.has_history != 0;
+    const bool fast = pc.fast_history != 0;
+
+    // Neighbourhood moments of the specular source (:118-132)
+    f3 m1 = mk3s(0.0f), m2 = mk3s(0.0f);
+    for (int x = -1; x <= 1; x++) {
+        for (int y = -1; y <= 1; y++) {
+            int cx = clampi((int)px + x, 0, bix - 1);
+            int cy = clampi((int)py + y, 0, biy - 1);
+            f3 s = S::ld3(io.spec_source_texture, (size_t)cy * w + cx);
+            m1 = m1 + s;
+            m2 = m2 + s * s;
+        }
+    }
+    m1 = m1 * (1.0f / 9.0f);
+    m2 = m2 * (1.0f / 9.0f);
+
+    // Virtual (reflected-point) reprojection (:138-167)
+    f3 virtual_spec_history = mk3s(0.0f);
+    f3 virtual_spec_fast_history = mk3s(0.0f);
+    f3 position;
+    {
+        f4 ph = mul44(fp.inv_view_proj, mk4(suv_x * 2.0f - 1.0f, suv_y * 2.0f - 1.0f, depth, 1.0f));
+        position = mk3(ph.x / ph.w, ph.y / ph.w, ph.z / ph.w);
+        f3 view_ray_dir = normalize3(position - xyz(fp.view_pos));
+        float ray_len = S::ld1(io.spec_ray_len_texture, i);
+        f3 vpos = position + view_ray_dir * ray_len;
+        f4 hr = mul44(fp.prev_view_proj, mk4(vpos.x, vpos.y, vpos.z, 1.0f));
+        float nx = hr.x / hr.w - fp.jitter[2];
+        float ny = hr.y / hr.w - fp.jitter[3];
+        float vu = (nx + fp.jitter[0]) * 0.5f + 0.5f;
+        float vv = (ny + fp.jitter[1]) * 0.5f + 0.5f;
+        bool valid = (vu == saturate(vu)) && (vv == saturate(vv)) && has_history;
+        if (valid) {
+            virtual_spec_history = sample_bilinear3<S>(io.spec_history_texture, w, h, vu, vv);
+            if (fast) virtual_spec_fast_history = sample_bilinear3<S>(io.spec_fast_history_texture, w, h, vu, vv);
+        }
+    }
+
+    // Reprojection (:170-189)
+    float mvx, mvy, reproj_depth;
+    {
+        f4 hp = mul44(fp.view_proj, mk4(position.x, position.y, position.z, 1.0f));
+        float nx = hp.x / hp.w - fp.jitter[0];
+        float ny = hp.y / hp.w - fp.jitter[1];
+        f4 hr = mul44(fp.prev_view_proj, mk4(position.x, position.y, position.z, 1.0f));
+        float rx = hr.x / hr.w - fp.jitter[2];
+        float ry = hr.y / hr.w - fp.jitter[3];
+        mvx = (nx - rx) * 0.5f;
+        mvy = (ny - ry) * 0.5f;
+        reproj_depth = hr.z / hr.w;
+    }
+    float hu = suv_x - mvx;
+    float hv = suv_y - mvy;
+
+    f3 diff_history = mk3s(0.0f), spec_history = mk3s(0.0f);
+    f2 sec_moments_history = mk2(0.0f, 0.0f);
+    float history_len = 0.0f;
+    f3 diff_fast_history = mk3s(0.0f), spec_fast_history = mk3s(0.0f);
+    bool valid = (hu == saturate(hu)) && (hv == saturate(hv)) && has_history;
+    if (valid) {
+        float pcx = hu * bsx - 0.5f;
+        float pcy = hv * bsy - 0.5f;
+        float frx = pcx - floorf(pcx);
+        float fry = pcy - floorf(pcy);
+        int bx = (int)pcx, by = (int)pcy;  // truncation, as int2() in the reference
+        float depth_tol = fwidth.x * 4.0f + 1e-4f;
+        float normal_tol = fwidth.y * 16.0f + 1e-2f;
+        float normal_tol_sq = normal_tol * normal_tol;
+        float b = fry, r = frx, t = 1.0f - b, l = 1.0f - r;
+        float wts[2][2] = {{t * l, t * r}, {b * l, b * r}};
+        f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f), dfsum = mk3s(0.0f), sfsum = mk3s(0.0f);
+        f2 msum = mk2(0.0f, 0.0f);
+        float lsum = 0.0f, wsum = 0.0f;
+        for (int x = 0; x < 2; ++x) {
+            for (int y = 0; y < 2; ++y) {
+                int cx = bx + x, cy = by + y;
+                bool coord_valid = cx >= 0 && cy >= 0 && cx <= bix - 1 && cy <= biy - 1;
+                if (!coord_valid) continue;
+                size_t j = (size_t)cy * w + cx;
+                f3 n = load_normal(io.prev_gbuffer_texture, plane, j);
+                f3 nd = n - gbn.normal;
+                bool ok = (fabsf(io.prev_depth_texture[j] - reproj_depth) < depth_tol) &&
+                          (dot3(nd, nd) < normal_tol_sq);
+                if (ok) {
+                    float wgt = wts[y][x];
+                    wsum += wgt;
+                    dsum = dsum + S::ld3(io.diff_history_texture, j) * wgt;
+                    ssum = ssum + S::ld3(io.spec_history_texture, j) * wgt;
+                    f2 mh = S::ld2(io.moments_history_texture, j);
+                    msum = mk2(msum.x + wgt * mh.x, msum.y + wgt * mh.y);
+                    lsum += wgt * history_len_load(io.history_len_texture[j]);
+                    if (fast) {
+                        dfsum = dfsum + S::ld3(io.diff_fast_history_texture, j) * wgt;
+                        sfsum = sfsum + S::ld3(io.spec_fast_history_texture, j) * wgt;
+                    }
+                }
+            }
+        }
+        valid = wsum > 0.00390625f;
+        if (valid) {
+            float inv = rcp(wsum);
+            diff_history = dsum * inv;
+            spec_history = ssum * inv;
+            sec_moments_history = mk2(msum.x * inv, msum.y * inv);
+            history_len = lsum * inv;
+            diff_fast_history = dfsum * inv;
+            spec_fast_history = sfsum * inv;
+        }
+    }
+    if (!valid) {
+        diff_history = mk3s(0.0f);
+        spec_history = mk3s(0.0f);
+        sec_moments_history = mk2(0.0f, 0.0f);
+        history_len = 0.0f;
+        diff_fast_history = mk3s(0.0f);
+        spec_fast_history = mk3s(0.0f);
+    }
+    history_len += kHistoryLenUnit;
+
+    f3 diff_source = S::ld3(io.diff_source_texture, i);
+    f3 spec_source = S::ld3(io.spec_source_texture, i);
+    float lx = luminance(diff_source), ly = luminance(spec_source);
+
+    const float BF_DIFF = 1.0f / 20.0f;
+    const float BF_FAST = 1.0f / 3.0f;
+    float blurness = sqrtf(saturate(gbn.perceptual_roughness * 6.0f));
+    float BF_SPEC = lerp(7.0f / 8.0f, BF_DIFF, blurness);
+    float BF_SPEC_FAST = lerp(1.0f, BF_FAST, blurness);
+    float len_alpha = kHistoryLenUnit / history_len;
+    float ax = fmaxf(BF_DIFF, len_alpha), ay = fmaxf(BF_SPEC, len_alpha);
+    f3 diff_filtered = lerp3(diff_history, diff_source, ax);
+    f2 mom = mk2(lerp(sec_moments_history.x, lx * lx, ax), lerp(sec_moments_history.y, ly * ly, ay));
+    float afx = fmaxf(BF_FAST, len_alpha), afy = fmaxf(BF_SPEC_FAST, len_alpha);
+    f3 diff_filtered_fast = lerp3(diff_fast_history, diff_source, afx);
+
+    // Dual-source blend [Stachowiak 2018] (:303-331)
+    f3 spec_dev = max3s(sqrt3(abs3(m2 - m1 * m1)), 1e-19f);
+    float wv = dsr_weight(virtual_spec_history, m1, spec_dev);
+    float wh = dsr_weight(spec_history, m1, spec_dev);
+    f3 dual = virtual_spec_history * wv + spec_history * wh;
+    dual = dual / fmaxf(wv + wh, 1e-37f);
+    f3 spec_filtered = lerp3(dual, spec_source, ay);
+    f3 spec_filtered_fast = mk3s(0.0f);
+    if (fast) {
+        float wvf = dsr_weight(virtual_spec_fast_history, m1, spec_dev);
+        float whf = dsr_weight(spec_fast_history, m1, spec_dev);
+        f3 dualf = virtual_spec_fast_history * wvf + spec_fast_history * whf;
+        dualf = dualf / fmaxf(wvf + whf, 1e-37f);
+        spec_filtered_fast = lerp3(dualf, spec_source, afy);
+    }
+
+    S::st3(io.rw_diff_texture, i, diff_filtered);
+    S::st3(io.rw_spec_texture, i, spec_filtered);
+    S::st2(io.rw_moments_texture, i, mom);
+    io.rw_history_len_texture[i] = history_len_store(history_len);
+    if (fast) {
+        S::st3(io.rw_diff_fast_history_texture, i, diff_filtered_fast);
+        S::st3(io.rw_spec_fast_history_texture, i, spec_filtered_fast);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Quad max of the history length (QuadReadAcrossX/Y), lanes 4q..4q+3.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float quad_max(float v) {
+    v = fmaxf(v, __shfl_xor(v, 1));
+    v = fmaxf(v, __shfl_xor(v, 2));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Disocclusion fix (relax_disocclusion_fix.hlsl)
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    const bool inside = px < w && py < h;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+    float len = inside ? history_len_load(io.history_len_texture[i]) : 0.0f;
+    float qmax = quad_max(len);
+    if (!inside) return;
+    if (len == 0.0f) {
+        S::st3(io.rw_diff_texture, i, mk3s(0.0f));
+        S::st3(io.rw_spec_texture, i, mk3s(0.0f));
+        S::st2(io.rw_moments_or_variance_texture, i, mk2(0.0f, 0.0f));
+        return;
+    }
+    len = qmax;
+    if (len < 4.0f * kHistoryLenUnit) {
+        float depth_center = io.depth_texture[i];
+        f3 nc = load_normal(io.gbuffer_texture, plane, i);
+        f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
+        EdgeStop es = EdgeStop::create(depth_center, fw.x, nc);
+        es.sigma_z = 5.0f;
+        es.sigma_n_log2 = 5;  // 32
+        f2 wsum = mk2(0.0f, 0.0f);
+        f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
+        f2 msum = mk2(0.0f, 0.0f);
+        const int stride = (int)((14.0f * kHistoryLenUnit) / len);
+        for (int y = -2; y <= 2; y++) {
+            for (int x = -2; x <= 2; x++) {
+                int ox = x * stride, oy = y * stride;
+                int cx = (int)px + ox, cy = (int)py + oy;
+                if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
+                size_t j = (size_t)cy * w + cx;
+                float d = io.depth_texture[j];
+                f3 n = load_normal(io.gbuffer_texture, plane, j);
+                float fox = (float)ox, foy = (float)oy;
+                f2 wt = es.weight(sqrtf(fox * fox + foy * foy), d, n, mk2(0.0f, 0.0f));
+                f3 dv = S::ld3(io.diff_texture, j);
+                f3 sv = S::ld3(io.spec_texture, j);
+                f2 mv = S::ld2(io.moments_texture, j);
+                wsum = mk2(wsum.x + wt.x, wsum.y + wt.y);
+                dsum = dsum + dv * wt.x;
+                ssum = ssum + sv * wt.y;
+                msum = mk2(msum.x + mv.x * wt.x, msum.y + mv.y * wt.y);
+            }
+        }
+        S::st3(io.rw_diff_texture, i, dsum / wsum.x);
+        S::st3(io.rw_spec_texture, i, ssum / wsum.y);
+        S::st2(io.rw_moments_or_variance_texture, i, mk2(msum.x / wsum.x, msum.y / wsum.y));
+    } else {
+        S::st3(io.rw_diff_texture, i, S::ld3(io.diff_texture, i));
+        S::st3(io.rw_spec_texture, i, S::ld3(io.spec_texture, i));
+        S::st2(io.rw_moments_or_variance_texture, i, S::ld2(io.moments_texture, i));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spatial fallback / variance estimation (svgf_spatial_fallback_filter.hlsl)
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    const bool inside = px < w && py < h;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+    float len = inside ? history_len_load(io.history_len_texture[i]) : 0.0f;
+    float qmax = quad_max(len);
+    if (!inside) return;
+    if (len == 0.0f) {
+        S::st3(io.rw_diff_texture, i, mk3s(0.0f));
+        S::st3(io.rw_spec_texture, i, mk3s(0.0f));
+        S::st2(io.rw_moments_or_variance_texture, i, mk2(0.0f, 0.0f));
+        return;
+    }
+    len = qmax;
+    f3 diff, spec;
+    f2 sm;
+    if (len < 4.0f * kHistoryLenUnit) {
+        float depth_center = io.depth_texture[i];
+        f3 nc = load_normal(io.gbuffer_texture, plane, i);
+        f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
+        EdgeStop es = EdgeStop::create(depth_center, fw.x * 3.0f, nc);
+        float wsx = 0.0f, wsy = 0.0f;
+        f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
+        f2 msum = mk2(0.0f, 0.0f);
+        for (int y = -3; y <= 3; y++) {
+            for (int x = -3; x <= 3; x++) {
+                int cx = (int)px + x, cy = (int)py + y;
+                if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
+                size_t j = (size_t)cy * w + cx;
+                float d = io.depth_texture[j];
+                f3 n = load_normal(io.gbuffer_texture, plane, j);
+                float fx = (float)x, fy = (float)y;
+                float wt = es.weight(sqrtf(fx * fx + fy * fy), d, n, mk2(0.0f, 0.0f)).x;
+                f3 dv = S::ld3(io.diff_texture, j);
+                f3 sv = S::ld3(io.spec_texture, j);
+                f2 mv = S::ld2(io.moments_texture, j);
+                wsx += wt;
+                wsy += wt;
+                dsum = dsum + dv * wt;
+                ssum = ssum + sv * wt;
+                msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
+            }
+        }
+        diff = dsum / wsx;
+        spec = ssum / wsy;
+        sm = mk2(msum.x / wsx, msum.y / wsy);
+    } else {
+        diff = S::ld3(io.diff_texture, i);
+        spec = S::ld3(io.spec_texture, i);
+        sm = S::ld2(io.moments_texture, i);
+    }
+    float l1x = luminance(diff), l1y = luminance(spec);
+    f2 var = mk2(fmaxf(sm.x - l1x * l1x, 0.0f), fmaxf(sm.y - l1y * l1y, 0.0f));
+    float boost = fmaxf(1.0f, (4.0f * kHistoryLenUnit) / len);
+    var = mk2(var.x * boost, var.y * boost);
+    S::st3(io.rw_diff_texture, i, diff);
+    S::st3(io.rw_spec_texture, i, spec);
+    S::st2(io.rw_moments_or_variance_texture, i, var);
+}
+
+// ---------------------------------------------------------------------------
+// Fast history clamp (relax_fast_history_clamp.hlsl), in place on rw_*
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 rgb_to_ycocg(f3 c) {
+    float y = (c.x * 0.25f + c.y * 0.5f) + c.z * 0.25f;
+    float co = c.x * 0.5f - c.z * 0.5f;
+    float cg = (c.x * -0.25f + c.y * 0.5f) - c.z * 0.25f;
+    return mk3(y, co, cg);
+}
+__device__ __forceinline__ f3 ycocg_to_rgb(f3 c) {
+    float tmp = c.x - c.z;
+    return mk3(tmp + c.y, c.x + c.z, tmp - c.y);
+}
+__device__ __forceinline__ f3 clamp3(f3 v, f3 lo, f3 hi) { return min3v(max3v(v, lo), hi); }
+
+template <class S>
+__global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    f3 dm1 = mk3s(0.0f), dm2 = mk3s(0.0f), sm1 = mk3s(0.0f), sm2 = mk3s(0.0f);
+    for (int y = -1; y <= 1; y++) {
+        for (int x = -1; x <= 1; x++) {
+            int cx = clampi((int)px + x, 0, (int)w - 1), cy = clampi((int)py + y, 0, (int)h - 1);
+            size_t j = (size_t)cy * w + cx;
+            f3 d = rgb_to_ycocg(S::ld3(io.diff_fast_history_texture, j));
+            f3 s = rgb_to_ycocg(S::ld3(io.spec_fast_history_texture, j));
+            dm1 = dm1 + d;
+            dm2 = dm2 + d * d;
+            sm1 = sm1 + s;
+            sm2 = sm2 + s * s;
+        }
+    }
+    const float ws = 9.0f;
+    dm1 = dm1 / ws;
+    dm2 = dm2 / ws;
+    sm1 = sm1 / ws;
+    sm2 = sm2 / ws;
+    f3 ddev = sqrt3(dm2 - dm1 * dm1) * 0.75f;
+    f3 sdev = sqrt3(sm2 - sm1 * sm1) * 0.75f;
+    f3 d = rgb_to_ycocg(S::ld3(io.rw_diff_texture, i));
+    f3 s = rgb_to_ycocg(S::ld3(io.rw_spec_texture, i));
+    d = clamp3(d, dm1 - ddev, dm1 + ddev);
+    s = clamp3(s, sm1 - sdev, sm1 + sdev);
+    S::st3(io.rw_diff_texture, i, ycocg_to_rgb(d));
+    S::st3(io.rw_spec_texture, i, ycocg_to_rgb(s));
+}
+
+// ---------------------------------------------------------------------------
+// A-trous iteration (svgf_atrous_filter.hlsl).  Taps with x != 0 AND y != 0
+// plus the explicit centre (the reference's `not_center` test drops the
+// whole centre row and column).
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(64) k_atrous(vgi_denoise_atrous_io io, int step) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+
+    float depth_center = io.depth_texture[i];
+    f3 nc = load_normal(io.gbuffer_texture, plane, i);
+    f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
+    f3 diff_center = S::ld3(io.diff_texture, i);
+    f3 spec_center = S::ld3(io.spec_texture, i);
+    f2 lumi_center = mk2(luminance(diff_center), luminance(spec_center));
+    f2 var_center = S::ld2(io.variance_texture, i);
+    EdgeStop es = EdgeStop::create(depth_center, fw.x, nc);
+    es.use_lumi = true;
+    es.lumi_center = lumi_center;
+    es.lumi_std = mk2(sqrtf(var_center.x), sqrtf(var_center.y));
+
+    const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    f3 dsum = diff_center, ssum = spec_center;
+    f2 vsum = var_center;
+    f2 wsum = mk2(1.0f, 1.0f);
+#pragma unroll
+    for (int y = -2; y <= 2; y++) {
+        float kwy = kw[y < 0 ? -y : y];
+#pragma unroll
+        for (int x = -2; x <= 2; x++) {
+            if (x == 0 || y == 0) continue;
+            int ox = x * step, oy = y * step;
+            int cx = (int)px + ox, cy = (int)py + oy;
+            if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
+            size_t j = (size_t)cy * w + cx;
+            float kwxy = kwy * kw[x < 0 ? -x : x];
+            f3 dv = S::ld3(io.diff_texture, j);
+            f3 sv = S::ld3(io.spec_texture, j);
+            float d = io.depth_texture[j];
+            f3 n = load_normal(io.gbuffer_texture, plane, j);
+            f2 lumi = mk2(luminance(dv), luminance(sv));
+            float fox = (float)ox, foy = (float)oy;
+            f2 e = es.weight(sqrtf(fox * fox + foy * foy), d, n, lumi);
+            float wx = kwxy * e.x, wy = kwxy * e.y;
+            f2 vv = S::ld2(io.variance_texture, j);
+            dsum = dsum + dv * wx;
+            ssum = ssum + sv * wy;
+            vsum = mk2(vsum.x + (wx * wx) * vv.x, vsum.y + (wy * wy) * vv.y);
+            wsum = mk2(wsum.x + wx, wsum.y + wy);
+        }
+    }
+    S::st3(io.rw_diff_texture, i, dsum / wsum.x);
+    S::st3(io.rw_spec_texture, i, ssum / wsum.y);
+    S::st2(io.rw_variance_texture, i, mk2(vsum.x / (wsum.x * wsum.x), vsum.y / (wsum.y * wsum.y)));
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static bool storage_ok(uint32_t s) { return s == VGI_STORAGE_F32 || s == VGI_STORAGE_F16; }
+
+// Algorithmic bytes per pixel (each used field read once, each output
+// written once): radiance R, pair P, scalar S bytes by storage mode.
+static double px_bytes(uint32_t storage, int nR, int nP, int nS, int fixed) {
+    return (double)(nR * (storage == VGI_STORAGE_F16 ? 6 : 12) + nP * (storage == VGI_STORAGE_F16 ? 4 : 8) +
+                    nS * (storage == VGI_STORAGE_F16 ? 2 : 4) + fixed);
+}
+
+vgi_status launch_denoise_temporal(const FrameParams& fp, const vgi_denoise_temporal_io& io,
+                                   const vgi_pc_denoise_temporal& pc, hipStream_t st) {
+    // reads: depth 4, L1+L2 8, diff/spec src, ray len, prev depth 4, prev L1 4,
+    // diff/spec/moments history, len 1; writes diff, spec, moments, len 1
+    ProfScope prof("denoise_temporal", st,
+                   px_bytes(io.storage, 6, 2, 1, 4 + 8 + 4 + 4 + 1 + 1) * io.width * io.height);
+    dim3 g = tile_grid(io.width, io.height);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL(k_temporal<StorageF32>, g, dim3(64), 0, st, fp, io, pc);
+    else
+        hipLaunchKernelGGL(k_temporal<StorageF16>, g, dim3(64), 0, st, fp, io, pc);
+    return check_launch("k_temporal");
+}
+vgi_status launch_denoise_disocclusion(const vgi_denoise_spatial_io& io, hipStream_t st) {
+    ProfScope prof("denoise_disocclusion_fix", st, px_bytes(io.storage, 4, 2, 0, 1) * io.width * io.height);
+    dim3 g = tile_grid(io.width, io.height);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL(k_disocclusion<StorageF32>, g, dim3(64), 0, st, io);
+    else
+        hipLaunchKernelGGL(k_disocclusion<StorageF16>, g, dim3(64), 0, st, io);
+    return check_launch("k_disocclusion");
+}
+vgi_status launch_denoise_fallback(const vgi_denoise_spatial_io& io, hipStream_t st) {
+    ProfScope prof("denoise_spatial_fallback", st, px_bytes(io.storage, 4, 2, 0, 1) * io.width * io.height);
+    dim3 g = tile_grid(io.width, io.height);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL(k_fallback<StorageF32>, g, dim3(64), 0, st, io);
+    else
+        hipLaunchKernelGGL(k_fallback<StorageF16>, g, dim3(64), 0, st, io);
+    return check_launch("k_fallback");
+}
+vgi_status launch_denoise_fast_clamp(const vgi_denoise_fast_clamp_io& io, hipStream_t st) {
+    ProfScope prof("denoise_fast_history_clamp", st, px_bytes(io.storage, 6, 0, 0, 0) * io.width * io.height);
+    dim3 g = tile_grid(io.width, io.height);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL(k_fast_clamp<StorageF32>, g, dim3(64), 0, st, io);
+    else
+        hipLaunchKernelGGL(k_fast_clamp<StorageF16>, g, dim3(64), 0, st, io);
+    return check_launch("k_fast_clamp");
+}
+vgi_status launch_denoise_atrous(const vgi_denoise_atrous_io& io, int step, hipStream_t st) {
+    // reads depth 4, L1+L2 8, diff, spec, variance; writes diff, spec, variance
+    ProfScope prof("denoise_atrous", st, px_bytes(io.storage, 4, 2, 0, 12) * io.width * io.height);
+    dim3 g = tile_grid(io.width, io.height);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(64), 0, st, io, step);
+    else
+        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(64), 0, st, io, step);
+    return check_launch("k_atrous");
+}
+
+}  // namespace vgi
+
+// ---------------------------------------------------------------------------
+// C ABI: per-pass entry points
+// ---------------------------------------------------------------------------
+using namespace vgi;
+
+extern "C" vgi_status vgi_pass_denoise_temporal(const vgi_frame_params* fp, const vgi_denoise_temporal_io* io,
+                                                const vgi_pc_denoise_temporal* pc, void* stream) {
+    VGI_REQUIRE(fp && io && pc, "vgi_pass_denoise_temporal: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_denoise_temporal: bad size %ux%u", io->width,
+                io->height);
+    VGI_REQUIRE(storage_ok(io->storage), "vgi_pass_denoise_temporal: bad storage mode %u", io->storage);
+    VGI_REQUIRE((uint32_t)pc->buffer_size[0] == io->width && (uint32_t)pc->buffer_size[1] == io->height,
+                "vgi_pass_denoise_temporal: buffer_size does not match the image size");
+    VGI_REQUIRE(io->depth_texture && io->gbuffer_texture && io->diff_source_texture && io->spec_source_texture &&
+                    io->spec_ray_len_texture && io->rw_diff_texture && io->rw_spec_texture &&
+                    io->rw_moments_texture && io->rw_history_len_texture,
+                "vgi_pass_denoise_temporal: missing binding");
+    if (pc->has_history)
+        VGI_REQUIRE(io->prev_depth_texture && io->prev_gbuffer_texture && io->diff_history_texture &&
+                        io->spec_history_texture && io->moments_history_texture && io->history_len_texture,
+                    "vgi_pass_denoise_temporal: has_history=1 but a history binding is NULL");
+    if (pc->fast_history)
+        VGI_REQUIRE(io->rw_diff_fast_history_texture && io->rw_spec_fast_history_texture &&
+                        (!pc->has_history || (io->diff_fast_history_texture && io->spec_fast_history_texture)),
+                    "vgi_pass_denoise_temporal: fast_history=1 but a fast-history binding is NULL");
+    FrameParams f;
+    static_assert(sizeof(vgi_frame_params) == sizeof(FrameParams), "layout");
+    memcpy(&f, fp, sizeof(f));
+    return launch_denoise_temporal(f, *io, *pc, (hipStream_t)stream);
+}
+
+static vgi_status check_spatial_io(const vgi_denoise_spatial_io* io, const char* name) {
+    VGI_REQUIRE(io, "%s: null argument", name);
+    VGI_REQUIRE(valid_dims(io->width, io->height), "%s: bad size %ux%u", name, io->width, io->height);
+    VGI_REQUIRE(storage_ok(io->storage), "%s: bad storage mode %u", name, io->storage);
+    VGI_REQUIRE(io->depth_texture && io->gbuffer_texture && io->history_len_texture && io->diff_texture &&
+                    io->spec_texture && io->moments_texture && io->rw_diff_texture && io->rw_spec_texture &&
+                    io->rw_moments_or_variance_texture,
+                "%s: missing binding", name);
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_pass_denoise_disocclusion_fix(const vgi_denoise_spatial_io* io, void* stream) {
+    vgi_status s = check_spatial_io(io, "vgi_pass_denoise_disocclusion_fix");
+    if (s != VGI_OK) return s;
+    return launch_denoise_disocclusion(*io, (hipStream_t)stream);
+}
+
+extern "C" vgi_status vgi_pass_denoise_spatial_fallback(const vgi_denoise_spatial_io* io, void* stream) {
+    vgi_status s = check_spatial_io(io, "vgi_pass_denoise_spatial_fallback");
+    if (s != VGI_OK) return s;
+    return launch_denoise_fallback(*io, (hipStream_t)stream);
+}
+
+extern "C" vgi_status vgi_pass_denoise_fast_history_clamp(const vgi_denoise_fast_clamp_io* io, void* stream) {
+    VGI_REQUIRE(io, "vgi_pass_denoise_fast_history_clamp: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_denoise_fast_history_clamp: bad size");
+    VGI_REQUIRE(storage_ok(io->storage), "vgi_pass_denoise_fast_history_clamp: bad storage mode");
+    VGI_REQUIRE(io->diff_fast_history_texture && io->spec_fast_history_texture && io->rw_diff_texture &&
+                    io->rw_spec_texture,
+                "vgi_pass_denoise_fast_history_clamp: missing binding");
+    return launch_denoise_fast_clamp(*io, (hipStream_t)stream);
+}
+
+extern "C" vgi_status vgi_pass_denoise_atrous(const vgi_denoise_atrous_io* io, const vgi_pc_denoise_atrous* pc,
+                                              void* stream) {
+    VGI_REQUIRE(io && pc, "vgi_pass_denoise_atrous: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_denoise_atrous: bad size %ux%u", io->width,
+                io->height);
+    VGI_REQUIRE(storage_ok(io->storage), "vgi_pass_denoise_atrous: bad storage mode %u", io->storage);
+    VGI_REQUIRE(pc->step_size >= 1 && pc->step_size <= 4096, "vgi_pass_denoise_atrous: bad step_size %d",
+                pc->step_size);
+    VGI_REQUIRE(io->depth_texture && io->gbuffer_texture && io->diff_texture && io->spec_texture &&
+                    io->variance_texture && io->rw_diff_texture && io->rw_spec_texture && io->rw_variance_texture,
+                "vgi_pass_denoise_atrous: missing binding");
+    return launch_denoise_atrous(*io, pc->step_size, (hipStream_t)stream);
+}

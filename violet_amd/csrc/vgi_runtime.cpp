@@ -1,0 +1,146 @@
+// vgi_runtime.cpp — error reporting for the C ABI (thread-local last error).
+#include "vgi_runtime.h"
+
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace vgi {
+
+static thread_local char g_last_error[512] = "";
+
+vgi_status set_error(vgi_status code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+vgi_status check_hip(hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory) return set_error(VGI_E_NOMEM, "%s: %s", what, hipGetErrorString(e));
+    return set_error(VGI_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+vgi_status check_launch(const char* kernel) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(VGI_E_HIP, "launch of %s failed: %s", kernel, hipGetErrorString(e));
+    return VGI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-pass profiler
+// ---------------------------------------------------------------------------
+namespace {
+struct Pending {
+    int id;
+    hipEvent_t a, b;
+    double bytes;
+};
+struct Accum {
+    std::string name;
+    double ms = 0;
+    uint64_t count = 0;
+    double bytes = 0;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<Accum> g_prof_acc;
+std::vector<Pending> g_prof_pending;
+std::vector<hipEvent_t> g_prof_pool;
+
+hipEvent_t take_event() {
+    if (!g_prof_pool.empty()) {
+        hipEvent_t e = g_prof_pool.back();
+        g_prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+int name_id(const char* name) {
+    for (size_t i = 0; i < g_prof_acc.size(); ++i)
+        if (g_prof_acc[i].name == name) return (int)i;
+    g_prof_acc.push_back(Accum{name});
+    return (int)g_prof_acc.size() - 1;
+}
+void drain_locked() {
+    for (auto& p : g_prof_pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            g_prof_acc[p.id].ms += ms;
+            g_prof_acc[p.id].count += 1;
+            g_prof_acc[p.id].bytes += p.bytes;
+        }
+        g_prof_pool.push_back(p.a);
+        g_prof_pool.push_back(p.b);
+    }
+    g_prof_pending.clear();
+}
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s, double bytes) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_on) return;
+    a_ = take_event();
+    if (!a_) return;
+    if (hipEventRecord(a_, s) != hipSuccess) {
+        g_prof_pool.push_back(a_);
+        a_ = nullptr;
+        return;
+    }
+    id_ = name_id(name);
+    s_ = s;
+    bytes_ = bytes;
+}
+
+ProfScope::~ProfScope() {
+    if (!a_) return;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    hipEvent_t b = take_event();
+    if (!b || hipEventRecord(b, s_) != hipSuccess) {
+        g_prof_pool.push_back(a_);
+        if (b) g_prof_pool.push_back(b);
+        return;
+    }
+    g_prof_pending.push_back(Pending{id_, a_, b, bytes_});
+}
+
+}  // namespace vgi
+
+extern "C" vgi_status vgi_profiler_enable(uint32_t enable) {
+    std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
+    vgi::g_prof_on = enable != 0;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_profiler_reset(void) {
+    std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
+    vgi::drain_locked();
+    vgi::g_prof_acc.clear();
+    return VGI_OK;
+}
+
+extern "C" int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap) {
+    std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
+    vgi::drain_locked();
+    int32_t n = 0;
+    for (auto& a : vgi::g_prof_acc) {
+        if (out && n < cap) {
+            memset(&out[n], 0, sizeof(out[n]));
+            snprintf(out[n].name, sizeof(out[n].name), "%s", a.name.c_str());
+            out[n].total_ms = a.ms;
+            out[n].count = a.count;
+            out[n].bytes = a.bytes;
+        }
+        ++n;
+    }
+    return n;
+}
+
+extern "C" const char* vgi_last_error(void) { return vgi::g_last_error; }
+
+extern "C" const char* vgi_version(void) { return "vgi 0.1 (gfx950)"; }

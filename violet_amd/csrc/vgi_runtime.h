@@ -1,0 +1,71 @@
+/*
+ * vgi_runtime.h — host-side helpers shared by the C-ABI translation units:
+ * error reporting (thread-local message + status codes), HIP call checking,
+ * and the tile mapping used by every per-pixel kernel.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/vgi.h"
+#include "vgi_common.h"
+
+namespace vgi {
+
+vgi_status set_error(vgi_status code, const char* fmt, ...);
+vgi_status check_hip(hipError_t e, const char* what);
+
+#define VGI_TRY_HIP(expr)                                        \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return ::vgi::check_hip(_e, #expr); \
+    } while (0)
+
+#define VGI_REQUIRE(cond, ...)                                              \
+    do {                                                                    \
+        if (!(cond)) return ::vgi::set_error(VGI_E_INVALID, __VA_ARGS__); \
+    } while (0)
+
+// Launch-after check: reports a launch configuration error synchronously.
+vgi_status check_launch(const char* kernel);
+
+constexpr uint32_t kMaxDim = 16384;
+
+// Per-pass GPU timer (active only while vgi_profiler_enable(1)); records a
+// hipEvent before construction's caller launches and one at destruction.
+class ProfScope {
+   public:
+    ProfScope(const char* name, hipStream_t s, double bytes);
+    ~ProfScope();
+
+   private:
+    int id_ = -1;
+    hipStream_t s_ = nullptr;
+    hipEvent_t a_ = nullptr;
+    double bytes_ = 0;
+};
+
+inline bool valid_dims(uint32_t w, uint32_t h) { return w > 0 && h > 0 && w <= kMaxDim && h <= kMaxDim; }
+
+inline dim3 tile_grid(uint32_t w, uint32_t h) { return dim3((w + 7) / 8, (h + 7) / 8, 1); }
+
+}  // namespace vgi
+
+#if defined(__HIPCC__)
+namespace vgi {
+// Thread -> pixel mapping for an 8x8 tile processed by one wave64: lanes are
+// grouped into 2x2 quads (lane>>2 = quad index in a 4x4 grid of quads,
+// lane&3 = position in the quad), as in the quad remap of
+// relax_disocclusion_fix.hlsl:36-40.  Quad members are lanes 4q..4q+3, so a
+// quad reduction is two xor-shuffles.
+__device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) {
+    uint32_t lane = threadIdx.x;
+    uint32_t q = lane >> 2;
+    uint32_t qi = lane & 3u;
+    px = blockIdx.x * 8u + ((q & 3u) << 1) + (qi & 1u);
+    py = blockIdx.y * 8u + ((q >> 2) << 1) + (qi >> 1);
+}
+}  // namespace vgi
+#endif

@@ -1,0 +1,86 @@
+/*
+ * vgi_storage.h — per-pixel storage policies.
+ *
+ * A storage policy decides how the floating-point per-pixel fields of the
+ * hot path (radiance RGB, moments/variance pairs, ray length, AO) are laid
+ * out in HBM.  Integer encodings (G-buffer layers, packed reservoirs, hit
+ * pos/normal, history length as an 8-bit count) are identical in every
+ * mode.  All images are row-major w*h arrays of their element type
+ * ("array of structs"); a wave of 64 lanes covering an 8x8 tile therefore
+ * reads eight contiguous 8-pixel segments per field.
+ *
+ *   StorageF32: radiance = 3 x f32 (12 B), pair = 2 x f32 (8 B), scalar f32
+ *   StorageF16: radiance = 3 x f16 (6 B),  pair = 2 x f16 (4 B), scalar f16
+ *
+ * Arithmetic is always f32.
+ */
+#pragma once
+#include "vgi_common.h"
+
+namespace vgi {
+
+struct StorageF32 {
+    static constexpr int kMode = 0;
+    typedef float scalar_t;
+    static constexpr int kRadianceBytes = 12;
+    static constexpr int kPairBytes = 8;
+    static constexpr int kScalarBytes = 4;
+    VGI_HD static f3 ld3(const void* p, size_t i) {
+        const float* f = (const float*)p + 3 * i;
+        return mk3(f[0], f[1], f[2]);
+    }
+    VGI_HD static void st3(void* p, size_t i, f3 v) {
+        float* f = (float*)p + 3 * i;
+        f[0] = v.x;
+        f[1] = v.y;
+        f[2] = v.z;
+    }
+    VGI_HD static f2 ld2(const void* p, size_t i) {
+        const float* f = (const float*)p + 2 * i;
+        return mk2(f[0], f[1]);
+    }
+    VGI_HD static void st2(void* p, size_t i, f2 v) {
+        float* f = (float*)p + 2 * i;
+        f[0] = v.x;
+        f[1] = v.y;
+    }
+    VGI_HD static float ld1(const void* p, size_t i) { return ((const float*)p)[i]; }
+    VGI_HD static void st1(void* p, size_t i, float v) { ((float*)p)[i] = v; }
+};
+
+struct StorageF16 {
+    static constexpr int kMode = 1;
+    typedef uint16_t scalar_t;
+    static constexpr int kRadianceBytes = 6;
+    static constexpr int kPairBytes = 4;
+    static constexpr int kScalarBytes = 2;
+    VGI_HD static f3 ld3(const void* p, size_t i) {
+        const uint16_t* h = (const uint16_t*)p + 3 * i;
+        return mk3(f16tof32(h[0]), f16tof32(h[1]), f16tof32(h[2]));
+    }
+    VGI_HD static void st3(void* p, size_t i, f3 v) {
+        uint16_t* h = (uint16_t*)p + 3 * i;
+        h[0] = (uint16_t)f32tof16(v.x);
+        h[1] = (uint16_t)f32tof16(v.y);
+        h[2] = (uint16_t)f32tof16(v.z);
+    }
+    VGI_HD static f2 ld2(const void* p, size_t i) {
+        uint32_t u = ((const uint32_t*)p)[i];
+        return mk2(f16tof32(u & 0xFFFFu), f16tof32(u >> 16));
+    }
+    VGI_HD static void st2(void* p, size_t i, f2 v) {
+        ((uint32_t*)p)[i] = f32tof16(v.x) | (f32tof16(v.y) << 16);
+    }
+    VGI_HD static float ld1(const void* p, size_t i) { return f16tof32(((const uint16_t*)p)[i]); }
+    VGI_HD static void st1(void* p, size_t i, float v) { ((uint16_t*)p)[i] = (uint16_t)f32tof16(v); }
+};
+
+// History length: R8_UNORM in the reference (denoising.rs:122-125).  Stored
+// as the 8-bit count round(len * 255) (round half to even), read as k/255.
+VGI_HD uint8_t history_len_store(float len) {
+    float s = saturate(len) * 255.0f;
+    return (uint8_t)rintf(s);
+}
+VGI_HD float history_len_load(uint8_t k) { return (float)k / 255.0f; }
+
+}  // namespace vgi
