@@ -228,6 +228,54 @@ vgi_status vgi_denoiser_set_config(vgi_denoiser* d, const vgi_denoiser_config* c
 vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_params* fp, const vgi_denoiser_input* in,
                                    vgi_denoiser_output* out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Scene and software ray tracing.
+ * Replaces the BLAS/TLAS build of crates/violet/src/render_scene.rs:
+ * 1170-1210 (BLAS per geometry group, UINT16 indices, f32x3 positions) and
+ * :1557-1712 (TLAS over instances) with a host-built binary BVH in HBM, and
+ * TraceRay (restir/raytrace.inc.hlsl:42-53,74-82) with HIP traversal.
+ * Instances are baked to world space by the caller; indices are u32.
+ * ---------------------------------------------------------------------- */
+typedef struct vgi_scene vgi_scene;
+
+/* MaterialParams factors (shader/scene_bindings.hlsl:24-51); textures are
+ * not sampled (untextured scenes: every texture fetch returns 1). */
+typedef struct vgi_material {
+    float base_color[4];
+    float metallic;
+    float roughness; /* perceptual roughness */
+    float pad[2];
+} vgi_material;
+
+typedef struct vgi_scene_desc {
+    const float* positions;          /* world space, 3 floats per vertex */
+    const float* normals;            /* per-vertex normals (3 floats) or NULL = geometric normal */
+    uint32_t num_vertices;
+    const uint32_t* indices;         /* 3 per triangle */
+    uint32_t num_triangles;
+    const uint32_t* triangle_material; /* per triangle or NULL (= material 0) */
+    const vgi_material* materials;
+    uint32_t num_materials;
+} vgi_scene_desc;
+
+typedef struct vgi_scene_stats {
+    uint32_t num_triangles, num_vertices, num_nodes, depth;
+    uint64_t device_bytes;
+} vgi_scene_stats;
+
+/* Host arrays in, BVH built on the host, uploaded to the current device. */
+vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** out);
+vgi_status vgi_scene_destroy(vgi_scene* s);
+vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out);
+
+/* Batch ray query (TraceRay without shading).  rays: 8 floats per ray
+ * {o.x,o.y,o.z,tmin, d.x,d.y,d.z,tmax} in device memory; hits: 4 words per
+ * ray {t (f32), u (f32), v (f32), triangle id (u32, 0xFFFFFFFF = miss)}.
+ * any_hit != 0 gives shadow-ray semantics (first accepted hit, t and
+ * barycentrics unspecified, id = some hit triangle). */
+vgi_status vgi_trace_rays(const vgi_scene* scene, const float* rays, uint32_t num_rays, uint32_t any_hit,
+                          uint32_t* hits, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,9 @@ def load(bits: int = 64) -> C.CDLL:
         lib.oracle_denoise_fallback.argtypes = [I, I] + [P] * 9
         lib.oracle_denoise_fast_clamp.argtypes = [I, I] + [P] * 4
         lib.oracle_denoise_atrous.argtypes = [I, I, I] + [P] * 8
+        for n in ("oracle_trace_brute", "oracle_trace_bvh"):
+            getattr(lib, n).argtypes = [P, P, U32, P, U32, I, P, P]
+        lib.oracle_num_threads.restype = I
         _libs[bits] = lib
     return _libs[bits]
 
@@ -147,6 +150,21 @@ class Oracle:
                                        _np_ptr(self.r(diff)), _np_ptr(self.r(spec)), _np_ptr(self.r(var)),
                                        _np_ptr(od), _np_ptr(osp), _np_ptr(ov))
         return dict(diff=od, spec=osp, variance=ov)
+
+
+def trace(bits: int, positions, indices, rays, any_hit: bool = False, brute: bool = True):
+    """Closest/any hit of f32 rays (N,8) against the triangle soup.
+    Returns (thv (N,3) REAL, ids (N,) u32)."""
+    lib = load(bits)
+    P_ = np.ascontiguousarray(positions, np.float32)
+    I_ = np.ascontiguousarray(indices, np.uint32)
+    R_ = np.ascontiguousarray(rays, np.float32)
+    n = len(R_)
+    thv = np.zeros((n, 3), real_dtype(bits))
+    ids = np.zeros(n, np.uint32)
+    fn = lib.oracle_trace_brute if brute else lib.oracle_trace_bvh
+    fn(_np_ptr(P_), _np_ptr(I_), len(I_), _np_ptr(R_), n, int(any_hit), _np_ptr(thv), _np_ptr(ids))
+    return thv, ids
 
 
 class OracleDenoiser:

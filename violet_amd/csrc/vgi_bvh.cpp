@@ -1,0 +1,315 @@
+/*
+ * vgi_bvh.cpp — scene upload and the host BVH builder (binned SAH).
+ *
+ * Replaces violet's driver-built BLAS/TLAS (render_scene.rs:1170-1210 BLAS
+ * per geometry group, :1557-1712 TLAS over instances).  Instances are baked
+ * to world space by the caller, so one binary BVH over all triangles serves
+ * every ray.  Build: 3-axis binned SAH (32 bins); leaves hold at most 15
+ * triangles; below depth 40 the split switches to the object median so the
+ * depth (and the traversal stack) stays bounded by kBvhMaxDepth.
+ */
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "vgi_bvh.h"
+#include "vgi_internal.h"
+
+using namespace vgi;
+
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    void grow(const float* p) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    double area() const {
+        double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.0;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Prim {
+    Box box;
+    float c[3];
+    uint32_t id;
+};
+
+constexpr int kBins = 32;
+constexpr int kMaxLeaf = 15;
+constexpr int kMedianDepth = 36;
+
+struct Builder {
+    std::vector<Prim> prims;
+    std::vector<BvhNode> nodes;
+    std::vector<uint32_t> order;
+    int max_depth = 0;
+
+    static void pad(Box& b) {
+        float s = 1.0f;
+        for (int a = 0; a < 3; ++a) s = std::max(s, std::max(std::fabs(b.lo[a]), std::fabs(b.hi[a])));
+        float p = s * (1.0f / 65536.0f);
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] -= p;
+            b.hi[a] += p;
+        }
+    }
+
+    int32_t make_leaf(int begin, int end) {
+        uint32_t first = (uint32_t)order.size();
+        for (int i = begin; i < end; ++i) order.push_back(prims[i].id);
+        return ~(int32_t)((first << 4) | (uint32_t)(end - begin));
+    }
+
+    int32_t build(int begin, int end, int depth, Box& out) {
+        max_depth = std::max(max_depth, depth);
+        Box box, cbox;
+        for (int i = begin; i < end; ++i) {
+            box.grow(prims[i].box);
+            cbox.grow(prims[i].c);
+        }
+        out = box;
+        int n = end - begin;
+        if (n <= 2 && n <= kMaxLeaf) return make_leaf(begin, end);
+        int mid = -1;
+        if (depth < kMedianDepth) {
+            // binned SAH over 3 axes
+            double best_cost = INFINITY;
+            int best_axis = -1, best_split = -1;
+            for (int a = 0; a < 3; ++a) {
+                float ext = cbox.hi[a] - cbox.lo[a];
+                if (!(ext > 0.0f)) continue;
+                Box bb[kBins];
+                int cnt[kBins] = {0};
+                float k = (float)kBins / ext;
+                for (int i = begin; i < end; ++i) {
+                    int b = (int)((prims[i].c[a] - cbox.lo[a]) * k);
+                    b = std::min(std::max(b, 0), kBins - 1);
+                    cnt[b]++;
+                    bb[b].grow(prims[i].box);
+                }
+                double right_area[kBins];
+                int right_cnt[kBins];
+                Box acc;
+                int c = 0;
+                for (int b = kBins - 1; b > 0; --b) {
+                    acc.grow(bb[b]);
+                    c += cnt[b];
+                    right_area[b] = acc.area();
+                    right_cnt[b] = c;
+                }
+                Box lacc;
+                int lc = 0;
+                for (int b = 1; b < kBins; ++b) {
+                    lacc.grow(bb[b - 1]);
+                    lc += cnt[b - 1];
+                    if (lc == 0 || right_cnt[b] == 0) continue;
+                    double cost = lacc.area() * lc + right_area[b] * right_cnt[b];
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = a;
+                        best_split = b;
+                    }
+                }
+            }
+            double parent = box.area();
+            double leaf_cost = (double)n;
+            double split_cost = parent > 0 ? 0.125 + best_cost / parent : INFINITY;
+            if (best_axis >= 0 && !(n <= 4 && split_cost >= leaf_cost)) {
+                float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+                float k = (float)kBins / ext;
+                auto it = std::partition(prims.begin() + begin, prims.begin() + end, [&](const Prim& p) {
+                    int b = (int)((p.c[best_axis] - cbox.lo[best_axis]) * k);
+                    b = std::min(std::max(b, 0), kBins - 1);
+                    return b < best_split;
+                });
+                mid = (int)(it - prims.begin());
+                if (mid == begin || mid == end) mid = -1;
+            } else if (n <= kMaxLeaf && best_axis >= 0) {
+                return make_leaf(begin, end);
+            } else if (n <= kMaxLeaf && best_axis < 0) {
+                return make_leaf(begin, end);
+            }
+        }
+        if (mid < 0) {
+            if (n <= kMaxLeaf && depth >= kMedianDepth) return make_leaf(begin, end);
+            // object median along the widest centroid axis
+            int a = 0;
+            float best = -1;
+            for (int k = 0; k < 3; ++k) {
+                float e = cbox.hi[k] - cbox.lo[k];
+                if (e > best) {
+                    best = e;
+                    a = k;
+                }
+            }
+            mid = begin + n / 2;
+            std::nth_element(prims.begin() + begin, prims.begin() + mid, prims.begin() + end,
+                             [a](const Prim& x, const Prim& y) { return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.id < y.id); });
+        }
+        int32_t idx = (int32_t)nodes.size();
+        nodes.push_back(BvhNode{});
+        Box lb, rb;
+        int32_t l = build(begin, mid, depth + 1, lb);
+        int32_t r = build(mid, end, depth + 1, rb);
+        pad(lb);
+        pad(rb);
+        BvhNode& nd = nodes[idx];
+        nd.c0xy = float4_{lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]};
+        nd.c1xy = float4_{rb.lo[0], rb.hi[0], rb.lo[1], rb.hi[1]};
+        nd.z = float4_{lb.lo[2], lb.hi[2], rb.lo[2], rb.hi[2]};
+        nd.link[0] = l;
+        nd.link[1] = r;
+        nd.pad[0] = nd.pad[1] = 0;
+        return idx;
+    }
+};
+
+}  // namespace
+
+struct vgi_scene {
+    DevBuf nodes, tris, shade, positions, normals, materials;
+    SceneView view{};
+    uint32_t num_vertices = 0, num_triangles = 0, depth = 0;
+};
+
+namespace vgi {
+const SceneView& scene_view(const vgi_scene* s) { return s->view; }
+}  // namespace vgi
+
+template <class T>
+static vgi_status upload(DevBuf& b, const std::vector<T>& v) {
+    vgi_status s = b.alloc(v.size() * sizeof(T));
+    if (s != VGI_OK) return s;
+    if (!v.empty()) VGI_TRY_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** out) {
+    VGI_REQUIRE(desc && out, "vgi_scene_create: null argument");
+    *out = nullptr;
+    VGI_REQUIRE(desc->positions && desc->indices && desc->num_vertices > 0 && desc->num_triangles > 0,
+                "vgi_scene_create: empty scene");
+    VGI_REQUIRE(desc->num_triangles < (1u << 27), "vgi_scene_create: too many triangles (%u)", desc->num_triangles);
+    VGI_REQUIRE(desc->materials && desc->num_materials > 0, "vgi_scene_create: no materials");
+    const uint32_t nt = desc->num_triangles, nv = desc->num_vertices;
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        VGI_REQUIRE(desc->indices[i] < nv, "vgi_scene_create: index %u out of range at %u", desc->indices[i], i);
+    if (desc->triangle_material)
+        for (uint32_t i = 0; i < nt; ++i)
+            VGI_REQUIRE(desc->triangle_material[i] < desc->num_materials,
+                        "vgi_scene_create: material index out of range at triangle %u", i);
+
+    Builder b;
+    b.prims.resize(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        Prim& p = b.prims[t];
+        for (int k = 0; k < 3; ++k) p.box.grow(desc->positions + 3 * desc->indices[3 * t + k]);
+        for (int a = 0; a < 3; ++a) p.c[a] = 0.5f * (p.box.lo[a] + p.box.hi[a]);
+        p.id = t;
+    }
+    b.nodes.reserve(2 * (size_t)nt / 4 + 4);
+    b.order.reserve(nt);
+    Box root_box;
+    int32_t root = b.build(0, (int)nt, 0, root_box);
+    if (root < 0) {
+        // tiny scene: wrap the single leaf in a node with an empty sibling
+        BvhNode nd{};
+        Box lb = root_box;
+        Builder::pad(lb);
+        nd.c0xy = float4_{lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]};
+        nd.c1xy = float4_{1.0f, -1.0f, 1.0f, -1.0f};
+        nd.z = float4_{lb.lo[2], lb.hi[2], 1.0f, -1.0f};
+        nd.link[0] = root;
+        nd.link[1] = root;
+        b.nodes.push_back(nd);
+        root = 0;
+    }
+    if (b.max_depth + 1 > kBvhMaxDepth)
+        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: BVH depth %d exceeds %d", b.max_depth + 1, kBvhMaxDepth);
+
+    std::vector<BvhTri> tris(nt);
+    std::vector<TriShade> shade(nt);
+    for (uint32_t k = 0; k < nt; ++k) {
+        uint32_t t = b.order[k];
+        const float* p0 = desc->positions + 3 * desc->indices[3 * t + 0];
+        const float* p1 = desc->positions + 3 * desc->indices[3 * t + 1];
+        const float* p2 = desc->positions + 3 * desc->indices[3 * t + 2];
+        tris[k].v0 = float4_{p0[0], p0[1], p0[2], asfloat(t)};
+        tris[k].e1 = float4_{p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], 0.0f};
+        tris[k].e2 = float4_{p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f};
+    }
+    for (uint32_t t = 0; t < nt; ++t)
+        shade[t] = TriShade{desc->indices[3 * t], desc->indices[3 * t + 1], desc->indices[3 * t + 2],
+                            desc->triangle_material ? desc->triangle_material[t] : 0u};
+    std::vector<float> pos(desc->positions, desc->positions + 3 * (size_t)nv);
+    std::vector<float> nrm;
+    if (desc->normals) {
+        nrm.assign(desc->normals, desc->normals + 3 * (size_t)nv);
+    } else {
+        // flat shading: per-vertex normal slot unused; use the geometric normal
+        nrm.assign(3 * (size_t)nv, 0.0f);
+    }
+    std::vector<Material> mats(desc->num_materials);
+    for (uint32_t m = 0; m < desc->num_materials; ++m) {
+        const vgi_material& s = desc->materials[m];
+        Material& d = mats[m];
+        for (int k = 0; k < 4; ++k) d.base_color[k] = s.base_color[k];
+        d.metallic = s.metallic;
+        d.roughness = s.roughness;
+        d.pad[0] = d.pad[1] = 0.0f;
+    }
+
+    vgi_scene* sc = new (std::nothrow) vgi_scene();
+    if (!sc) return set_error(VGI_E_NOMEM, "vgi_scene_create: out of host memory");
+    vgi_status st;
+    if ((st = upload(sc->nodes, b.nodes)) != VGI_OK || (st = upload(sc->tris, tris)) != VGI_OK ||
+        (st = upload(sc->shade, shade)) != VGI_OK || (st = upload(sc->positions, pos)) != VGI_OK ||
+        (st = upload(sc->normals, nrm)) != VGI_OK || (st = upload(sc->materials, mats)) != VGI_OK) {
+        delete sc;
+        return st;
+    }
+    sc->num_vertices = nv;
+    sc->num_triangles = nt;
+    sc->depth = (uint32_t)b.max_depth + 1;
+    sc->view.nodes = sc->nodes.as<BvhNode>();
+    sc->view.tris = sc->tris.as<BvhTri>();
+    sc->view.shade = sc->shade.as<TriShade>();
+    sc->view.positions = sc->positions.as<float>();
+    sc->view.normals = desc->normals ? sc->normals.as<float>() : nullptr;
+    sc->view.materials = sc->materials.as<Material>();
+    sc->view.num_nodes = (uint32_t)b.nodes.size();
+    sc->view.num_tris = nt;
+    sc->view.root = root;
+    *out = sc;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_scene_destroy(vgi_scene* s) {
+    delete s;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out) {
+    VGI_REQUIRE(s && out, "vgi_scene_info: null argument");
+    out->num_triangles = s->num_triangles;
+    out->num_vertices = s->num_vertices;
+    out->num_nodes = s->view.num_nodes;
+    out->depth = s->depth;
+    out->device_bytes = (uint64_t)(s->nodes.bytes + s->tris.bytes + s->shade.bytes + s->positions.bytes +
+                                   s->normals.bytes + s->materials.bytes);
+    return VGI_OK;
+}

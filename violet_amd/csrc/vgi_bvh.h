@@ -1,0 +1,96 @@
+/*
+ * vgi_bvh.h — software ray tracing on gfx950 (CDNA4 has no ray-tracing
+ * hardware): device layout of the scene BVH and the traversal routines that
+ * replace violet's VK_KHR_acceleration_structure + TraceRay
+ * (restir/raytrace.inc.hlsl:31-56,67-82; render_scene.rs:1170-1712).
+ *
+ * Layout (built on the host by vgi_bvh.cpp, uploaded once per scene):
+ *   nodes : binary BVH, one 64-byte node holds both children's boxes
+ *           (x/y of child 0, x/y of child 1, z of both, child links), so a
+ *           wave fetches a node with four 16-byte loads.  A child link >= 0
+ *           is an inner node index; < 0 encodes a leaf ~((first << 4) | n).
+ *   tris  : leaf-ordered triangles as {v0, e1 = v1-v0, e2 = v2-v0} (3 x
+ *           float4, 48 bytes); v0.w carries the scene triangle id.
+ * Child boxes are padded by a small relative margin so the slab test is
+ * conservative: a triangle the intersection routine accepts is never pruned
+ * by its ancestors' boxes, and the closest hit (smallest t, ties to the
+ * lowest triangle id) equals a brute-force loop over all triangles.
+ *
+ * Hit semantics follow violet: opaque, two-sided, t in [tmin, tmax];
+ * shadow rays accept the first hit (RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH).
+ */
+#pragma once
+#include "vgi_common.h"
+
+namespace vgi {
+
+struct BvhNode {
+    float4_ c0xy;  // c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y
+    float4_ c1xy;  // c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y
+    float4_ z;     // c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z
+    int32_t link[2];
+    int32_t pad[2];
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+
+struct BvhTri {
+    float4_ v0;  // w = triangle id (bits)
+    float4_ e1;
+    float4_ e2;
+};
+static_assert(sizeof(BvhTri) == 48, "BvhTri layout");
+
+// Per-triangle shading data, indexed by scene triangle id.
+struct TriShade {
+    uint32_t i0, i1, i2;  // vertex indices
+    uint32_t material;
+};
+
+// MaterialParams factors (scene_bindings.hlsl:24-51), untextured.
+struct Material {
+    float base_color[4];
+    float metallic;
+    float roughness;
+    float pad[2];
+};
+
+struct SceneView {
+    const BvhNode* nodes;
+    const BvhTri* tris;
+    const TriShade* shade;
+    const float* positions;  // 3 floats per vertex
+    const float* normals;    // 3 floats per vertex
+    const Material* materials;
+    uint32_t num_nodes;
+    uint32_t num_tris;
+    int32_t root;  // link of the root (node 0, or a leaf for tiny scenes)
+};
+
+constexpr float kRayTMin = 0.0005f;  // raytrace.inc.hlsl:39,70
+constexpr float kRayTMax = 1000.0f;  // raytrace.inc.hlsl:40,71
+constexpr int kBvhMaxDepth = 64;     // builder guarantees depth <= this (= stack size)
+
+struct Hit {
+    float t;
+    float u, v;  // barycentric weights of vertex 1 and vertex 2
+    uint32_t tri;  // scene triangle id; 0xFFFFFFFF = miss
+};
+
+// Moller-Trumbore, two-sided.  Fixed evaluation order (bit-identical on the
+// host and the device).
+VGI_HD bool intersect_tri(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float tmin, float& t, float& u, float& v) {
+    f3 pvec = cross3(d, e2);
+    float det = dot3(e1, pvec);
+    if (det == 0.0f) return false;
+    float inv_det = 1.0f / det;
+    f3 tvec = o - v0;
+    u = dot3(tvec, pvec) * inv_det;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    f3 qvec = cross3(tvec, e1);
+    v = dot3(d, qvec) * inv_det;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    t = dot3(e2, qvec) * inv_det;
+    return t >= tmin;
+}
+
+}  // namespace vgi

@@ -1,0 +1,131 @@
+/*
+ * vgi_traverse.h — device-side BVH traversal (closest hit and any hit).
+ *
+ * One lane per ray.  The traversal stack lives in LDS, interleaved across
+ * the 64 lanes of the wave (entry k of lane l at [k*64 + l]) so that a push
+ * or pop by all lanes touches 64 consecutive dwords (no bank conflicts).
+ * Near child first; the far child is pushed.  Boxes are tested with the
+ * slab method in exact IEEE single precision; NaNs from 0*inf drop out of
+ * fminf/fmaxf, which only widens the interval (conservative).
+ */
+#pragma once
+#include "vgi_bvh.h"
+
+namespace vgi {
+
+struct RayInv {
+    f3 o, d, idir;
+};
+
+__device__ __forceinline__ RayInv make_ray(f3 o, f3 d) {
+    RayInv r;
+    r.o = o;
+    r.d = d;
+    r.idir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    return r;
+}
+
+// Returns the entry distance of the box on [tmin, tmax], or +inf if missed.
+__device__ __forceinline__ float slab(const RayInv& r, float lox, float hix, float loy, float hiy, float loz, float hiz,
+                                      float tmin, float tmax) {
+    float tx0 = (lox - r.o.x) * r.idir.x, tx1 = (hix - r.o.x) * r.idir.x;
+    float ty0 = (loy - r.o.y) * r.idir.y, ty1 = (hiy - r.o.y) * r.idir.y;
+    float tz0 = (loz - r.o.z) * r.idir.z, tz1 = (hiz - r.o.z) * r.idir.z;
+    float tn = fmaxf(fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1)), tmin);
+    float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), tmax);
+    return tn <= tf ? tn : INFINITY;
+}
+
+__device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, int32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float4 a = p[0];
+    float4 b = p[1];
+    float4 c = p[2];
+    int4 d = *reinterpret_cast<const int4*>(p + 3);
+    BvhNode n;
+    n.c0xy = float4_{a.x, a.y, a.z, a.w};
+    n.c1xy = float4_{b.x, b.y, b.z, b.w};
+    n.z = float4_{c.x, c.y, c.z, c.w};
+    n.link[0] = d.x;
+    n.link[1] = d.y;
+    return n;
+}
+
+// Closest hit over [tmin, tmax]; ties in t resolve to the lowest triangle id.
+// `stack` points at this lane's slot 0 (stride 64 between entries).
+template <bool kAnyHit>
+__device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+    RayInv r = make_ray(o, d);
+    Hit h;
+    h.t = tmax;
+    h.u = h.v = 0.0f;
+    h.tri = 0xFFFFFFFFu;
+    int sp = 0;
+    int32_t link = s.root;
+    // Each node and leaf of a tree is reached at most once; the budget only
+    // guards against a corrupted BVH turning into an endless loop.
+    uint32_t budget = 2u * s.num_nodes + 8u;
+    while (budget-- != 0u) {
+        if (link >= 0) {
+            BvhNode n = load_node(s.nodes, link);
+            float t0 = slab(r, n.c0xy.x, n.c0xy.y, n.c0xy.z, n.c0xy.w, n.z.x, n.z.y, tmin, h.t);
+            float t1 = slab(r, n.c1xy.x, n.c1xy.y, n.c1xy.z, n.c1xy.w, n.z.z, n.z.w, tmin, h.t);
+            bool b0 = t0 != INFINITY, b1 = t1 != INFINITY;
+            if (b0 && b1) {
+                int32_t nearl = n.link[0], farl = n.link[1];
+                if (t1 < t0) {
+                    nearl = n.link[1];
+                    farl = n.link[0];
+                }
+                if (sp < kBvhMaxDepth) {  // always true for a BVH from vgi_scene_create
+                    stack[sp * 64] = farl;
+                    ++sp;
+                }
+                link = nearl;
+                continue;
+            }
+            if (b0 || b1) {
+                link = b0 ? n.link[0] : n.link[1];
+                continue;
+            }
+        } else {
+            uint32_t enc = (uint32_t)(~link);
+            uint32_t first = enc >> 4, cnt = enc & 15u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(s.tris + first + k);
+                float4 a = tp[0], e1 = tp[1], e2 = tp[2];
+                float t, u, v;
+                if (intersect_tri(o, d, mk3(a.x, a.y, a.z), mk3(e1.x, e1.y, e1.z), mk3(e2.x, e2.y, e2.z), tmin, t, u,
+                                  v)) {
+                    uint32_t id = __float_as_uint(a.w);
+                    if (t < h.t || (t == h.t && id < h.tri)) {
+                        h.t = t;
+                        h.u = u;
+                        h.v = v;
+                        h.tri = id;
+                        if (kAnyHit) return h;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        link = stack[sp * 64];
+    }
+    return h;
+}
+
+__device__ __forceinline__ Hit trace_closest(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+    return traverse<false>(s, o, d, tmin, tmax, stack);
+}
+// Shadow ray: true if something is hit on [tmin, tmax] (trace_shadow returns !this).
+__device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+    return traverse<true>(s, o, d, tmin, tmax, stack).tri != 0xFFFFFFFFu;
+}
+
+// Per-wave LDS traversal stack: kBvhMaxDepth entries x 64 lanes.
+#define VGI_TRAVERSAL_STACK(name)                        \
+    __shared__ int32_t name##_mem[kBvhMaxDepth * 64];    \
+    int32_t* name = name##_mem + (threadIdx.x & 63u)
+
+}  // namespace vgi
