@@ -276,6 +276,315 @@ vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out);
 vgi_status vgi_trace_rays(const vgi_scene* scene, const float* rays, uint32_t num_rays, uint32_t any_hit,
                           uint32_t* hits, void* stream);
 
+/* ------------------------------------------------------------------------
+ * ReSTIR GI passes (shader/restir/).
+ * ---------------------------------------------------------------------- */
+
+/* HashGridCell (restir/hash_grid.inc.hlsl:112-122): 16 bytes.  The
+ * reference host allocates 20 B per cell (restir_renderer.rs:27-28); the
+ * HLSL struct is 16 B and that packed layout is used here. */
+typedef struct vgi_hash_grid_cell {
+    uint32_t checksum; /* 0 = empty */
+    float radiance[3];
+} vgi_hash_grid_cell;
+
+/* HashGridQuery (restir/hash_grid.inc.hlsl:186-207): 24 bytes. */
+typedef struct vgi_hash_grid_query {
+    float hit_position[3];
+    uint32_t hit_normal_encoded; /* oct u32 of the geometric normal */
+    uint32_t cell_hash;
+    uint32_t cell_checksum;
+} vgi_hash_grid_query;
+
+#define VGI_HASH_GRID_NUM_CELLS (65536u * 16u)
+#define VGI_HASH_GRID_MAX_QUERIES (128u * 1024u)
+
+/* Resources of restir/raytrace.inc.hlsl:17-28 bound by every radiance-trace
+ * pass (restir_renderer.rs:99-121 restir_raytrace_resrouces +
+ * hg_cache_resources).  The sky cube is replaced by a constant sky
+ * radiance.  prev_* may be NULL when the trace pass has has_prev_frame = 0. */
+typedef struct vgi_trace_resources {
+    const vgi_scene* scene_tlas;
+    float sky_radiance[3];
+    uint32_t storage;                           /* storage mode of prev_indirect_diffuse_texture */
+    const void* prev_indirect_diffuse_texture;  /* radiance, previous frame's denoised diffuse */
+    const float* prev_depth_texture;
+    uint32_t prev_width, prev_height;           /* size of the two prev_* images */
+    const vgi_hash_grid_cell* hash_grid_storage_buffer;
+    vgi_hash_grid_query* rw_hash_grid_query_buffer;
+    uint32_t* rw_hash_grid_query_counter_buffer;
+} vgi_trace_resources;
+
+/* restir/ind_diff_sample_gen.hlsl:11-23; restir_renderer.rs:478-492 */
+typedef struct vgi_ind_diff_sample_gen_io {
+    uint32_t width, height, storage;
+    const uint32_t* gbuffer_color;
+    const float* gbuffer_depth;
+    uint32_t* rw_hit_pos_normal_texture; /* uint2 per pixel */
+    void* rw_hit_radiance_texture;       /* radiance */
+} vgi_ind_diff_sample_gen_io;
+typedef struct vgi_pc_sample_gen {
+    uint32_t frame_rand;
+    uint32_t has_prev_frame;
+} vgi_pc_sample_gen;
+vgi_status vgi_pass_ind_diff_sample_gen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                        const vgi_ind_diff_sample_gen_io* io, const vgi_pc_sample_gen* pc,
+                                        void* stream);
+
+/* restir/ind_diff_sample_validate.hlsl:83-88; restir_renderer.rs:499-512
+ * (depth_texture = previous frame's depth; rw_* = previous reservoir). */
+typedef struct vgi_ind_diff_sample_validate_io {
+    uint32_t width, height, storage;
+    const float* depth_texture;
+    uint32_t* rw_reservoir_texture;
+    uint32_t* rw_hit_pos_normal_texture;
+    void* rw_hit_radiance_texture;
+} vgi_ind_diff_sample_validate_io;
+vgi_status vgi_pass_ind_diff_sample_validate(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                             const vgi_ind_diff_sample_validate_io* io, void* stream);
+
+/* restir/ind_spec_sample_gen.hlsl:12-26; restir_renderer.rs:685-701 */
+typedef struct vgi_ind_spec_sample_gen_io {
+    uint32_t width, height, storage;
+    const uint32_t* gbuffer_color;
+    const float* gbuffer_depth;
+    float* rw_hit_pos_texture;     /* float4 per pixel */
+    void* rw_hit_radiance_texture; /* radiance */
+} vgi_ind_spec_sample_gen_io;
+vgi_status vgi_pass_ind_spec_sample_gen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                        const vgi_ind_spec_sample_gen_io* io, const vgi_pc_sample_gen* pc,
+                                        void* stream);
+
+/* restir/ind_spec_sample_validate.hlsl:140-142; restir_renderer.rs:722-737
+ * (reads prev_depth_texture of the trace resources). */
+typedef struct vgi_ind_spec_sample_validate_io {
+    uint32_t width, height, storage;
+    uint32_t* rw_prev_reservoir_texture; /* uint2 */
+    float* rw_prev_hit_pos_texture;      /* float4 */
+    void* rw_prev_hit_radiance_texture;  /* radiance */
+} vgi_ind_spec_sample_validate_io;
+vgi_status vgi_pass_ind_spec_sample_validate(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                             const vgi_ind_spec_sample_validate_io* io, void* stream);
+
+/* restir/raytraced_ao_gen.hlsl:5-16; restir_renderer.rs:528-545 */
+typedef struct vgi_ao_gen_io {
+    uint32_t width, height, storage;
+    const uint32_t* hit_pos_normal_texture; /* may be NULL if !has_new_sample */
+    const float* depth_buffer;
+    const void* prev_ao_texture;            /* pair (x, a); may be NULL if !has_prev_frame */
+    void* rw_ao_texture;                    /* pair (x, a) */
+} vgi_ao_gen_io;
+typedef struct vgi_pc_ao_gen {
+    uint32_t has_new_sample;
+    uint32_t has_prev_frame;
+    float radius_ws;
+} vgi_pc_ao_gen;
+vgi_status vgi_pass_raytraced_ao_gen(const vgi_frame_params* fp, const vgi_ao_gen_io* io, const vgi_pc_ao_gen* pc,
+                                     void* stream);
+
+/* restir/raytraced_ao_filter.hlsl:4-6; restir_renderer.rs:551-560 */
+typedef struct vgi_ao_filter_io {
+    uint32_t width, height, storage;
+    const float* depth_buffer;
+    const void* ao_texture;
+    void* rw_filtered_ao_texture;
+} vgi_ao_filter_io;
+vgi_status vgi_pass_raytraced_ao_filter(const vgi_ao_filter_io* io, void* stream);
+
+/* restir/ind_diff_temporal_resample.hlsl:15-36; restir_renderer.rs:589-607 */
+typedef struct vgi_ind_diff_temporal_io {
+    uint32_t width, height, storage;
+    const float* prev_gbuffer_depth;
+    const float* gbuffer_depth;
+    const uint32_t* new_hit_pos_normal_texture;
+    const void* new_hit_radiance_texture;
+    const uint32_t* prev_reservoir_texture;
+    const uint32_t* prev_hit_pos_normal_texture;
+    const void* prev_hit_radiance_texture;
+    uint32_t* rw_reservoir_texture;
+    uint32_t* rw_hit_pos_normal_texture;
+    void* rw_hit_radiance_texture;
+} vgi_ind_diff_temporal_io;
+typedef struct vgi_pc_ind_diff_temporal {
+    uint32_t frame_rand;
+    uint32_t has_prev_frame;
+    uint32_t has_new_sample;
+} vgi_pc_ind_diff_temporal;
+vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
+                                               const vgi_pc_ind_diff_temporal* pc, void* stream);
+
+/* restir/ind_diff_spatial_resample.hlsl:29-49; restir_renderer.rs:632-652 */
+typedef struct vgi_ind_diff_spatial_io {
+    uint32_t width, height, storage;
+    const uint32_t* gbuffer_color;
+    const float* gbuffer_depth;
+    const void* ao_texture; /* the filtered AO pair; .x is used */
+    const uint32_t* temp_reservoir_texture;
+    const uint32_t* temp_hit_pos_normal_texture;
+    const void* temp_hit_radiance_texture;
+    uint32_t* rw_reservoir_texture;
+    uint32_t* rw_hit_pos_normal_texture;
+    void* rw_hit_radiance_texture;
+    void* rw_lighting_texture;
+} vgi_ind_diff_spatial_io;
+typedef struct vgi_pc_frame_rand {
+    uint32_t frame_rand;
+} vgi_pc_frame_rand;
+vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
+                                              const vgi_pc_frame_rand* pc, void* stream);
+
+/* restir/ind_spec_temporal_resample.hlsl:26-44; restir_renderer.rs:743-761
+ * (rw_hit_pos/rw_hit_radiance hold the new sample on entry). */
+typedef struct vgi_ind_spec_temporal_io {
+    uint32_t width, height, storage;
+    const uint32_t* prev_gbuffer_color;
+    const float* prev_gbuffer_depth;
+    const uint32_t* gbuffer_color;
+    const float* gbuffer_depth;
+    const uint32_t* prev_reservoir_texture; /* uint2 */
+    const float* prev_hit_pos_texture;      /* float4 */
+    const void* prev_hit_radiance_texture;
+    uint32_t* rw_reservoir_texture;         /* uint2 */
+    float* rw_hit_pos_texture;              /* float4 */
+    void* rw_hit_radiance_texture;
+} vgi_ind_spec_temporal_io;
+typedef struct vgi_pc_ind_spec_temporal {
+    uint32_t frame_rand;
+    uint32_t has_prev_frame;
+} vgi_pc_ind_spec_temporal;
+vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params* fp, const vgi_ind_spec_temporal_io* io,
+                                               const vgi_pc_ind_spec_temporal* pc, void* stream);
+
+/* restir/ind_spec_spatial_resample.hlsl:23-35; restir_renderer.rs:770-784 */
+typedef struct vgi_ind_spec_spatial_io {
+    uint32_t width, height, storage;
+    const uint32_t* gbuffer_color;
+    const float* gbuffer_depth;
+    const uint32_t* reservoir_texture;
+    const float* hit_pos_texture;
+    const void* hit_radiance_texture;
+    void* rw_lighting_texture;
+    void* rw_ray_len_texture; /* scalar */
+} vgi_ind_spec_spatial_io;
+vgi_status vgi_pass_ind_spec_spatial_resample(const vgi_frame_params* fp, const vgi_ind_spec_spatial_io* io,
+                                              const vgi_pc_frame_rand* pc, void* stream);
+
+/* restir/hash_grid_decay.hlsl:5-6; restir_renderer.rs:322-329 */
+typedef struct vgi_hash_grid_decay_io {
+    uint8_t* rw_decay_buffer;                 /* 1 byte per cell */
+    vgi_hash_grid_cell* rw_storage_buffer;
+} vgi_hash_grid_decay_io;
+vgi_status vgi_pass_hash_grid_decay(const vgi_hash_grid_decay_io* io, void* stream);
+
+/* restir/hash_grid_query_select.hlsl:9-20; restir_renderer.rs:366-373 */
+typedef struct vgi_hash_grid_query_select_io {
+    const vgi_hash_grid_query* hash_grid_query_buffer;
+    const uint32_t* hash_grid_query_counter_buffer;
+    vgi_hash_grid_cell* rw_hash_grid_storage_buffer;
+    uint32_t* rw_hash_grid_query_selection_buffer; /* 1 word per cell */
+} vgi_hash_grid_query_select_io;
+typedef struct vgi_pc_frame_hash {
+    uint32_t frame_hash;
+} vgi_pc_frame_hash;
+vgi_status vgi_pass_hash_grid_query_select(const vgi_hash_grid_query_select_io* io, const vgi_pc_frame_hash* pc,
+                                           void* stream);
+
+/* restir/hash_grid_query_compact.hlsl:63-66; restir_renderer.rs:384-391 */
+typedef struct vgi_hash_grid_query_compact_io {
+    const uint32_t* hash_grid_query_selection_buffer;
+    uint32_t* rw_compact_query_index_buffer;
+    uint32_t* rw_compact_query_cell_addr_buffer;
+    uint32_t* rw_compact_query_counter_buffer;
+} vgi_hash_grid_query_compact_io;
+vgi_status vgi_pass_hash_grid_query_compact(const vgi_hash_grid_query_compact_io* io, void* stream);
+
+/* restir/hash_grid_raygen.hlsl:125-139; restir_renderer.rs:405-426.  The
+ * trace resources' query buffer/counter are the *new* query buffer; the
+ * trace reads rw_hash_grid_storage_buffer (as in the reference). */
+typedef struct vgi_hash_grid_raygen_io {
+    vgi_hash_grid_cell* rw_hash_grid_storage_buffer;
+    const vgi_hash_grid_query* hash_grid_query_buffer;
+    const uint32_t* compact_query_counter_buffer;
+    const uint32_t* compact_query_index_buffer;
+    const uint32_t* compact_query_cell_addr_buffer;
+    uint8_t* rw_hash_grid_decay_buffer;
+} vgi_hash_grid_raygen_io;
+vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                     const vgi_hash_grid_raygen_io* io, const vgi_pc_frame_hash* pc, void* stream);
+
+/* util/clear_buffer.hlsl:3-17; util_passes.rs:9-41 */
+vgi_status vgi_pass_clear_buffer(uint32_t* buffer, uint32_t length, uint32_t value, void* stream);
+
+/* Primary-visibility G-buffer producer (input side of the hot path; stands
+ * in for the raster pass gbuffer_pass.rs:45-137 / mesh_gbuffer.hlsl:62-126):
+ * one ray per pixel centre, reverse-Z depth of the hit, G-buffer layers
+ * encoded as gbuffer.hlsl:54-77 with fwidth_z / fwidth_n from differences
+ * inside each aligned 2x2 quad (a partner without geometry counts as 0). */
+typedef struct vgi_gbuffer_io {
+    uint32_t width, height;
+    float* rw_depth;       /* f32, 0 = no geometry */
+    uint32_t* rw_gbuffer;  /* 4 layers */
+} vgi_gbuffer_io;
+vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const vgi_scene* scene, const vgi_gbuffer_io* io,
+                                    void* stream);
+
+/* ------------------------------------------------------------------------
+ * Renderer sequencer: crates/violet-restir/src/restir_renderer.rs
+ * `RestirRenderer::add_passes` (:258-1028).  Records the whole GI pass chain
+ * of one frame in violet's order, with violet's FrameRand seed consumption
+ * (restir_renderer.rs:130-143) and history flags, on per-view persistent
+ * HBM state (the RGTemporal resources of restir_renderer.rs:146-184).
+ * ---------------------------------------------------------------------- */
+typedef struct vgi_renderer vgi_renderer;
+
+typedef struct vgi_renderer_config {
+    uint32_t storage;                /* VGI_STORAGE_F32 / VGI_STORAGE_F16 */
+    uint32_t denoise;                /* default 1 (restir_renderer.rs:66) */
+    uint32_t ind_diff_validate;      /* default 1 */
+    uint32_t ind_spec_validate;      /* default 1 */
+    uint32_t hash_grid_cache_decay;  /* default 1 */
+    float ao_radius;                 /* default 2.0 */
+    float sky_radiance[3];           /* constant sky (replaces the sky cube) */
+    vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
+} vgi_renderer_config;
+
+/* Outputs of one frame: device images owned by the renderer, valid until
+ * the next add_passes on this renderer. */
+typedef struct vgi_renderer_output {
+    const void* indirect_diffuse;   /* denoised (or raw when the denoiser did not run) */
+    const void* indirect_specular;
+    const float* depth;
+    const uint32_t* gbuffer;
+    uint32_t denoised;              /* 1 if the denoiser ran this frame */
+    uint32_t validation_frame;      /* 1 if this was a diffuse validation frame */
+    uint32_t hash_grid_updated;     /* 1 if the cache update passes ran */
+} vgi_renderer_output;
+
+vgi_status vgi_renderer_default_config(vgi_renderer_config* cfg);
+vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const vgi_renderer_config* cfg, vgi_renderer** out);
+vgi_status vgi_renderer_destroy(vgi_renderer* r);
+/* Drops all history (the state after a reset/eviction: every has_prev_* = 0,
+ * hash grid cleared on the next frame). */
+vgi_status vgi_renderer_reset(vgi_renderer* r);
+/* restir_renderer.rs:258-1028 (G-buffer from primary rays on `scene`). */
+vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp,
+                                   uint32_t frame_index, vgi_renderer_output* out, void* stream);
+/* Named per-view images (violet's debug views / temporal resources), for
+ * inspection, tests and state export: "depth", "gbuffer", "prev_depth",
+ * "diff_reservoir", "diff_hit_pos_normal", "diff_hit_radiance" (spatial
+ * output), "diff_temporal_reservoir", "diff_temporal_hit_pos_normal",
+ * "diff_temporal_hit_radiance", "diff_new_hit_pos_normal",
+ * "diff_new_hit_radiance", "ao", "ao_filtered", "indirect_diffuse_raw",
+ * "spec_reservoir", "spec_hit_pos", "spec_hit_radiance",
+ * "indirect_specular_raw", "spec_ray_len", "hash_grid_storage",
+ * "hash_grid_decay", "hash_grid_queries", "hash_grid_query_counter". */
+vgi_status vgi_renderer_get_image(const vgi_renderer* r, const char* name, void** ptr, size_t* bytes);
+/* Per-view state snapshot (checkpoint/resume, serving eviction): copies all
+ * temporal state to/from a host blob of vgi_renderer_state_size() bytes. */
+size_t vgi_renderer_state_size(const vgi_renderer* r);
+vgi_status vgi_renderer_export_state(const vgi_renderer* r, void* host_blob, size_t bytes, void* stream);
+vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* host_blob, size_t bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -214,3 +214,233 @@ class OracleDenoiser:
             d, s, v = a["diff"], a["spec"], a["variance"]
         return dict(diffuse=d, specular=s, history_len=t["history_len"], variance=fb["variance"],
                     temporal=t, fallback=fb)
+
+
+# ---------------------------------------------------------------------------
+# ReSTIR passes
+# ---------------------------------------------------------------------------
+class _TraceRes(C.Structure):
+    _fields_ = [("scene", P), ("sky", C.c_float * 3), ("prev_ind_diff", P), ("prev_depth", P), ("pw", C.c_int32),
+                ("ph", C.c_int32), ("hg_cells", P), ("queries", P), ("query_counter", P)]
+
+
+def _restir_sigs(lib):
+    lib.oracle_scene_create.restype = P
+    lib.oracle_scene_create.argtypes = [P, P, P, U32, P, P]
+    lib.oracle_scene_destroy.argtypes = [P]
+    TR = C.POINTER(_TraceRes)
+    lib.oracle_trace_radiance.argtypes = [P, TR, P, U32, I, P]
+    lib.oracle_diff_sample_gen.argtypes = [P, TR, I, I, P, P, U32, I, P, P]
+    lib.oracle_diff_sample_validate.argtypes = [P, TR, I, I, P, P, P, P]
+    lib.oracle_spec_sample_gen.argtypes = [P, TR, I, I, P, P, U32, I, P, P]
+    lib.oracle_spec_sample_validate.argtypes = [P, TR, I, I, P, P, P]
+    lib.oracle_ao_gen.argtypes = [P, I, I, P, P, P, I, I, C.c_double if lib._bits == 64 else C.c_float, P]
+    lib.oracle_ao_filter.argtypes = [I, I, P, P, P]
+    lib.oracle_diff_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, I, P, P, P]
+    lib.oracle_diff_spatial.argtypes = [P, I, I, P, P, P, P, P, P, U32, P, P, P, P]
+    lib.oracle_spec_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, P, P, P]
+    lib.oracle_spec_spatial.argtypes = [P, I, I, P, P, P, P, P, U32, P, P]
+    lib.oracle_gbuffer.argtypes = [P, P, I, I, P, P]
+    lib.oracle_hg_decay.argtypes = [P, P]
+    lib.oracle_hg_select.argtypes = [P, U32, P, P, U32]
+    lib.oracle_hg_compact.restype = U32
+    lib.oracle_hg_compact.argtypes = [P, P, P]
+    lib.oracle_hg_raygen.argtypes = [P, TR, P, P, U32, P, P, P, U32]
+
+
+def _fp_arr(fp):
+    return fp if isinstance(fp, np.ndarray) else np.frombuffer(bytes(fp), dtype=np.float32).copy()
+
+
+class OracleScene:
+    """The oracle's view of a scene.Mesh (own median-split BVH)."""
+
+    def __init__(self, bits: int, mesh):
+        self.lib = load(bits)
+        if not hasattr(self.lib, "_restir"):
+            self.lib._bits = bits
+            _restir_sigs(self.lib)
+            self.lib._restir = True
+        self.P = np.ascontiguousarray(mesh.positions, np.float32)
+        self.N = np.ascontiguousarray(mesh.normals, np.float32)
+        self.I = np.ascontiguousarray(mesh.indices, np.uint32)
+        self.TM = np.ascontiguousarray(mesh.tri_material, np.uint32)
+        mats = np.zeros((len(mesh.albedo), 8), np.float32)
+        mats[:, :3] = mesh.albedo
+        mats[:, 3] = 1.0
+        mats[:, 4] = mesh.metallic
+        mats[:, 5] = mesh.roughness
+        self.mats = mats
+        self.h = self.lib.oracle_scene_create(_np_ptr(self.P), _np_ptr(self.N), _np_ptr(self.I), len(self.I),
+                                              _np_ptr(self.TM), _np_ptr(self.mats))
+
+    def __del__(self):
+        try:
+            self.lib.oracle_scene_destroy(self.h)
+        except Exception:
+            pass
+
+
+class OracleRestir:
+    """Typed wrappers of the ReSTIR passes for one precision build."""
+
+    NQ = 128 * 1024
+
+    def __init__(self, bits: int, scene: OracleScene, sky=(1.0, 1.0, 1.0)):
+        self.bits = bits
+        self.lib = load(bits)
+        if not hasattr(self.lib, "_restir"):
+            self.lib._bits = bits
+            _restir_sigs(self.lib)
+            self.lib._restir = True
+        self.rt_dtype = real_dtype(bits)
+        self.scene = scene
+        self.sky = sky
+
+    def r(self, a):
+        return None if a is None else np.ascontiguousarray(a, dtype=self.rt_dtype)
+
+    def trace_res(self, hg_cells, queries, counter, prev_ind_diff=None, prev_depth=None):
+        tr = _TraceRes()
+        tr.scene = self.scene.h
+        tr.sky[:] = list(self.sky)
+        self._keep = [self.r(prev_ind_diff), self.r(prev_depth)]
+        tr.prev_ind_diff = _np_ptr(self._keep[0])
+        tr.prev_depth = _np_ptr(self._keep[1])
+        if prev_depth is not None:
+            tr.ph, tr.pw = prev_depth.shape
+        tr.hg_cells = _np_ptr(hg_cells)
+        tr.queries = _np_ptr(queries)
+        tr.query_counter = _np_ptr(counter)
+        return tr
+
+    def gbuffer(self, fp, w, h):
+        depth = np.zeros((h, w), self.rt_dtype)
+        gb = np.zeros((4, h, w), np.uint32)
+        self.lib.oracle_gbuffer(_np_ptr(_fp_arr(fp)), self.scene.h, w, h, _np_ptr(depth), _np_ptr(gb))
+        return depth, gb
+
+    def diff_sample_gen(self, fp, tr, gb, depth, frame_rand, has_prev):
+        h, w = depth.shape
+        hit = np.zeros((h, w, 2), np.uint32)
+        rad = np.zeros((h, w, 3), self.rt_dtype)
+        self.lib.oracle_diff_sample_gen(_np_ptr(_fp_arr(fp)), C.byref(tr), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
+                                        _np_ptr(self.r(depth)), frame_rand, int(has_prev), _np_ptr(hit), _np_ptr(rad))
+        return hit, rad
+
+    def diff_sample_validate(self, fp, tr, prev_depth, rsv, hit, rad):
+        h, w = prev_depth.shape
+        rsv, hit, rad = rsv.copy(), hit.copy(), self.r(rad).copy()
+        self.lib.oracle_diff_sample_validate(_np_ptr(_fp_arr(fp)), C.byref(tr), w, h, _np_ptr(self.r(prev_depth)),
+                                             _np_ptr(rsv), _np_ptr(hit), _np_ptr(rad))
+        return rsv, hit, rad
+
+    def spec_sample_gen(self, fp, tr, gb, depth, frame_rand, has_prev, hitpos_in=None):
+        h, w = depth.shape
+        hp = np.zeros((h, w, 4), self.rt_dtype) if hitpos_in is None else self.r(hitpos_in).copy()
+        rad = np.zeros((h, w, 3), self.rt_dtype)
+        self.lib.oracle_spec_sample_gen(_np_ptr(_fp_arr(fp)), C.byref(tr), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
+                                        _np_ptr(self.r(depth)), frame_rand, int(has_prev), _np_ptr(hp), _np_ptr(rad))
+        return hp, rad
+
+    def spec_sample_validate(self, fp, tr, rsv, hitpos, rad):
+        h, w = rad.shape[:2]
+        rsv, hp, rd = rsv.copy(), self.r(hitpos).copy(), self.r(rad).copy()
+        self.lib.oracle_spec_sample_validate(_np_ptr(_fp_arr(fp)), C.byref(tr), w, h, _np_ptr(rsv), _np_ptr(hp), _np_ptr(rd))
+        return rsv, hp, rd
+
+    def ao_gen(self, fp, hit, depth, prev_ao, has_new, has_prev, radius=2.0):
+        h, w = depth.shape
+        out = np.zeros((h, w, 2), self.rt_dtype)
+        self.lib.oracle_ao_gen(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(hit), _np_ptr(self.r(depth)), _np_ptr(self.r(prev_ao)),
+                               int(has_new), int(has_prev), radius, _np_ptr(out))
+        return out
+
+    def ao_filter(self, depth, ao):
+        h, w = depth.shape
+        out = np.zeros((h, w, 2), self.rt_dtype)
+        self.lib.oracle_ao_filter(w, h, _np_ptr(self.r(depth)), _np_ptr(self.r(ao)), _np_ptr(out))
+        return out
+
+    def diff_temporal(self, fp, prev_depth, depth, new_hit, new_rad, prev_rsv, prev_hit, prev_rad, frame_rand, has_prev,
+                      has_new):
+        h, w = depth.shape
+        rsv = np.zeros((h, w), np.uint32)
+        hit = np.zeros((h, w, 2), np.uint32)
+        rad = np.zeros((h, w, 3), self.rt_dtype)
+        a = lambda x: None if x is None else np.ascontiguousarray(x, np.uint32)  # noqa: E731
+        self.lib.oracle_diff_temporal(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(self.r(prev_depth)), _np_ptr(self.r(depth)),
+                                      _np_ptr(a(new_hit)), _np_ptr(self.r(new_rad)), _np_ptr(a(prev_rsv)),
+                                      _np_ptr(a(prev_hit)), _np_ptr(self.r(prev_rad)), frame_rand, int(has_prev),
+                                      int(has_new), _np_ptr(rsv), _np_ptr(hit), _np_ptr(rad))
+        return rsv, hit, rad
+
+    def diff_spatial(self, fp, gb, depth, ao, rsv, hit, rad, frame_rand):
+        h, w = depth.shape
+        o_rsv = np.zeros((h, w), np.uint32)
+        o_hit = np.zeros((h, w, 2), np.uint32)
+        o_rad = np.zeros((h, w, 3), self.rt_dtype)
+        o_l = np.zeros((h, w, 3), self.rt_dtype)
+        self.lib.oracle_diff_spatial(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
+                                     _np_ptr(self.r(depth)), _np_ptr(self.r(ao)), _np_ptr(np.ascontiguousarray(rsv, np.uint32)),
+                                     _np_ptr(np.ascontiguousarray(hit, np.uint32)), _np_ptr(self.r(rad)), frame_rand,
+                                     _np_ptr(o_rsv), _np_ptr(o_hit), _np_ptr(o_rad), _np_ptr(o_l))
+        return o_rsv, o_hit, o_rad, o_l
+
+    def spec_temporal(self, fp, prev_gb, prev_depth, gb, depth, prev_rsv, prev_hp, prev_rad, new_hp, new_rad, frame_rand,
+                      has_prev):
+        h, w = depth.shape
+        rsv = np.zeros((h, w, 2), np.uint32)
+        hp = self.r(new_hp).copy()
+        rad = self.r(new_rad).copy()
+        a = lambda x: None if x is None else np.ascontiguousarray(x, np.uint32)  # noqa: E731
+        self.lib.oracle_spec_temporal(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(a(prev_gb)), _np_ptr(self.r(prev_depth)),
+                                      _np_ptr(a(gb)), _np_ptr(self.r(depth)), _np_ptr(a(prev_rsv)), _np_ptr(self.r(prev_hp)),
+                                      _np_ptr(self.r(prev_rad)), frame_rand, int(has_prev), _np_ptr(rsv), _np_ptr(hp),
+                                      _np_ptr(rad))
+        return rsv, hp, rad
+
+    def spec_spatial(self, fp, gb, depth, rsv, hp, rad, frame_rand):
+        h, w = depth.shape
+        light = np.zeros((h, w, 3), self.rt_dtype)
+        rl = np.zeros((h, w), self.rt_dtype)
+        self.lib.oracle_spec_spatial(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
+                                     _np_ptr(self.r(depth)), _np_ptr(np.ascontiguousarray(rsv, np.uint32)),
+                                     _np_ptr(self.r(hp)), _np_ptr(self.r(rad)), frame_rand, _np_ptr(light), _np_ptr(rl))
+        return light, rl
+
+    # hash grid
+    def hg_decay(self, decay, cells):
+        decay, cells = decay.copy(), cells.copy()
+        self.lib.oracle_hg_decay(_np_ptr(decay), _np_ptr(cells))
+        return decay, cells
+
+    def hg_select(self, queries, counter, cells, frame_hash):
+        cells = cells.copy()
+        sel = np.zeros(65536 * 16, np.uint32)
+        self.lib.oracle_hg_select(_np_ptr(np.ascontiguousarray(queries, np.uint32)), int(counter), _np_ptr(cells),
+                                  _np_ptr(sel), frame_hash)
+        return cells, sel
+
+    def hg_compact(self, sel):
+        idx = np.zeros(self.NQ, np.uint32)
+        addr = np.zeros(self.NQ, np.uint32)
+        n = self.lib.oracle_hg_compact(_np_ptr(np.ascontiguousarray(sel, np.uint32)), _np_ptr(idx), _np_ptr(addr))
+        return idx[:n], addr[:n]
+
+    def hg_raygen(self, fp, tr, cells, queries, cidx, caddr, decay, frame_hash):
+        cells, decay = cells.copy(), decay.copy()
+        cidx = np.ascontiguousarray(cidx, np.uint32)
+        caddr = np.ascontiguousarray(caddr, np.uint32)
+        self.lib.oracle_hg_raygen(_np_ptr(_fp_arr(fp)), C.byref(tr), _np_ptr(cells), _np_ptr(np.ascontiguousarray(queries, np.uint32)),
+                                  len(cidx), _np_ptr(cidx), _np_ptr(caddr), _np_ptr(decay), frame_hash)
+        return cells, decay
+
+
+def frame_rand_seeds(frame_index: int, n: int) -> list[int]:
+    """FrameRand (restir_renderer.rs:130-143): the first n values of the jenkins chain."""
+    out, s = [], frame_index & 0xFFFFFFFF
+    for _ in range(n):
+        s = int(load(64).oracle_jenkins(s))
+        out.append(s)
+    return out

@@ -1,0 +1,1058 @@
+/*
+ * oracle_restir.c — ORACLE (test infrastructure only).  CPU restatement of
+ * violet's ReSTIR GI passes, one function per HLSL pass, pixel loops in
+ * dispatch order:
+ *
+ *   oracle_diff_sample_gen      restir/ind_diff_sample_gen.hlsl:25-67
+ *   oracle_diff_sample_validate restir/ind_diff_sample_validate.hlsl:90-145
+ *   oracle_spec_sample_gen      restir/ind_spec_sample_gen.hlsl:28-128
+ *   oracle_spec_sample_validate restir/ind_spec_sample_validate.hlsl:144-191
+ *   oracle_ao_gen / _filter     restir/raytraced_ao_gen.hlsl:18-81, raytraced_ao_filter.hlsl:13-70
+ *   oracle_diff_temporal        restir/ind_diff_temporal_resample.hlsl:49-200
+ *   oracle_diff_spatial         restir/ind_diff_spatial_resample.hlsl:94-337
+ *   oracle_spec_temporal        restir/ind_spec_temporal_resample.hlsl:146-362
+ *   oracle_spec_spatial         restir/ind_spec_spatial_resample.hlsl:169-448
+ *   trace_radiance / shading    restir/raytrace.inc.hlsl:65-208,
+ *                               raytrace/geometry_rchit.inc.hlsl:376-472
+ *   hash-grid passes            restir/hash_grid_{decay,query_select,query_compact,raygen}.hlsl
+ *
+ * Scene: world-space f32 positions/normals, u32 indices, per-triangle
+ * material {base_color rgba, metallic, roughness, pad, pad}; queried through
+ * the oracle's own median-split BVH (oracle_trace.inc).
+ * Hash-grid queries are appended in pixel order (the GPU appends in
+ * scheduling order; tests compare them as sets).
+ */
+#include "oracle_trace.inc"
+
+typedef struct {
+    obvh* bvh;
+    const float* P;
+    const float* N; /* may be NULL */
+    const uint32_t* I;
+    const uint32_t* TM;
+    const float* mats; /* 8 floats per material */
+    uint32_t nt;
+} oscene;
+
+oscene* oracle_scene_create(const float* P, const float* N, const uint32_t* I, uint32_t nt, const uint32_t* TM,
+                            const float* mats) {
+    oscene* s = (oscene*)calloc(1, sizeof(oscene));
+    s->bvh = obvh_create(P, I, nt);
+    s->P = P;
+    s->N = N;
+    s->I = I;
+    s->TM = TM;
+    s->mats = mats;
+    s->nt = nt;
+    return s;
+}
+void oracle_scene_destroy(oscene* s) {
+    if (!s) return;
+    obvh_destroy(s->bvh);
+    free(s);
+}
+
+typedef struct {
+    oscene* scene;
+    float sky[3];
+    const real* prev_ind_diff; /* radiance (3 reals per pixel) */
+    const real* prev_depth;
+    int32_t pw, ph;
+    const uint32_t* hg_cells; /* 4 words per cell: checksum, rgb (f32 bits) */
+    uint32_t* queries;        /* 6 words per query */
+    uint32_t* query_counter;
+} otrace_res;
+
+typedef struct {
+    v3 pos, normal, radiance;
+} otrace_out;
+
+#define HG_MAX_QUERIES (128u * 1024u)
+
+static inline v3 vtx(const float* A, uint32_t i) { return V3(A[3 * i], A[3 * i + 1], A[3 * i + 2]); }
+
+/* hash_grid.inc.hlsl:32-105 VertexDescriptor keys */
+static inline void o_hg_key(v3 pos, v3 cam, uint32_t* hash, uint32_t* checksum) {
+    v3 dd = sub3(pos, cam);
+    real d2 = dot3(dd, dd);
+    double l = 0.5 * log2((double)d2);
+    if (!(l > 0.0)) l = 0.0;
+    uint32_t lod = (uint32_t)l;
+    if (lod > 31) lod = 31;
+    real cell = (RC(1.0) / RC(16.0)) * (real)(1u << lod);
+    real inv = RC(1.0) / cell;
+    /* quantise the single-precision position (the stored value) */
+    float fx = (float)pos.x, fy = (float)pos.y, fz = (float)pos.z;
+    int32_t qx = (int32_t)floor((double)fx * (double)(float)inv);
+    int32_t qy = (int32_t)floor((double)fy * (double)(float)inv);
+    int32_t qz = (int32_t)floor((double)fz * (double)(float)inv);
+    uint32_t ux = (uint32_t)qx, uy = (uint32_t)qy, uz = (uint32_t)qz;
+    uint32_t px_ = ux ^ (uy << 22);
+    uint32_t py_ = (uy >> 10) ^ (uz << 22);
+    *hash = o_xxh_eval(o_xxh_add(o_xxh_add(o_xxh_init(px_), py_), lod)) & 0xFFFFu;
+    *checksum = o_xxh_eval(o_xxh_add(o_xxh_add(o_xxh_add(o_xxh_init(uy), uz), ux), lod)) | 1u;
+}
+
+static inline int o_hg_find(const uint32_t* cells, uint32_t hash, uint32_t checksum, v3* rad, uint32_t* addr) {
+    for (uint32_t a = hash << 4; a < (hash << 4) + 16; ++a) {
+        const uint32_t* c = cells + 4 * (size_t)a;
+        if (c[0] == checksum) {
+            *rad = V3(bits_f(c[1]), bits_f(c[2]), bits_f(c[3]));
+            *addr = a;
+            return 1;
+        }
+        if (c[0] == 0) break;
+    }
+    return 0;
+}
+
+/* brdf.hlsl */
+static inline real o_pow5(real x) {
+#if defined(ORACLE_F32)
+    real x2 = x * x, x4 = x2 * x2;
+    return x4 * x;
+#else
+    return pow(x, 5.0);
+#endif
+}
+static inline v3 o_F_schlick(real u, v3 f0) {
+    real f = o_pow5(RC(1.0) - u);
+    return V3(f + f0.x * (RC(1.0) - f), f + f0.y * (RC(1.0) - f), f + f0.z * (RC(1.0) - f));
+}
+static inline real o_D_ggx(real NoH, real r) {
+    real a = NoH * r;
+    real k = r / ((RC(1.0) - NoH * NoH) + a * a);
+    return (k * k) * O_ONE_OVER_PI;
+}
+static inline real o_vis_smith(real NoL, real NoV, real r) {
+    real r2 = r * r;
+    real uL = NoV * R_SQRT((-r2 * NoL + NoL) * NoL + r2);
+    real uV = NoL * R_SQRT((-r2 * NoV + NoV) * NoV + r2);
+    return RC(0.5) / (uL + uV);
+}
+static inline real o_g2_over_g1(real NoL, real NoV, real r) {
+    real r2 = r * r;
+    real kV = R_SQRT((-r2 * NoV + NoV) * NoV + r2) / NoV;
+    real kL = R_SQRT((-r2 * NoL + NoL) * NoL + r2) / NoL;
+    return (RC(1.0) + kV) / (kV + kL);
+}
+static inline real o_lambda_pomc(real c, real r) {
+    real r2 = r * r;
+    return (c + R_SQRT((-r2 * c + c) * c + r2)) / RC(2.0);
+}
+static inline v3 o_f0(v3 base, real metal) {
+    return V3(lerp1(RC(0.04), base.x, metal), lerp1(RC(0.04), base.y, metal), lerp1(RC(0.04), base.z, metal));
+}
+static inline v3 o_eval_ggx_lambert(v3 v, v3 l, v3 n, real pr, v3 rho, v3 f0) {
+    v3 h = norm3(add3(v, l));
+    real NoL = sat(dot3(n, l)), NoV = sat(dot3(n, v)), NoH = sat(dot3(n, h)), LoH = sat(dot3(l, h));
+    real r = pr * pr;
+    real D = o_D_ggx(NoH, r);
+    v3 F = o_F_schlick(LoH, f0);
+    real V = o_vis_smith(NoL, NoV, r);
+    real dv = D * V;
+    if (dv > RC(1e19)) dv = RC(1e19);
+    v3 Fr = scale3(F, dv);
+    v3 Fd = scale3(rho, O_ONE_OVER_PI);
+    return add3(Fd, Fr);
+}
+static inline v3 o_ggx_integral(real c, real r, v3 f0) {
+    real a = r * r, a3 = a * (a * a), c2 = c * c, c3 = c * c2;
+    real k0 = RC(1.0) * (RC(0.99044) * RC(1.0) + RC(-1.28514) * c) + a * (RC(1.29678) * RC(1.0) + RC(-0.755907) * c);
+    k0 = k0 / ((RC(1.0) * ((RC(1.0) * RC(1.0) + RC(2.92338) * c) + RC(59.4188) * c3) +
+                a * ((RC(20.3225) * RC(1.0) + RC(-27.0302) * c) + RC(222.592) * c3)) +
+               a3 * ((RC(121.563) * RC(1.0) + RC(626.13) * c) + RC(316.627) * c3));
+    real k1 = RC(1.0) * (RC(0.0365463) * RC(1.0) + RC(3.32707) * c) + a * (RC(9.0632) * RC(1.0) + RC(-9.04756) * c);
+    k1 = k1 / ((RC(1.0) * ((RC(1.0) * RC(1.0) + RC(3.59685) * c2) + RC(-1.36772) * c3) +
+                a * ((RC(9.04401) * RC(1.0) + RC(-16.3174) * c2) + RC(9.22949) * c3)) +
+               a3 * ((RC(5.56589) * RC(1.0) + RC(19.7886) * c2) + RC(-20.2123) * c3));
+    return V3(k0 + k1 * f0.x, k0 + k1 * f0.y, k0 + k1 * f0.z);
+}
+
+/* sampling.hlsl */
+static inline v4 o_rot_to_z(v3 v) {
+    if (v.z < RC(-0.99999)) return V4(1, 0, 0, 0);
+    v4 q = V4(v.y, -v.x, RC(0.0), RC(1.0) + v.z);
+    real inv = RC(1.0) / R_SQRT(((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w);
+    return V4(q.x * inv, q.y * inv, q.z * inv, q.w * inv);
+}
+static inline v4 o_inv_rot(v4 q) { return V4(-q.x, -q.y, -q.z, q.w); }
+static inline v3 o_rotate(v4 q, v3 v) {
+    v3 ax = V3(q.x, q.y, q.z);
+    v3 a = scale3(ax, RC(2.0) * dot3(ax, v));
+    v3 b = scale3(v, q.w * q.w - dot3(ax, ax));
+    v3 c = scale3(cross3(ax, v), RC(2.0) * q.w);
+    return add3(add3(a, b), c);
+}
+
+/* TraceRay + geometry_rchit.inc.hlsl + raytrace.inc.hlsl:65-208 */
+static otrace_out o_trace_radiance(const otrace_res* rt, const ofp* fp, v3 o, v3 d, int has_prev) {
+    otrace_out r;
+    const oscene* s = rt->scene;
+    ohit h = obvh_trace(s->bvh, o, d, RC(0.0005), RC(1000.0), 0);
+    if (h.id == 0xFFFFFFFFu) {
+        r.radiance = V3(rt->sky[0], rt->sky[1], rt->sky[2]);
+        r.pos = add3(o, scale3(d, RC(1000.0)));
+        r.normal = neg3(d);
+        return r;
+    }
+    uint32_t i0 = s->I[3 * h.id], i1 = s->I[3 * h.id + 1], i2 = s->I[3 * h.id + 2];
+    v3 p0 = vtx(s->P, i0), p1 = vtx(s->P, i1), p2 = vtx(s->P, i2);
+    real w0 = (RC(1.0) - h.u) - h.v;
+    v3 pos = add3(add3(scale3(p0, w0), scale3(p1, h.u)), scale3(p2, h.v));
+    v3 ng = norm3(cross3(sub3(p1, p0), sub3(p2, p0)));
+    v3 n = ng;
+    if (s->N) n = norm3(add3(add3(scale3(vtx(s->N, i0), w0), scale3(vtx(s->N, i1), h.u)), scale3(vtx(s->N, i2), h.v)));
+    if (dot3(ng, d) > RC(0.0)) {
+        ng = neg3(ng);
+        n = neg3(n);
+    }
+    const float* m = s->mats + 8 * (s->TM ? s->TM[h.id] : 0);
+    v3 base = V3(m[0], m[1], m[2]);
+    real metal = m[4], rough = m[5];
+    v3 rho = scale3(base, RC(1.0) - metal);
+    v3 sun_dir = o_xyz(fp->sun_dir), sun = o_xyz(fp->sun_inten);
+    v3 rad = V3s(0);
+    ohit sh = obvh_trace(s->bvh, pos, sun_dir, RC(0.0005), RC(1000.0), 1);
+    if (sh.id == 0xFFFFFFFFu) {
+        real pr = R_FMAX(rough, RC(0.045));
+        pr = R_FMAX(pr, RC(0.5));
+        real NoL = sat(dot3(n, sun_dir));
+        rad = mul3(scale3(o_eval_ggx_lambert(neg3(d), sun_dir, n, pr, rho, o_f0(base, metal)), NoL), sun);
+    }
+    int world = 1;
+    if (has_prev) {
+        v4 ph = o_mul44(fp->prev_view_proj, V4(pos.x, pos.y, pos.z, RC(1.0)));
+        real nx = ph.x / ph.w - fp->jitter[2], ny = ph.y / ph.w - fp->jitter[3];
+        real rd = ph.z / ph.w;
+        if (R_FABS(nx) < RC(1.0) && R_FABS(ny) < RC(1.0)) {
+            real u = nx * RC(0.5) + RC(0.5), v = ny * RC(0.5) + RC(0.5);
+            int ppx = (int)R_FLOOR(u * (real)rt->pw), ppy = (int)R_FLOOR(v * (real)rt->ph);
+            if (ppx > rt->pw - 1) ppx = rt->pw - 1;
+            if (ppy > rt->ph - 1) ppy = rt->ph - 1;
+            size_t pi = (size_t)ppy * rt->pw + ppx;
+            if (R_FABS(rd - rt->prev_depth[pi]) < RC(0.0001)) {
+                const real* c = rt->prev_ind_diff + 3 * pi;
+                rad = add3(rad, mul3(rho, V3(c[0], c[1], c[2])));
+                world = 0;
+            }
+        }
+    }
+    if (world) {
+        uint32_t hh, ck, addr;
+        o_hg_key(pos, o_xyz(fp->view_pos), &hh, &ck);
+        v3 cr;
+        if (o_hg_find(rt->hg_cells, hh, ck, &cr, &addr)) rad = add3(rad, mul3(rho, cr));
+        uint32_t qi;
+#pragma omp atomic capture
+        qi = (*rt->query_counter)++;
+        if (qi < HG_MAX_QUERIES) {
+            uint32_t* q = rt->queries + 6 * (size_t)qi;
+            q[0] = f_bits((float)pos.x);
+            q[1] = f_bits((float)pos.y);
+            q[2] = f_bits((float)pos.z);
+            q[3] = o_oct_enc_u32(ng);
+            q[4] = hh;
+            q[5] = ck;
+        }
+    }
+    r.pos = pos;
+    r.normal = n;
+    r.radiance = rad;
+    return r;
+}
+
+static inline v3 ld3r(const real* p, size_t i) { return V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+static inline void st3r(real* p, size_t i, v3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+
+#define DEPTH_ERR (RC(1.0) / RC(16777216.0))
+
+/* Batch trace_radiance (tests): rays {o, -, d, -}, out 9 reals per ray. */
+void oracle_trace_radiance(const float* fpf, const otrace_res* rt, const float* rays, uint32_t n, int has_prev,
+                           real* out) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (uint32_t k = 0; k < n; ++k) {
+        const float* r = rays + 8 * (size_t)k;
+        otrace_out t = o_trace_radiance(rt, &fp, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), has_prev);
+        real* o = out + 9 * (size_t)k;
+        o[0] = t.pos.x, o[1] = t.pos.y, o[2] = t.pos.z;
+        o[3] = t.normal.x, o[4] = t.normal.y, o[5] = t.normal.z;
+        o[6] = t.radiance.x, o[7] = t.radiance.y, o[8] = t.radiance.z;
+    }
+}
+
+void oracle_diff_sample_gen(const float* fpf, const otrace_res* rt, int w, int h, const uint32_t* gb, const real* depth,
+                            uint32_t frame_rand, int has_prev, uint32_t* o_hit, real* o_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real dpt = depth[i];
+            if (dpt == RC(0.0)) {
+                o_hit[2 * i] = o_hit[2 * i + 1] = 0;
+                st3r(o_rad, i, V3s(0));
+                continue;
+            }
+            v3 nrm = o_gb_normal(gb[plane + i]);
+            v3 pos = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, dpt + DEPTH_ERR);
+            uint32_t rng = o_lcg_init(px, py, frame_rand);
+            real u0 = o_rand(&rng), u1 = o_rand(&rng);
+            real cos_t = RC(1.0) - u0, sin_t = R_SQRT(RC(1.0) - cos_t * cos_t), sp, cp;
+            r_sincos(O_TWO_PI * u1, &sp, &cp);
+            v3 l = V3(cp * sin_t, sp * sin_t, cos_t);
+            v3 dir = o_rotate(o_inv_rot(o_rot_to_z(nrm)), l);
+            otrace_out t = o_trace_radiance(rt, &fp, pos, dir, has_prev);
+            o_hit_enc(t.pos, t.normal, &o_hit[2 * i], &o_hit[2 * i + 1]);
+            st3r(o_rad, i, t.radiance);
+        }
+}
+
+void oracle_diff_sample_validate(const float* fpf, const otrace_res* rt, int w, int h, const real* prev_depth,
+                                 uint32_t* rw_rsv, uint32_t* rw_hit, real* rw_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real dpt = prev_depth[i];
+            if (dpt == RC(0.0)) continue;
+            v3 hp, hn;
+            o_hit_dec(rw_hit[2 * i], rw_hit[2 * i + 1], &hp, &hn);
+            v3 prev_rad = ld3r(rw_rad, i);
+            v3 org = o_depth_to_pos(fp.prev_inv_view_proj, px, py, w, h, dpt);
+            v3 dir = norm3(sub3(hp, org));
+            otrace_out t = o_trace_radiance(rt, &fp, org, dir, 1);
+            real lp = lum(prev_rad);
+            real ld = lum(abs3(sub3(t.radiance, prev_rad)));
+            real len = len3(sub3(hp, org));
+            int moved = cmax3(abs3(sub3(hp, t.pos))) > R_FMAX(len / RC(128.0), RC(0.01));
+            real ms = sat(RC(1.0) - (RC(1.0) * ld) / lp);
+            if (moved) ms = RC(0.0);
+            if (lp > RC(0.0) && ms <= (RC(1.0) - RC(0.5) / RC(32.0))) {
+                uint32_t M;
+                float W;
+                o_rsv_dec(rw_rsv[i], &M, &W);
+                M = r2u(R_FLOOR((real)M * ms));
+                rw_rsv[i] = o_rsv_enc(M, W);
+            }
+            o_hit_enc(t.pos, t.normal, &rw_hit[2 * i], &rw_hit[2 * i + 1]);
+            st3r(rw_rad, i, t.radiance);
+        }
+}
+
+static v3 o_spec_dir(v3 view_dir, v3 n, real a, uint32_t* rng) {
+    v4 rot = o_rot_to_z(n);
+    v3 V = o_rotate(rot, view_dir);
+    v3 H;
+    if (a != RC(0.0)) {
+        real ux = o_rand(rng), uy = o_rand(rng);
+        v3 vh = norm3(V3(a * V.x, a * V.y, V.z));
+        real lensq = vh.x * vh.x + vh.y * vh.y;
+        v3 T1 = lensq > RC(0.0) ? scale3(V3(-vh.y, vh.x, RC(0.0)), RC(1.0) / R_SQRT(lensq)) : V3(1, 0, 0);
+        v3 T2 = cross3(vh, T1);
+        real r = R_SQRT(ux), sp, cp;
+        r_sincos(O_TWO_PI * uy, &sp, &cp);
+        real t1 = r * cp, t2 = r * sp;
+        real s = RC(0.5) * (RC(1.0) + vh.z);
+        t2 = lerp1(R_SQRT(RC(1.0) - t1 * t1), t2, s);
+        v3 nh = add3(add3(scale3(T1, t1), scale3(T2, t2)), scale3(vh, R_SQRT(R_FMAX(RC(0.0), (RC(1.0) - t1 * t1) - t2 * t2))));
+        H = norm3(V3(a * nh.x, a * nh.y, R_FMAX(RC(0.0), nh.z)));
+    } else {
+        H = V3(0, 0, 1);
+    }
+    v3 I = neg3(V);
+    v3 L = sub3(I, scale3(H, RC(2.0) * dot3(H, I)));
+    return o_rotate(o_inv_rot(rot), L);
+}
+
+void oracle_spec_sample_gen(const float* fpf, const otrace_res* rt, int w, int h, const uint32_t* gb, const real* depth,
+                            uint32_t frame_rand, int has_prev, real* o_hitpos, real* o_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real dpt = depth[i];
+            if (dpt == RC(0.0)) {
+                st3r(o_rad, i, V3s(0));
+                continue;
+            }
+            v3 nrm = o_gb_normal(gb[plane + i]);
+            real pr = o_gb_rough(gb[plane + i]);
+            v3 pos = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, dpt + DEPTH_ERR);
+            uint32_t rng = o_lcg_init(px, py, frame_rand);
+            v3 vd = norm3(sub3(o_xyz(fp.view_pos), pos));
+            real a = R_FMAX(pr * pr, RC(0.002));
+            v3 dir = o_spec_dir(vd, nrm, a, &rng);
+            otrace_out t = o_trace_radiance(rt, &fp, pos, dir, has_prev);
+            real* hp = o_hitpos + 4 * i;
+            hp[0] = t.pos.x, hp[1] = t.pos.y, hp[2] = t.pos.z, hp[3] = RC(1.0);
+            st3r(o_rad, i, t.radiance);
+        }
+}
+
+void oracle_spec_sample_validate(const float* fpf, const otrace_res* rt, int w, int h, uint32_t* rw_rsv, real* rw_hitpos,
+                                 real* rw_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real pd = rt->prev_depth[i];
+            if (pd == RC(0.0)) continue;
+            v3 pos = o_depth_to_pos(fp.prev_inv_view_proj, px, py, w, h, pd + DEPTH_ERR);
+            real* hp = rw_hitpos + 4 * i;
+            v3 php = V3(hp[0], hp[1], hp[2]);
+            real ptf = hp[3];
+            v3 prad = ld3r(rw_rad, i);
+            otrace_out t = o_trace_radiance(rt, &fp, pos, norm3(sub3(php, pos)), 1);
+            real mr = R_FMAX(cmax3(t.radiance), cmax3(prad));
+            v3 dr = abs3(sub3(t.radiance, prad));
+            real lim = RC(0.5) * mr;
+            if (dr.x > lim || dr.y > lim || dr.z > lim) {
+                rw_rsv[2 * i] = 1;
+                hp[0] = t.pos.x, hp[1] = t.pos.y, hp[2] = t.pos.z, hp[3] = ptf;
+                st3r(rw_rad, i, t.radiance);
+            }
+        }
+}
+
+/* ---- AO ---- */
+static real bil1(const real* t, int w, int h, real u, real v) {
+    real tx = u * (real)w - RC(0.5), ty = v * (real)h - RC(0.5);
+    real x0 = R_FLOOR(tx), y0 = R_FLOOR(ty), fx = tx - x0, fy = ty - y0;
+    int xa = clampi((int)x0, 0, w - 1), xb = clampi((int)x0 + 1, 0, w - 1);
+    int ya = clampi((int)y0, 0, h - 1), yb = clampi((int)y0 + 1, 0, h - 1);
+    real top = t[(size_t)ya * w + xa] * (RC(1.0) - fx) + t[(size_t)ya * w + xb] * fx;
+    real bot = t[(size_t)yb * w + xa] * (RC(1.0) - fx) + t[(size_t)yb * w + xb] * fx;
+    return top * (RC(1.0) - fy) + bot * fy;
+}
+static v2 bil2(const real* t, int w, int h, real u, real v) {
+    real tx = u * (real)w - RC(0.5), ty = v * (real)h - RC(0.5);
+    real x0 = R_FLOOR(tx), y0 = R_FLOOR(ty), fx = tx - x0, fy = ty - y0;
+    int xa = clampi((int)x0, 0, w - 1), xb = clampi((int)x0 + 1, 0, w - 1);
+    int ya = clampi((int)y0, 0, h - 1), yb = clampi((int)y0 + 1, 0, h - 1);
+    v2 r;
+    for (int c = 0; c < 2; ++c) {
+        real top = t[2 * ((size_t)ya * w + xa) + c] * (RC(1.0) - fx) + t[2 * ((size_t)ya * w + xb) + c] * fx;
+        real bot = t[2 * ((size_t)yb * w + xa) + c] * (RC(1.0) - fx) + t[2 * ((size_t)yb * w + xb) + c] * fx;
+        real val = top * (RC(1.0) - fy) + bot * fy;
+        if (c == 0) r.x = val; else r.y = val;
+    }
+    return r;
+}
+
+void oracle_ao_gen(const float* fpf, int w, int h, const uint32_t* hit, const real* depth, const real* prev_ao,
+                   int has_new, int has_prev, real radius, real* o_ao) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                o_ao[2 * i] = o_ao[2 * i + 1] = 0;
+                continue;
+            }
+            v3 p = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d);
+            v4 hp = o_mul44(fp.view_proj, V4(p.x, p.y, p.z, RC(1.0)));
+            v4 hr = o_mul44(fp.prev_view_proj, V4(p.x, p.y, p.z, RC(1.0)));
+            real mvx = ((hp.x / hp.w - fp.jitter[0]) - (hr.x / hr.w - fp.jitter[2])) * RC(0.5);
+            real mvy = ((hp.y / hp.w - fp.jitter[1]) - (hr.y / hr.w - fp.jitter[3])) * RC(0.5);
+            real hu = ((real)px + RC(0.5)) / (real)w - mvx, hv = ((real)py + RC(0.5)) / (real)h - mvy;
+            v2 hist = V2(0, 0);
+            if (hu == sat(hu) && hv == sat(hv) && has_prev) hist = bil2(prev_ao, w, h, hu, hv);
+            if (has_new) {
+                v3 hpos, hn;
+                o_hit_dec(hit[2 * i], hit[2 * i + 1], &hpos, &hn);
+                real dist = len3(sub3(hpos, p));
+                real ao = dist < radius ? RC(0.0) : RC(1.0);
+                hist.x = lerp1(ao, hist.x, ((RC(32.0) - RC(1.0)) / RC(32.0)) * hist.y);
+                hist.y = RC(1.0);
+            } else {
+                hist.x = hist.y > RC(0.0) ? hist.x : RC(1.0);
+            }
+            o_ao[2 * i] = hist.x;
+            o_ao[2 * i + 1] = hist.y;
+        }
+}
+
+static inline real med3c(real a, real b, real c) { return R_FMIN(R_FMAX(a, b), c); }
+
+void oracle_ao_filter(int w, int h, const real* depth, const real* ao, real* o_ao) {
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                o_ao[2 * i] = o_ao[2 * i + 1] = 0;
+                continue;
+            }
+            for (int c = 0; c < 2; ++c) {
+                real center = ao[2 * i + c];
+                real cmed[3], cmin[3], cmax[3];
+                for (int y = -1; y <= 1; y++) {
+                    real row[3];
+                    for (int x = -1; x <= 1; x++) {
+                        size_t j = (size_t)clampi(py + y, 0, h - 1) * w + clampi(px + x, 0, w - 1);
+                        row[x + 1] = R_FABS(d - depth[j]) > RC(0.0005) ? center : ao[2 * j + c];
+                    }
+                    cmed[y + 1] = med3c(row[0], row[1], row[2]);
+                    cmin[y + 1] = R_FMIN(R_FMIN(row[0], row[1]), row[2]);
+                    cmax[y + 1] = R_FMAX(R_FMAX(row[0], row[1]), row[2]);
+                }
+                real A = R_FMIN(R_FMIN(cmax[0], cmax[1]), cmax[2]);
+                real B = R_FMAX(R_FMAX(cmin[0], cmin[1]), cmin[2]);
+                real C = med3c(cmed[0], cmed[1], cmed[2]);
+                real med = med3c(C, A, B);
+                real mn = R_FMIN(R_FMIN(cmin[0], cmin[1]), cmin[2]);
+                o_ao[2 * i + c] = lerp1(med, mn, RC(0.5));
+            }
+        }
+}
+
+/* ---- diffuse temporal ---- */
+void oracle_diff_temporal(const float* fpf, int w, int h, const real* prev_depth, const real* depth,
+                          const uint32_t* new_hit, const real* new_rad, const uint32_t* prev_rsv, const uint32_t* prev_hit,
+                          const real* prev_rad, uint32_t frame_rand, int has_prev, int has_new, uint32_t* o_rsv,
+                          uint32_t* o_hit, real* o_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            v3 p = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d);
+            v4 hr = o_mul44(fp.prev_view_proj, V4(p.x, p.y, p.z, RC(1.0)));
+            real nx = hr.x / hr.w - fp.jitter[2], ny = hr.y / hr.w - fp.jitter[3], nz = hr.z / hr.w - RC(0.0);
+            real u = nx * RC(0.5) + RC(0.5), v = ny * RC(0.5) + RC(0.5);
+            int sample_prev = has_prev && R_FABS(nx) < RC(1.0) && R_FABS(ny) < RC(1.0);
+            if (sample_prev && R_FABS(bil1(prev_depth, w, h, u, v) - nz) > RC(0.001)) sample_prev = 0;
+            uint32_t M = 0, ex = 0, ey = 0;
+            float Wf = 0.0f;
+            real W = 0;
+            v3 rad = V3s(0);
+            if (sample_prev) {
+                int ppx = (int)r2u(u * (real)w), ppy = (int)r2u(v * (real)h);
+                if (ppx > w - 1) ppx = w - 1;
+                if (ppy > h - 1) ppy = h - 1;
+                size_t j = (size_t)ppy * w + ppx;
+                o_rsv_dec(prev_rsv[j], &M, &Wf);
+                W = Wf;
+                rad = ld3r(prev_rad, j);
+                ex = prev_hit[2 * j];
+                ey = prev_hit[2 * j + 1];
+                if (M == 0) {
+                    rad = V3s(0);
+                    ex = ey = 0;
+                }
+                if (M > 16) M = 16;
+            }
+            if (has_new) {
+                real tp = lum(rad);
+                real pw = (W * tp) * (real)M;
+                v3 nr = ld3r(new_rad, i);
+                real nt = lum(nr);
+                real nw = nt * O_TWO_PI;
+                uint32_t rng = o_lcg_init(px, py, frame_rand);
+                real ws = pw + nw;
+                real chance = nw / ws;
+                if (o_rand(&rng) < chance) {
+                    ex = new_hit[2 * i];
+                    ey = new_hit[2 * i + 1];
+                    rad = nr;
+                    tp = nt;
+                }
+                M += 1;
+                W = tp > RC(0.0) ? ws / (tp * (real)M) : RC(0.0);
+            }
+            o_rsv[i] = o_rsv_enc(M, (float)W);
+            o_hit[2 * i] = ex;
+            o_hit[2 * i + 1] = ey;
+            st3r(o_rad, i, rad);
+        }
+}
+
+/* ---- diffuse spatial ---- */
+typedef struct {
+    uint32_t M;
+    real W, target;
+    v3 rad, pos, nrm;
+} owr;
+
+static owr o_diff_load(const uint32_t* rsv, const uint32_t* hit, const real* rad, size_t j, v3 pp, v3 pn) {
+    owr r;
+    float Wf;
+    o_rsv_dec(rsv[j], &r.M, &Wf);
+    r.W = Wf;
+    o_hit_dec(hit[2 * j], hit[2 * j + 1], &r.pos, &r.nrm);
+    r.rad = ld3r(rad, j);
+    r.target = sat(dot3(pn, norm3(sub3(r.pos, pp)))) * lum(r.rad);
+    return r;
+}
+
+void oracle_diff_spatial(const float* fpf, int w, int h, const uint32_t* gb, const real* depth, const real* ao,
+                         const uint32_t* t_rsv, const uint32_t* t_hit, const real* t_rad, uint32_t frame_rand,
+                         uint32_t* o_rsv, uint32_t* o_hit, real* o_rad, real* o_light) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    /* spiral constants (ind_diff_spatial_resample.hlsl:148-151, 186), folded in double */
+    const double PI_ = (double)(float)3.14159265359;
+    const float Bf = (float)(log(1.61803398) / (0.5 * PI_));
+    const float Af = (float)(1.0 / exp((double)Bf * (double)(2.0f * (float)PI_)));
+    real theta[9], rr[9];
+    for (int k = 0; k < 9; ++k) {
+        theta[k] = O_TWO_PI * (((real)k + RC(1.0)) / RC(9.0));
+#if defined(ORACLE_F32)
+        rr[k] = (float)((double)Af * exp((double)Bf * (double)theta[k]));
+#else
+        rr[k] = (double)Af * exp((double)Bf * theta[k]);
+#endif
+    }
+    const real T = (real)(float)(0.6186 * 9.0);
+    const real cos_tol = (real)(float)cos((double)(float)(PI_ * (25.0 / 180.0)));
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                o_rsv[i] = 0;
+                o_hit[2 * i] = o_hit[2 * i + 1] = 0;
+                st3r(o_rad, i, V3s(0));
+                st3r(o_light, i, V3s(0));
+                continue;
+            }
+            v3 p = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d);
+            v3 n = o_gb_normal(gb[plane + i]);
+            real a = ao[2 * i];
+            owr res = o_diff_load(t_rsv, t_hit, t_rad, i, p, n);
+            real ws = (res.W * res.target) * (real)res.M;
+            real radius = (real)(w < h ? w : h) * RC(0.05);
+            radius = R_FMAX(radius * a, RC(1.5));
+            uint32_t rng = o_lcg_init(px, py, frame_rand);
+            real ra = o_rand(&rng) * O_TWO_PI;
+            for (int k = 0; k < 9; ++k) {
+                real s, c;
+                r_sincos(theta[k] * T + ra, &s, &c);
+                real x = rr[k] * c, y = rr[k] * s;
+                int ox = (int)(x * radius), oy = (int)(y * radius);
+                int cx = clampi(px + ox, 0, w - 1), cy = clampi(py + oy, 0, h - 1);
+                size_t j = (size_t)cy * w + cx;
+                v3 nn = o_gb_normal(gb[plane + j]);
+                real dn = depth[j];
+                int diff = dot3(n, nn) < cos_tol;
+                v3 pn = o_depth_to_pos(fp.inv_view_proj, cx, cy, w, h, dn);
+                diff = diff || R_FABS(dot3(sub3(pn, p), n)) > RC(0.05);
+                if (diff) {
+                    radius = R_FMAX(radius * RC(0.5), RC(1.5));
+                    continue;
+                }
+                owr nb = o_diff_load(t_rsv, t_hit, t_rad, j, p, n);
+                v3 oqq = sub3(pn, nb.pos), orq = sub3(p, nb.pos);
+                real L2qq = dot3(oqq, oqq), L2rq = dot3(orq, orq);
+                real crq = dot3(nb.nrm, scale3(orq, RC(1.0) / R_SQRT(L2rq)));
+                real cqq = dot3(nb.nrm, scale3(oqq, RC(1.0) / R_SQRT(L2qq)));
+                real jac = (crq / cqq) * (L2qq / L2rq);
+                jac = R_FMIN(R_FMAX(jac, RC(0.0)), RC(10.0));
+                jac = R_FMIN(R_FMAX(jac, RC(0.0)), RC(1.0));
+                real tn = nb.target * jac;
+                if (tn <= RC(0.0)) continue;
+                uint32_t nM = nb.M < 255 ? nb.M : 255;
+                real wn = (nb.W * tn) * (real)nM;
+                ws += wn;
+                real chance = wn / ws;
+                if (o_rand(&rng) < chance) {
+                    res.nrm = nb.nrm;
+                    res.pos = nb.pos;
+                    res.rad = nb.rad;
+                    res.target = nb.target;
+                }
+                res.M += nM;
+            }
+            if (res.target > RC(0.0)) res.W = ws / (res.target * (real)res.M);
+            o_rsv[i] = o_rsv_enc(res.M, (float)res.W);
+            o_hit_enc(res.pos, res.nrm, &o_hit[2 * i], &o_hit[2 * i + 1]);
+            st3r(o_rad, i, res.rad);
+            real NoL = sat(dot3(n, norm3(sub3(res.pos, p))));
+            st3r(o_light, i, scale3(res.rad, (O_ONE_OVER_PI * NoL) * res.W));
+        }
+}
+
+/* ---- specular ---- */
+typedef struct {
+    real src_pdf;
+    v3 f;
+} ospec;
+
+static ospec o_calc_things(v3 view_pos, v3 p, v3 n, real pr, v3 f0, v3 hit) {
+    ospec r;
+    real a = R_FMAX(pr * pr, RC(0.002));
+    v3 vd = norm3(sub3(view_pos, p)), ld = norm3(sub3(hit, p)), H = norm3(add3(vd, ld));
+    real NoV = sat(dot3(n, vd)), NoH = sat(dot3(n, H));
+    real D = o_D_ggx(NoH, a);
+    r.src_pdf = (RC(0.25) * D) / o_lambda_pomc(NoV, a);
+    real NoL = sat(dot3(n, ld)), LoH = sat(dot3(ld, H));
+    v3 F = o_F_schlick(LoH, f0);
+    real g = o_g2_over_g1(NoL, NoV, a);
+    if (NoL <= RC(0.0)) g = RC(0.0);
+    if (NoV <= RC(0.0)) g = RC(0.0);
+    r.f = scale3(F, g);
+    return r;
+}
+static inline real o_spec_target(v3 view_pos, v3 p, v3 n, real pr, v3 f0, v3 hit, v3 rad) {
+    ospec s = o_calc_things(view_pos, p, n, pr, f0, hit);
+    return lum(mul3(rad, s.f)) * s.src_pdf;
+}
+
+void oracle_spec_temporal(const float* fpf, int w, int h, const uint32_t* prev_gb, const real* prev_depth,
+                          const uint32_t* gb, const real* depth, const uint32_t* prev_rsv, const real* prev_hitpos,
+                          const real* prev_rad, uint32_t frame_rand, int has_prev, uint32_t* rw_rsv, real* rw_hitpos,
+                          real* rw_rad) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                rw_rsv[2 * i] = rw_rsv[2 * i + 1] = 0;
+                continue;
+            }
+            v3 p = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d);
+            v4 hr = o_mul44(fp.prev_view_proj, V4(p.x, p.y, p.z, RC(1.0)));
+            real nx = hr.x / hr.w - fp.jitter[2], ny = hr.y / hr.w - fp.jitter[3], nz = hr.z / hr.w - RC(0.0);
+            real u = nx * RC(0.5) + RC(0.5), v = ny * RC(0.5) + RC(0.5);
+            int sp_ = (u == sat(u)) && (v == sat(v)) && has_prev;
+            if (sp_ && R_FABS(bil1(prev_depth, w, h, u, v) - nz) > RC(0.001)) sp_ = 0;
+            uint32_t M = 0;
+            real W = 0, tf = 0;
+            v3 rpos = V3s(0), rrad = V3s(0);
+            v3 pg_col = V3s(0), pg_n = V3s(0);
+            real pg_metal = 0, pg_rough = 0;
+            if (sp_) {
+                int ppx = (int)r2u(u * (real)w), ppy = (int)r2u(v * (real)h);
+                if (ppx > w - 1) ppx = w - 1;
+                if (ppy > h - 1) ppy = h - 1;
+                size_t j = (size_t)ppy * w + ppx;
+                M = prev_rsv[2 * j];
+                W = bits_f(prev_rsv[2 * j + 1]);
+                const real* hp = prev_hitpos + 4 * j;
+                rpos = V3(hp[0], hp[1], hp[2]);
+                tf = hp[3];
+                rrad = ld3r(prev_rad, j);
+                pg_col = o_gb_color(prev_gb[j]);
+                pg_metal = o_gb_metal(prev_gb[j]);
+                pg_n = o_gb_normal(prev_gb[plane + j]);
+                pg_rough = o_gb_rough(prev_gb[plane + j]);
+            }
+            const real* nh = rw_hitpos + 4 * i;
+            v3 npos = V3(nh[0], nh[1], nh[2]);
+            v3 nrad = ld3r(rw_rad, i);
+            v3 col = o_gb_color(gb[i]);
+            real metal = o_gb_metal(gb[i]);
+            v3 n = o_gb_normal(gb[plane + i]);
+            real pr = o_gb_rough(gb[plane + i]);
+            uint32_t Mmax = r2u(RC(16.0) * pr);
+            if (M > Mmax) M = Mmax;
+            v3 f0 = o_f0(col, metal);
+            ospec cur = o_calc_things(o_xyz(fp.view_pos), p, n, pr, f0, npos);
+            real nw = lum(mul3(nrad, cur.f));
+            real ntf = nw * cur.src_pdf;
+            real pw = W * (tf * (real)M);
+            if (M > 0) {
+                real ptc = o_spec_target(o_xyz(fp.prev_view_pos), p, pg_n, pg_rough, o_f0(pg_col, pg_metal), npos, nrad);
+                real cm = ntf / (ntf + ptc * (real)M);
+                if (ntf == RC(0.0)) cm = RC(0.0);
+                nw = cm * nw;
+                real ctp = o_spec_target(o_xyz(fp.view_pos), p, n, pr, f0, rpos, rrad);
+                real mn = tf * (real)M;
+                real pm = mn / (mn + ctp);
+                if (mn == RC(0.0)) pm = RC(0.0);
+                pw = pm * (ctp * W);
+                tf = ctp;
+            }
+            uint32_t rng = o_lcg_init(px, py, frame_rand);
+            real ws = pw + nw;
+            real cp = pw / ws;
+            int use_prev = o_rand(&rng) < cp;
+            if (!use_prev) {
+                rpos = npos;
+                rrad = nrad;
+                tf = ntf;
+            }
+            M += 1;
+            W = tf > RC(0.0) ? ws / tf : RC(0.0);
+            rw_rsv[2 * i] = M;
+            rw_rsv[2 * i + 1] = f_bits((float)W);
+            real* o = rw_hitpos + 4 * i;
+            o[0] = rpos.x, o[1] = rpos.y, o[2] = rpos.z, o[3] = tf;
+            if (use_prev) st3r(rw_rad, i, rrad);
+        }
+}
+
+void oracle_spec_spatial(const float* fpf, int w, int h, const uint32_t* gb, const real* depth, const uint32_t* rsv,
+                         const real* hitpos, const real* rad, uint32_t frame_rand, real* o_light, real* o_raylen) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    const double PI_ = (double)(float)3.14159265359;
+    const real cos_tol = (real)(float)cos((double)(float)(PI_ * (25.0 / 180.0)));
+    v3 vp = o_xyz(fp.view_pos);
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                st3r(o_light, i, V3s(0));
+                o_raylen[i] = 0;
+                continue;
+            }
+            v3 p = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d);
+            v3 col = o_gb_color(gb[i]);
+            real metal = o_gb_metal(gb[i]);
+            v3 n = o_gb_normal(gb[plane + i]);
+            real pr = o_gb_rough(gb[plane + i]);
+            uint32_t M = rsv[2 * i];
+            real W = bits_f(rsv[2 * i + 1]);
+            const real* hp = hitpos + 4 * i;
+            v3 hpos = V3(hp[0], hp[1], hp[2]);
+            v3 hrad = ld3r(rad, i);
+            v3 f0 = o_f0(col, metal);
+            real tf = hp[3];
+            uint32_t iters = r2u(RC(0.0) + pr * (RC(4.0) - RC(0.0)));
+            real radius = (real)(w < h ? w : h) * RC(0.05);
+            uint32_t rng = o_lcg_init(px, py, frame_rand);
+            for (uint32_t k = 0; k < iters; ++k) {
+                real ux = o_rand(&rng), uy = o_rand(&rng);
+                real r = R_SQRT(ux), s, c;
+                r_sincos(O_TWO_PI * uy, &s, &c);
+                real x = r * c, y = r * s;
+                int ox = (int)(x * radius), oy = (int)(y * radius);
+                int cx = clampi(px + ox, 0, w - 1), cy = clampi(py + oy, 0, h - 1);
+                if (cx == px && cy == py) continue;
+                size_t j = (size_t)cy * w + cx;
+                v3 ncol = o_gb_color(gb[j]);
+                real nmetal = o_gb_metal(gb[j]);
+                v3 nn = o_gb_normal(gb[plane + j]);
+                real npr = o_gb_rough(gb[plane + j]);
+                real dn = depth[j];
+                int diff = dot3(n, nn) < cos_tol;
+                v3 pn = o_depth_to_pos(fp.inv_view_proj, cx, cy, w, h, dn);
+                diff = diff || R_FABS(dot3(sub3(pn, p), n)) > RC(0.05);
+                if (diff) {
+                    radius = R_FMAX(radius * RC(0.5), RC(1.5));
+                    continue;
+                }
+                uint32_t Mn = rsv[2 * j];
+                real Wn = bits_f(rsv[2 * j + 1]);
+                const real* hpn = hitpos + 4 * j;
+                v3 hposn = V3(hpn[0], hpn[1], hpn[2]);
+                v3 hradn = ld3r(rad, j);
+                real tfn = hpn[3];
+                real c_on_n = o_spec_target(vp, p, n, pr, f0, hposn, hradn);
+                real mnb = (tfn * (real)Mn) / (tfn * (real)Mn + c_on_n * (real)M);
+                if (tfn * (real)Mn == RC(0.0)) mnb = RC(0.0);
+                real wnb = (mnb * tfn) * Wn;
+                real n_on_c = o_spec_target(vp, pn, nn, npr, o_f0(ncol, nmetal), hpos, hrad);
+                real mc = (tf * (real)M) / (tf * (real)M + n_on_c * (real)Mn);
+                if (tf * (real)M == RC(0.0)) mc = RC(0.0);
+                real wc = (mc * tf) * W;
+                real ws = wc + wnb;
+                real chance = wnb / ws;
+                if (o_rand(&rng) < chance) {
+                    hpos = hposn;
+                    hrad = hradn;
+                    tf = tfn;
+                }
+                M += Mn;
+                W = tf > RC(0.0) ? ws / tf : RC(0.0);
+            }
+            o_raylen[i] = len3(sub3(hpos, p));
+            real a = R_FMAX(pr * pr, RC(0.002));
+            v3 sel = norm3(sub3(hpos, p)), vd = norm3(sub3(vp, p)), H = norm3(add3(vd, sel));
+            real NoL = sat(dot3(n, sel)), NoV = sat(dot3(n, vd)), NoH = sat(dot3(n, H)), LoH = sat(dot3(sel, H));
+            v3 F = o_F_schlick(LoH, f0);
+            real D = o_D_ggx(NoH, a);
+            real vndf = (RC(0.25) * D) / o_lambda_pomc(NoV, a);
+            real g = o_g2_over_g1(NoL, NoV, a);
+            if (NoL <= RC(0.0)) g = RC(0.0);
+            if (NoV <= RC(0.0)) g = RC(0.0);
+            v3 L = mul3(hrad, scale3(scale3(F, g), vndf * W));
+            v3 refl = o_ggx_integral(NoV, a, f0);
+            refl = V3(R_FMAX(refl.x, RC(1.0) / RC(65536.0)), R_FMAX(refl.y, RC(1.0) / RC(65536.0)),
+                      R_FMAX(refl.z, RC(1.0) / RC(65536.0)));
+            st3r(o_light, i, div3(L, refl));
+        }
+}
+
+/* ---- primary-visibility G-buffer (the producer that stands in for the
+ * raster pass gbuffer_pass.rs:45-137 / mesh_gbuffer.hlsl:62-126) ---- */
+static inline uint32_t enc_unorm(real x, real s) { return r2u(x * s); }
+
+void oracle_gbuffer(const float* fpf, const oscene* s, int w, int h, real* o_depth, uint32_t* o_gb) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    real* nrm = (real*)calloc(3 * plane, sizeof(real));
+    uint8_t* geo = (uint8_t*)calloc(plane, 1);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real ndx = ((real)px + RC(0.5)) / (real)w * RC(2.0) - RC(1.0);
+            real ndy = ((real)py + RC(0.5)) / (real)h * RC(2.0) - RC(1.0);
+            v4 e = o_mul44(fp.inv_view_proj, V4(ndx, ndy, RC(1.0), RC(1.0)));
+            v3 o = o_xyz(fp.view_pos);
+            v3 d = norm3(sub3(V3(e.x / e.w, e.y / e.w, e.z / e.w), o));
+            ohit hh = obvh_trace(s->bvh, o, d, RC(0.05), RC(100000.0), 0);
+            o_depth[i] = 0;
+            o_gb[i] = o_gb[plane + i] = o_gb[2 * plane + i] = o_gb[3 * plane + i] = 0;
+            if (hh.id == 0xFFFFFFFFu) continue;
+            uint32_t i0 = s->I[3 * hh.id], i1 = s->I[3 * hh.id + 1], i2 = s->I[3 * hh.id + 2];
+            v3 p0 = vtx(s->P, i0), p1 = vtx(s->P, i1), p2 = vtx(s->P, i2);
+            real w0 = (RC(1.0) - hh.u) - hh.v;
+            v3 pos = add3(add3(scale3(p0, w0), scale3(p1, hh.u)), scale3(p2, hh.v));
+            v3 ng = norm3(cross3(sub3(p1, p0), sub3(p2, p0)));
+            v3 n = ng;
+            if (s->N) n = norm3(add3(add3(scale3(vtx(s->N, i0), w0), scale3(vtx(s->N, i1), hh.u)), scale3(vtx(s->N, i2), hh.v)));
+            if (dot3(ng, d) > RC(0.0)) n = neg3(n);
+            v4 c = o_mul44(fp.view_proj, V4(pos.x, pos.y, pos.z, RC(1.0)));
+            real dep = c.z / c.w;
+            if (!(dep > RC(0.0))) continue;
+            const float* m = s->mats + 8 * (s->TM ? s->TM[hh.id] : 0);
+            o_depth[i] = dep;
+            geo[i] = 1;
+            nrm[3 * i] = n.x, nrm[3 * i + 1] = n.y, nrm[3 * i + 2] = n.z;
+            o_gb[i] = enc_unorm(m[0], RC(255.0)) | (enc_unorm(m[1], RC(255.0)) << 8) | (enc_unorm(m[2], RC(255.0)) << 16) |
+                      (enc_unorm(m[4], RC(255.0)) << 24);
+            v2 oe = o_oct_enc(n);
+            o_gb[plane + i] = enc_unorm(oe.x, RC(4095.0)) | (enc_unorm(oe.y, RC(4095.0)) << 12) |
+                              (enc_unorm(m[5], RC(255.0)) << 24);
+            o_gb[3 * plane + i] = 1;
+        }
+    /* fwidth from fine differences inside aligned 2x2 quads */
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            if (!geo[i]) continue;
+            int qx = px ^ 1, qy = py ^ 1;
+            int okx = qx < w && geo[(size_t)py * w + qx];
+            int oky = qy < h && geo[(size_t)qy * w + px];
+            size_t jx = (size_t)py * w + (qx < w ? qx : px), jy = (size_t)(qy < h ? qy : py) * w + px;
+            real fz = (okx ? R_FABS(o_depth[jx] - o_depth[i]) : RC(0.0)) + (oky ? R_FABS(o_depth[jy] - o_depth[i]) : RC(0.0));
+            v3 ni = V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+            v3 ax = okx ? abs3(sub3(V3(nrm[3 * jx], nrm[3 * jx + 1], nrm[3 * jx + 2]), ni)) : V3s(0);
+            v3 ay = oky ? abs3(sub3(V3(nrm[3 * jy], nrm[3 * jy + 1], nrm[3 * jy + 2]), ni)) : V3s(0);
+            real fn = len3(add3(ax, ay));
+            o_gb[2 * plane + i] = o_double_to_half((double)(float)fz) | (o_double_to_half((double)(float)fn) << 16);
+        }
+    free(nrm);
+    free(geo);
+}
+
+/* ---- hash grid maintenance ---- */
+void oracle_hg_decay(uint8_t* decay, uint32_t* cells) {
+    for (uint32_t b = 0; b < 65536u; ++b) {
+        uint32_t off = b * 16u, occ = 0, set = 0, slice[4] = {0, 0, 0, 0};
+        while (set < 4) {
+            uint32_t base = off + set * 4, packed = 0;
+            memcpy(&packed, decay + base, 4);
+            if (packed == 0) break;
+            uint32_t upd = 0, dv = 0;
+            for (uint32_t i = 0; i < 4; ++i) {
+                dv = (packed >> (8 * i)) & 0xFFu;
+                if (dv == 0) break;
+                dv -= 1;
+                if (dv == 0) {
+                    memset(cells + 4 * (size_t)(base + i), 0, 16);
+                } else {
+                    upd |= dv << (8 * i);
+                    occ |= 1u << (set * 4 + i);
+                }
+            }
+            slice[set++] = upd;
+            if (dv == 0) break;
+        }
+        while (occ) {
+            uint32_t last = 31u - (uint32_t)__builtin_clz(occ);
+            uint32_t fs = (uint32_t)__builtin_ctz(~occ);
+            if (!(fs < last)) break;
+            uint32_t dv = slice[last >> 2] >> ((last & 3) * 8);
+            slice[last >> 2] ^= dv << ((last & 3) * 8);
+            slice[fs >> 2] |= dv << ((fs & 3) * 8);
+            memcpy(cells + 4 * (size_t)(off + fs), cells + 4 * (size_t)(off + last), 16);
+            memset(cells + 4 * (size_t)(off + last), 0, 16);
+            occ |= 1u << fs;
+            occ ^= 1u << last;
+        }
+        for (uint32_t i = 0; i < set; ++i) memcpy(decay + off + 4 * i, &slice[i], 4);
+    }
+}
+
+/* Sequential restatement: queries processed in index order (one valid
+ * schedule of the reference's concurrent CAS inserts / atomicMax). */
+void oracle_hg_select(const uint32_t* queries, uint32_t counter, uint32_t* cells, uint32_t* selection,
+                      uint32_t frame_hash) {
+    uint32_t n = counter < HG_MAX_QUERIES ? counter : HG_MAX_QUERIES;
+    for (uint32_t id = 0; id < n; ++id) {
+        uint32_t hash = queries[6 * (size_t)id + 4], ck = queries[6 * (size_t)id + 5];
+        int found = 0;
+        uint32_t addr = 0;
+        for (uint32_t a = hash << 4; a < (hash << 4) + 16; ++a) {
+            uint32_t slot = cells[4 * (size_t)a];
+            if (slot == ck) { found = 1; addr = a; break; }
+            if (slot == 0) { cells[4 * (size_t)a] = ck; found = 1; addr = a; break; }
+        }
+        if (!found) continue;
+        uint32_t st = o_jenkins(id ^ frame_hash);
+        uint32_t r = o_xorshift(&st);
+        uint32_t word = (r & (0xFFFFFFFFu << 18)) | (1u << 31) | id;
+        if (word > selection[addr]) selection[addr] = word;
+    }
+}
+
+uint32_t oracle_hg_compact(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr) {
+    uint32_t n = 0;
+    for (uint32_t c = 0; c < 65536u * 16u; ++c)
+        if (selection[c] != 0) {
+            out_index[n] = selection[c] & 0x1FFFFu;
+            out_addr[n] = c;
+            ++n;
+        }
+    return n;
+}
+
+/* One cosine ray per compacted query, in compacted order; the cache lookup
+ * of the trace reads `cells` as updated so far (sequential schedule). */
+void oracle_hg_raygen(const float* fpf, const otrace_res* rt, uint32_t* cells, const uint32_t* queries, uint32_t count,
+                      const uint32_t* cidx, const uint32_t* caddr, uint8_t* decay, uint32_t frame_hash) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    for (uint32_t id = 0; id < count && id < HG_MAX_QUERIES; ++id) {
+        const uint32_t* q = queries + 6 * (size_t)cidx[id];
+        uint32_t cell = caddr[id];
+        uint32_t rng = o_jenkins(id ^ frame_hash);
+        v3 n = o_oct_dec_u32(q[3]);
+        real u0 = o_rand(&rng), u1 = o_rand(&rng);
+        real st_, sc;
+        r_sincos(O_TWO_PI * u1, &st_, &sc);
+        real sin_t = R_SQRT(u0), cos_t = R_SQRT(RC(1.0) - u0);
+        v3 l = V3(sin_t * sc, sin_t * st_, cos_t);
+        v3 dir = o_rotate(o_inv_rot(o_rot_to_z(n)), l);
+        otrace_out t = o_trace_radiance(rt, &fp, V3(bits_f(q[0]), bits_f(q[1]), bits_f(q[2])), dir, 1);
+        uint32_t* c = cells + 4 * (size_t)cell;
+        const real bf = RC(1.0) / RC(16.0);
+        v3 prev = V3(bits_f(c[1]), bits_f(c[2]), bits_f(c[3]));
+        v3 b = add3(scale3(t.radiance, bf), scale3(prev, RC(1.0) - bf));
+        c[1] = f_bits((float)b.x);
+        c[2] = f_bits((float)b.y);
+        c[3] = f_bits((float)b.z);
+        decay[cell] = 10;
+    }
+}

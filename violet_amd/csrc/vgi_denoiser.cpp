@@ -11,33 +11,7 @@
  */
 #include <new>
 
-#include "vgi_internal.h"
-
-namespace vgi {
-
-struct ColorSet {
-    DevBuf diff, spec, pair;
-};
-
-}  // namespace vgi
-
-struct vgi_denoiser {
-    uint32_t w = 0, h = 0, storage = 0;
-    vgi_denoiser_config cfg{1, 0, 3};
-    vgi::ColorSet temporal[2];
-    vgi::DevBuf temporal_len[2];
-    vgi::ColorSet fix[2];
-    vgi::DevBuf fast_diff[2], fast_spec[2];
-    vgi::ColorSet pp[2];  // fallback output + a-trous ping-pong (pair = variance)
-    int cur = 0;           // set index written this frame
-    bool has_history = false;
-    bool has_fast_history = false;
-    // history views (point into the set written last frame)
-    const void* hist_diff = nullptr;
-    const void* hist_spec = nullptr;
-    const void* hist_mom = nullptr;
-    const uint8_t* hist_len = nullptr;
-};
+#include "vgi_denoiser_impl.h"
 
 using namespace vgi;
 

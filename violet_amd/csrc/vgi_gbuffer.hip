@@ -1,0 +1,88 @@
+/*
+ * vgi_gbuffer.hip — primary-visibility G-buffer producer.
+ *
+ * Stands in for violet's raster G-buffer pass (gbuffer_pass.rs:45-137,
+ * mesh_gbuffer.hlsl:62-126): one ray through each pixel centre, the hit's
+ * reverse-Z depth (view_proj applied to the interpolated hit position), and
+ * the four G-buffer layers encoded as gbuffer.hlsl:54-77.  fwidth_z and
+ * fwidth_n are |ddx| + |ddy| with fine differences inside each aligned 2x2
+ * quad, taken with lane shuffles: the quad's four pixels are lanes 4q..4q+3
+ * (x partner = lane ^ 1, y partner = lane ^ 2), exactly the lanes a raster
+ * pixel shader's derivatives come from.  A partner without geometry (or
+ * outside the image) contributes 0.
+ */
+#include "vgi_internal.h"
+#include "vgi_radiance.h"
+
+namespace vgi {
+const SceneView& scene_view(const vgi_scene* s);
+
+__global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    const bool inside = px < w && py < h;
+    bool geo = false;
+    float depth = 0.0f;
+    f3 n = mk3s(0.0f);
+    SurfaceHit sh;
+    if (inside) {
+        float ndx = ((float)px + 0.5f) / (float)w * 2.0f - 1.0f;
+        float ndy = ((float)py + 0.5f) / (float)h * 2.0f - 1.0f;
+        f4 e = mul44(fp.inv_view_proj, mk4(ndx, ndy, 1.0f, 1.0f));
+        f3 end = mk3(e.x / e.w, e.y / e.w, e.z / e.w);
+        f3 o = xyz(fp.view_pos);
+        f3 d = normalize3(end - o);
+        Hit hit = trace_closest(s, o, d, 0.05f, 100000.0f, stack);
+        sh = shade_hit(s, hit, d);
+        if (!sh.missed) {
+            f4 c = mul44(fp.view_proj, mk4(sh.position.x, sh.position.y, sh.position.z, 1.0f));
+            depth = c.z / c.w;
+            geo = depth > 0.0f;
+            n = sh.normal;
+        }
+    }
+    // quad partners (all 64 lanes take part in the shuffles)
+    float dx = __shfl_xor(depth, 1), dy = __shfl_xor(depth, 2);
+    int gx = __shfl_xor((int)geo, 1), gy = __shfl_xor((int)geo, 2);
+    f3 nx = mk3(__shfl_xor(n.x, 1), __shfl_xor(n.y, 1), __shfl_xor(n.z, 1));
+    f3 ny = mk3(__shfl_xor(n.x, 2), __shfl_xor(n.y, 2), __shfl_xor(n.z, 2));
+    if (!inside) return;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+    if (!geo) {
+        io.rw_depth[i] = 0.0f;
+        io.rw_gbuffer[i] = 0u;
+        io.rw_gbuffer[plane + i] = 0u;
+        io.rw_gbuffer[2 * plane + i] = 0u;
+        io.rw_gbuffer[3 * plane + i] = 0u;
+        return;
+    }
+    float fz = (gx ? fabsf(dx - depth) : 0.0f) + (gy ? fabsf(dy - depth) : 0.0f);
+    f3 ax = gx ? abs3(nx - n) : mk3s(0.0f);
+    f3 ay = gy ? abs3(ny - n) : mk3s(0.0f);
+    float fn = length3(ax + ay);
+    io.rw_depth[i] = depth;
+    io.rw_gbuffer[i] = encode_gbuffer_color(sh.base_color, sh.metallic);
+    io.rw_gbuffer[plane + i] = encode_gbuffer_normal(n, sh.roughness);
+    io.rw_gbuffer[2 * plane + i] = encode_gbuffer_fwidth(fz, fn);
+    io.rw_gbuffer[3 * plane + i] = 1u;
+}
+
+}  // namespace vgi
+
+using namespace vgi;
+
+extern "C" vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const vgi_scene* scene,
+                                               const vgi_gbuffer_io* io, void* stream) {
+    VGI_REQUIRE(fp && scene && io, "vgi_pass_gbuffer_primary: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_gbuffer_primary: bad size");
+    VGI_REQUIRE(io->rw_depth && io->rw_gbuffer, "vgi_pass_gbuffer_primary: missing binding");
+    FrameParams f;
+    memcpy(&f, fp, sizeof(f));
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("gbuffer_primary", st, (double)io->width * io->height * 20.0);
+    hipLaunchKernelGGL(k_gbuffer, tile_grid(io->width, io->height), dim3(64), 0, st, f, scene_view(scene), *io);
+    return check_launch("k_gbuffer");
+}

@@ -1,0 +1,172 @@
+/*
+ * vgi_hashgrid.hip — world-space radiance cache maintenance passes:
+ *
+ *   k_hg_decay    restir/hash_grid_decay.hlsl:13-128   (one lane per bucket)
+ *   k_hg_select   restir/hash_grid_query_select.hlsl:22-56 (one lane per query)
+ *   k_hg_compact  restir/hash_grid_query_compact.hlsl:78-118
+ *
+ * Storage: 65536 buckets x 16 cells of {u32 checksum, f32 rgb} (16 MiB),
+ * decay 1 byte per cell (read four cells per 32-bit word).  Insertion uses a
+ * CAS on the checksum and the per-cell query choice an atomicMax, as in the
+ * reference; which query index / which slot wins depends on scheduling, so
+ * only the resulting sets are deterministic.  Compaction is a wave-ballot
+ * stream compaction: one global atomic per wave of 64 cells.
+ */
+#include "vgi_internal.h"
+#include "vgi_radiance.h"
+
+namespace vgi {
+
+__global__ void __launch_bounds__(64) k_hg_decay(uint32_t* decay_words, HashCell* storage) {
+    const uint32_t bucket = blockIdx.x * 64u + threadIdx.x;
+    if (bucket >= kHashGridNumBuckets) return;
+    const uint32_t addr_offset = bucket * 16u;
+    uint32_t occ = 0;
+    uint32_t slice[4] = {0u, 0u, 0u, 0u};
+    uint32_t set = 0;
+    for (; set < 4u;) {
+        uint32_t set_addr = addr_offset + set * 4u;
+        uint32_t packed = decay_words[set_addr >> 2];
+        if (packed == 0u) break;
+        uint32_t updated = 0u;
+        uint32_t decay = 0u;
+        for (uint32_t i = 0; i < 4u; ++i) {
+            uint32_t shift = i * 8u;
+            decay = (packed >> shift) & 0xFFu;
+            if (decay == 0u) break;
+            decay = decay - 1u;
+            if (decay == 0u) {
+                HashCell e;
+                e.checksum = 0u;
+                e.r = e.g = e.b = 0.0f;
+                storage[set_addr + i] = e;
+            } else {
+                updated |= (decay << shift);
+                occ |= (1u << (set * 4u + i));
+            }
+        }
+        slice[set] = updated;
+        ++set;
+        if (decay == 0u) break;  // (also taken when the set's last cell just expired)
+    }
+    if (occ != 0u) {
+        while (true) {
+            uint32_t last = 31u - (uint32_t)__clz(occ);
+            uint32_t first_space = (uint32_t)__ffs(~occ) - 1u;
+            if (first_space < last) {
+                uint32_t src_set = last >> 2, dst_set = first_space >> 2;
+                uint32_t last_shift = (last & 3u) * 8u;
+                uint32_t d = slice[src_set] >> last_shift;
+                slice[src_set] ^= (d << last_shift);
+                uint32_t fs_shift = (first_space & 3u) * 8u;
+                slice[dst_set] |= (d << fs_shift);
+                HashCell c = storage[addr_offset + last];
+                storage[addr_offset + first_space] = c;
+                HashCell e;
+                e.checksum = 0u;
+                e.r = e.g = e.b = 0.0f;
+                storage[addr_offset + last] = e;
+                occ |= (1u << first_space);
+                occ ^= (1u << last);
+            } else {
+                break;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < set; ++i) decay_words[(addr_offset >> 2) + i] = slice[i];
+}
+
+// hash_grid.inc.hlsl:149-184
+__device__ __forceinline__ bool hash_grid_find_or_insert(HashCell* buf, uint32_t hash, uint32_t checksum,
+                                                         uint32_t& addr) {
+    uint32_t beg = hash << kHashGridBucketShift;
+    for (addr = beg; addr < beg + kHashGridBucketSize; ++addr) {
+        uint32_t slot = __atomic_load_n(&buf[addr].checksum, __ATOMIC_RELAXED);
+        if (slot == checksum) return true;
+        if (slot == 0u) {
+            uint32_t prev = atomicCAS(&buf[addr].checksum, 0u, checksum);
+            if (prev == 0u || prev == checksum) return true;
+        }
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, const uint32_t* counter, HashCell* storage,
+                                                  uint32_t* selection, uint32_t frame_hash) {
+    uint32_t id = blockIdx.x * 64u + threadIdx.x;
+    if (id >= kHashGridMaxQueries || id >= *counter) return;
+    HashQuery q = queries[id];
+    uint32_t addr;
+    if (!hash_grid_find_or_insert(storage, q.hash, q.checksum, addr)) return;
+    uint32_t rng = jenkins_hash(id ^ frame_hash);
+    uint32_t r = lcg_xorshift(rng);
+    uint32_t word = (r & (0xFFFFFFFFu << (kHashGridMaxQueriesShift + 1u))) | (1u << 31) | id;
+    atomicMax(&selection[addr], word);
+}
+
+__global__ void __launch_bounds__(64) k_hg_compact(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr,
+                                                   uint32_t* out_counter) {
+    uint32_t cell = blockIdx.x * 64u + threadIdx.x;
+    uint32_t word = cell < kHashGridNumCells ? selection[cell] : 0u;
+    bool active = word != 0u;
+    uint64_t mask = __ballot(active);
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t total = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (total != 0u) {
+        int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(out_counter, total);
+        base = __shfl(base, leader);
+    }
+    if (active) {
+        out_index[base + below] = word & ((1u << kHashGridMaxQueriesShift) - 1u);
+        out_addr[base + below] = cell;
+    }
+}
+
+}  // namespace vgi
+
+using namespace vgi;
+
+extern "C" vgi_status vgi_pass_hash_grid_decay(const vgi_hash_grid_decay_io* io, void* stream) {
+    VGI_REQUIRE(io && io->rw_decay_buffer && io->rw_storage_buffer, "vgi_pass_hash_grid_decay: missing binding");
+    VGI_REQUIRE(((uintptr_t)io->rw_decay_buffer & 3u) == 0, "vgi_pass_hash_grid_decay: decay buffer not 4-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("hash_grid_decay", st, (double)kHashGridNumCells * 2.0);
+    hipLaunchKernelGGL(k_hg_decay, dim3(kHashGridNumBuckets / 64), dim3(64), 0, st, (uint32_t*)io->rw_decay_buffer,
+                       reinterpret_cast<HashCell*>(io->rw_storage_buffer));
+    return check_launch("k_hg_decay");
+}
+
+extern "C" vgi_status vgi_pass_hash_grid_query_select(const vgi_hash_grid_query_select_io* io, const vgi_pc_frame_hash* pc,
+                                                      void* stream) {
+    VGI_REQUIRE(io && pc, "vgi_pass_hash_grid_query_select: null argument");
+    VGI_REQUIRE(io->hash_grid_query_buffer && io->hash_grid_query_counter_buffer && io->rw_hash_grid_storage_buffer &&
+                    io->rw_hash_grid_query_selection_buffer,
+                "vgi_pass_hash_grid_query_select: missing binding");
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("hash_grid_query_select", st, 0.0);
+    hipLaunchKernelGGL(k_hg_select, dim3(kHashGridMaxQueries / 64), dim3(64), 0, st,
+                       reinterpret_cast<const HashQuery*>(io->hash_grid_query_buffer), io->hash_grid_query_counter_buffer,
+                       reinterpret_cast<HashCell*>(io->rw_hash_grid_storage_buffer),
+                       io->rw_hash_grid_query_selection_buffer, pc->frame_hash);
+    return check_launch("k_hg_select");
+}
+
+extern "C" vgi_status vgi_pass_hash_grid_query_compact(const vgi_hash_grid_query_compact_io* io, void* stream) {
+    VGI_REQUIRE(io && io->hash_grid_query_selection_buffer && io->rw_compact_query_index_buffer &&
+                    io->rw_compact_query_cell_addr_buffer && io->rw_compact_query_counter_buffer,
+                "vgi_pass_hash_grid_query_compact: missing binding");
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("hash_grid_query_compact", st, (double)kHashGridNumCells * 4.0);
+    hipLaunchKernelGGL(k_hg_compact, dim3(kHashGridNumCells / 64), dim3(64), 0, st, io->hash_grid_query_selection_buffer,
+                       io->rw_compact_query_index_buffer, io->rw_compact_query_cell_addr_buffer,
+                       io->rw_compact_query_counter_buffer);
+    return check_launch("k_hg_compact");
+}
+
+extern "C" vgi_status vgi_pass_clear_buffer(uint32_t* buffer, uint32_t length, uint32_t value, void* stream) {
+    VGI_REQUIRE(buffer || length == 0, "vgi_pass_clear_buffer: null buffer");
+    return fill_u32(buffer, length, value, (hipStream_t)stream);
+}

@@ -1,0 +1,341 @@
+/*
+ * vgi_sample.hip — the radiance-tracing passes of the ReSTIR GI path:
+ *
+ *   k_diff_sample_gen      restir/ind_diff_sample_gen.hlsl:25-67
+ *   k_diff_sample_validate restir/ind_diff_sample_validate.hlsl:90-145
+ *   k_spec_sample_gen      restir/ind_spec_sample_gen.hlsl:28-128
+ *   k_spec_sample_validate restir/ind_spec_sample_validate.hlsl:144-191
+ *   k_hash_grid_raygen     restir/hash_grid_raygen.hlsl:141-204
+ *
+ * One lane per pixel (or per compacted cache query).  Each lane traces one
+ * GI ray plus one sun shadow ray through the LDS-stack traversal.  Pixels
+ * without geometry leave the wave early, so the remaining lanes are the
+ * active rays.
+ */
+#include "vgi_internal.h"
+#include "vgi_radiance.h"
+
+namespace vgi {
+const SceneView& scene_view(const vgi_scene* s);
+
+constexpr float kDepthError = 1.0f / 16777216.0f;  // 24-bit depth bias of the sample passes
+
+__device__ __forceinline__ f3 position_biased(const float* m, uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                                              float depth) {
+    return depth_to_position(m, px, py, w, h, depth + kDepthError);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io,
+                                                        vgi_pc_sample_gen pc) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    float depth = io.gbuffer_depth[i];
+    if (depth == 0.0f) {
+        io.rw_hit_pos_normal_texture[2 * i] = 0u;
+        io.rw_hit_pos_normal_texture[2 * i + 1] = 0u;
+        S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
+        return;
+    }
+    GBufferNormal g = decode_gbuffer_normal(io.gbuffer_color[(size_t)w * h + i]);
+    f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
+    uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
+    float u0 = lcg_rand(rng);
+    float u1 = lcg_rand(rng);
+    f3 dir = sample_hemisphere_uniform_with_normal(u0, u1, g.normal);
+    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
+    uint32_t ex, ey;
+    hit_encode(r.position_ws, r.normal_ws, ex, ey);
+    io.rw_hit_pos_normal_texture[2 * i] = ex;
+    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, TraceRes rt,
+                                                             vgi_ind_diff_sample_validate_io io) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    float depth = io.depth_texture[i];
+    if (depth == 0.0f) return;
+    HitPosNormal hit = hit_decode(io.rw_hit_pos_normal_texture[2 * i], io.rw_hit_pos_normal_texture[2 * i + 1]);
+    f3 prev_rad = S::ld3(io.rw_hit_radiance_texture, i);
+    f3 origin = depth_to_position(fp.prev_inv_view_proj, px, py, w, h, depth);
+    f3 dir = normalize3(hit.pos - origin);
+    RadianceTraceResult r = trace_radiance<S>(rt, fp, origin, dir, true, stack);
+    float lumi_prev = luminance(prev_rad);
+    float lumi_diff = luminance(abs3(r.radiance - prev_rad));
+    float prev_ray_len = length3(hit.pos - origin);
+    bool pos_changed = component_max(abs3(hit.pos - r.position_ws)) > fmaxf(prev_ray_len / 128.0f, 0.01f);
+    float m_scale = saturate(1.0f - (1.0f * lumi_diff) / lumi_prev);
+    m_scale = pos_changed ? 0.0f : m_scale;
+    if ((lumi_prev > 0.0f) && (m_scale <= (1.0f - 0.5f / 32.0f))) {
+        ReservoirSimple rs = reservoir_decode_32b(io.rw_reservoir_texture[i]);
+        rs.M = f2u(floorf((float)rs.M * m_scale));
+        io.rw_reservoir_texture[i] = reservoir_encode_32b(rs);
+    }
+    uint32_t ex, ey;
+    hit_encode(r.position_ws, r.normal_ws, ex, ey);
+    io.rw_hit_pos_normal_texture[2 * i] = ex;
+    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+}
+
+// GGX VNDF sample [Heitz 2014] of ind_spec_sample_gen.hlsl:67-120; returns the
+// reflected world-space direction.
+__device__ __forceinline__ f3 sample_spec_dir(f3 view_dir, f3 normal, float roughness, uint32_t& rng) {
+    f4 rot = get_rotation_to_z_from(normal);
+    f3 V = rotate_point(rot, view_dir);
+    f3 H;
+    if (roughness != 0.0f) {
+        float ux = lcg_rand(rng);
+        float uy = lcg_rand(rng);
+        f3 v_h = normalize3(mk3(roughness * V.x, roughness * V.y, V.z));
+        float lensq = v_h.x * v_h.x + v_h.y * v_h.y;
+        f3 T1 = lensq > 0.0f ? mk3(-v_h.y, v_h.x, 0.0f) * rsqrt(lensq) : mk3(1.0f, 0.0f, 0.0f);
+        f3 T2 = cross3(v_h, T1);
+        float r = sqrtf(ux);
+        float phi = kTwoPi * uy;
+        float sp, cp;
+        vgi_sincosf(phi, &sp, &cp);
+        float t1 = r * cp;
+        float t2 = r * sp;
+        float s = 0.5f * (1.0f + v_h.z);
+        t2 = lerp(sqrtf(1.0f - t1 * t1), t2, s);
+        f3 n_h = (T1 * t1 + T2 * t2) + v_h * sqrtf(fmaxf(0.0f, (1.0f - t1 * t1) - t2 * t2));
+        H = normalize3(mk3(roughness * n_h.x, roughness * n_h.y, fmaxf(0.0f, n_h.z)));
+    } else {
+        H = mk3(0.0f, 0.0f, 1.0f);
+    }
+    f3 I = -V;
+    f3 L = I - H * (2.0f * dot3(H, I));  // reflect(-V, H)
+    return rotate_point(invert_rotation(rot), L);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io,
+                                                        vgi_pc_sample_gen pc) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    const size_t plane = (size_t)w * h;
+    float depth = io.gbuffer_depth[i];
+    if (depth == 0.0f) {
+        S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
+        return;
+    }
+    GBufferNormal g = decode_gbuffer_normal(io.gbuffer_color[plane + i]);
+    f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
+    uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
+    f3 view_dir = normalize3(xyz(fp.view_pos) - position);
+    float roughness = fmaxf(g.perceptual_roughness * g.perceptual_roughness, 0.002f);
+    f3 dir = sample_spec_dir(view_dir, g.normal, roughness, rng);
+    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
+    float* hp = io.rw_hit_pos_texture + 4 * i;
+    hp[0] = r.position_ws.x;
+    hp[1] = r.position_ws.y;
+    hp[2] = r.position_ws.z;
+    hp[3] = 1.0f;
+    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, TraceRes rt,
+                                                             vgi_ind_spec_sample_validate_io io) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t w = io.width, h = io.height;
+    if (px >= w || py >= h) return;
+    const size_t i = (size_t)py * w + px;
+    float prev_depth = rt.prev_depth[i];
+    if (prev_depth == 0.0f) return;
+    f3 position = position_biased(fp.prev_inv_view_proj, px, py, w, h, prev_depth);
+    uint32_t M = io.rw_prev_reservoir_texture[2 * i];
+    uint32_t Wbits = io.rw_prev_reservoir_texture[2 * i + 1];
+    const float* hp = io.rw_prev_hit_pos_texture + 4 * i;
+    f3 prev_hit_pos = mk3(hp[0], hp[1], hp[2]);
+    float prev_target = hp[3];
+    f3 prev_rad = S::ld3(io.rw_prev_hit_radiance_texture, i);
+    f3 dir = normalize3(prev_hit_pos - position);
+    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, true, stack);
+    float max_rad = fmaxf(component_max(r.radiance), component_max(prev_rad));
+    f3 dr = abs3(r.radiance - prev_rad);
+    float lim = 0.5f * max_rad;
+    bool changed = dr.x > lim || dr.y > lim || dr.z > lim;
+    if (changed) {
+        (void)M;
+        io.rw_prev_reservoir_texture[2 * i] = 1u;
+        io.rw_prev_reservoir_texture[2 * i + 1] = Wbits;
+        float* o = io.rw_prev_hit_pos_texture + 4 * i;
+        o[0] = r.position_ws.x;
+        o[1] = r.position_ws.y;
+        o[2] = r.position_ws.z;
+        o[3] = prev_target;
+        S::st3(io.rw_prev_hit_radiance_texture, i, r.radiance);
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io,
+                                                         uint32_t frame_hash) {
+    VGI_TRAVERSAL_STACK(stack);
+    uint32_t id = blockIdx.x * 64u + threadIdx.x;
+    if (id >= *io.compact_query_counter_buffer) return;
+    uint32_t qi = io.compact_query_index_buffer[id];
+    uint32_t cell = io.compact_query_cell_addr_buffer[id];
+    uint32_t rng = jenkins_hash(id ^ frame_hash);
+    const HashQuery q = reinterpret_cast<const HashQuery*>(io.hash_grid_query_buffer)[qi];
+    f3 normal = normal_decode_oct_u32(q.normal);
+    float u0 = lcg_rand(rng);
+    float u1 = lcg_rand(rng);
+    f3 local = sample_hemisphere_cosine(u0, u1);
+    f3 dir = rotate_point(invert_rotation(get_rotation_to_z_from(normal)), local);
+    RadianceTraceResult r = trace_radiance<S>(rt, fp, mk3(q.px, q.py, q.pz), dir, true, stack);
+    HashCell* storage = reinterpret_cast<HashCell*>(io.rw_hash_grid_storage_buffer);
+    f3 prev = mk3(storage[cell].r, storage[cell].g, storage[cell].b);
+    const float bf = 1.0f / 16.0f;
+    f3 blend = r.radiance * bf + prev * (1.0f - bf);
+    storage[cell].r = blend.x;
+    storage[cell].g = blend.y;
+    storage[cell].b = blend.z;
+    io.rw_hash_grid_decay_buffer[cell] = 10;
+}
+
+// ---------------------------------------------------------------------------
+static vgi_status make_trace_res(const vgi_trace_resources* r, bool need_prev, TraceRes& t) {
+    VGI_REQUIRE(r && r->scene_tlas, "trace resources: no scene");
+    VGI_REQUIRE(r->hash_grid_storage_buffer && r->rw_hash_grid_query_buffer && r->rw_hash_grid_query_counter_buffer,
+                "trace resources: hash-grid buffers missing");
+    VGI_REQUIRE(r->storage == VGI_STORAGE_F32 || r->storage == VGI_STORAGE_F16, "trace resources: bad storage");
+    if (need_prev)
+        VGI_REQUIRE(r->prev_indirect_diffuse_texture && r->prev_depth_texture,
+                    "trace resources: has_prev_frame=1 but prev_* is NULL");
+    t.scene = scene_view(r->scene_tlas);
+    t.sky = mk3(r->sky_radiance[0], r->sky_radiance[1], r->sky_radiance[2]);
+    t.prev_indirect_diffuse = r->prev_indirect_diffuse_texture;
+    t.prev_depth = r->prev_depth_texture;
+    t.pw = r->prev_width;
+    t.ph = r->prev_height;
+    if (need_prev) VGI_REQUIRE(valid_dims(t.pw, t.ph), "trace resources: bad prev_width/prev_height");
+    t.hg_storage = reinterpret_cast<const HashCell*>(r->hash_grid_storage_buffer);
+    t.queries = reinterpret_cast<HashQuery*>(r->rw_hash_grid_query_buffer);
+    t.query_counter = r->rw_hash_grid_query_counter_buffer;
+    return VGI_OK;
+}
+
+static FrameParams to_fp(const vgi_frame_params* fp) {
+    FrameParams f;
+    memcpy(&f, fp, sizeof(f));
+    return f;
+}
+
+#define VGI_DISPATCH_STORAGE(mode, kernel, grid, ...)                                          \
+    do {                                                                                       \
+        if ((mode) == VGI_STORAGE_F32)                                                         \
+            hipLaunchKernelGGL(kernel<StorageF32>, grid, dim3(64), 0, st, __VA_ARGS__);        \
+        else                                                                                   \
+            hipLaunchKernelGGL(kernel<StorageF16>, grid, dim3(64), 0, st, __VA_ARGS__);        \
+    } while (0)
+
+}  // namespace vgi
+
+using namespace vgi;
+
+extern "C" vgi_status vgi_pass_ind_diff_sample_gen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                   const vgi_ind_diff_sample_gen_io* io, const vgi_pc_sample_gen* pc,
+                                                   void* stream) {
+    VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_sample_gen: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_ind_diff_sample_gen: bad size");
+    VGI_REQUIRE(io->storage == rt->storage, "vgi_pass_ind_diff_sample_gen: storage mismatch");
+    VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->rw_hit_pos_normal_texture && io->rw_hit_radiance_texture,
+                "vgi_pass_ind_diff_sample_gen: missing binding");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, pc->has_prev_frame != 0, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("ind_diff_sample_gen", st, (double)io->width * io->height * (8 + 4 + 8 + radiance_bytes(io->storage)));
+    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    return check_launch("k_diff_sample_gen");
+}
+
+extern "C" vgi_status vgi_pass_ind_diff_sample_validate(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                        const vgi_ind_diff_sample_validate_io* io, void* stream) {
+    VGI_REQUIRE(fp && io, "vgi_pass_ind_diff_sample_validate: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_ind_diff_sample_validate: bad size");
+    VGI_REQUIRE(io->storage == rt->storage, "vgi_pass_ind_diff_sample_validate: storage mismatch");
+    VGI_REQUIRE(io->depth_texture && io->rw_reservoir_texture && io->rw_hit_pos_normal_texture &&
+                    io->rw_hit_radiance_texture,
+                "vgi_pass_ind_diff_sample_validate: missing binding");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, true, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("ind_diff_sample_validate", st,
+                   (double)io->width * io->height * (4 + 2 * (4 + 8 + radiance_bytes(io->storage))));
+    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    return check_launch("k_diff_sample_validate");
+}
+
+extern "C" vgi_status vgi_pass_ind_spec_sample_gen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                   const vgi_ind_spec_sample_gen_io* io, const vgi_pc_sample_gen* pc,
+                                                   void* stream) {
+    VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_spec_sample_gen: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_ind_spec_sample_gen: bad size");
+    VGI_REQUIRE(io->storage == rt->storage, "vgi_pass_ind_spec_sample_gen: storage mismatch");
+    VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->rw_hit_pos_texture && io->rw_hit_radiance_texture,
+                "vgi_pass_ind_spec_sample_gen: missing binding");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, pc->has_prev_frame != 0, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("ind_spec_sample_gen", st, (double)io->width * io->height * (4 + 4 + 16 + radiance_bytes(io->storage)));
+    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    return check_launch("k_spec_sample_gen");
+}
+
+extern "C" vgi_status vgi_pass_ind_spec_sample_validate(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                        const vgi_ind_spec_sample_validate_io* io, void* stream) {
+    VGI_REQUIRE(fp && io, "vgi_pass_ind_spec_sample_validate: null argument");
+    VGI_REQUIRE(valid_dims(io->width, io->height), "vgi_pass_ind_spec_sample_validate: bad size");
+    VGI_REQUIRE(io->storage == rt->storage, "vgi_pass_ind_spec_sample_validate: storage mismatch");
+    VGI_REQUIRE(io->rw_prev_reservoir_texture && io->rw_prev_hit_pos_texture && io->rw_prev_hit_radiance_texture,
+                "vgi_pass_ind_spec_sample_validate: missing binding");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, true, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("ind_spec_sample_validate", st,
+                   (double)io->width * io->height * (4 + 2 * (8 + 16 + radiance_bytes(io->storage))));
+    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    return check_launch("k_spec_sample_validate");
+}
+
+extern "C" vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                const vgi_hash_grid_raygen_io* io, const vgi_pc_frame_hash* pc,
+                                                void* stream) {
+    VGI_REQUIRE(fp && io && pc && rt, "vgi_pass_hash_grid_raygen: null argument");
+    VGI_REQUIRE(io->rw_hash_grid_storage_buffer && io->hash_grid_query_buffer && io->compact_query_counter_buffer &&
+                    io->compact_query_index_buffer && io->compact_query_cell_addr_buffer && io->rw_hash_grid_decay_buffer,
+                "vgi_pass_hash_grid_raygen: missing binding");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, true, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    // ray per compacted query: query 24 B + index 4 + addr 4 + cell rmw 32 + decay 1
+    ProfScope prof("hash_grid_raygen", st, 0.0);
+    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kHashGridMaxQueries / 64), to_fp(fp), t, *io,
+                         pc->frame_hash);
+    return check_launch("k_hash_grid_raygen");
+}
