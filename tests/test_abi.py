@@ -38,7 +38,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_covers_header():
     from violet_amd import _lib
-    import violet_amd.gpu  # noqa: F401  (registers its entry points)
+    import violet_amd.gpu  # noqa: F401  (these modules register their entry points)
+    import violet_amd.renderer  # noqa: F401
+    import violet_amd.scene  # noqa: F401
 
     names = declared_functions()
     missing = [n for n in names if n not in _lib.exported_symbols()]
