@@ -113,27 +113,123 @@ def cpu_baseline_denoise(args, budget_s: float = 20.0):
                        f"FP32 twin of the oracle, {el:.1f} s")
 
 
+def orbit_view(frame: int, w: int, h: int, radius=30.0, height=9.0, deg_per_frame=6.0):
+    """Camera orbit around the scene centre (60 frames = one turn)."""
+    import math
+
+    from violet_amd import synth
+
+    a = math.radians(-90.0 + deg_per_frame * frame)
+    pos = np.array([radius * math.cos(a), radius * math.sin(a), height])
+    d = -pos / np.linalg.norm(pos)
+    yaw = math.degrees(math.atan2(d[0], d[1]))
+    pitch = math.degrees(math.asin(d[2])) + 8.0
+    return synth.View(synth.Camera(pos, yaw, pitch), w, h)
+
+
+def frame_params_seq(w, h, n):
+    from violet_amd import synth
+
+    views = [orbit_view(f, w, h) for f in range(n + 1)]
+    return [synth.make_frame_params(views[f], views[f - 1] if f else views[-1]) for f in range(n)]
+
+
+def run_frame(args):
+    import torch
+
+    from violet_amd import _lib, renderer, scene
+
+    w, h = args.width, args.height
+    log(f"[bench] building the {args.triangles}-triangle scene and its BVH")
+    mesh = scene.make_scene(args.triangles, seed=11)
+    sc = scene.Scene(mesh)
+    st = sc.stats()
+    log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
+    cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16 if args.storage == "f16" else _lib.STORAGE_F32,
+                                atrous_iterations=args.iterations)
+    r = renderer.RestirRenderer(w, h, cfg)
+    fps = frame_params_seq(w, h, args.frames)
+
+    def step(k: int):
+        r.add_passes(sc, fps[k % len(fps)], k)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    lat = []
+    base = args.warmup
+    for k in range(min(args.steps, 120)):
+        t0 = time.perf_counter()
+        step(base + k)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    base += min(args.steps, 120)
+    _lib.call("vgi_profiler_reset")
+    _lib.call("vgi_profiler_enable", 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(base + k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("vgi_profiler_enable", 0)
+    prof = _lib.profiler_read()
+    return dict(elapsed=elapsed, lat=lat, prof=prof, pixels=w * h, mesh=mesh)
+
+
+def cpu_baseline_frame(args, mesh, budget_s: float = 20.0):
+    """FP32 twin of the oracle (OpenMP) rendering frames of the same workload
+    at a reduced resolution, bounded to about `budget_s` seconds."""
+    import oracle as O
+
+    threads = O.load(32).oracle_num_threads()
+    w, h = args.cpu_width, args.cpu_height
+    fps = frame_params_seq(w, h, args.frames)
+    R = O.OracleRenderer(32, mesh, w, h, atrous_iterations=args.iterations)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        R.add_passes(fps[n % len(fps)], n)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= len(fps):
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n * w * h / el / 1e6, unit="Mpix/s", cores=threads, kind="port",
+                sample=f"{n} full frames (GI trace, ReSTIR, hash grid, denoise) of the {len(mesh.indices)}-triangle "
+                       f"orbit workload at {w}x{h}, FP32 twin of the oracle, {el:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--workload", default="denoise", choices=["denoise"])
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--frames", type=int, default=8)
-    ap.add_argument("--iterations", type=int, default=5)
+    ap.add_argument("--workload", default="frame", choices=["frame", "denoise"])
+    ap.add_argument("--width", type=int, default=None)
+    ap.add_argument("--height", type=int, default=None)
+    ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--iterations", type=int, default=None)
+    ap.add_argument("--triangles", type=int, default=200_000)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16"])
-    ap.add_argument("--cpu-width", type=int, default=640)
-    ap.add_argument("--cpu-height", type=int, default=360)
+    ap.add_argument("--cpu-width", type=int, default=None)
+    ap.add_argument("--cpu-height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    frame = args.workload == "frame"
+    # full frame: 1280x720, 60-frame orbit, 3 a-trous iterations (the reference
+    # default); denoise only: 1920x1080, 8-frame pan, 5 iterations
+    args.width = args.width or (1280 if frame else 1920)
+    args.height = args.height or (720 if frame else 1080)
+    args.frames = args.frames or (60 if frame else 8)
+    args.iterations = args.iterations if args.iterations is not None else (3 if frame else 5)
+    args.cpu_width = args.cpu_width or (160 if frame else 640)
+    args.cpu_height = args.cpu_height or (90 if frame else 360)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
               "path is described in DESIGN.md and is not built", file=sys.stderr)
         sys.exit(2)
 
-    r = run_denoise(args)
+    r = run_frame(args) if frame else run_denoise(args)
     mpix = args.steps * r["pixels"] / r["elapsed"] / 1e6
     prof = r["prof"]
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
@@ -143,8 +239,13 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     lat = np.array(r["lat"])
     passes = {k: dict(avg_us=round(v["ms"] / max(v["count"], 1) * 1e3, 2), launches=v["count"],
-                      gbs=round(v["bytes"] / max(v["ms"] * 1e-3, 1e-12) / 1e9, 1)) for k, v in prof.items()}
-    cpu = None if args.no_cpu_baseline else cpu_baseline_denoise(args)
+                      gbs=round(v["bytes"] / max(v["ms"] * 1e-3, 1e-12) / 1e9, 1),
+                      share=round(v["ms"] / max(sum(x["ms"] for x in prof.values()), 1e-12), 4))
+              for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    if args.no_cpu_baseline:
+        cpu = None
+    else:
+        cpu = cpu_baseline_frame(args, r["mesh"]) if frame else cpu_baseline_denoise(args)
     out = {
         "metric": "ReSTIR GI+denoise Mpix/s @1080p; p50/p99 frame latency, batched vs plain",
         "value": round(mpix, 2),
@@ -159,10 +260,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.storage == "f32" else "f16-storage/f32-math",
-        "data": "synthetic (seeded plane + 20 spheres G-buffer, clean field x Exp(1) noise, 2 px/frame pan)",
-        "config": {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
-                   "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
-                   "storage": args.storage, "parallelism": "single-gpu"},
+        "data": ("synthetic (seeded boxes + icospheres on a plane, sun + constant sky, 60-frame camera orbit)"
+                 if frame else
+                 "synthetic (seeded plane + 20 spheres G-buffer, clean field x Exp(1) noise, 2 px/frame pan)"),
+        "config": ({"workload": f"restir_gi_frame_{args.width}x{args.height}", "width": args.width,
+                    "height": args.height, "triangles": int(len(r["mesh"].indices)), "frames_cycled": args.frames,
+                    "gi_rays_per_px": "1 diffuse + 1 specular (+1 shadow each)", "atrous_iterations": args.iterations,
+                    "restir_params": "reference defaults (M cap 16, 9 spatial taps, radius 0.05*min(w,h))",
+                    "storage": args.storage, "parallelism": "single-gpu"} if frame else
+                   {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
+                    "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
+                    "storage": args.storage, "parallelism": "single-gpu"}),
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": bytes_per_launch},

@@ -444,3 +444,79 @@ def frame_rand_seeds(frame_index: int, n: int) -> list[int]:
         s = int(load(64).oracle_jenkins(s))
         out.append(s)
     return out
+
+
+class OracleRenderer:
+    """Mirror of restir_renderer.rs `RestirRenderer::add_passes` (:258-1028)
+    over the oracle passes (same pass order, FrameRand seed chain, history
+    flags and feedback as the GPU renderer)."""
+
+    NC = 65536 * 16
+
+    def __init__(self, bits: int, mesh, width: int, height: int, sky=(1.0, 1.0, 1.0), denoise=True,
+                 ind_diff_validate=True, ind_spec_validate=True, hash_grid_cache_decay=True, ao_radius=2.0,
+                 atrous_iterations=3):
+        self.bits = bits
+        self.scene = OracleScene(bits, mesh)
+        self.R = OracleRestir(bits, self.scene, sky)
+        self.den = OracleDenoiser(bits, atrous_iterations=atrous_iterations)
+        self.w, self.h = width, height
+        self.denoise, self.ind_diff_validate, self.ind_spec_validate = denoise, ind_diff_validate, ind_spec_validate
+        self.decay_on, self.ao_radius = hash_grid_cache_decay, ao_radius
+        self.cells = None
+        self.prev = None  # dict of last frame's state
+
+    def add_passes(self, fp, frame_index: int):
+        R, w, h = self.R, self.w, self.h
+        seeds = iter(frame_rand_seeds(frame_index, 8))
+        depth, gb = R.gbuffer(fp, w, h)
+        if self.cells is None:
+            self.cells = np.zeros((self.NC, 4), np.uint32)
+            self.decay = np.zeros(self.NC, np.uint8)
+            self.q = np.zeros((131072, 6), np.uint32)
+            self.cnt = np.zeros(1, np.uint32)
+        if self.decay_on:
+            self.decay, self.cells = R.hg_decay(self.decay, self.cells)
+        p = self.prev
+        has_prev = p is not None
+        hg = False
+        if has_prev:
+            self.cells, sel = R.hg_select(self.q, int(self.cnt[0]), self.cells, next(seeds))
+            cidx, caddr = R.hg_compact(sel)
+            nq = np.zeros((131072, 6), np.uint32)
+            ncnt = np.zeros(1, np.uint32)
+            tr = R.trace_res(self.cells, nq, ncnt, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"])
+            self.cells, self.decay = R.hg_raygen(fp, tr, self.cells, self.q, cidx, caddr, self.decay, next(seeds))
+            self.q, self.cnt = nq, ncnt
+            hg = True
+        validation = has_prev and (frame_index & 3) == 0 and self.ind_diff_validate
+        tr = R.trace_res(self.cells, self.q, self.cnt, prev_ind_diff=p["out_diffuse"] if has_prev else None,
+                         prev_depth=p["depth"] if has_prev else None)
+        if not validation:
+            hit, rad = R.diff_sample_gen(fp, tr, gb, depth, next(seeds), has_prev)
+            prev_rsv = (p["d_rsv"], p["d_hit"], p["d_rad"]) if has_prev else (None, None, None)
+        else:
+            hit, rad = None, None
+            prev_rsv = R.diff_sample_validate(fp, tr, p["depth"], p["d_rsv"], p["d_hit"], p["d_rad"])
+        ao = R.ao_gen(fp, hit, depth, p["ao"] if has_prev else None, not validation, has_prev, self.ao_radius)
+        aof = R.ao_filter(depth, ao)
+        t = R.diff_temporal(fp, p["depth"] if has_prev else None, depth, hit, rad, *prev_rsv, next(seeds), has_prev,
+                            not validation)
+        d_rsv, d_hit, d_rad, d_light = R.diff_spatial(fp, gb, depth, aof, *t, next(seeds))
+        hp, srad = R.spec_sample_gen(fp, tr, gb, depth, next(seeds), has_prev)
+        ps = (p["s_rsv"], p["s_hp"], p["s_rad"]) if has_prev else (None, None, None)
+        if has_prev and self.ind_spec_validate:
+            ps = R.spec_sample_validate(fp, tr, *ps)
+        s_rsv, s_hp, s_rad = R.spec_temporal(fp, p["gb"] if has_prev else None, p["depth"] if has_prev else None, gb,
+                                             depth, *ps, hp, srad, next(seeds), has_prev)
+        s_light, ray_len = R.spec_spatial(fp, gb, depth, s_rsv, s_hp, s_rad, next(seeds))
+        if self.denoise and has_prev:
+            o = self.den.add_passes(fp, depth, gb, p["depth"], p["gb"], d_light, s_light, ray_len)
+            out_d, out_s = o["diffuse"], o["specular"]
+        else:
+            self.den.reset()
+            out_d, out_s = d_light, s_light
+        self.prev = dict(depth=depth, gb=gb, d_rsv=d_rsv, d_hit=d_hit, d_rad=d_rad, s_rsv=s_rsv, s_hp=s_hp,
+                         s_rad=s_rad, ao=ao, out_diffuse=out_d)
+        return dict(diffuse=out_d, specular=out_s, depth=depth, gbuffer=gb, hash_grid_updated=hg,
+                    validation=validation, diff_light=d_light, spec_light=s_light)
