@@ -287,6 +287,7 @@ void oracle_diff_sample_gen(const float* fpf, const otrace_res* rt, int w, int h
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -314,6 +315,7 @@ void oracle_diff_sample_validate(const float* fpf, const otrace_res* rt, int w, 
                                  uint32_t* rw_rsv, uint32_t* rw_hit, real* rw_rad) {
     ofp fp;
     o_load_fp(fpf, &fp);
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -373,6 +375,7 @@ void oracle_spec_sample_gen(const float* fpf, const otrace_res* rt, int w, int h
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -399,6 +402,7 @@ void oracle_spec_sample_validate(const float* fpf, const otrace_res* rt, int w, 
                                  real* rw_rad) {
     ofp fp;
     o_load_fp(fpf, &fp);
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -450,6 +454,7 @@ void oracle_ao_gen(const float* fpf, int w, int h, const uint32_t* hit, const re
                    int has_new, int has_prev, real radius, real* o_ao) {
     ofp fp;
     o_load_fp(fpf, &fp);
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -484,6 +489,7 @@ void oracle_ao_gen(const float* fpf, int w, int h, const uint32_t* hit, const re
 static inline real med3c(real a, real b, real c) { return R_FMIN(R_FMAX(a, b), c); }
 
 void oracle_ao_filter(int w, int h, const real* depth, const real* ao, real* o_ao) {
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -522,6 +528,7 @@ void oracle_diff_temporal(const float* fpf, int w, int h, const real* prev_depth
                           uint32_t* o_hit, real* o_rad) {
     ofp fp;
     o_load_fp(fpf, &fp);
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -616,6 +623,7 @@ void oracle_diff_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
     }
     const real T = (real)(float)(0.6186 * 9.0);
     const real cos_tol = (real)(float)cos((double)(float)(PI_ * (25.0 / 180.0)));
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -716,6 +724,7 @@ void oracle_spec_temporal(const float* fpf, int w, int h, const uint32_t* prev_g
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
@@ -804,6 +813,7 @@ void oracle_spec_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
     const double PI_ = (double)(float)3.14159265359;
     const real cos_tol = (real)(float)cos((double)(float)(PI_ * (25.0 / 180.0)));
     v3 vp = o_xyz(fp.view_pos);
+#pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
