@@ -294,6 +294,7 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     sc->view.num_nodes = (uint32_t)b.nodes.size();
     sc->view.num_tris = nt;
     sc->view.root = root;
+    sc->view.stack_depth = sc->depth;
     *out = sc;
     return VGI_OK;
 }

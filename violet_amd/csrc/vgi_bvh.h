@@ -63,8 +63,12 @@ struct SceneView {
     const Material* materials;
     uint32_t num_nodes;
     uint32_t num_tris;
-    int32_t root;  // link of the root (node 0, or a leaf for tiny scenes)
+    int32_t root;           // link of the root (node 0, or a leaf for tiny scenes)
+    uint32_t stack_depth;   // levels of the tree = most traversal-stack entries a ray can hold
 };
+
+// Dynamic LDS bytes of a 64-lane traversal stack for this scene.
+inline size_t traversal_lds_bytes(const SceneView& s) { return (size_t)s.stack_depth * 64u * sizeof(int32_t); }
 
 constexpr float kRayTMin = 0.0005f;  // raytrace.inc.hlsl:39,70
 constexpr float kRayTMax = 1000.0f;  // raytrace.inc.hlsl:40,71

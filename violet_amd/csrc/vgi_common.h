@@ -163,6 +163,12 @@ struct XXHash32 {
 // binary16 conversions (round to nearest even), bit-identical on both sides
 // ---------------------------------------------------------------------------
 VGI_HD uint32_t f32tof16(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_cvt_f16_f32 rounds to nearest even like the host code below; NaNs are
+    // canonicalised to the host code's quiet NaN.
+    if (f != f) return ((asuint(f) >> 16) & 0x8000u) | 0x7E00u;
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
+#endif
     uint32_t x = asuint(f);
     uint32_t sign = (x >> 16) & 0x8000u;
     uint32_t absx = x & 0x7FFFFFFFu;
@@ -189,6 +195,9 @@ VGI_HD uint32_t f32tof16(float f) {
     return sign | q;
 }
 VGI_HD float f16tof32(uint32_t h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);  // exact
+#endif
     h &= 0xFFFFu;
     uint32_t sign = (h & 0x8000u) << 16;
     uint32_t e = (h >> 10) & 0x1Fu;

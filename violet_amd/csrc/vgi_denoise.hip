@@ -17,6 +17,8 @@
  * outputs of the temporal, disocclusion and fallback passes unwritten
  * (undefined pooled-texture contents); these kernels write zeros there.
  */
+#include <algorithm>
+
 #include "vgi_runtime.h"
 #include "vgi_storage.h"
 
@@ -32,44 +34,25 @@ __device__ __forceinline__ float pow_pow2(float x, int k) {
     return x;
 }
 
-// svgf.inc.hlsl:6-84 EdgeStoppingFunc
-struct EdgeStop {
-    float depth_center;
-    float fwidth_z;
-    f3 normal_center;
-    f2 lumi_center;
-    f2 lumi_std;
-    bool use_lumi;
-    float sigma_z;
-    int sigma_n_log2;  // sigma_n = 2^sigma_n_log2
+constexpr float kLog2e = 1.4426950408889634f;
 
-    __device__ __forceinline__ static EdgeStop create(float depth, float fwidth_z, f3 normal) {
-        EdgeStop e;
-        e.depth_center = depth;
-        e.fwidth_z = fwidth_z;
-        e.normal_center = normal;
-        e.lumi_center = mk2(0.0f, 0.0f);
-        e.lumi_std = mk2(0.0f, 0.0f);
-        e.use_lumi = false;
-        e.sigma_z = 1.0f;
-        e.sigma_n_log2 = 7;  // 128
-        return e;
-    }
-    __device__ __forceinline__ f2 weight(float tap_dist, float depth, f3 normal, f2 lumi) const {
-        const float sigma_l = 4.0f;
-        const float epsilon = 1e-8f;
-        float dist_z = fabsf(depth - depth_center);
-        float weight_z_log = -dist_z / ((sigma_z * fwidth_z) * tap_dist + epsilon);
-        float n_dot = saturate(dot3(normal_center, normal));
-        float weight_n = pow_pow2(n_dot, sigma_n_log2);
-        float wlx = 0.0f, wly = 0.0f;
-        if (use_lumi) {
-            wlx = -fabsf(lumi.x - lumi_center.x) * rcp(sigma_l * lumi_std.x + epsilon);
-            wly = -fabsf(lumi.y - lumi_center.y) * rcp(sigma_l * lumi_std.y + epsilon);
-        }
-        return mk2(weight_n * expf(wlx + weight_z_log), weight_n * expf(wly + weight_z_log));
-    }
-};
+// G-buffer oct 12:12 normal with reciprocal multiplies and a hardware rsqrt.
+__device__ __forceinline__ f3 decode_normal_fast(uint32_t enc) {
+    const float k = 2.0f / 4095.0f;
+    float fx = (float)(enc & 0xFFFu) * k - 1.0f;
+    float fy = (float)((enc >> 12) & 0xFFFu) * k - 1.0f;
+    float fz = (1.0f - fabsf(fx)) - fabsf(fy);
+    float t = saturate(-fz);
+    fx += (fx >= 0.0f) ? -t : t;
+    fy += (fy >= 0.0f) ? -t : t;
+    float inv = __builtin_amdgcn_rsqf(fx * fx + fy * fy + fz * fz);
+    return mk3(fx * inv, fy * inv, fz * inv);
+}
+
+// EdgeStoppingFunc (svgf.inc.hlsl:6-84) is evaluated inline by the spatial
+// passes: w = pow(sat(n.n_c), sigma_n) * exp(-|z - z_c| / (sigma_z fwidth_z
+// dist + 1e-8) - |l - l_c| / (sigma_l std + 1e-8)), with exp(a) as
+// exp2(a log2 e) and the divisions as reciprocal multiplies.
 
 // Bilinear fetch with clamp-to-edge addressing (sampler_linear_clamp), in
 // exact f32: texel t = uv*size - 0.5, weights from t - floor(t).
@@ -322,38 +305,41 @@ __global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) 
     }
     len = qmax;
     if (len < 4.0f * kHistoryLenUnit) {
-        float depth_center = io.depth_texture[i];
-        f3 nc = load_normal(io.gbuffer_texture, plane, i);
-        f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
-        EdgeStop es = EdgeStop::create(depth_center, fw.x, nc);
-        es.sigma_z = 5.0f;
-        es.sigma_n_log2 = 5;  // 32
+        // EdgeStoppingFunc with sigma_z = 5, sigma_n = 32 and no luminance
+        // term: w = pow(sat(n.n_c), 32) * exp(-|z - z_c| / (5 fwidth_z dist + 1e-8))
+        const float depth_center = io.depth_texture[i];
+        const f3 nc = decode_normal_fast(io.gbuffer_texture[plane + i]);
+        const float fwz = 5.0f * decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x;
         f2 wsum = mk2(0.0f, 0.0f);
         f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
         f2 msum = mk2(0.0f, 0.0f);
         const int stride = (int)((14.0f * kHistoryLenUnit) / len);
+        const float fs = (float)stride;
+#pragma unroll
         for (int y = -2; y <= 2; y++) {
+#pragma unroll
             for (int x = -2; x <= 2; x++) {
-                int ox = x * stride, oy = y * stride;
-                int cx = (int)px + ox, cy = (int)py + oy;
+                int cx = (int)px + x * stride, cy = (int)py + y * stride;
                 if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
                 size_t j = (size_t)cy * w + cx;
-                float d = io.depth_texture[j];
-                f3 n = load_normal(io.gbuffer_texture, plane, j);
-                float fox = (float)ox, foy = (float)oy;
-                f2 wt = es.weight(sqrtf(fox * fox + foy * foy), d, n, mk2(0.0f, 0.0f));
+                const float dist = fs * sqrtf((float)(x * x + y * y));  // folded per tap
+                float dz = fabsf(io.depth_texture[j] - depth_center);
+                f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+                float wn = pow_pow2(saturate(dot3(nc, n)), 5);
+                float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
                 f3 sv = S::ld3(io.spec_texture, j);
                 f2 mv = S::ld2(io.moments_texture, j);
-                wsum = mk2(wsum.x + wt.x, wsum.y + wt.y);
-                dsum = dsum + dv * wt.x;
-                ssum = ssum + sv * wt.y;
-                msum = mk2(msum.x + mv.x * wt.x, msum.y + mv.y * wt.y);
+                wsum = mk2(wsum.x + wt, wsum.y + wt);
+                dsum = dsum + dv * wt;
+                ssum = ssum + sv * wt;
+                msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
             }
         }
-        S::st3(io.rw_diff_texture, i, dsum / wsum.x);
-        S::st3(io.rw_spec_texture, i, ssum / wsum.y);
-        S::st2(io.rw_moments_or_variance_texture, i, mk2(msum.x / wsum.x, msum.y / wsum.y));
+        const float iw = __builtin_amdgcn_rcpf(wsum.x);
+        S::st3(io.rw_diff_texture, i, dsum * iw);
+        S::st3(io.rw_spec_texture, i, ssum * iw);
+        S::st2(io.rw_moments_or_variance_texture, i, mk2(msum.x * iw, msum.y * iw));
     } else {
         S::st3(io.rw_diff_texture, i, S::ld3(io.diff_texture, i));
         S::st3(io.rw_spec_texture, i, S::ld3(io.spec_texture, i));
@@ -385,35 +371,39 @@ __global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
     f3 diff, spec;
     f2 sm;
     if (len < 4.0f * kHistoryLenUnit) {
-        float depth_center = io.depth_texture[i];
-        f3 nc = load_normal(io.gbuffer_texture, plane, i);
-        f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
-        EdgeStop es = EdgeStop::create(depth_center, fw.x * 3.0f, nc);
-        float wsx = 0.0f, wsy = 0.0f;
+        // EdgeStoppingFunc with fwidth_z * 3, sigma_n = 128 and no luminance
+        // term, over a 7x7 window
+        const float depth_center = io.depth_texture[i];
+        const f3 nc = decode_normal_fast(io.gbuffer_texture[plane + i]);
+        const float fwz = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x * 3.0f;
+        float wsum = 0.0f;
         f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
         f2 msum = mk2(0.0f, 0.0f);
+#pragma unroll
         for (int y = -3; y <= 3; y++) {
+#pragma unroll
             for (int x = -3; x <= 3; x++) {
                 int cx = (int)px + x, cy = (int)py + y;
                 if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
                 size_t j = (size_t)cy * w + cx;
-                float d = io.depth_texture[j];
-                f3 n = load_normal(io.gbuffer_texture, plane, j);
-                float fx = (float)x, fy = (float)y;
-                float wt = es.weight(sqrtf(fx * fx + fy * fy), d, n, mk2(0.0f, 0.0f)).x;
+                const float dist = sqrtf((float)(x * x + y * y));  // folded per tap
+                float dz = fabsf(io.depth_texture[j] - depth_center);
+                f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+                float wn = pow_pow2(saturate(dot3(nc, n)), 7);
+                float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
                 f3 sv = S::ld3(io.spec_texture, j);
                 f2 mv = S::ld2(io.moments_texture, j);
-                wsx += wt;
-                wsy += wt;
+                wsum += wt;
                 dsum = dsum + dv * wt;
                 ssum = ssum + sv * wt;
                 msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
             }
         }
-        diff = dsum / wsx;
-        spec = ssum / wsy;
-        sm = mk2(msum.x / wsx, msum.y / wsy);
+        const float iw = __builtin_amdgcn_rcpf(wsum);
+        diff = dsum * iw;
+        spec = ssum * iw;
+        sm = mk2(msum.x * iw, msum.y * iw);
     } else {
         diff = S::ld3(io.diff_texture, i);
         spec = S::ld3(io.spec_texture, i);
@@ -482,61 +472,106 @@ __global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io)
 // A-trous iteration (svgf_atrous_filter.hlsl).  Taps with x != 0 AND y != 0
 // plus the explicit centre (the reference's `not_center` test drops the
 // whole centre row and column).
+//
+// Lattice tiling: a 256-thread workgroup filters a 16x16 block of pixels
+// that share the residue (px mod step, py mod step).  The 5x5 taps at
+// spacing `step` are then the block's lattice neighbours, so the 20x20
+// lattice footprint is staged once in LDS (normal decoded, luminances
+// computed) and every tap is an LDS read.  Out-of-image footprint entries
+// hold a zero normal and zero colour: their edge weight is exactly 0, so
+// they add nothing, as a skipped tap adds nothing.
+//
+// Arithmetic is FP32 with hardware transcendentals (exp2, rsqrt, rcp): the
+// pass's contract is the FP64-oracle tolerance, not bit equality.
 // ---------------------------------------------------------------------------
+constexpr int kAtrousB = 16;                   // block edge in lattice points
+constexpr int kAtrousF = kAtrousB + 4;         // footprint edge (2 each side)
+constexpr int kAtrousN = kAtrousF * kAtrousF;  // footprint entries
+
+enum AtrousField { kFDepth, kFNx, kFNy, kFNz, kFLd, kFLs, kFDr, kFDg, kFDb, kFSr, kFSg, kFSb, kFVx, kFVy, kFCount };
+
 template <class S>
-__global__ void __launch_bounds__(64) k_atrous(vgi_denoise_atrous_io io, int step) {
-    uint32_t px, py;
-    tile_pixel(px, py);
-    const uint32_t w = io.width, h = io.height;
+__global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
+    __shared__ float lds[kFCount][kAtrousN];
+    const int w = (int)io.width, h = (int)io.height;
+    const size_t plane = (size_t)w * h;
+    const int rx = (int)(blockIdx.x / nbx), bx = (int)(blockIdx.x % nbx);
+    const int ry = (int)(blockIdx.y / nby), by = (int)(blockIdx.y % nby);
+    const int t = (int)threadIdx.x;
+
+    for (int e = t; e < kAtrousN; e += 256) {
+        int gx = bx * kAtrousB + e % kAtrousF - 2, gy = by * kAtrousB + e / kAtrousF - 2;
+        int cx = rx + step * gx, cy = ry + step * gy;
+        float v[kFCount];
+#pragma unroll
+        for (int k = 0; k < kFCount; ++k) v[k] = 0.0f;
+        if (gx >= 0 && gy >= 0 && cx < w && cy < h) {
+            size_t j = (size_t)cy * w + cx;
+            f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+            f3 dv = S::ld3(io.diff_texture, j);
+            f3 sv = S::ld3(io.spec_texture, j);
+            f2 vv = S::ld2(io.variance_texture, j);
+            v[kFDepth] = io.depth_texture[j];
+            v[kFNx] = n.x, v[kFNy] = n.y, v[kFNz] = n.z;
+            v[kFLd] = luminance(dv), v[kFLs] = luminance(sv);
+            v[kFDr] = dv.x, v[kFDg] = dv.y, v[kFDb] = dv.z;
+            v[kFSr] = sv.x, v[kFSg] = sv.y, v[kFSb] = sv.z;
+            v[kFVx] = vv.x, v[kFVy] = vv.y;
+        }
+#pragma unroll
+        for (int k = 0; k < kFCount; ++k) lds[k][e] = v[k];
+    }
+    __syncthreads();
+
+    const int li = t & (kAtrousB - 1), lj = t / kAtrousB;
+    const int px = rx + step * (bx * kAtrousB + li), py = ry + step * (by * kAtrousB + lj);
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    const size_t plane = (size_t)w * h;
+    const int c = (lj + 2) * kAtrousF + (li + 2);
 
-    float depth_center = io.depth_texture[i];
-    f3 nc = load_normal(io.gbuffer_texture, plane, i);
-    f2 fw = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
-    f3 diff_center = S::ld3(io.diff_texture, i);
-    f3 spec_center = S::ld3(io.spec_texture, i);
-    f2 lumi_center = mk2(luminance(diff_center), luminance(spec_center));
-    f2 var_center = S::ld2(io.variance_texture, i);
-    EdgeStop es = EdgeStop::create(depth_center, fw.x, nc);
-    es.use_lumi = true;
-    es.lumi_center = lumi_center;
-    es.lumi_std = mk2(sqrtf(var_center.x), sqrtf(var_center.y));
-
+    const float depth_center = lds[kFDepth][c];
+    const f3 nc = mk3(lds[kFNx][c], lds[kFNy][c], lds[kFNz][c]);
+    const f2 lc = mk2(lds[kFLd][c], lds[kFLs][c]);
+    f3 dsum = mk3(lds[kFDr][c], lds[kFDg][c], lds[kFDb][c]);
+    f3 ssum = mk3(lds[kFSr][c], lds[kFSg][c], lds[kFSb][c]);
+    f2 vsum = mk2(lds[kFVx][c], lds[kFVy][c]);
+    const float fwz = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x;
+    // exp(-|dl| / (sigma_l*std + eps) - dz / (fwidth_z*dist + eps)) as exp2 of
+    // pre-scaled terms (sigma_l = 4, sigma_z = 1, eps = 1e-8)
+    const float kl_d = kLog2e * __builtin_amdgcn_rcpf(4.0f * sqrtf(vsum.x) + 1e-8f);
+    const float kl_s = kLog2e * __builtin_amdgcn_rcpf(4.0f * sqrtf(vsum.y) + 1e-8f);
+    float kz[3];  // by tap class |x|+|y| - 2: distances sqrt(2), sqrt(5), sqrt(8) times step
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float a = (float)((q == 2 ? 2 : 1) * step), b = (float)((q == 0 ? 1 : 2) * step);
+        kz[q] = kLog2e * __builtin_amdgcn_rcpf(fwz * sqrtf(a * a + b * b) + 1e-8f);
+    }
     const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
-    f3 dsum = diff_center, ssum = spec_center;
-    f2 vsum = var_center;
     f2 wsum = mk2(1.0f, 1.0f);
 #pragma unroll
     for (int y = -2; y <= 2; y++) {
-        float kwy = kw[y < 0 ? -y : y];
 #pragma unroll
         for (int x = -2; x <= 2; x++) {
             if (x == 0 || y == 0) continue;
-            int ox = x * step, oy = y * step;
-            int cx = (int)px + ox, cy = (int)py + oy;
-            if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
-            size_t j = (size_t)cy * w + cx;
-            float kwxy = kwy * kw[x < 0 ? -x : x];
-            f3 dv = S::ld3(io.diff_texture, j);
-            f3 sv = S::ld3(io.spec_texture, j);
-            float d = io.depth_texture[j];
-            f3 n = load_normal(io.gbuffer_texture, plane, j);
-            f2 lumi = mk2(luminance(dv), luminance(sv));
-            float fox = (float)ox, foy = (float)oy;
-            f2 e = es.weight(sqrtf(fox * fox + foy * foy), d, n, lumi);
-            float wx = kwxy * e.x, wy = kwxy * e.y;
-            f2 vv = S::ld2(io.variance_texture, j);
-            dsum = dsum + dv * wx;
-            ssum = ssum + sv * wy;
-            vsum = mk2(vsum.x + (wx * wx) * vv.x, vsum.y + (wy * wy) * vv.y);
+            const int ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
+            const float kwxy = kw[ay] * kw[ax];
+            const int j = c + y * kAtrousF + x;
+            float dz = fabsf(lds[kFDepth][j] - depth_center);
+            float ndot = saturate(nc.x * lds[kFNx][j] + nc.y * lds[kFNy][j] + nc.z * lds[kFNz][j]);
+            float wn = pow_pow2(ndot, 7);  // sigma_n = 128
+            float ez = dz * kz[ax + ay - 2];
+            float wx = kwxy * (wn * __builtin_amdgcn_exp2f(-(fabsf(lds[kFLd][j] - lc.x) * kl_d) - ez));
+            float wy = kwxy * (wn * __builtin_amdgcn_exp2f(-(fabsf(lds[kFLs][j] - lc.y) * kl_s) - ez));
+            dsum = dsum + mk3(lds[kFDr][j], lds[kFDg][j], lds[kFDb][j]) * wx;
+            ssum = ssum + mk3(lds[kFSr][j], lds[kFSg][j], lds[kFSb][j]) * wy;
+            vsum = mk2(vsum.x + (wx * wx) * lds[kFVx][j], vsum.y + (wy * wy) * lds[kFVy][j]);
             wsum = mk2(wsum.x + wx, wsum.y + wy);
         }
     }
-    S::st3(io.rw_diff_texture, i, dsum / wsum.x);
-    S::st3(io.rw_spec_texture, i, ssum / wsum.y);
-    S::st2(io.rw_variance_texture, i, mk2(vsum.x / (wsum.x * wsum.x), vsum.y / (wsum.y * wsum.y)));
+    const float id = __builtin_amdgcn_rcpf(wsum.x), is = __builtin_amdgcn_rcpf(wsum.y);
+    S::st3(io.rw_diff_texture, i, dsum * id);
+    S::st3(io.rw_spec_texture, i, ssum * is);
+    S::st2(io.rw_variance_texture, i, mk2(vsum.x * (id * id), vsum.y * (is * is)));
 }
 
 // ---------------------------------------------------------------------------
@@ -594,11 +629,16 @@ vgi_status launch_denoise_fast_clamp(const vgi_denoise_fast_clamp_io& io, hipStr
 vgi_status launch_denoise_atrous(const vgi_denoise_atrous_io& io, int step, hipStream_t st) {
     // reads depth 4, L1+L2 8, diff, spec, variance; writes diff, spec, variance
     ProfScope prof("denoise_atrous", st, px_bytes(io.storage, 4, 2, 0, 12) * io.width * io.height);
-    dim3 g = tile_grid(io.width, io.height);
+    // A step at or beyond the larger image edge puts every tap outside the
+    // image, exactly as that edge does: clamp it to bound the residue count.
+    const uint32_t s = (uint32_t)step < std::max(io.width, io.height) ? (uint32_t)step : std::max(io.width, io.height);
+    const uint32_t lw = (io.width + s - 1) / s, lh = (io.height + s - 1) / s;  // lattice extent of residue 0
+    const uint32_t nbx = (lw + kAtrousB - 1) / kAtrousB, nby = (lh + kAtrousB - 1) / kAtrousB;
+    dim3 g(nbx * s, nby * s, 1);
     if (io.storage == VGI_STORAGE_F32)
-        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(64), 0, st, io, step);
+        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(256), 0, st, io, (int)s, nbx, nby);
     else
-        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(64), 0, st, io, step);
+        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(256), 0, st, io, (int)s, nbx, nby);
     return check_launch("k_atrous");
 }
 

@@ -83,6 +83,7 @@ extern "C" vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const
     memcpy(&f, fp, sizeof(f));
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("gbuffer_primary", st, (double)io->width * io->height * 20.0);
-    hipLaunchKernelGGL(k_gbuffer, tile_grid(io->width, io->height), dim3(64), 0, st, f, scene_view(scene), *io);
+    hipLaunchKernelGGL(k_gbuffer, tile_grid(io->width, io->height), dim3(64), traversal_lds_bytes(scene_view(scene)), st, f,
+                       scene_view(scene), *io);
     return check_launch("k_gbuffer");
 }

@@ -241,12 +241,14 @@ static FrameParams to_fp(const vgi_frame_params* fp) {
     return f;
 }
 
-#define VGI_DISPATCH_STORAGE(mode, kernel, grid, ...)                                          \
-    do {                                                                                       \
-        if ((mode) == VGI_STORAGE_F32)                                                         \
-            hipLaunchKernelGGL(kernel<StorageF32>, grid, dim3(64), 0, st, __VA_ARGS__);        \
-        else                                                                                   \
-            hipLaunchKernelGGL(kernel<StorageF16>, grid, dim3(64), 0, st, __VA_ARGS__);        \
+// Every kernel here traces rays: launch with the scene's LDS traversal stack.
+#define VGI_DISPATCH_STORAGE(mode, kernel, grid, ...)                                                     \
+    do {                                                                                                  \
+        const size_t lds_ = traversal_lds_bytes(t.scene);                                                 \
+        if ((mode) == VGI_STORAGE_F32)                                                                    \
+            hipLaunchKernelGGL(kernel<StorageF32>, grid, dim3(64), lds_, st, __VA_ARGS__);                \
+        else                                                                                              \
+            hipLaunchKernelGGL(kernel<StorageF16>, grid, dim3(64), lds_, st, __VA_ARGS__);                \
     } while (0)
 
 }  // namespace vgi

@@ -34,7 +34,7 @@ extern "C" vgi_status vgi_trace_rays(const vgi_scene* scene, const float* rays, 
     if (num_rays == 0) return VGI_OK;
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof(any_hit ? "trace_rays_any" : "trace_rays_closest", st, 0.0);
-    hipLaunchKernelGGL(k_trace_rays, dim3((num_rays + 63) / 64), dim3(64), 0, st, scene_view(scene), rays, num_rays,
+    hipLaunchKernelGGL(k_trace_rays, dim3((num_rays + 63) / 64), dim3(64), traversal_lds_bytes(scene_view(scene)), st, scene_view(scene), rays, num_rays,
                        any_hit, hits);
     return check_launch("k_trace_rays");
 }

@@ -77,7 +77,7 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
                     nearl = n.link[1];
                     farl = n.link[0];
                 }
-                if (sp < kBvhMaxDepth) {  // always true for a BVH from vgi_scene_create
+                if (sp < (int)s.stack_depth) {  // always true: the stack never outgrows the tree depth
                     stack[sp * 64] = farl;
                     ++sp;
                 }
@@ -123,9 +123,12 @@ __device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float 
     return traverse<true>(s, o, d, tmin, tmax, stack).tri != 0xFFFFFFFFu;
 }
 
-// Per-wave LDS traversal stack: kBvhMaxDepth entries x 64 lanes.
+// Per-wave LDS traversal stack: SceneView::stack_depth entries x 64 lanes in
+// dynamic LDS (launch with traversal_lds_bytes(scene) bytes), so a shallow
+// tree leaves LDS for more resident waves.  A ray at a node of depth d holds
+// at most d stacked far children, so stack_depth entries always suffice.
 #define VGI_TRAVERSAL_STACK(name)                        \
-    __shared__ int32_t name##_mem[kBvhMaxDepth * 64];    \
+    extern __shared__ int32_t name##_mem[];              \
     int32_t* name = name##_mem + (threadIdx.x & 63u)
 
 }  // namespace vgi
