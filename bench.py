@@ -25,6 +25,23 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
+# Resampling parameters: the 1280x720 frame-loop workload's (temporal M cap
+# 30, 5 spatial taps in a 30 px radius) or the reference's constants.
+RESAMPLE = {
+    "frame_loop": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0),
+    "reference": dict(temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0),
+}
+RESAMPLE_DESC = {
+    "frame_loop": "M cap 30 (specular 30*roughness), 5 spatial taps, 30 px radius",
+    "reference": "reference constants (M cap 16, 9 spatial taps, radius 0.05*min(w,h))",
+}
+
+
+def _lib_const(name: str):
+    from violet_amd import _lib
+
+    return getattr(_lib, name)
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -146,7 +163,7 @@ def run_frame(args):
     st = sc.stats()
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16 if args.storage == "f16" else _lib.STORAGE_F32,
-                                atrous_iterations=args.iterations)
+                                atrous_iterations=args.iterations, **RESAMPLE[args.resample])
     r = renderer.RestirRenderer(w, h, cfg)
     fps = frame_params_seq(w, h, args.frames)
 
@@ -174,7 +191,19 @@ def run_frame(args):
     elapsed = time.perf_counter() - t0
     _lib.call("vgi_profiler_enable", 0)
     prof = _lib.profiler_read()
-    return dict(elapsed=elapsed, lat=lat, prof=prof, pixels=w * h, mesh=mesh)
+    # traversal statistics: the instrumented trace kernels over a few more
+    # frames of the same orbit (outside the timed region)
+    base += args.steps
+    _lib.call("vgi_profiler_reset")
+    _lib.call("vgi_trace_stats_enable", 1)
+    _lib.call("vgi_profiler_enable", 1)
+    for k in range(args.count_frames):
+        step(base + k)
+    torch.cuda.synchronize()
+    _lib.call("vgi_profiler_enable", 0)
+    _lib.call("vgi_trace_stats_enable", 0)
+    counts = _lib.profiler_read()
+    return dict(elapsed=elapsed, lat=lat, prof=prof, counts=counts, pixels=w * h, mesh=mesh)
 
 
 def cpu_baseline_frame(args, mesh, budget_s: float = 20.0):
@@ -185,7 +214,7 @@ def cpu_baseline_frame(args, mesh, budget_s: float = 20.0):
     threads = O.load(32).oracle_num_threads()
     w, h = args.cpu_width, args.cpu_height
     fps = frame_params_seq(w, h, args.frames)
-    R = O.OracleRenderer(32, mesh, w, h, atrous_iterations=args.iterations)
+    R = O.OracleRenderer(32, mesh, w, h, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -214,6 +243,8 @@ def main():
     ap.add_argument("--cpu-width", type=int, default=None)
     ap.add_argument("--cpu-height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--resample", default="frame_loop", choices=sorted(RESAMPLE))
+    ap.add_argument("--count-frames", type=int, default=8, help="frames run with traversal statistics")
     args = ap.parse_args()
     frame = args.workload == "frame"
     # full frame: 1280x720, 60-frame orbit, 3 a-trous iterations (the reference
@@ -232,7 +263,13 @@ def main():
     r = run_frame(args) if frame else run_denoise(args)
     mpix = args.steps * r["pixels"] / r["elapsed"] / 1e6
     prof = r["prof"]
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    counts = r.get("counts", {})
+    trace_passes = {k for k, v in counts.items() if v["rays_closest"] + v["rays_any"] > 0}
+    # the roofline is the dominant HBM-bound pixel pass; trace passes are
+    # latency/divergence bound and are reported in Mrays/s with their
+    # counted traversal bytes instead
+    pixel_prof = {k: v for k, v in prof.items() if k not in trace_passes}
+    dom = max(pixel_prof.items(), key=lambda kv: kv[1]["ms"])
     dname, d = dom
     avg_ms = d["ms"] / max(d["count"], 1)
     bytes_per_launch = d["bytes"] / max(d["count"], 1)
@@ -242,6 +279,20 @@ def main():
                       gbs=round(v["bytes"] / max(v["ms"] * 1e-3, 1e-12) / 1e9, 1),
                       share=round(v["ms"] / max(sum(x["ms"] for x in prof.values()), 1e-12), 4))
               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    trace = {}
+    for k in sorted(trace_passes, key=lambda n: -prof.get(n, {"ms": 0})["ms"]):
+        c, t = counts[k], prof.get(k)
+        if not t or not c["count"]:
+            continue
+        rays = (c["rays_closest"] + c["rays_any"]) / c["count"]
+        tb = (c["nodes"] * _lib_const("BVH_NODE_BYTES") + c["tris"] * _lib_const("BVH_TRI_BYTES")) / c["count"]
+        us = t["ms"] / max(t["count"], 1) * 1e3
+        trace[k] = dict(avg_us=round(us, 2), rays_per_launch=round(rays), mrays_per_s=round(rays / us, 1),
+                        closest_share=round(c["rays_closest"] / max(c["rays_closest"] + c["rays_any"], 1), 3),
+                        nodes_per_ray=round(c["nodes"] / max(c["rays_closest"] + c["rays_any"], 1), 2),
+                        tris_per_ray=round(c["tris"] / max(c["rays_closest"] + c["rays_any"], 1), 2),
+                        traversal_bytes_per_ray=round(tb / max(rays, 1), 1),
+                        traversal_gbs=round(tb / (us * 1e-6) / 1e9, 1))
     if args.no_cpu_baseline:
         cpu = None
     else:
@@ -266,7 +317,7 @@ def main():
         "config": ({"workload": f"restir_gi_frame_{args.width}x{args.height}", "width": args.width,
                     "height": args.height, "triangles": int(len(r["mesh"].indices)), "frames_cycled": args.frames,
                     "gi_rays_per_px": "1 diffuse + 1 specular (+1 shadow each)", "atrous_iterations": args.iterations,
-                    "restir_params": "reference defaults (M cap 16, 9 spatial taps, radius 0.05*min(w,h))",
+                    "restir_params": RESAMPLE_DESC[args.resample],
                     "storage": args.storage, "parallelism": "single-gpu"} if frame else
                    {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
                     "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
@@ -275,6 +326,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": bytes_per_launch},
         "passes": passes,
+        "trace": trace,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -68,10 +68,25 @@ typedef struct vgi_profile_entry {
     double total_ms;     /* summed kernel time */
     uint64_t count;      /* number of launches */
     double bytes;        /* summed algorithmic bytes of those launches */
+    /* traversal counts of those launches (zero unless trace stats are on) */
+    double rays_closest; /* closest-hit BVH traversals */
+    double rays_any;     /* any-hit (shadow) BVH traversals */
+    double nodes;        /* interior nodes visited (VGI_BVH_NODE_BYTES each) */
+    double tris;         /* triangles tested (VGI_BVH_TRI_BYTES each) */
 } vgi_profile_entry;
 vgi_status vgi_profiler_enable(uint32_t enable);
 vgi_status vgi_profiler_reset(void);
 int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap);
+
+/* Traversal statistics (no reference counterpart; measurement only).  When
+ * enabled, every ray-tracing pass launches an instrumented build of its
+ * kernel that counts rays, interior nodes visited and triangles tested, and
+ * waits for it, adding the counts to the pass's profiler entry (the
+ * profiler must be enabled too).  Counts do not depend on scheduling.
+ * Allocates a small counter buffer on the current device when turned on. */
+#define VGI_BVH_NODE_BYTES 64
+#define VGI_BVH_TRI_BYTES 48
+vgi_status vgi_trace_stats_enable(uint32_t enable);
 
 /* FrameParams (shader/frame_bindings.hlsl:6-25; filled by
  * crates/violet-app/src/render_loop.rs:108-158).  384 bytes, matrices
@@ -409,6 +424,7 @@ typedef struct vgi_pc_ind_diff_temporal {
     uint32_t frame_rand;
     uint32_t has_prev_frame;
     uint32_t has_new_sample;
+    uint32_t m_cap;     /* history M clamp; 0 = the reference's M_MAX 16 (:136) */
 } vgi_pc_ind_diff_temporal;
 vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
                                                const vgi_pc_ind_diff_temporal* pc, void* stream);
@@ -430,8 +446,15 @@ typedef struct vgi_ind_diff_spatial_io {
 typedef struct vgi_pc_frame_rand {
     uint32_t frame_rand;
 } vgi_pc_frame_rand;
+/* num_taps: golden-spiral taps (MAX_ITERATION, :12; 0 = 9, at most 16);
+ * radius_px: initial search radius (0 = INIT_RADIUS_FACTOR 0.05 * min(w, h)). */
+typedef struct vgi_pc_ind_diff_spatial {
+    uint32_t frame_rand;
+    uint32_t num_taps;
+    float radius_px;
+} vgi_pc_ind_diff_spatial;
 vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
-                                              const vgi_pc_frame_rand* pc, void* stream);
+                                              const vgi_pc_ind_diff_spatial* pc, void* stream);
 
 /* restir/ind_spec_temporal_resample.hlsl:26-44; restir_renderer.rs:743-761
  * (rw_hit_pos/rw_hit_radiance hold the new sample on entry). */
@@ -451,6 +474,7 @@ typedef struct vgi_ind_spec_temporal_io {
 typedef struct vgi_pc_ind_spec_temporal {
     uint32_t frame_rand;
     uint32_t has_prev_frame;
+    uint32_t m_cap;     /* M clamp is uint(m_cap * roughness); 0 = M_MAX 16 (:252) */
 } vgi_pc_ind_spec_temporal;
 vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params* fp, const vgi_ind_spec_temporal_io* io,
                                                const vgi_pc_ind_spec_temporal* pc, void* stream);
@@ -466,8 +490,13 @@ typedef struct vgi_ind_spec_spatial_io {
     void* rw_lighting_texture;
     void* rw_ray_len_texture; /* scalar */
 } vgi_ind_spec_spatial_io;
+/* radius_px: search radius (0 = INIT_RADIUS_FACTOR 0.05 * min(w, h), :13). */
+typedef struct vgi_pc_ind_spec_spatial {
+    uint32_t frame_rand;
+    float radius_px;
+} vgi_pc_ind_spec_spatial;
 vgi_status vgi_pass_ind_spec_spatial_resample(const vgi_frame_params* fp, const vgi_ind_spec_spatial_io* io,
-                                              const vgi_pc_frame_rand* pc, void* stream);
+                                              const vgi_pc_ind_spec_spatial* pc, void* stream);
 
 /* restir/hash_grid_decay.hlsl:5-6; restir_renderer.rs:322-329 */
 typedef struct vgi_hash_grid_decay_io {
@@ -545,6 +574,10 @@ typedef struct vgi_renderer_config {
     uint32_t hash_grid_cache_decay;  /* default 1 */
     float ao_radius;                 /* default 2.0 */
     float sky_radiance[3];           /* constant sky (replaces the sky cube) */
+    /* resampling parameters; 0 = the reference's constants */
+    uint32_t temporal_m_cap;         /* diffuse M clamp; specular uint(cap * roughness); 0 = 16 */
+    uint32_t diff_spatial_taps;      /* diffuse spatial golden-spiral taps; 0 = 9, at most 16 */
+    float spatial_radius_px;         /* spatial search radius; 0 = 0.05 * min(w, h) */
     vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
 } vgi_renderer_config;
 

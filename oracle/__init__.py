@@ -236,10 +236,11 @@ def _restir_sigs(lib):
     lib.oracle_spec_sample_validate.argtypes = [P, TR, I, I, P, P, P]
     lib.oracle_ao_gen.argtypes = [P, I, I, P, P, P, I, I, C.c_double if lib._bits == 64 else C.c_float, P]
     lib.oracle_ao_filter.argtypes = [I, I, P, P, P]
-    lib.oracle_diff_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, I, P, P, P]
-    lib.oracle_diff_spatial.argtypes = [P, I, I, P, P, P, P, P, P, U32, P, P, P, P]
-    lib.oracle_spec_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, P, P, P]
-    lib.oracle_spec_spatial.argtypes = [P, I, I, P, P, P, P, P, U32, P, P]
+    R = C.c_double if lib._bits == 64 else C.c_float
+    lib.oracle_diff_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, I, P, P, P, U32]
+    lib.oracle_diff_spatial.argtypes = [P, I, I, P, P, P, P, P, P, U32, P, P, P, P, I, R]
+    lib.oracle_spec_temporal.argtypes = [P, I, I, P, P, P, P, P, P, P, U32, I, P, P, P, U32]
+    lib.oracle_spec_spatial.argtypes = [P, I, I, P, P, P, P, P, U32, P, P, R]
     lib.oracle_gbuffer.argtypes = [P, P, I, I, P, P]
     lib.oracle_hg_decay.argtypes = [P, P]
     lib.oracle_hg_select.argtypes = [P, U32, P, P, U32]
@@ -363,7 +364,7 @@ class OracleRestir:
         return out
 
     def diff_temporal(self, fp, prev_depth, depth, new_hit, new_rad, prev_rsv, prev_hit, prev_rad, frame_rand, has_prev,
-                      has_new):
+                      has_new, m_cap=0):
         h, w = depth.shape
         rsv = np.zeros((h, w), np.uint32)
         hit = np.zeros((h, w, 2), np.uint32)
@@ -372,10 +373,10 @@ class OracleRestir:
         self.lib.oracle_diff_temporal(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(self.r(prev_depth)), _np_ptr(self.r(depth)),
                                       _np_ptr(a(new_hit)), _np_ptr(self.r(new_rad)), _np_ptr(a(prev_rsv)),
                                       _np_ptr(a(prev_hit)), _np_ptr(self.r(prev_rad)), frame_rand, int(has_prev),
-                                      int(has_new), _np_ptr(rsv), _np_ptr(hit), _np_ptr(rad))
+                                      int(has_new), _np_ptr(rsv), _np_ptr(hit), _np_ptr(rad), int(m_cap))
         return rsv, hit, rad
 
-    def diff_spatial(self, fp, gb, depth, ao, rsv, hit, rad, frame_rand):
+    def diff_spatial(self, fp, gb, depth, ao, rsv, hit, rad, frame_rand, num_taps=0, radius_px=0.0):
         h, w = depth.shape
         o_rsv = np.zeros((h, w), np.uint32)
         o_hit = np.zeros((h, w, 2), np.uint32)
@@ -384,11 +385,12 @@ class OracleRestir:
         self.lib.oracle_diff_spatial(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
                                      _np_ptr(self.r(depth)), _np_ptr(self.r(ao)), _np_ptr(np.ascontiguousarray(rsv, np.uint32)),
                                      _np_ptr(np.ascontiguousarray(hit, np.uint32)), _np_ptr(self.r(rad)), frame_rand,
-                                     _np_ptr(o_rsv), _np_ptr(o_hit), _np_ptr(o_rad), _np_ptr(o_l))
+                                     _np_ptr(o_rsv), _np_ptr(o_hit), _np_ptr(o_rad), _np_ptr(o_l), int(num_taps),
+                                     float(radius_px))
         return o_rsv, o_hit, o_rad, o_l
 
     def spec_temporal(self, fp, prev_gb, prev_depth, gb, depth, prev_rsv, prev_hp, prev_rad, new_hp, new_rad, frame_rand,
-                      has_prev):
+                      has_prev, m_cap=0):
         h, w = depth.shape
         rsv = np.zeros((h, w, 2), np.uint32)
         hp = self.r(new_hp).copy()
@@ -397,16 +399,17 @@ class OracleRestir:
         self.lib.oracle_spec_temporal(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(a(prev_gb)), _np_ptr(self.r(prev_depth)),
                                       _np_ptr(a(gb)), _np_ptr(self.r(depth)), _np_ptr(a(prev_rsv)), _np_ptr(self.r(prev_hp)),
                                       _np_ptr(self.r(prev_rad)), frame_rand, int(has_prev), _np_ptr(rsv), _np_ptr(hp),
-                                      _np_ptr(rad))
+                                      _np_ptr(rad), int(m_cap))
         return rsv, hp, rad
 
-    def spec_spatial(self, fp, gb, depth, rsv, hp, rad, frame_rand):
+    def spec_spatial(self, fp, gb, depth, rsv, hp, rad, frame_rand, radius_px=0.0):
         h, w = depth.shape
         light = np.zeros((h, w, 3), self.rt_dtype)
         rl = np.zeros((h, w), self.rt_dtype)
         self.lib.oracle_spec_spatial(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
                                      _np_ptr(self.r(depth)), _np_ptr(np.ascontiguousarray(rsv, np.uint32)),
-                                     _np_ptr(self.r(hp)), _np_ptr(self.r(rad)), frame_rand, _np_ptr(light), _np_ptr(rl))
+                                     _np_ptr(self.r(hp)), _np_ptr(self.r(rad)), frame_rand, _np_ptr(light), _np_ptr(rl),
+                                     float(radius_px))
         return light, rl
 
     # hash grid
@@ -455,7 +458,7 @@ class OracleRenderer:
 
     def __init__(self, bits: int, mesh, width: int, height: int, sky=(1.0, 1.0, 1.0), denoise=True,
                  ind_diff_validate=True, ind_spec_validate=True, hash_grid_cache_decay=True, ao_radius=2.0,
-                 atrous_iterations=3):
+                 atrous_iterations=3, temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0):
         self.bits = bits
         self.scene = OracleScene(bits, mesh)
         self.R = OracleRestir(bits, self.scene, sky)
@@ -463,6 +466,7 @@ class OracleRenderer:
         self.w, self.h = width, height
         self.denoise, self.ind_diff_validate, self.ind_spec_validate = denoise, ind_diff_validate, ind_spec_validate
         self.decay_on, self.ao_radius = hash_grid_cache_decay, ao_radius
+        self.m_cap, self.taps, self.radius_px = temporal_m_cap, diff_spatial_taps, spatial_radius_px
         self.cells = None
         self.prev = None  # dict of last frame's state
 
@@ -501,15 +505,15 @@ class OracleRenderer:
         ao = R.ao_gen(fp, hit, depth, p["ao"] if has_prev else None, not validation, has_prev, self.ao_radius)
         aof = R.ao_filter(depth, ao)
         t = R.diff_temporal(fp, p["depth"] if has_prev else None, depth, hit, rad, *prev_rsv, next(seeds), has_prev,
-                            not validation)
-        d_rsv, d_hit, d_rad, d_light = R.diff_spatial(fp, gb, depth, aof, *t, next(seeds))
+                            not validation, self.m_cap)
+        d_rsv, d_hit, d_rad, d_light = R.diff_spatial(fp, gb, depth, aof, *t, next(seeds), self.taps, self.radius_px)
         hp, srad = R.spec_sample_gen(fp, tr, gb, depth, next(seeds), has_prev)
         ps = (p["s_rsv"], p["s_hp"], p["s_rad"]) if has_prev else (None, None, None)
         if has_prev and self.ind_spec_validate:
             ps = R.spec_sample_validate(fp, tr, *ps)
         s_rsv, s_hp, s_rad = R.spec_temporal(fp, p["gb"] if has_prev else None, p["depth"] if has_prev else None, gb,
-                                             depth, *ps, hp, srad, next(seeds), has_prev)
-        s_light, ray_len = R.spec_spatial(fp, gb, depth, s_rsv, s_hp, s_rad, next(seeds))
+                                             depth, *ps, hp, srad, next(seeds), has_prev, self.m_cap)
+        s_light, ray_len = R.spec_spatial(fp, gb, depth, s_rsv, s_hp, s_rad, next(seeds), self.radius_px)
         if self.denoise and has_prev:
             o = self.den.add_passes(fp, depth, gb, p["depth"], p["gb"], d_light, s_light, ray_len)
             out_d, out_s = o["diffuse"], o["specular"]

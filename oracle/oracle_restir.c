@@ -525,7 +525,9 @@ void oracle_ao_filter(int w, int h, const real* depth, const real* ao, real* o_a
 void oracle_diff_temporal(const float* fpf, int w, int h, const real* prev_depth, const real* depth,
                           const uint32_t* new_hit, const real* new_rad, const uint32_t* prev_rsv, const uint32_t* prev_hit,
                           const real* prev_rad, uint32_t frame_rand, int has_prev, int has_new, uint32_t* o_rsv,
-                          uint32_t* o_hit, real* o_rad) {
+                          uint32_t* o_hit, real* o_rad, uint32_t m_cap) {
+    /* M_MAX (ind_diff_temporal_resample.hlsl:136) unless set */
+    const uint32_t Mcap = m_cap ? m_cap : 16u;
     ofp fp;
     o_load_fp(fpf, &fp);
 #pragma omp parallel for schedule(dynamic, 2)
@@ -557,7 +559,7 @@ void oracle_diff_temporal(const float* fpf, int w, int h, const real* prev_depth
                     rad = V3s(0);
                     ex = ey = 0;
                 }
-                if (M > 16) M = 16;
+                if (M > Mcap) M = Mcap;
             }
             if (has_new) {
                 real tp = lum(rad);
@@ -604,7 +606,11 @@ static owr o_diff_load(const uint32_t* rsv, const uint32_t* hit, const real* rad
 
 void oracle_diff_spatial(const float* fpf, int w, int h, const uint32_t* gb, const real* depth, const real* ao,
                          const uint32_t* t_rsv, const uint32_t* t_hit, const real* t_rad, uint32_t frame_rand,
-                         uint32_t* o_rsv, uint32_t* o_hit, real* o_rad, real* o_light) {
+                         uint32_t* o_rsv, uint32_t* o_hit, real* o_rad, real* o_light, int num_taps, real radius_px) {
+    /* MAX_ITERATION (ind_diff_spatial_resample.hlsl) and the initial radius
+     * INIT_RADIUS_FACTOR * min(w, h) unless set */
+    const int N = num_taps > 0 ? num_taps : 9;
+    const real radius0 = radius_px > RC(0.0) ? radius_px : (real)(w < h ? w : h) * RC(0.05);
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
@@ -612,16 +618,16 @@ void oracle_diff_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
     const double PI_ = (double)(float)3.14159265359;
     const float Bf = (float)(log(1.61803398) / (0.5 * PI_));
     const float Af = (float)(1.0 / exp((double)Bf * (double)(2.0f * (float)PI_)));
-    real theta[9], rr[9];
-    for (int k = 0; k < 9; ++k) {
-        theta[k] = O_TWO_PI * (((real)k + RC(1.0)) / RC(9.0));
+    real theta[16], rr[16];
+    for (int k = 0; k < N && k < 16; ++k) {
+        theta[k] = O_TWO_PI * (((real)k + RC(1.0)) / (real)N);
 #if defined(ORACLE_F32)
         rr[k] = (float)((double)Af * exp((double)Bf * (double)theta[k]));
 #else
         rr[k] = (double)Af * exp((double)Bf * theta[k]);
 #endif
     }
-    const real T = (real)(float)(0.6186 * 9.0);
+    const real T = (real)(float)(0.6186 * (double)N);
     const real cos_tol = (real)(float)cos((double)(float)(PI_ * (25.0 / 180.0)));
 #pragma omp parallel for schedule(dynamic, 2)
     for (int py = 0; py < h; ++py)
@@ -640,11 +646,10 @@ void oracle_diff_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
             real a = ao[2 * i];
             owr res = o_diff_load(t_rsv, t_hit, t_rad, i, p, n);
             real ws = (res.W * res.target) * (real)res.M;
-            real radius = (real)(w < h ? w : h) * RC(0.05);
-            radius = R_FMAX(radius * a, RC(1.5));
+            real radius = R_FMAX(radius0 * a, RC(1.5));
             uint32_t rng = o_lcg_init(px, py, frame_rand);
             real ra = o_rand(&rng) * O_TWO_PI;
-            for (int k = 0; k < 9; ++k) {
+            for (int k = 0; k < N; ++k) {
                 real s, c;
                 r_sincos(theta[k] * T + ra, &s, &c);
                 real x = rr[k] * c, y = rr[k] * s;
@@ -720,7 +725,9 @@ static inline real o_spec_target(v3 view_pos, v3 p, v3 n, real pr, v3 f0, v3 hit
 void oracle_spec_temporal(const float* fpf, int w, int h, const uint32_t* prev_gb, const real* prev_depth,
                           const uint32_t* gb, const real* depth, const uint32_t* prev_rsv, const real* prev_hitpos,
                           const real* prev_rad, uint32_t frame_rand, int has_prev, uint32_t* rw_rsv, real* rw_hitpos,
-                          real* rw_rad) {
+                          real* rw_rad, uint32_t m_cap) {
+    /* M_MAX (ind_spec_temporal_resample.hlsl:252) unless set */
+    const real Mcap = (real)(m_cap ? m_cap : 16u);
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
@@ -767,7 +774,7 @@ void oracle_spec_temporal(const float* fpf, int w, int h, const uint32_t* prev_g
             real metal = o_gb_metal(gb[i]);
             v3 n = o_gb_normal(gb[plane + i]);
             real pr = o_gb_rough(gb[plane + i]);
-            uint32_t Mmax = r2u(RC(16.0) * pr);
+            uint32_t Mmax = r2u(Mcap * pr);
             if (M > Mmax) M = Mmax;
             v3 f0 = o_f0(col, metal);
             ospec cur = o_calc_things(o_xyz(fp.view_pos), p, n, pr, f0, npos);
@@ -806,7 +813,10 @@ void oracle_spec_temporal(const float* fpf, int w, int h, const uint32_t* prev_g
 }
 
 void oracle_spec_spatial(const float* fpf, int w, int h, const uint32_t* gb, const real* depth, const uint32_t* rsv,
-                         const real* hitpos, const real* rad, uint32_t frame_rand, real* o_light, real* o_raylen) {
+                         const real* hitpos, const real* rad, uint32_t frame_rand, real* o_light, real* o_raylen,
+                         real radius_px) {
+    /* INIT_RADIUS_FACTOR * min(w, h) (ind_spec_spatial_resample.hlsl:13,213) unless set */
+    const real radius0 = radius_px > RC(0.0) ? radius_px : (real)(w < h ? w : h) * RC(0.05);
     ofp fp;
     o_load_fp(fpf, &fp);
     size_t plane = (size_t)w * h;
@@ -836,7 +846,7 @@ void oracle_spec_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
             v3 f0 = o_f0(col, metal);
             real tf = hp[3];
             uint32_t iters = r2u(RC(0.0) + pr * (RC(4.0) - RC(0.0)));
-            real radius = (real)(w < h ? w : h) * RC(0.05);
+            real radius = radius0;
             uint32_t rng = o_lcg_init(px, py, frame_rand);
             for (uint32_t k = 0; k < iters; ++k) {
                 real ux = o_rand(&rng), uy = o_rand(&rng);

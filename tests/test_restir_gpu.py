@@ -9,6 +9,10 @@ sequencer) against the oracle.
   recorded inputs; reservoir words, selected samples and hit encodings must
   be bit-identical, float outputs bit-identical (they use IEEE operations
   only), and within the FP64 oracle's tolerance.
+
+Every test runs twice: with the reference's resampling constants (M cap 16,
+9 spiral taps, radius 0.05*min(w,h)) and with the 1280x720 frame-loop
+workload's (M cap 30, 5 taps, 30 px).
 """
 from __future__ import annotations
 
@@ -23,11 +27,20 @@ pytestmark = pytest.mark.gpu
 W, H = 96, 64
 
 
-@pytest.fixture(scope="module")
-def run():
+PARAMS = {
+    "reference": dict(temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0),
+    "frame_loop": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(PARAMS))
+def run(request):
+    from violet_amd import renderer
+
     mesh = make_mesh()
-    fps, outs, snaps, r, sc = run_gpu_frames(mesh, W, H, 6)
-    return dict(mesh=mesh, fps=fps, outs=outs, snaps=snaps)
+    prm = PARAMS[request.param]
+    fps, outs, snaps, r, sc = run_gpu_frames(mesh, W, H, 6, config=renderer.RestirConfig(**prm))
+    return dict(mesh=mesh, fps=fps, outs=outs, snaps=snaps, prm=prm)
 
 
 @pytest.fixture(scope="module")
@@ -71,10 +84,13 @@ def test_frame0_bit_exact_pipeline(run, o32):
     bits_equal(s["ao"], ao, "ao")
     aof = o32.ao_filter(depth, ao)
     bits_equal(s["ao_filtered"], aof, "ao filtered")
-    trsv, thit, trad = o32.diff_temporal(fp, None, depth, hit, rad, None, None, None, seeds["diff_temporal"], False, True)
+    P = run["prm"]
+    trsv, thit, trad = o32.diff_temporal(fp, None, depth, hit, rad, None, None, None, seeds["diff_temporal"], False, True,
+                                         P["temporal_m_cap"])
     bits_equal(s["diff_temporal_reservoir"], trsv, "diff temporal reservoir")
     bits_equal(s["diff_temporal_hit_pos_normal"], thit, "diff temporal hit")
-    srsv, shit, srad, slight = o32.diff_spatial(fp, gb, depth, aof, trsv, thit, trad, seeds["diff_spatial"])
+    srsv, shit, srad, slight = o32.diff_spatial(fp, gb, depth, aof, trsv, thit, trad, seeds["diff_spatial"],
+                                                P["diff_spatial_taps"], P["spatial_radius_px"])
     bits_equal(s["diff_reservoir"], srsv, "diff spatial reservoir")
     bits_equal(s["diff_hit_pos_normal"], shit, "diff spatial hit")
     bits_equal(s["diff_hit_radiance"], srad, "diff spatial radiance")
@@ -82,10 +98,11 @@ def test_frame0_bit_exact_pipeline(run, o32):
     hp, sr = o32.spec_sample_gen(fp, tr, gb, depth, seeds["spec_gen"], False)
     geo = depth > 0
     bits_equal(s["spec_hit_radiance"][geo], sr[geo], "spec new radiance")
-    ursv, uhp, urad = o32.spec_temporal(fp, None, None, gb, depth, None, None, None, hp, sr, seeds["spec_temporal"], False)
+    ursv, uhp, urad = o32.spec_temporal(fp, None, None, gb, depth, None, None, None, hp, sr, seeds["spec_temporal"], False,
+                                        P["temporal_m_cap"])
     bits_equal(s["spec_reservoir"], ursv, "spec temporal reservoir")
     bits_equal(s["spec_hit_pos"][geo], uhp[geo], "spec temporal hit pos")
-    light, rl = o32.spec_spatial(fp, gb, depth, ursv, uhp, urad, seeds["spec_spatial"])
+    light, rl = o32.spec_spatial(fp, gb, depth, ursv, uhp, urad, seeds["spec_spatial"], P["spatial_radius_px"])
     bits_equal(s["indirect_specular_raw"], light, "spec lighting")
     bits_equal(s["spec_ray_len"], rl, "spec ray length")
     # hash-grid queries appended by the two trace passes: same multiset
@@ -99,6 +116,7 @@ def test_resampling_passes_replayed(run, o32, o64, k):
     """Per-pass replay on recorded inputs of frame k (bit-exact vs FP32 twin,
     tolerance vs FP64 oracle)."""
     s, p, fp = run["snaps"][k], run["snaps"][k - 1], run["fps"][k]
+    P = run["prm"]
     fl = s["flags"]
     seeds = seeds_for_frame(k, fl["hg"], fl["validation"])
     depth, gb = s["depth"], s["gbuffer"]
@@ -112,14 +130,14 @@ def test_resampling_passes_replayed(run, o32, o64, k):
     # validation pass rewrote it in place this frame)
     if not fl["validation"]:
         args = (fp, p["depth"], depth, s["diff_new_hit_pos_normal"], s["diff_new_hit_radiance"], p["diff_reservoir"],
-                p["diff_hit_pos_normal"], p["diff_hit_radiance"], seeds["diff_temporal"], True, True)
+                p["diff_hit_pos_normal"], p["diff_hit_radiance"], seeds["diff_temporal"], True, True, P["temporal_m_cap"])
         t32 = o32.diff_temporal(*args)
         bits_equal(s["diff_temporal_reservoir"], t32[0], f"diff temporal reservoir[{k}]")
         bits_equal(s["diff_temporal_hit_pos_normal"], t32[1], f"diff temporal hit[{k}]")
         bits_equal(s["diff_temporal_hit_radiance"], t32[2], f"diff temporal radiance[{k}]")
     # diffuse spatial
     sp = (fp, gb, depth, s["ao_filtered"], s["diff_temporal_reservoir"], s["diff_temporal_hit_pos_normal"],
-          s["diff_temporal_hit_radiance"], seeds["diff_spatial"])
+          s["diff_temporal_hit_radiance"], seeds["diff_spatial"], P["diff_spatial_taps"], P["spatial_radius_px"])
     d32 = o32.diff_spatial(*sp)
     bits_equal(s["diff_reservoir"], d32[0], f"diff spatial reservoir[{k}]")
     bits_equal(s["diff_hit_pos_normal"], d32[1], f"diff spatial hit[{k}]")
@@ -132,7 +150,8 @@ def test_resampling_passes_replayed(run, o32, o64, k):
     assert same.mean() > 0.98, f"FP64 selection agreement {same.mean():.4f}"
     assert_close_frac(s["indirect_diffuse_raw"][same], d64[3][same], f"diff lighting vs FP64 [{k}]")
     # specular spatial (its inputs are the in-place temporal results)
-    ss = (fp, gb, depth, s["spec_reservoir"], s["spec_hit_pos"], s["spec_hit_radiance"], seeds["spec_spatial"])
+    ss = (fp, gb, depth, s["spec_reservoir"], s["spec_hit_pos"], s["spec_hit_radiance"], seeds["spec_spatial"],
+          P["spatial_radius_px"])
     l32, r32 = o32.spec_spatial(*ss)
     bits_equal(s["indirect_specular_raw"], l32, f"spec lighting[{k}]")
     bits_equal(s["spec_ray_len"], r32, f"spec ray len[{k}]")

@@ -75,3 +75,39 @@ def test_closest_hit_large_scene_vs_oracle_bvh(vgi):
     assert np.array_equal(g_id, o_id), f"hit ids differ on {np.mean(g_id != o_id):.3%} of rays"
     hit = g_id != 0xFFFFFFFF
     assert np.array_equal(g_thv[hit].view(np.uint32), o_thv[hit].astype(np.float32).view(np.uint32))
+
+
+def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
+    """The instrumented traversal (trace statistics) returns the same hits and
+    counts what the BVH implies: with a 2-triangle scene the root is a leaf,
+    so every ray tests exactly 2 triangles and visits no interior node."""
+    from violet_amd import _lib, scene
+
+    tiny = scene.Mesh(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32),
+                      normals=np.tile(np.array([0, 0, 1], np.float32), (4, 1)),
+                      indices=np.array([[0, 1, 2], [0, 1, 3]], np.uint32), tri_material=np.zeros(2, np.uint32),
+                      albedo=np.full((1, 3), 0.5), metallic=np.zeros(1), roughness=np.full(1, 0.5))
+    big = scene.make_scene(4000, seed=5, extent=8.0)
+    rays = _rays(5000, 9, extent=8.0)
+    for mesh, exact in ((tiny, True), (big, False)):
+        sc = scene.Scene(mesh)
+        ref = [_gpu_trace(sc, rays, a) for a in (False, True)]
+        _lib.call("vgi_profiler_reset")
+        _lib.call("vgi_trace_stats_enable", 1)
+        _lib.call("vgi_profiler_enable", 1)
+        try:
+            got = [_gpu_trace(sc, rays, a) for a in (False, True)]
+        finally:
+            _lib.call("vgi_profiler_enable", 0)
+            _lib.call("vgi_trace_stats_enable", 0)
+        prof = _lib.profiler_read()
+        for (rt, ri), (gt, gi) in zip(ref, got):
+            assert np.array_equal(ri, gi) and np.array_equal(rt.view(np.uint32), gt.view(np.uint32))
+        c, a = prof["trace_rays_closest"], prof["trace_rays_any"]
+        assert c["rays_closest"] == len(rays) and c["rays_any"] == 0
+        assert a["rays_any"] == len(rays) and a["rays_closest"] == 0
+        if exact:
+            assert c["nodes"] == 0 and c["tris"] == 2 * len(rays)
+        else:
+            assert c["nodes"] > len(rays) and c["tris"] > 0
+            assert a["nodes"] <= c["nodes"]  # any-hit stops early

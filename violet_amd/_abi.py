@@ -43,17 +43,20 @@ DiffTemporalIO = _struct("DiffTemporalIO", WH + [
     ("new_hit_radiance_texture", vp), ("prev_reservoir_texture", vp), ("prev_hit_pos_normal_texture", vp),
     ("prev_hit_radiance_texture", vp), ("rw_reservoir_texture", vp), ("rw_hit_pos_normal_texture", vp),
     ("rw_hit_radiance_texture", vp)])
-PCDiffTemporal = _struct("PCDiffTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("has_new_sample", u32)])
+PCDiffTemporal = _struct("PCDiffTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("has_new_sample", u32),
+                                            ("m_cap", u32)])
 DiffSpatialIO = _struct("DiffSpatialIO", WH + [
     ("gbuffer_color", vp), ("gbuffer_depth", vp), ("ao_texture", vp), ("temp_reservoir_texture", vp),
     ("temp_hit_pos_normal_texture", vp), ("temp_hit_radiance_texture", vp), ("rw_reservoir_texture", vp),
     ("rw_hit_pos_normal_texture", vp), ("rw_hit_radiance_texture", vp), ("rw_lighting_texture", vp)])
 PCFrameRand = _struct("PCFrameRand", [("frame_rand", u32)])
+PCDiffSpatial = _struct("PCDiffSpatial", [("frame_rand", u32), ("num_taps", u32), ("radius_px", f32)])
+PCSpecSpatial = _struct("PCSpecSpatial", [("frame_rand", u32), ("radius_px", f32)])
 SpecTemporalIO = _struct("SpecTemporalIO", WH + [
     ("prev_gbuffer_color", vp), ("prev_gbuffer_depth", vp), ("gbuffer_color", vp), ("gbuffer_depth", vp),
     ("prev_reservoir_texture", vp), ("prev_hit_pos_texture", vp), ("prev_hit_radiance_texture", vp),
     ("rw_reservoir_texture", vp), ("rw_hit_pos_texture", vp), ("rw_hit_radiance_texture", vp)])
-PCSpecTemporal = _struct("PCSpecTemporal", [("frame_rand", u32), ("has_prev_frame", u32)])
+PCSpecTemporal = _struct("PCSpecTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("m_cap", u32)])
 SpecSpatialIO = _struct("SpecSpatialIO", WH + [
     ("gbuffer_color", vp), ("gbuffer_depth", vp), ("reservoir_texture", vp), ("hit_pos_texture", vp),
     ("hit_radiance_texture", vp), ("rw_lighting_texture", vp), ("rw_ray_len_texture", vp)])
@@ -73,6 +76,7 @@ GBufferIO = _struct("GBufferIO", [("width", u32), ("height", u32), ("rw_depth", 
 RendererConfig = _struct("RendererConfig", [
     ("storage", u32), ("denoise", u32), ("ind_diff_validate", u32), ("ind_spec_validate", u32),
     ("hash_grid_cache_decay", u32), ("ao_radius", f32), ("sky_radiance", f32 * 3),
+    ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32),
     ("denoiser", _lib.DenoiserConfig)])
 RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
@@ -88,9 +92,9 @@ _SIGS = {
     "vgi_pass_raytraced_ao_gen": [FPp, P(AOGenIO), P(PCAOGen), vp],
     "vgi_pass_raytraced_ao_filter": [P(AOFilterIO), vp],
     "vgi_pass_ind_diff_temporal_resample": [FPp, P(DiffTemporalIO), P(PCDiffTemporal), vp],
-    "vgi_pass_ind_diff_spatial_resample": [FPp, P(DiffSpatialIO), P(PCFrameRand), vp],
+    "vgi_pass_ind_diff_spatial_resample": [FPp, P(DiffSpatialIO), P(PCDiffSpatial), vp],
     "vgi_pass_ind_spec_temporal_resample": [FPp, P(SpecTemporalIO), P(PCSpecTemporal), vp],
-    "vgi_pass_ind_spec_spatial_resample": [FPp, P(SpecSpatialIO), P(PCFrameRand), vp],
+    "vgi_pass_ind_spec_spatial_resample": [FPp, P(SpecSpatialIO), P(PCSpecSpatial), vp],
     "vgi_pass_hash_grid_decay": [P(HGDecayIO), vp],
     "vgi_pass_hash_grid_query_select": [P(HGSelectIO), P(PCFrameHash), vp],
     "vgi_pass_hash_grid_query_compact": [P(HGCompactIO), vp],

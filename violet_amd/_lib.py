@@ -120,7 +120,12 @@ class DenoiserOutput(C.Structure):
 
 
 class ProfileEntry(C.Structure):
-    _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("count", C.c_uint64), ("bytes", C.c_double)]
+    _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("count", C.c_uint64), ("bytes", C.c_double),
+                ("rays_closest", C.c_double), ("rays_any", C.c_double), ("nodes", C.c_double), ("tris", C.c_double)]
+
+
+BVH_NODE_BYTES = 64  # VGI_BVH_NODE_BYTES
+BVH_TRI_BYTES = 48  # VGI_BVH_TRI_BYTES
 
 
 # name -> (restype, argtypes)
@@ -130,6 +135,7 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_profiler_enable": (C.c_int32, [u32]),
     "vgi_profiler_reset": (C.c_int32, []),
     "vgi_profiler_read": (C.c_int32, [C.POINTER(ProfileEntry), C.c_int32]),
+    "vgi_trace_stats_enable": (C.c_int32, [u32]),
     "vgi_pass_denoise_temporal": (C.c_int32, [C.POINTER(FrameParams), C.POINTER(DenoiseTemporalIO),
                                               C.POINTER(PCDenoiseTemporal), vp]),
     "vgi_pass_denoise_disocclusion_fix": (C.c_int32, [C.POINTER(DenoiseSpatialIO), vp]),
@@ -179,8 +185,12 @@ def exported_symbols() -> list[str]:
 
 
 def profiler_read() -> dict[str, dict]:
-    """Per-pass GPU time / launches / algorithmic bytes since the last reset."""
+    """Per-pass GPU time / launches / algorithmic bytes since the last reset,
+    plus traversal counts (rays_closest, rays_any, nodes, tris) when trace
+    statistics were on."""
     n = lib().vgi_profiler_read(None, 0)
     arr = (ProfileEntry * max(n, 1))()
     n = lib().vgi_profiler_read(arr, n)
-    return {arr[i].name.decode(): dict(ms=arr[i].total_ms, count=arr[i].count, bytes=arr[i].bytes) for i in range(n)}
+    return {arr[i].name.decode(): dict(ms=arr[i].total_ms, count=arr[i].count, bytes=arr[i].bytes,
+                                       rays_closest=arr[i].rays_closest, rays_any=arr[i].rays_any,
+                                       nodes=arr[i].nodes, tris=arr[i].tris) for i in range(n)}

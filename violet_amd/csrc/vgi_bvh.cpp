@@ -295,6 +295,7 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     sc->view.num_tris = nt;
     sc->view.root = root;
     sc->view.stack_depth = sc->depth;
+    sc->view.counters = nullptr;
     *out = sc;
     return VGI_OK;
 }

@@ -65,7 +65,9 @@ struct SceneView {
     uint32_t num_tris;
     int32_t root;           // link of the root (node 0, or a leaf for tiny scenes)
     uint32_t stack_depth;   // levels of the tree = most traversal-stack entries a ray can hold
+    unsigned long long* counters;  // traversal statistics slots (instrumented kernels only)
 };
+
 
 // Dynamic LDS bytes of a 64-lane traversal stack for this scene.
 inline size_t traversal_lds_bytes(const SceneView& s) { return (size_t)s.stack_depth * 64u * sizeof(int32_t); }

@@ -17,6 +17,7 @@
 namespace vgi {
 const SceneView& scene_view(const vgi_scene* s);
 
+template <bool kCount>
 __global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
@@ -34,7 +35,7 @@ __global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi
         f3 end = mk3(e.x / e.w, e.y / e.w, e.z / e.w);
         f3 o = xyz(fp.view_pos);
         f3 d = normalize3(end - o);
-        Hit hit = trace_closest(s, o, d, 0.05f, 100000.0f, stack);
+        Hit hit = trace_closest<kCount>(s, o, d, 0.05f, 100000.0f, stack);
         sh = shade_hit(s, hit, d);
         if (!sh.missed) {
             f4 c = mul44(fp.view_proj, mk4(sh.position.x, sh.position.y, sh.position.z, 1.0f));
@@ -83,7 +84,13 @@ extern "C" vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const
     memcpy(&f, fp, sizeof(f));
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("gbuffer_primary", st, (double)io->width * io->height * 20.0);
-    hipLaunchKernelGGL(k_gbuffer, tile_grid(io->width, io->height), dim3(64), traversal_lds_bytes(scene_view(scene)), st, f,
-                       scene_view(scene), *io);
+    SceneView sv = scene_view(scene);
+    sv.counters = trace_counters();
+    const dim3 g = tile_grid(io->width, io->height);
+    if (sv.counters)
+        hipLaunchKernelGGL(k_gbuffer<true>, g, dim3(64), traversal_lds_bytes(sv), st, f, sv, *io);
+    else
+        hipLaunchKernelGGL(k_gbuffer<false>, g, dim3(64), traversal_lds_bytes(sv), st, f, sv, *io);
+    trace_stats_collect("gbuffer_primary", st);
     return check_launch("k_gbuffer");
 }

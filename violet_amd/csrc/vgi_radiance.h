@@ -106,11 +106,11 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
 
 // raytrace.inc.hlsl:65-208.  `S` is the storage policy of the previous
 // frame's denoised diffuse (the screen-space radiance cache).
-template <class S>
+template <class S, bool kCount = false>
 __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt, const FrameParams& fp, f3 o, f3 d,
                                                               bool has_prev_frame, int32_t* stack) {
     RadianceTraceResult ret;
-    Hit hit = trace_closest(rt.scene, o, d, kRayTMin, kRayTMax, stack);
+    Hit hit = trace_closest<kCount>(rt.scene, o, d, kRayTMin, kRayTMax, stack);
     SurfaceHit sh = shade_hit(rt.scene, hit, d);
     if (sh.missed) {
         ret.radiance = rt.sky;
@@ -124,7 +124,7 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
     f3 sun_dir = xyz(fp.sun_dir);
     f3 sun_inten = xyz(fp.sun_inten);
     f3 radiance = mk3s(0.0f);
-    bool missed = !trace_any(rt.scene, sh.position, sun_dir, kRayTMin, kRayTMax, stack);
+    bool missed = !trace_any<kCount>(rt.scene, sh.position, sun_dir, kRayTMin, kRayTMax, stack);
     if (missed) {
         float pr = fmaxf(sh.roughness, 0.045f);
         pr = fmaxf(pr, 0.5f);  // RAYTRACE_INC_FIREFLY_SUPPRESS

@@ -130,6 +130,8 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     else
         vgi_renderer_default_config(&c);
     VGI_REQUIRE(c.storage == VGI_STORAGE_F32 || c.storage == VGI_STORAGE_F16, "vgi_renderer_create: bad storage");
+    VGI_REQUIRE(c.diff_spatial_taps <= 16u, "vgi_renderer_create: diff_spatial_taps %u > 16", c.diff_spatial_taps);
+    VGI_REQUIRE(c.spatial_radius_px >= 0.0f && c.spatial_radius_px <= 16384.0f, "vgi_renderer_create: bad spatial_radius_px");
     vgi_renderer* r = new (std::nothrow) vgi_renderer();
     if (!r) return set_error(VGI_E_NOMEM, "vgi_renderer_create: out of host memory");
     r->w = width;
@@ -291,13 +293,13 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
         // has_prev_frame = has_prev_diffuse_reservoir (restir_renderer.rs:605); the
         // previous depth exists whenever the previous reservoir does
         vgi_pc_ind_diff_temporal pc{frame_rand_next(frame_rand), (has_prev_diff_rsv && has_prev_depth) ? 1u : 0u,
-                                    has_new_sample};
+                                    has_new_sample, r->cfg.temporal_m_cap};
         CALL(vgi_pass_ind_diff_temporal_resample(fp, &io, &pc, stream));
         vgi_ind_diff_spatial_io sio{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(), r->ao_filtered.p,
                                     r->dtm_rsv.as<uint32_t>(), r->dtm_hit.as<uint32_t>(), r->dtm_rad.p,
                                     r->dsp_rsv[c].as<uint32_t>(), r->dsp_hit[c].as<uint32_t>(), r->dsp_rad[c].p,
                                     r->diff_light[c].p};
-        vgi_pc_frame_rand spc{frame_rand_next(frame_rand)};
+        vgi_pc_ind_diff_spatial spc{frame_rand_next(frame_rand), r->cfg.diff_spatial_taps, r->cfg.spatial_radius_px};
         CALL(vgi_pass_ind_diff_spatial_resample(fp, &sio, &spc, stream));
     }
 
@@ -323,12 +325,12 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
                                     r->depth[c].as<float>(), hp ? r->spec_rsv[p].as<uint32_t>() : nullptr,
                                     hp ? r->spec_hitpos[p].as<float>() : nullptr, hp ? r->spec_rad[p].p : nullptr,
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
-        vgi_pc_ind_spec_temporal pc{frame_rand_next(frame_rand), hp ? 1u : 0u};
+        vgi_pc_ind_spec_temporal pc{frame_rand_next(frame_rand), hp ? 1u : 0u, r->cfg.temporal_m_cap};
         CALL(vgi_pass_ind_spec_temporal_resample(fp, &io, &pc, stream));
         vgi_ind_spec_spatial_io sio{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p,
                                     r->spec_light.p, r->spec_raylen.p};
-        vgi_pc_frame_rand spc{frame_rand_next(frame_rand)};
+        vgi_pc_ind_spec_spatial spc{frame_rand_next(frame_rand), r->cfg.spatial_radius_px};
         CALL(vgi_pass_ind_spec_spatial_resample(fp, &sio, &spc, stream));
     }
 

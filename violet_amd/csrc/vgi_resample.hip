@@ -178,7 +178,8 @@ __global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_di
             rad = mk3s(0.0f);
             ex = ey = 0u;
         }
-        M = M < 16u ? M : 16u;
+        const uint32_t m_cap = pc.m_cap ? pc.m_cap : 16u;  // M_MAX
+        M = M < m_cap ? M : m_cap;
     }
     if (pc.has_new_sample) {
         float target = luminance(rad);
@@ -210,11 +211,13 @@ __global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_di
 // ---------------------------------------------------------------------------
 // Indirect diffuse spatial resampling (golden-spiral taps)
 // ---------------------------------------------------------------------------
+constexpr int kMaxSpiralTaps = 16;
 struct SpiralTable {
-    float theta[9];   // TWO_PI * (i+1)/9
-    float r[9];       // A * exp(B * theta_i)
-    float T;          // 0.6186 * 9
-    float cos_tol;    // cos(25 deg)
+    float theta[kMaxSpiralTaps];  // TWO_PI * (i+1)/n
+    float r[kMaxSpiralTaps];      // A * exp(B * theta_i)
+    float T;                      // 0.6186 * n
+    float cos_tol;                // cos(25 deg)
+    int n;                        // taps (MAX_ITERATION)
 };
 
 struct DiffWR {
@@ -241,7 +244,7 @@ __device__ __forceinline__ DiffWR diff_load(const vgi_ind_diff_spatial_io& io, s
 
 template <class S>
 __global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_diff_spatial_io io, uint32_t frame_rand,
-                                                     SpiralTable tab) {
+                                                     float radius0, SpiralTable tab) {
     uint32_t px, py;
     tile_pixel(px, py);
     const uint32_t w = io.width, h = io.height;
@@ -262,11 +265,10 @@ __global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_dif
     float ao = S::ld2(io.ao_texture, i).x;
     DiffWR res = diff_load<S>(io, i, position, nrm);
     float w_sum = (res.W * res.target) * (float)res.M;
-    float radius = (float)(w < h ? w : h) * 0.05f;
-    radius = fmaxf(radius * ao, 1.5f);
+    float radius = fmaxf(radius0 * ao, 1.5f);
     uint32_t rng = lcg_init_with_seed(px, py, frame_rand);
     float rand_angle = lcg_rand(rng) * kTwoPi;
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < tab.n; k++) {
         float sv, cv;
         vgi_sincosf(tab.theta[k] * tab.T + rand_angle, &sv, &cv);
         float x = tab.r[k] * cv;
@@ -419,7 +421,7 @@ __global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_sp
     f3 new_pos = mk3(nhp[0], nhp[1], nhp[2]);
     f3 new_rad = S::ld3(io.rw_hit_radiance_texture, i);
     GB g = load_gb(io.gbuffer_color, plane, i);
-    uint32_t M_max = f2u(16.0f * g.roughness);
+    uint32_t M_max = f2u((float)(pc.m_cap ? pc.m_cap : 16u) * g.roughness);
     M = M < M_max ? M : M_max;
     f3 f0 = get_specular_f0(g.color, g.metallic);
     SpecSampling cur = calc_things(xyz(fp.view_pos), position, g.normal, g.roughness, f0, new_pos);
@@ -476,7 +478,7 @@ struct SpecTF {
 
 template <class S>
 __global__ void __launch_bounds__(64) k_spec_spatial(FrameParams fp, vgi_ind_spec_spatial_io io, uint32_t frame_rand,
-                                                     float cos_tol) {
+                                                     float radius0, float cos_tol) {
     uint32_t px, py;
     tile_pixel(px, py);
     const uint32_t w = io.width, h = io.height;
@@ -500,7 +502,7 @@ __global__ void __launch_bounds__(64) k_spec_spatial(FrameParams fp, vgi_ind_spe
     SpecTF center{xyz(fp.view_pos), position, g.normal, f0, g.roughness};
     float res_tf = hp[3];
     const uint32_t num_iteration = f2u(0.0f + g.roughness * (4.0f - 0.0f));
-    float radius = (float)(w < h ? w : h) * 0.05f;
+    float radius = radius0;
     uint32_t rng = lcg_init_with_seed(px, py, frame_rand);
     for (uint32_t k = 0; k < num_iteration; k++) {
         float ux = lcg_rand(rng);
@@ -583,20 +585,26 @@ static FrameParams to_fp(const vgi_frame_params* fp) {
 }
 
 // Constants of the reference spiral (ind_diff_spatial_resample.hlsl:148-151,
-// 186), folded in double and rounded to f32 once.
-SpiralTable make_spiral_table() {
+// 186) for n taps, folded in double and rounded to f32 once.
+SpiralTable make_spiral_table(int n) {
     SpiralTable t;
     const double PI = (double)(float)3.14159265359;
     const float B = (float)(std::log(1.61803398) / (0.5 * PI));
     const float A = (float)(1.0 / std::exp((double)B * (double)(2.0f * (float)PI)));
-    for (int i = 0; i < 9; ++i) {
-        float theta = kTwoPi * (((float)i + 1.0f) / 9.0f);
+    for (int i = 0; i < kMaxSpiralTaps; ++i) {
+        float theta = i < n ? kTwoPi * (((float)i + 1.0f) / (float)n) : 0.0f;
         t.theta[i] = theta;
-        t.r[i] = (float)((double)A * std::exp((double)B * (double)theta));
+        t.r[i] = i < n ? (float)((double)A * std::exp((double)B * (double)theta)) : 0.0f;
     }
-    t.T = (float)(0.6186 * 9.0);
+    t.T = (float)(0.6186 * (double)n);
     t.cos_tol = (float)std::cos((double)(float)(PI * (25.0 / 180.0)));
+    t.n = n;
     return t;
+}
+
+// Initial spatial search radius: INIT_RADIUS_FACTOR * min(w, h) unless set.
+static float spatial_radius(float radius_px, uint32_t w, uint32_t h) {
+    return radius_px > 0.0f ? radius_px : (float)(w < h ? w : h) * 0.05f;
 }
 
 #define VGI_LAUNCH_S(mode, kernel, grid, ...)                                             \
@@ -658,7 +666,7 @@ extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params
 }
 
 extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
-                                                         const vgi_pc_frame_rand* pc, void* stream) {
+                                                         const vgi_pc_ind_diff_spatial* pc, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_spatial_resample: null argument");
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_diff_spatial_resample: bad size/storage");
@@ -666,12 +674,17 @@ extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params*
                     io->temp_hit_pos_normal_texture && io->temp_hit_radiance_texture && io->rw_reservoir_texture &&
                     io->rw_hit_pos_normal_texture && io->rw_hit_radiance_texture && io->rw_lighting_texture,
                 "vgi_pass_ind_diff_spatial_resample: missing binding");
-    static const SpiralTable tab = make_spiral_table();
+    VGI_REQUIRE(pc->num_taps <= (uint32_t)kMaxSpiralTaps, "vgi_pass_ind_diff_spatial_resample: num_taps %u > %d",
+                pc->num_taps, kMaxSpiralTaps);
+    VGI_REQUIRE(pc->radius_px >= 0.0f && pc->radius_px <= 16384.0f,
+                "vgi_pass_ind_diff_spatial_resample: bad radius_px");
+    const SpiralTable tab = make_spiral_table(pc->num_taps ? (int)pc->num_taps : 9);
     hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_diff_spatial_resample", st,
                    (double)io->width * io->height * (4 + 4 + pair_bytes(io->storage) / 2 + 2 * (4 + 8 + R) + R));
-    VGI_LAUNCH_S(io->storage, k_diff_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand, tab);
+    VGI_LAUNCH_S(io->storage, k_diff_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand,
+                 spatial_radius(pc->radius_px, io->width, io->height), tab);
     return check_launch("k_diff_spatial");
 }
 
@@ -695,19 +708,21 @@ extern "C" vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params
 }
 
 extern "C" vgi_status vgi_pass_ind_spec_spatial_resample(const vgi_frame_params* fp, const vgi_ind_spec_spatial_io* io,
-                                                         const vgi_pc_frame_rand* pc, void* stream) {
+                                                         const vgi_pc_ind_spec_spatial* pc, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_spec_spatial_resample: null argument");
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_spec_spatial_resample: bad size/storage");
     VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->reservoir_texture && io->hit_pos_texture &&
                     io->hit_radiance_texture && io->rw_lighting_texture && io->rw_ray_len_texture,
                 "vgi_pass_ind_spec_spatial_resample: missing binding");
-    static const SpiralTable tab = make_spiral_table();
+    VGI_REQUIRE(pc->radius_px >= 0.0f && pc->radius_px <= 16384.0f,
+                "vgi_pass_ind_spec_spatial_resample: bad radius_px");
+    static const SpiralTable tab = make_spiral_table(9);
     hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_spec_spatial_resample", st,
                    (double)io->width * io->height * (4 + 8 + 8 + 16 + R + R + scalar_bytes(io->storage)));
     VGI_LAUNCH_S(io->storage, k_spec_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand,
-                 tab.cos_tol);
+                 spatial_radius(pc->radius_px, io->width, io->height), tab.cos_tol);
     return check_launch("k_spec_spatial");
 }

@@ -44,6 +44,7 @@ struct Accum {
     double ms = 0;
     uint64_t count = 0;
     double bytes = 0;
+    double trace[4] = {0, 0, 0, 0};  // closest rays, any-hit rays, nodes, tris
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
@@ -109,7 +110,52 @@ ProfScope::~ProfScope() {
     g_prof_pending.push_back(Pending{id_, a_, b, bytes_});
 }
 
+// ---------------------------------------------------------------------------
+// traversal statistics
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_stats_mu;
+unsigned long long* g_stats_dev = nullptr;
+constexpr size_t kStatsWords = 4u * (size_t)kTraceCounterSlots;
+}  // namespace
+
+unsigned long long* trace_counters() {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    return g_stats_dev;
+}
+
+void trace_stats_collect(const char* name, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    if (!g_stats_dev) return;
+    unsigned long long host[kStatsWords];
+    if (hipStreamSynchronize(s) != hipSuccess) return;
+    if (hipMemcpy(host, g_stats_dev, sizeof(host), hipMemcpyDeviceToHost) != hipSuccess) return;
+    double sum[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < kStatsWords; ++i) sum[i & 3u] += (double)host[i];
+    memset(host, 0, sizeof(host));
+    (void)hipMemcpy(g_stats_dev, host, sizeof(host), hipMemcpyHostToDevice);
+    std::lock_guard<std::mutex> lp(g_prof_mu);
+    if (!g_prof_on) return;
+    Accum& a = g_prof_acc[name_id(name)];
+    for (int k = 0; k < 4; ++k) a.trace[k] += sum[k];
+}
+
 }  // namespace vgi
+
+extern "C" vgi_status vgi_trace_stats_enable(uint32_t enable) {
+    std::lock_guard<std::mutex> lk(vgi::g_stats_mu);
+    if (enable && !vgi::g_stats_dev) {
+        void* p = nullptr;
+        VGI_TRY_HIP(hipMalloc(&p, vgi::kStatsWords * sizeof(unsigned long long)));
+        VGI_TRY_HIP(hipMemset(p, 0, vgi::kStatsWords * sizeof(unsigned long long)));
+        vgi::g_stats_dev = (unsigned long long*)p;
+    } else if (!enable && vgi::g_stats_dev) {
+        VGI_TRY_HIP(hipDeviceSynchronize());
+        VGI_TRY_HIP(hipFree(vgi::g_stats_dev));
+        vgi::g_stats_dev = nullptr;
+    }
+    return VGI_OK;
+}
 
 extern "C" vgi_status vgi_profiler_enable(uint32_t enable) {
     std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
@@ -135,6 +181,10 @@ extern "C" int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap) {
             out[n].total_ms = a.ms;
             out[n].count = a.count;
             out[n].bytes = a.bytes;
+            out[n].rays_closest = a.trace[0];
+            out[n].rays_any = a.trace[1];
+            out[n].nodes = a.trace[2];
+            out[n].tris = a.trace[3];
         }
         ++n;
     }

@@ -47,6 +47,15 @@ class ProfScope {
     double bytes_ = 0;
 };
 
+// Traversal statistics: kTraceCounterSlots slots of {closest rays, any-hit
+// rays, interior nodes, triangles}, spread over slots to limit contention.
+constexpr int kTraceCounterSlots = 64;
+// Traversal statistics: the device counter buffer while they are enabled
+// (else null), and the per-pass collection after an instrumented launch
+// (waits for `s`, adds the counts to the pass's profiler entry, clears them).
+unsigned long long* trace_counters();
+void trace_stats_collect(const char* name, hipStream_t s);
+
 inline bool valid_dims(uint32_t w, uint32_t h) { return w > 0 && h > 0 && w <= kMaxDim && h <= kMaxDim; }
 
 inline dim3 tile_grid(uint32_t w, uint32_t h) { return dim3((w + 7) / 8, (h + 7) / 8, 1); }

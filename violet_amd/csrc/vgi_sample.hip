@@ -25,7 +25,7 @@ __device__ __forceinline__ f3 position_biased(const float* m, uint32_t px, uint3
     return depth_to_position(m, px, py, w, h, depth + kDepthError);
 }
 
-template <class S>
+template <class S, bool kCount>
 __global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io,
                                                         vgi_pc_sample_gen pc) {
     VGI_TRAVERSAL_STACK(stack);
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes
     float u0 = lcg_rand(rng);
     float u1 = lcg_rand(rng);
     f3 dir = sample_hemisphere_uniform_with_normal(u0, u1, g.normal);
-    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
     uint32_t ex, ey;
     hit_encode(r.position_ws, r.normal_ws, ex, ey);
     io.rw_hit_pos_normal_texture[2 * i] = ex;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes
     S::st3(io.rw_hit_radiance_texture, i, r.radiance);
 }
 
-template <class S>
+template <class S, bool kCount>
 __global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, TraceRes rt,
                                                              vgi_ind_diff_sample_validate_io io) {
     VGI_TRAVERSAL_STACK(stack);
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, Tra
     f3 prev_rad = S::ld3(io.rw_hit_radiance_texture, i);
     f3 origin = depth_to_position(fp.prev_inv_view_proj, px, py, w, h, depth);
     f3 dir = normalize3(hit.pos - origin);
-    RadianceTraceResult r = trace_radiance<S>(rt, fp, origin, dir, true, stack);
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, origin, dir, true, stack);
     float lumi_prev = luminance(prev_rad);
     float lumi_diff = luminance(abs3(r.radiance - prev_rad));
     float prev_ray_len = length3(hit.pos - origin);
@@ -120,7 +120,7 @@ __device__ __forceinline__ f3 sample_spec_dir(f3 view_dir, f3 normal, float roug
     return rotate_point(invert_rotation(rot), L);
 }
 
-template <class S>
+template <class S, bool kCount>
 __global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io,
                                                         vgi_pc_sample_gen pc) {
     VGI_TRAVERSAL_STACK(stack);
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes
     f3 view_dir = normalize3(xyz(fp.view_pos) - position);
     float roughness = fmaxf(g.perceptual_roughness * g.perceptual_roughness, 0.002f);
     f3 dir = sample_spec_dir(view_dir, g.normal, roughness, rng);
-    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
     float* hp = io.rw_hit_pos_texture + 4 * i;
     hp[0] = r.position_ws.x;
     hp[1] = r.position_ws.y;
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes
     S::st3(io.rw_hit_radiance_texture, i, r.radiance);
 }
 
-template <class S>
+template <class S, bool kCount>
 __global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, TraceRes rt,
                                                              vgi_ind_spec_sample_validate_io io) {
     VGI_TRAVERSAL_STACK(stack);
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, Tra
     float prev_target = hp[3];
     f3 prev_rad = S::ld3(io.rw_prev_hit_radiance_texture, i);
     f3 dir = normalize3(prev_hit_pos - position);
-    RadianceTraceResult r = trace_radiance<S>(rt, fp, position, dir, true, stack);
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, true, stack);
     float max_rad = fmaxf(component_max(r.radiance), component_max(prev_rad));
     f3 dr = abs3(r.radiance - prev_rad);
     float lim = 0.5f * max_rad;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, Tra
     }
 }
 
-template <class S>
+template <class S, bool kCount>
 __global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io,
                                                          uint32_t frame_hash) {
     VGI_TRAVERSAL_STACK(stack);
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRe
     float u1 = lcg_rand(rng);
     f3 local = sample_hemisphere_cosine(u0, u1);
     f3 dir = rotate_point(invert_rotation(get_rotation_to_z_from(normal)), local);
-    RadianceTraceResult r = trace_radiance<S>(rt, fp, mk3(q.px, q.py, q.pz), dir, true, stack);
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, mk3(q.px, q.py, q.pz), dir, true, stack);
     HashCell* storage = reinterpret_cast<HashCell*>(io.rw_hash_grid_storage_buffer);
     f3 prev = mk3(storage[cell].r, storage[cell].g, storage[cell].b);
     const float bf = 1.0f / 16.0f;
@@ -223,6 +223,7 @@ static vgi_status make_trace_res(const vgi_trace_resources* r, bool need_prev, T
         VGI_REQUIRE(r->prev_indirect_diffuse_texture && r->prev_depth_texture,
                     "trace resources: has_prev_frame=1 but prev_* is NULL");
     t.scene = scene_view(r->scene_tlas);
+    t.scene.counters = trace_counters();
     t.sky = mk3(r->sky_radiance[0], r->sky_radiance[1], r->sky_radiance[2]);
     t.prev_indirect_diffuse = r->prev_indirect_diffuse_texture;
     t.prev_depth = r->prev_depth_texture;
@@ -241,14 +242,23 @@ static FrameParams to_fp(const vgi_frame_params* fp) {
     return f;
 }
 
-// Every kernel here traces rays: launch with the scene's LDS traversal stack.
-#define VGI_DISPATCH_STORAGE(mode, kernel, grid, ...)                                                     \
-    do {                                                                                                  \
-        const size_t lds_ = traversal_lds_bytes(t.scene);                                                 \
-        if ((mode) == VGI_STORAGE_F32)                                                                    \
-            hipLaunchKernelGGL(kernel<StorageF32>, grid, dim3(64), lds_, st, __VA_ARGS__);                \
-        else                                                                                              \
-            hipLaunchKernelGGL(kernel<StorageF16>, grid, dim3(64), lds_, st, __VA_ARGS__);                \
+// Every kernel here traces rays: launch with the scene's LDS traversal stack,
+// and the instrumented build when traversal statistics are on.
+#define VGI_DISPATCH_STORAGE(mode, kernel, grid, ...)                                                      \
+    do {                                                                                                   \
+        const size_t lds_ = traversal_lds_bytes(t.scene);                                                  \
+        const bool f32_ = (mode) == VGI_STORAGE_F32;                                                       \
+        if (t.scene.counters) {                                                                            \
+            if (f32_)                                                                                      \
+                hipLaunchKernelGGL((kernel<StorageF32, true>), grid, dim3(64), lds_, st, __VA_ARGS__);     \
+            else                                                                                           \
+                hipLaunchKernelGGL((kernel<StorageF16, true>), grid, dim3(64), lds_, st, __VA_ARGS__);     \
+        } else {                                                                                           \
+            if (f32_)                                                                                      \
+                hipLaunchKernelGGL((kernel<StorageF32, false>), grid, dim3(64), lds_, st, __VA_ARGS__);    \
+            else                                                                                           \
+                hipLaunchKernelGGL((kernel<StorageF16, false>), grid, dim3(64), lds_, st, __VA_ARGS__);    \
+        }                                                                                                  \
     } while (0)
 
 }  // namespace vgi
@@ -269,6 +279,7 @@ extern "C" vgi_status vgi_pass_ind_diff_sample_gen(const vgi_frame_params* fp, c
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_diff_sample_gen", st, (double)io->width * io->height * (8 + 4 + 8 + radiance_bytes(io->storage)));
     VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    trace_stats_collect("ind_diff_sample_gen", st);
     return check_launch("k_diff_sample_gen");
 }
 
@@ -287,6 +298,7 @@ extern "C" vgi_status vgi_pass_ind_diff_sample_validate(const vgi_frame_params* 
     ProfScope prof("ind_diff_sample_validate", st,
                    (double)io->width * io->height * (4 + 2 * (4 + 8 + radiance_bytes(io->storage))));
     VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    trace_stats_collect("ind_diff_sample_validate", st);
     return check_launch("k_diff_sample_validate");
 }
 
@@ -304,6 +316,7 @@ extern "C" vgi_status vgi_pass_ind_spec_sample_gen(const vgi_frame_params* fp, c
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_spec_sample_gen", st, (double)io->width * io->height * (4 + 4 + 16 + radiance_bytes(io->storage)));
     VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    trace_stats_collect("ind_spec_sample_gen", st);
     return check_launch("k_spec_sample_gen");
 }
 
@@ -321,6 +334,7 @@ extern "C" vgi_status vgi_pass_ind_spec_sample_validate(const vgi_frame_params* 
     ProfScope prof("ind_spec_sample_validate", st,
                    (double)io->width * io->height * (4 + 2 * (8 + 16 + radiance_bytes(io->storage))));
     VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    trace_stats_collect("ind_spec_sample_validate", st);
     return check_launch("k_spec_sample_validate");
 }
 
@@ -339,5 +353,6 @@ extern "C" vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, cons
     ProfScope prof("hash_grid_raygen", st, 0.0);
     VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kHashGridMaxQueries / 64), to_fp(fp), t, *io,
                          pc->frame_hash);
+    trace_stats_collect("hash_grid_raygen", st);
     return check_launch("k_hash_grid_raygen");
 }

@@ -51,9 +51,18 @@ __device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, int32_t i) {
     return n;
 }
 
+__device__ __forceinline__ void count_traversal(const SceneView& s, bool any_hit, uint32_t nodes, uint32_t tris) {
+    unsigned long long* c = s.counters + 4u * ((blockIdx.x + 7u * blockIdx.y) % (unsigned)kTraceCounterSlots);
+    atomicAdd(c + (any_hit ? 1 : 0), 1ull);
+    atomicAdd(c + 2, (unsigned long long)nodes);
+    atomicAdd(c + 3, (unsigned long long)tris);
+}
+
 // Closest hit over [tmin, tmax]; ties in t resolve to the lowest triangle id.
 // `stack` points at this lane's slot 0 (stride 64 between entries).
-template <bool kAnyHit>
+// kCount: the instrumented build (traversal statistics) that adds this ray's
+// node and triangle counts to s.counters.
+template <bool kAnyHit, bool kCount = false>
 __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
     RayInv r = make_ray(o, d);
     Hit h;
@@ -65,8 +74,10 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
     // Each node and leaf of a tree is reached at most once; the budget only
     // guards against a corrupted BVH turning into an endless loop.
     uint32_t budget = 2u * s.num_nodes + 8u;
+    uint32_t n_nodes = 0, n_tris = 0;
     while (budget-- != 0u) {
         if (link >= 0) {
+            if (kCount) ++n_nodes;
             BvhNode n = load_node(s.nodes, link);
             float t0 = slab(r, n.c0xy.x, n.c0xy.y, n.c0xy.z, n.c0xy.w, n.z.x, n.z.y, tmin, h.t);
             float t1 = slab(r, n.c1xy.x, n.c1xy.y, n.c1xy.z, n.c1xy.w, n.z.z, n.z.w, tmin, h.t);
@@ -92,6 +103,7 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             uint32_t enc = (uint32_t)(~link);
             uint32_t first = enc >> 4, cnt = enc & 15u;
             for (uint32_t k = 0; k < cnt; ++k) {
+                if (kCount) ++n_tris;
                 const float4* tp = reinterpret_cast<const float4*>(s.tris + first + k);
                 float4 a = tp[0], e1 = tp[1], e2 = tp[2];
                 float t, u, v;
@@ -103,7 +115,10 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
                         h.u = u;
                         h.v = v;
                         h.tri = id;
-                        if (kAnyHit) return h;
+                        if (kAnyHit) {
+                            if (kCount) count_traversal(s, true, n_nodes, n_tris);
+                            return h;
+                        }
                     }
                 }
             }
@@ -112,15 +127,18 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
         --sp;
         link = stack[sp * 64];
     }
+    if (kCount) count_traversal(s, kAnyHit, n_nodes, n_tris);
     return h;
 }
 
+template <bool kCount = false>
 __device__ __forceinline__ Hit trace_closest(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
-    return traverse<false>(s, o, d, tmin, tmax, stack);
+    return traverse<false, kCount>(s, o, d, tmin, tmax, stack);
 }
 // Shadow ray: true if something is hit on [tmin, tmax] (trace_shadow returns !this).
+template <bool kCount = false>
 __device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
-    return traverse<true>(s, o, d, tmin, tmax, stack).tri != 0xFFFFFFFFu;
+    return traverse<true, kCount>(s, o, d, tmin, tmax, stack).tri != 0xFFFFFFFFu;
 }
 
 // Per-wave LDS traversal stack: SceneView::stack_depth entries x 64 lanes in

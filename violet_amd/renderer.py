@@ -28,6 +28,12 @@ class RestirConfig:
     hash_grid_cache_decay: bool = True
     ao_radius: float = 2.0
     sky_radiance: tuple = (1.0, 1.0, 1.0)
+    # resampling parameters, 0 = the reference's constants: temporal M clamp
+    # (16; specular uint(cap * roughness)), diffuse spatial spiral taps (9),
+    # spatial search radius in pixels (0.05 * min(w, h))
+    temporal_m_cap: int = 0
+    diff_spatial_taps: int = 0
+    spatial_radius_px: float = 0.0
     disocclusion_fix: bool = True
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
@@ -41,6 +47,9 @@ class RestirConfig:
         c.hash_grid_cache_decay = int(self.hash_grid_cache_decay)
         c.ao_radius = self.ao_radius
         c.sky_radiance[:] = list(self.sky_radiance)
+        c.temporal_m_cap = self.temporal_m_cap
+        c.diff_spatial_taps = self.diff_spatial_taps
+        c.spatial_radius_px = self.spatial_radius_px
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
                                          int(self.atrous_iterations))
         return c
