@@ -134,6 +134,12 @@ typedef struct vgi_denoise_temporal_io {
     uint8_t* rw_history_len_texture;
     void* rw_diff_fast_history_texture;      /* written only if fast_history */
     void* rw_spec_fast_history_texture;      /* written only if fast_history */
+    /* FP16 storage only, both NULL or both set: the low f16 halves of the
+     * compensated moments (moment = hi + lo, hi in the moments texture,
+     * lo = f16(m - hi) the rounding residual).  NULL = plain f16 moments.
+     * No reference counterpart (the reference stores R16G16_SFLOAT). */
+    const void* moments_comp_history_texture; /* pair; may be NULL if !has_history */
+    void* rw_moments_comp_texture;            /* pair */
 } vgi_denoise_temporal_io;
 
 /* svgf_temporal_filter.hlsl:41-47 (host: denoising.rs:157-159). */
@@ -161,6 +167,11 @@ typedef struct vgi_denoise_spatial_io {
     void* rw_diff_texture;
     void* rw_spec_texture;
     void* rw_moments_or_variance_texture;
+    /* FP16 compensated moments (see vgi_denoise_temporal_io): read by both
+     * passes when set; the disocclusion fix also writes the residual of its
+     * output moments to rw_moments_comp_texture (NULL for the fallback). */
+    const void* moments_comp_texture;
+    void* rw_moments_comp_texture;
 } vgi_denoise_spatial_io;
 
 vgi_status vgi_pass_denoise_disocclusion_fix(const vgi_denoise_spatial_io* io, void* stream);
@@ -211,6 +222,7 @@ typedef struct vgi_denoiser_config {
     uint32_t disocclusion_fix;   /* default 1 (denoising.rs:54) */
     uint32_t fast_history_clamp; /* default 0 (denoising.rs:55) */
     uint32_t atrous_iterations;  /* default 3 (denoising.rs:56), clamped to 0..5 */
+    uint32_t compensated_moments; /* FP16 storage: keep moments as hi + lo f16 pairs (no reference counterpart) */
 } vgi_denoiser_config;
 
 /* denoising.rs:10-18 `Input` */
@@ -617,6 +629,35 @@ vgi_status vgi_renderer_get_image(const vgi_renderer* r, const char* name, void*
 size_t vgi_renderer_state_size(const vgi_renderer* r);
 vgi_status vgi_renderer_export_state(const vgi_renderer* r, void* host_blob, size_t bytes, void* stream);
 vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* host_blob, size_t bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched views.  No reference counterpart: violet renders one view per
+ * frame (restir_render_loop.rs:262-268).  Each view is a vgi_renderer with
+ * its own reservoirs, denoiser history and radiance cache; a batched frame
+ * advances every given view by one frame exactly as vgi_renderer_add_passes
+ * would, but runs each pass of all views as ONE persistent launch (a fixed
+ * grid walking the tiles of every view), optionally replayed from a
+ * captured HIP graph.  All views of a call share the scene and the storage
+ * mode; a view appears at most once per call.  The views are externally
+ * synchronized like single-view renderers.
+ * ------------------------------------------------------------------------ */
+typedef struct vgi_batch vgi_batch;
+typedef struct vgi_batch_config {
+    uint32_t max_views;      /* views per call, 1..64 */
+    uint32_t pixel_groups;   /* persistent 64-lane groups of the pixel and cache passes */
+    uint32_t trace_groups;   /* persistent 64-lane groups of the ray-tracing passes */
+    uint32_t atrous_groups;  /* persistent 256-lane groups of the a-trous passes */
+    uint32_t use_graph;      /* 1: capture the frame once as a HIP graph and replay it */
+} vgi_batch_config;
+/* Defaults: 64 views, 8 / 16 / 4 groups per compute unit, no graph. */
+vgi_status vgi_batch_default_config(vgi_batch_config* cfg);
+vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** out);
+vgi_status vgi_batch_destroy(vgi_batch* b);
+/* One frame of `num_views` views: views[i] renders fps[i] as frame
+ * frame_indices[i] and reports into outs[i].  Ordered on `stream`. */
+vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene, uint32_t num_views, vgi_renderer* const* views,
+                                const vgi_frame_params* fps, const uint32_t* frame_indices, vgi_renderer_output* outs,
+                                void* stream);
 
 #ifdef __cplusplus
 }

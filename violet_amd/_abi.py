@@ -82,6 +82,9 @@ RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
     ("validation_frame", u32), ("hash_grid_updated", u32)])
 
+BatchConfig = _struct("BatchConfig", [("max_views", u32), ("pixel_groups", u32), ("trace_groups", u32),
+                                     ("atrous_groups", u32), ("use_graph", u32)])
+
 FPp = P(FrameParams)
 TRp = P(TraceResources)
 _SIGS = {
@@ -109,6 +112,10 @@ _SIGS = {
     "vgi_renderer_get_image": [vp, C.c_char_p, P(vp), P(C.c_size_t)],
     "vgi_renderer_export_state": [vp, vp, C.c_size_t, vp],
     "vgi_renderer_import_state": [vp, vp, C.c_size_t, vp],
+    "vgi_batch_default_config": [P(BatchConfig)],
+    "vgi_batch_create": [P(BatchConfig), P(vp)],
+    "vgi_batch_destroy": [vp],
+    "vgi_batch_add_passes": [vp, vp, u32, P(vp), FPp, P(u32), P(RendererOutput), vp],
 }
 for _n, _a in _SIGS.items():
     _lib.register(_n, C.c_int32, _a)

@@ -67,6 +67,7 @@ class DenoiseTemporalIO(C.Structure):
         ("rw_diff_texture", vp), ("rw_spec_texture", vp), ("rw_moments_texture", vp),
         ("rw_history_len_texture", vp),
         ("rw_diff_fast_history_texture", vp), ("rw_spec_fast_history_texture", vp),
+        ("moments_comp_history_texture", vp), ("rw_moments_comp_texture", vp),
     ]
 
 
@@ -80,6 +81,7 @@ class DenoiseSpatialIO(C.Structure):
         ("depth_texture", vp), ("gbuffer_texture", vp), ("history_len_texture", vp),
         ("diff_texture", vp), ("spec_texture", vp), ("moments_texture", vp),
         ("rw_diff_texture", vp), ("rw_spec_texture", vp), ("rw_moments_or_variance_texture", vp),
+        ("moments_comp_texture", vp), ("rw_moments_comp_texture", vp),
     ]
 
 
@@ -105,7 +107,8 @@ class PCDenoiseAtrous(C.Structure):
 
 
 class DenoiserConfig(C.Structure):
-    _fields_ = [("disocclusion_fix", u32), ("fast_history_clamp", u32), ("atrous_iterations", u32)]
+    _fields_ = [("disocclusion_fix", u32), ("fast_history_clamp", u32), ("atrous_iterations", u32),
+                ("compensated_moments", u32)]
 
 
 class DenoiserInput(C.Structure):

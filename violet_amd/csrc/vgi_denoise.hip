@@ -19,6 +19,7 @@
  */
 #include <algorithm>
 
+#include "vgi_batch_kernel.h"
 #include "vgi_runtime.h"
 #include "vgi_storage.h"
 
@@ -78,6 +79,28 @@ __device__ __forceinline__ f3 load_normal(const uint32_t* gb, size_t plane, size
     return decode_gbuffer_normal(gb[plane + i]).normal;
 }
 
+// Second moments.  With FP16 storage and a residual texture `lo`, a moment
+// is kept compensated as hi + lo: hi = f16(m) and lo = f16(m - hi), which
+// carries the bits the f16 rounding of the EMA drops (about 22 significant
+// bits instead of 11 for values well inside the f16 normal range).
+template <class S>
+__device__ __forceinline__ f2 ld_moments(const void* hi, const void* lo, size_t i) {
+    f2 m = S::ld2(hi, i);
+    if (S::kMode == StorageF16::kMode && lo) {
+        f2 c = StorageF16::ld2(lo, i);
+        m = mk2(m.x + c.x, m.y + c.y);
+    }
+    return m;
+}
+template <class S>
+__device__ __forceinline__ void st_moments(void* hi, void* lo, size_t i, f2 m) {
+    S::st2(hi, i, m);
+    if (S::kMode == StorageF16::kMode && lo) {
+        const float hx = f16tof32(f32tof16(m.x)), hy = f16tof32(f32tof16(m.y));  // the stored high half
+        StorageF16::st2(lo, i, mk2(m.x - hx, m.y - hy));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Temporal filter (svgf_temporal_filter.hlsl)
 // ---------------------------------------------------------------------------
@@ -87,10 +110,9 @@ __device__ __forceinline__ float dsr_weight(f3 color, f3 mean, f3 deviation) {
 }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_temporal_io io,
-                                                 vgi_pc_denoise_temporal pc) {
+__device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi_denoise_temporal_io& io, const vgi_pc_denoise_temporal& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -101,7 +123,7 @@ __global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_tem
         io.rw_history_len_texture[i] = 0;
         S::st3(io.rw_diff_texture, i, mk3s(0.0f));
         S::st3(io.rw_spec_texture, i, mk3s(0.0f));
-        S::st2(io.rw_moments_texture, i, mk2(0.0f, 0.0f));
+        st_moments<S>(io.rw_moments_texture, io.rw_moments_comp_texture, i, mk2(0.0f, 0.0f));
         return;
     }
 
@@ -201,7 +223,7 @@ __global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_tem
                     wsum += wgt;
                     dsum = dsum + S::ld3(io.diff_history_texture, j) * wgt;
                     ssum = ssum + S::ld3(io.spec_history_texture, j) * wgt;
-                    f2 mh = S::ld2(io.moments_history_texture, j);
+                    f2 mh = ld_moments<S>(io.moments_history_texture, io.moments_comp_history_texture, j);
                     msum = mk2(msum.x + wgt * mh.x, msum.y + wgt * mh.y);
                     lsum += wgt * history_len_load(io.history_len_texture[j]);
                     if (fast) {
@@ -266,12 +288,17 @@ __global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_tem
 
     S::st3(io.rw_diff_texture, i, diff_filtered);
     S::st3(io.rw_spec_texture, i, spec_filtered);
-    S::st2(io.rw_moments_texture, i, mom);
+    st_moments<S>(io.rw_moments_texture, io.rw_moments_comp_texture, i, mom);
     io.rw_history_len_texture[i] = history_len_store(history_len);
     if (fast) {
         S::st3(io.rw_diff_fast_history_texture, i, diff_filtered_fast);
         S::st3(io.rw_spec_fast_history_texture, i, spec_filtered_fast);
     }
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_temporal(FrameParams fp, vgi_denoise_temporal_io io, vgi_pc_denoise_temporal pc) {
+    k_temporal_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -287,9 +314,9 @@ __device__ __forceinline__ float quad_max(float v) {
 // Disocclusion fix (relax_disocclusion_fix.hlsl)
 // ---------------------------------------------------------------------------
 template <class S>
-__global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) {
+__device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io& io, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
@@ -300,7 +327,7 @@ __global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) 
     if (len == 0.0f) {
         S::st3(io.rw_diff_texture, i, mk3s(0.0f));
         S::st3(io.rw_spec_texture, i, mk3s(0.0f));
-        S::st2(io.rw_moments_or_variance_texture, i, mk2(0.0f, 0.0f));
+        st_moments<S>(io.rw_moments_or_variance_texture, io.rw_moments_comp_texture, i, mk2(0.0f, 0.0f));
         return;
     }
     len = qmax;
@@ -329,7 +356,7 @@ __global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) 
                 float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
                 f3 sv = S::ld3(io.spec_texture, j);
-                f2 mv = S::ld2(io.moments_texture, j);
+                f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
                 wsum = mk2(wsum.x + wt, wsum.y + wt);
                 dsum = dsum + dv * wt;
                 ssum = ssum + sv * wt;
@@ -339,21 +366,28 @@ __global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) 
         const float iw = __builtin_amdgcn_rcpf(wsum.x);
         S::st3(io.rw_diff_texture, i, dsum * iw);
         S::st3(io.rw_spec_texture, i, ssum * iw);
-        S::st2(io.rw_moments_or_variance_texture, i, mk2(msum.x * iw, msum.y * iw));
+        st_moments<S>(io.rw_moments_or_variance_texture, io.rw_moments_comp_texture, i,
+                      mk2(msum.x * iw, msum.y * iw));
     } else {
         S::st3(io.rw_diff_texture, i, S::ld3(io.diff_texture, i));
         S::st3(io.rw_spec_texture, i, S::ld3(io.spec_texture, i));
-        S::st2(io.rw_moments_or_variance_texture, i, S::ld2(io.moments_texture, i));
+        st_moments<S>(io.rw_moments_or_variance_texture, io.rw_moments_comp_texture, i,
+                      ld_moments<S>(io.moments_texture, io.moments_comp_texture, i));
     }
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) {
+    k_disocclusion_body<S>(io, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
 // Spatial fallback / variance estimation (svgf_spatial_fallback_filter.hlsl)
 // ---------------------------------------------------------------------------
 template <class S>
-__global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
+__device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
@@ -393,7 +427,7 @@ __global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
                 float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
                 f3 sv = S::ld3(io.spec_texture, j);
-                f2 mv = S::ld2(io.moments_texture, j);
+                f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
                 wsum += wt;
                 dsum = dsum + dv * wt;
                 ssum = ssum + sv * wt;
@@ -407,7 +441,7 @@ __global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
     } else {
         diff = S::ld3(io.diff_texture, i);
         spec = S::ld3(io.spec_texture, i);
-        sm = S::ld2(io.moments_texture, i);
+        sm = ld_moments<S>(io.moments_texture, io.moments_comp_texture, i);
     }
     float l1x = luminance(diff), l1y = luminance(spec);
     f2 var = mk2(fmaxf(sm.x - l1x * l1x, 0.0f), fmaxf(sm.y - l1y * l1y, 0.0f));
@@ -416,6 +450,11 @@ __global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
     S::st3(io.rw_diff_texture, i, diff);
     S::st3(io.rw_spec_texture, i, spec);
     S::st2(io.rw_moments_or_variance_texture, i, var);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
+    k_fallback_body<S>(io, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -434,9 +473,9 @@ __device__ __forceinline__ f3 ycocg_to_rgb(f3 c) {
 __device__ __forceinline__ f3 clamp3(f3 v, f3 lo, f3 hi) { return min3v(max3v(v, lo), hi); }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io) {
+__device__ __forceinline__ void k_fast_clamp_body(const vgi_denoise_fast_clamp_io& io, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -468,6 +507,11 @@ __global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io)
     S::st3(io.rw_spec_texture, i, ycocg_to_rgb(s));
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io) {
+    k_fast_clamp_body<S>(io, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // A-trous iteration (svgf_atrous_filter.hlsl).  Taps with x != 0 AND y != 0
 // plus the explicit centre (the reference's `not_center` test drops the
@@ -491,12 +535,12 @@ constexpr int kAtrousN = kAtrousF * kAtrousF;  // footprint entries
 enum AtrousField { kFDepth, kFNx, kFNy, kFNz, kFLd, kFLs, kFDr, kFDg, kFDb, kFSr, kFSg, kFSb, kFVx, kFVy, kFCount };
 
 template <class S>
-__global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
+__device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, int step, uint32_t nbx, uint32_t nby, uint32_t bx_, uint32_t by_) {
     __shared__ float lds[kFCount][kAtrousN];
     const int w = (int)io.width, h = (int)io.height;
     const size_t plane = (size_t)w * h;
-    const int rx = (int)(blockIdx.x / nbx), bx = (int)(blockIdx.x % nbx);
-    const int ry = (int)(blockIdx.y / nby), by = (int)(blockIdx.y % nby);
+    const int rx = (int)(bx_ / nbx), bx = (int)(bx_ % nbx);
+    const int ry = (int)(by_ / nby), by = (int)(by_ % nby);
     const int t = (int)threadIdx.x;
 
     for (int e = t; e < kAtrousN; e += 256) {
@@ -574,6 +618,11 @@ __global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int st
     S::st2(io.rw_variance_texture, i, mk2(vsum.x * (id * id), vsum.y * (is * is)));
 }
 
+template <class S>
+__global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
+    k_atrous_body<S>(io, step, nbx, nby, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -626,20 +675,135 @@ vgi_status launch_denoise_fast_clamp(const vgi_denoise_fast_clamp_io& io, hipStr
         hipLaunchKernelGGL(k_fast_clamp<StorageF16>, g, dim3(64), 0, st, io);
     return check_launch("k_fast_clamp");
 }
+// Lattice blocks of one a-trous launch.  A step at or beyond the larger image
+// edge puts every tap outside the image, exactly as that edge does: it is
+// clamped there to bound the residue count.
+struct AtrousGrid {
+    uint32_t s, nbx, nby;
+};
+static AtrousGrid atrous_grid(uint32_t w, uint32_t h, uint32_t step) {
+    AtrousGrid g;
+    g.s = step < std::max(w, h) ? step : std::max(w, h);
+    const uint32_t lw = (w + g.s - 1) / g.s, lh = (h + g.s - 1) / g.s;  // lattice extent of residue 0
+    g.nbx = (lw + kAtrousB - 1) / kAtrousB;
+    g.nby = (lh + kAtrousB - 1) / kAtrousB;
+    return g;
+}
+
 vgi_status launch_denoise_atrous(const vgi_denoise_atrous_io& io, int step, hipStream_t st) {
     // reads depth 4, L1+L2 8, diff, spec, variance; writes diff, spec, variance
     ProfScope prof("denoise_atrous", st, px_bytes(io.storage, 4, 2, 0, 12) * io.width * io.height);
-    // A step at or beyond the larger image edge puts every tap outside the
-    // image, exactly as that edge does: clamp it to bound the residue count.
-    const uint32_t s = (uint32_t)step < std::max(io.width, io.height) ? (uint32_t)step : std::max(io.width, io.height);
-    const uint32_t lw = (io.width + s - 1) / s, lh = (io.height + s - 1) / s;  // lattice extent of residue 0
-    const uint32_t nbx = (lw + kAtrousB - 1) / kAtrousB, nby = (lh + kAtrousB - 1) / kAtrousB;
-    dim3 g(nbx * s, nby * s, 1);
+    const AtrousGrid ag = atrous_grid(io.width, io.height, (uint32_t)step);
+    dim3 g(ag.nbx * ag.s, ag.nby * ag.s, 1);
     if (io.storage == VGI_STORAGE_F32)
-        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(256), 0, st, io, (int)s, nbx, nby);
+        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(256), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
     else
-        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(256), 0, st, io, (int)s, nbx, nby);
+        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(256), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
     return check_launch("k_atrous");
+}
+
+// ---------------------------------------------------------------------------
+// Fused multi-view launches (vgi_batch.h)
+// ---------------------------------------------------------------------------
+template <class S>
+struct JobDenTemporal {
+    FrameParams fp;
+    vgi_denoise_temporal_io io;
+    vgi_pc_denoise_temporal pc;
+    __device__ static void run(const JobDenTemporal& j, uint32_t bx, uint32_t by) {
+        k_temporal_body<S>(j.fp, j.io, j.pc, bx, by);
+    }
+};
+template <class S>
+struct JobDenDisocclusion {
+    vgi_denoise_spatial_io io;
+    __device__ static void run(const JobDenDisocclusion& j, uint32_t bx, uint32_t by) {
+        k_disocclusion_body<S>(j.io, bx, by);
+    }
+};
+template <class S>
+struct JobDenFallback {
+    vgi_denoise_spatial_io io;
+    __device__ static void run(const JobDenFallback& j, uint32_t bx, uint32_t by) { k_fallback_body<S>(j.io, bx, by); }
+};
+template <class S>
+struct JobDenFastClamp {
+    vgi_denoise_fast_clamp_io io;
+    __device__ static void run(const JobDenFastClamp& j, uint32_t bx, uint32_t by) {
+        k_fast_clamp_body<S>(j.io, bx, by);
+    }
+};
+template <class S>
+struct JobDenAtrous {
+    vgi_denoise_atrous_io io;
+    int step;
+    uint32_t nbx, nby;
+    __device__ static void run(const JobDenAtrous& j, uint32_t bx, uint32_t by) {
+        k_atrous_body<S>(j.io, j.step, j.nbx, j.nby, bx, by);
+    }
+};
+
+static vgi_status fill_den_temporal(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDenTemporal>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        memcpy(&j.fp, &r.fp, sizeof(j.fp));
+        j.io = r.io.den_temporal;
+        j.pc = r.pc.den_temporal;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+template <template <class> class J>
+static vgi_status fill_den_spatial(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<J>(host);
+    for (int v = 0; v < n; ++v) {
+        t->jobs[v].io = recs[v]->io.den_spatial;
+        batch_add_pixel_tiles(t->hd, v, recs[v]->io.den_spatial.width, recs[v]->io.den_spatial.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_den_fast_clamp(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDenFastClamp>(host);
+    for (int v = 0; v < n; ++v) {
+        t->jobs[v].io = recs[v]->io.den_fast_clamp;
+        batch_add_pixel_tiles(t->hd, v, recs[v]->io.den_fast_clamp.width, recs[v]->io.den_fast_clamp.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_den_atrous(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDenAtrous>(host);
+    for (int v = 0; v < n; ++v) {
+        const vgi_denoise_atrous_io& io = recs[v]->io.den_atrous;
+        const int step = recs[v]->pc.den_atrous.step_size;
+        VGI_REQUIRE(step >= 1, "batched a-trous: bad step %d", step);
+        const AtrousGrid g = atrous_grid(io.width, io.height, (uint32_t)step);
+        t->jobs[v] = {io, (int)g.s, g.nbx, g.nby};
+        batch_add_tiles(t->hd, v, g.nbx * g.s, g.nby * g.s);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_denoise(int kind) {
+    static const BatchOps temporal{sizeof(BatchTable<JobDenTemporal<StorageF32>>), fill_den_temporal,
+                                   batch_launch<JobDenTemporal>, 64, false};
+    static const BatchOps disocclusion{sizeof(BatchTable<JobDenDisocclusion<StorageF32>>),
+                                       fill_den_spatial<JobDenDisocclusion>, batch_launch<JobDenDisocclusion>, 64, false};
+    static const BatchOps fallback{sizeof(BatchTable<JobDenFallback<StorageF32>>), fill_den_spatial<JobDenFallback>,
+                                   batch_launch<JobDenFallback>, 64, false};
+    static const BatchOps fast_clamp{sizeof(BatchTable<JobDenFastClamp<StorageF32>>), fill_den_fast_clamp,
+                                     batch_launch<JobDenFastClamp>, 64, false};
+    static const BatchOps atrous{sizeof(BatchTable<JobDenAtrous<StorageF32>>), fill_den_atrous,
+                                 batch_launch<JobDenAtrous, 256, true>, 256, false};
+    switch (kind) {
+        case kPassDenTemporal: return &temporal;
+        case kPassDenDisocclusion: return &disocclusion;
+        case kPassDenFallback: return &fallback;
+        case kPassDenFastClamp: return &fast_clamp;
+        case kPassDenAtrous: return &atrous;
+        default: return nullptr;
+    }
 }
 
 }  // namespace vgi

@@ -12,6 +12,7 @@
 #include <new>
 
 #include "vgi_denoiser_impl.h"
+#include "vgi_passes.h"
 
 using namespace vgi;
 
@@ -42,6 +43,10 @@ extern "C" vgi_status vgi_denoiser_create(uint32_t width, uint32_t height, uint3
         if ((st = alloc_set(d->temporal[k], n, storage)) != VGI_OK) break;
         if ((st = d->temporal_len[k].alloc(n)) != VGI_OK) break;
         if ((st = alloc_set(d->fix[k], n, storage)) != VGI_OK) break;
+        if (storage == VGI_STORAGE_F16) {
+            if ((st = d->temporal[k].comp.alloc(pair_bytes(storage) * n)) != VGI_OK) break;
+            if ((st = d->fix[k].comp.alloc(pair_bytes(storage) * n)) != VGI_OK) break;
+        }
         if ((st = alloc_set(d->pp[k], n, storage)) != VGI_OK) break;
         if ((st = d->fast_diff[k].alloc(radiance_bytes(storage) * n)) != VGI_OK) break;
         if ((st = d->fast_spec[k].alloc(radiance_bytes(storage) * n)) != VGI_OK) break;
@@ -72,16 +77,15 @@ extern "C" vgi_status vgi_denoiser_set_config(vgi_denoiser* d, const vgi_denoise
     return VGI_OK;
 }
 
-extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_params* fp,
-                                              const vgi_denoiser_input* in, vgi_denoiser_output* out,
-                                              void* stream) {
+vgi_status vgi_denoiser_plan(vgi_denoiser* d, const vgi_frame_params* fp, const vgi_denoiser_input* in,
+                             vgi_denoiser_output* out, PassSink& sink) {
     VGI_REQUIRE(d && fp && in && out, "vgi_denoiser_add_passes: null argument");
     VGI_REQUIRE(in->depth && in->gbuffer && in->diffuse && in->specular && in->specular_ray_len,
                 "vgi_denoiser_add_passes: missing input");
-    hipStream_t st = (hipStream_t)stream;
     const uint32_t iters = d->cfg.atrous_iterations > 5 ? 5 : d->cfg.atrous_iterations;  // denoising.rs:73
     const bool has_history = d->has_history && in->prev_depth && in->prev_gbuffer;
     const bool fast = d->cfg.fast_history_clamp != 0;
+    const bool comp = d->storage == VGI_STORAGE_F16 && d->cfg.compensated_moments != 0;
     const int c = d->cur;
     vgi_status s;
 
@@ -101,6 +105,7 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
         t.diff_history_texture = d->hist_diff;
         t.spec_history_texture = d->hist_spec;
         t.moments_history_texture = d->hist_mom;
+        t.moments_comp_history_texture = comp ? d->hist_comp : nullptr;  // NULL reads as a zero residual
         t.history_len_texture = d->hist_len;
     }
     const bool fast_hist_in = fast && d->has_fast_history && has_history;
@@ -108,8 +113,10 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
         // Without a fast history the reference binds a dummy (zero) texture;
         // the blend then starts from 0 exactly as for a disocclusion.
         if (!fast_hist_in) {
-            if ((s = fill_zero(d->fast_diff[1 - c].p, d->fast_diff[1 - c].bytes, st)) != VGI_OK) return s;
-            if ((s = fill_zero(d->fast_spec[1 - c].p, d->fast_spec[1 - c].bytes, st)) != VGI_OK) return s;
+            if ((s = sink.submit(kSlotDenClearFastDiff, make_clear(d->fast_diff[1 - c].p, d->fast_diff[1 - c].bytes))) != VGI_OK)
+                return s;
+            if ((s = sink.submit(kSlotDenClearFastSpec, make_clear(d->fast_spec[1 - c].p, d->fast_spec[1 - c].bytes))) != VGI_OK)
+                return s;
         }
         t.diff_fast_history_texture = d->fast_diff[1 - c].p;
         t.spec_fast_history_texture = d->fast_spec[1 - c].p;
@@ -119,13 +126,20 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
     t.rw_diff_texture = d->temporal[c].diff.p;
     t.rw_spec_texture = d->temporal[c].spec.p;
     t.rw_moments_texture = d->temporal[c].pair.p;
+    t.rw_moments_comp_texture = comp ? d->temporal[c].comp.p : nullptr;
     t.rw_history_len_texture = (uint8_t*)d->temporal_len[c].p;
     vgi_pc_denoise_temporal pc{{(float)d->w, (float)d->h}, has_history ? 1u : 0u, fast ? 1u : 0u};
-    if ((s = vgi_pass_denoise_temporal(fp, &t, &pc, stream)) != VGI_OK) return s;
+    {
+        PassRec rec = make_rec(kPassDenTemporal, fp);
+        rec.io.den_temporal = t;
+        rec.pc.den_temporal = pc;
+        if ((s = sink.submit(kSlotDenTemporal, rec)) != VGI_OK) return s;
+    }
 
     const void* cur_diff = d->temporal[c].diff.p;
     const void* cur_spec = d->temporal[c].spec.p;
     const void* cur_mom = d->temporal[c].pair.p;
+    const void* cur_comp = comp ? d->temporal[c].comp.p : nullptr;
     const uint8_t* cur_len = (const uint8_t*)d->temporal_len[c].p;
 
     // Pass: Disocclusion Fix (denoising.rs:164-188)
@@ -140,13 +154,18 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
         x.diff_texture = cur_diff;
         x.spec_texture = cur_spec;
         x.moments_texture = cur_mom;
+        x.moments_comp_texture = cur_comp;
         x.rw_diff_texture = d->fix[c].diff.p;
         x.rw_spec_texture = d->fix[c].spec.p;
         x.rw_moments_or_variance_texture = d->fix[c].pair.p;
-        if ((s = vgi_pass_denoise_disocclusion_fix(&x, stream)) != VGI_OK) return s;
+        x.rw_moments_comp_texture = comp ? d->fix[c].comp.p : nullptr;
+        PassRec rec = make_rec(kPassDenDisocclusion);
+        rec.io.den_spatial = x;
+        if ((s = sink.submit(kSlotDenDisocclusion, rec)) != VGI_OK) return s;
         cur_diff = d->fix[c].diff.p;
         cur_spec = d->fix[c].spec.p;
         cur_mom = d->fix[c].pair.p;
+        cur_comp = comp ? d->fix[c].comp.p : nullptr;
     }
 
     // Pass: Fast History Clamp (denoising.rs:192-207), in place
@@ -159,7 +178,9 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
         fc.spec_fast_history_texture = d->fast_spec[c].p;
         fc.rw_diff_texture = (void*)cur_diff;
         fc.rw_spec_texture = (void*)cur_spec;
-        if ((s = vgi_pass_denoise_fast_history_clamp(&fc, stream)) != VGI_OK) return s;
+        PassRec rec = make_rec(kPassDenFastClamp);
+        rec.io.den_fast_clamp = fc;
+        if ((s = sink.submit(kSlotDenFastClamp, rec)) != VGI_OK) return s;
     }
     d->has_fast_history = fast;
 
@@ -167,6 +188,7 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
     d->hist_diff = cur_diff;
     d->hist_spec = cur_spec;
     d->hist_mom = cur_mom;
+    d->hist_comp = cur_comp;
     d->hist_len = cur_len;
     d->has_history = true;
 
@@ -181,10 +203,15 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
     fb.diff_texture = cur_diff;
     fb.spec_texture = cur_spec;
     fb.moments_texture = cur_mom;
+    fb.moments_comp_texture = cur_comp;
     fb.rw_diff_texture = d->pp[0].diff.p;
     fb.rw_spec_texture = d->pp[0].spec.p;
     fb.rw_moments_or_variance_texture = d->pp[0].pair.p;
-    if ((s = vgi_pass_denoise_spatial_fallback(&fb, stream)) != VGI_OK) return s;
+    {
+        PassRec rec = make_rec(kPassDenFallback);
+        rec.io.den_spatial = fb;
+        if ((s = sink.submit(kSlotDenFallback, rec)) != VGI_OK) return s;
+    }
 
     // Passes: A-Trous x N (denoising.rs:244-289)
     int src = 0;
@@ -202,7 +229,10 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
         a.rw_spec_texture = d->pp[1 - src].spec.p;
         a.rw_variance_texture = d->pp[1 - src].pair.p;
         vgi_pc_denoise_atrous apc{(int32_t)(1u << it)};
-        if ((s = vgi_pass_denoise_atrous(&a, &apc, stream)) != VGI_OK) return s;
+        PassRec rec = make_rec(kPassDenAtrous);
+        rec.io.den_atrous = a;
+        rec.pc.den_atrous = apc;
+        if ((s = sink.submit(kSlotDenAtrous0 + (int)it, rec)) != VGI_OK) return s;
         src = 1 - src;
     }
     out->diffuse = d->pp[src].diff.p;
@@ -211,4 +241,11 @@ extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_p
     out->variance = d->pp[0].pair.p;  // DebugViews.variance = fallback output
     d->cur = 1 - c;
     return VGI_OK;
+}
+
+extern "C" vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_params* fp,
+                                              const vgi_denoiser_input* in, vgi_denoiser_output* out,
+                                              void* stream) {
+    DirectSink sink((hipStream_t)stream);
+    return vgi_denoiser_plan(d, fp, in, out, sink);
 }

@@ -11,13 +11,14 @@ namespace vgi {
 
 struct ColorSet {
     DevBuf diff, spec, pair;
+    DevBuf comp;  // FP16 storage: residual (low f16 halves) of compensated moments
 };
 
 }  // namespace vgi
 
 struct vgi_denoiser {
     uint32_t w = 0, h = 0, storage = 0;
-    vgi_denoiser_config cfg{1, 0, 3};
+    vgi_denoiser_config cfg{1, 0, 3, 1};
     vgi::ColorSet temporal[2];
     vgi::DevBuf temporal_len[2];
     vgi::ColorSet fix[2];
@@ -30,6 +31,7 @@ struct vgi_denoiser {
     const void* hist_diff = nullptr;
     const void* hist_spec = nullptr;
     const void* hist_mom = nullptr;
+    const void* hist_comp = nullptr;  // null when the history was written without compensation
     const uint8_t* hist_len = nullptr;
 };
 
@@ -41,10 +43,12 @@ inline void denoiser_buffers(vgi_denoiser* d, std::vector<DevBuf*>& out) {
         out.push_back(&d->temporal[k].diff);
         out.push_back(&d->temporal[k].spec);
         out.push_back(&d->temporal[k].pair);
+        out.push_back(&d->temporal[k].comp);
         out.push_back(&d->temporal_len[k]);
         out.push_back(&d->fix[k].diff);
         out.push_back(&d->fix[k].spec);
         out.push_back(&d->fix[k].pair);
+        out.push_back(&d->fix[k].comp);
         out.push_back(&d->fast_diff[k]);
         out.push_back(&d->fast_spec[k]);
         out.push_back(&d->pp[k].diff);

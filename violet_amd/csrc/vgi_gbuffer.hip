@@ -11,6 +11,7 @@
  * pixel shader's derivatives come from.  A partner without geometry (or
  * outside the image) contributes 0.
  */
+#include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
 #include "vgi_radiance.h"
 
@@ -18,10 +19,10 @@ namespace vgi {
 const SceneView& scene_view(const vgi_scene* s);
 
 template <bool kCount>
-__global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
+__device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const SceneView& s, const vgi_gbuffer_io& io, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     const bool inside = px < w && py < h;
     bool geo = false;
@@ -69,6 +70,42 @@ __global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi
     io.rw_gbuffer[plane + i] = encode_gbuffer_normal(n, sh.roughness);
     io.rw_gbuffer[2 * plane + i] = encode_gbuffer_fwidth(fz, fn);
     io.rw_gbuffer[3 * plane + i] = 1u;
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
+    k_gbuffer_body<kCount>(fp, s, io, blockIdx.x, blockIdx.y);
+}
+
+// Fused multi-view launch (vgi_batch.h): every view traces the shared scene.
+template <class>
+struct JobGBuffer {
+    FrameParams fp;
+    SceneView s;
+    vgi_gbuffer_io io;
+    __device__ static void run(const JobGBuffer& j, uint32_t bx, uint32_t by) {
+        k_gbuffer_body<false>(j.fp, j.s, j.io, bx, by);
+    }
+};
+
+static vgi_status fill_gbuffer(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobGBuffer>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        VGI_REQUIRE(r.scene && valid_dims(r.io.gbuffer.width, r.io.gbuffer.height), "batched G-buffer: bad job");
+        auto& j = t->jobs[v];
+        memcpy(&j.fp, &r.fp, sizeof(j.fp));
+        j.s = scene_view(r.scene);
+        j.s.counters = nullptr;
+        j.io = r.io.gbuffer;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_gbuffer(int kind) {
+    static const BatchOps g{sizeof(BatchTable<JobGBuffer<StorageF32>>), fill_gbuffer, batch_launch<JobGBuffer>, 64, true};
+    return kind == kPassGBuffer ? &g : nullptr;
 }
 
 }  // namespace vgi

@@ -12,13 +12,14 @@
  * only the resulting sets are deterministic.  Compaction is a wave-ballot
  * stream compaction: one global atomic per wave of 64 cells.
  */
+#include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
 #include "vgi_radiance.h"
 
 namespace vgi {
 
-__global__ void __launch_bounds__(64) k_hg_decay(uint32_t* decay_words, HashCell* storage) {
-    const uint32_t bucket = blockIdx.x * 64u + threadIdx.x;
+__device__ __forceinline__ void k_hg_decay_body(uint32_t* decay_words, HashCell* storage, uint32_t bx_, uint32_t by_) {
+    const uint32_t bucket = bx_ * 64u + threadIdx.x;
     if (bucket >= kHashGridNumBuckets) return;
     const uint32_t addr_offset = bucket * 16u;
     uint32_t occ = 0;
@@ -76,6 +77,10 @@ __global__ void __launch_bounds__(64) k_hg_decay(uint32_t* decay_words, HashCell
     for (uint32_t i = 0; i < set; ++i) decay_words[(addr_offset >> 2) + i] = slice[i];
 }
 
+__global__ void __launch_bounds__(64) k_hg_decay(uint32_t* decay_words, HashCell* storage) {
+    k_hg_decay_body(decay_words, storage, blockIdx.x, blockIdx.y);
+}
+
 // hash_grid.inc.hlsl:149-184
 __device__ __forceinline__ bool hash_grid_find_or_insert(HashCell* buf, uint32_t hash, uint32_t checksum,
                                                          uint32_t& addr) {
@@ -91,9 +96,8 @@ __device__ __forceinline__ bool hash_grid_find_or_insert(HashCell* buf, uint32_t
     return false;
 }
 
-__global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, const uint32_t* counter, HashCell* storage,
-                                                  uint32_t* selection, uint32_t frame_hash) {
-    uint32_t id = blockIdx.x * 64u + threadIdx.x;
+__device__ __forceinline__ void k_hg_select_body(const HashQuery* queries, const uint32_t* counter, HashCell* storage, uint32_t* selection, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
+    uint32_t id = bx_ * 64u + threadIdx.x;
     if (id >= kHashGridMaxQueries || id >= *counter) return;
     HashQuery q = queries[id];
     uint32_t addr;
@@ -104,9 +108,12 @@ __global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, cons
     atomicMax(&selection[addr], word);
 }
 
-__global__ void __launch_bounds__(64) k_hg_compact(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr,
-                                                   uint32_t* out_counter) {
-    uint32_t cell = blockIdx.x * 64u + threadIdx.x;
+__global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, const uint32_t* counter, HashCell* storage, uint32_t* selection, uint32_t frame_hash) {
+    k_hg_select_body(queries, counter, storage, selection, frame_hash, blockIdx.x, blockIdx.y);
+}
+
+__device__ __forceinline__ void k_hg_compact_body(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter, uint32_t bx_, uint32_t by_) {
+    uint32_t cell = bx_ * 64u + threadIdx.x;
     uint32_t word = cell < kHashGridNumCells ? selection[cell] : 0u;
     bool active = word != 0u;
     uint64_t mask = __ballot(active);
@@ -122,6 +129,88 @@ __global__ void __launch_bounds__(64) k_hg_compact(const uint32_t* selection, ui
     if (active) {
         out_index[base + below] = word & ((1u << kHashGridMaxQueriesShift) - 1u);
         out_addr[base + below] = cell;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_hg_compact(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter) {
+    k_hg_compact_body(selection, out_index, out_addr, out_counter, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Fused multi-view launches (vgi_batch.h); every view has its own cache.
+// ---------------------------------------------------------------------------
+template <class>
+struct JobHgDecay {
+    uint32_t* decay_words;
+    HashCell* storage;
+    __device__ static void run(const JobHgDecay& j, uint32_t bx, uint32_t by) {
+        k_hg_decay_body(j.decay_words, j.storage, bx, by);
+    }
+};
+template <class>
+struct JobHgSelect {
+    const HashQuery* queries;
+    const uint32_t* counter;
+    HashCell* storage;
+    uint32_t* selection;
+    uint32_t frame_hash;
+    __device__ static void run(const JobHgSelect& j, uint32_t bx, uint32_t by) {
+        k_hg_select_body(j.queries, j.counter, j.storage, j.selection, j.frame_hash, bx, by);
+    }
+};
+template <class>
+struct JobHgCompact {
+    const uint32_t* selection;
+    uint32_t *out_index, *out_addr, *out_counter;
+    __device__ static void run(const JobHgCompact& j, uint32_t bx, uint32_t by) {
+        k_hg_compact_body(j.selection, j.out_index, j.out_addr, j.out_counter, bx, by);
+    }
+};
+
+static vgi_status fill_hg_decay(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobHgDecay>(host);
+    for (int v = 0; v < n; ++v) {
+        const vgi_hash_grid_decay_io& io = recs[v]->io.hg_decay;
+        VGI_REQUIRE(((uintptr_t)io.rw_decay_buffer & 3u) == 0, "batched hash-grid decay: unaligned decay buffer");
+        t->jobs[v] = {(uint32_t*)io.rw_decay_buffer, reinterpret_cast<HashCell*>(io.rw_storage_buffer)};
+        batch_add_tiles(t->hd, v, kHashGridNumBuckets / 64, 1);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_hg_select(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobHgSelect>(host);
+    for (int v = 0; v < n; ++v) {
+        const vgi_hash_grid_query_select_io& io = recs[v]->io.hg_select;
+        t->jobs[v] = {reinterpret_cast<const HashQuery*>(io.hash_grid_query_buffer), io.hash_grid_query_counter_buffer,
+                      reinterpret_cast<HashCell*>(io.rw_hash_grid_storage_buffer), io.rw_hash_grid_query_selection_buffer,
+                      recs[v]->pc.frame_hash.frame_hash};
+        batch_add_tiles(t->hd, v, kHashGridMaxQueries / 64, 1);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_hg_compact(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobHgCompact>(host);
+    for (int v = 0; v < n; ++v) {
+        const vgi_hash_grid_query_compact_io& io = recs[v]->io.hg_compact;
+        t->jobs[v] = {io.hash_grid_query_selection_buffer, io.rw_compact_query_index_buffer,
+                      io.rw_compact_query_cell_addr_buffer, io.rw_compact_query_counter_buffer};
+        batch_add_tiles(t->hd, v, kHashGridNumCells / 64, 1);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_hashgrid(int kind) {
+    static const BatchOps decay{sizeof(BatchTable<JobHgDecay<StorageF32>>), fill_hg_decay, batch_launch<JobHgDecay>, 64,
+                                false};
+    static const BatchOps select{sizeof(BatchTable<JobHgSelect<StorageF32>>), fill_hg_select, batch_launch<JobHgSelect>,
+                                 64, false};
+    static const BatchOps compact{sizeof(BatchTable<JobHgCompact<StorageF32>>), fill_hg_compact,
+                                  batch_launch<JobHgCompact>, 64, false};
+    switch (kind) {
+        case kPassHgDecay: return &decay;
+        case kPassHgSelect: return &select;
+        case kPassHgCompact: return &compact;
+        default: return nullptr;
     }
 }
 

@@ -1,0 +1,40 @@
+/*
+ * vgi_passes.cpp — the direct sink: one pass record -> its C entry point.
+ */
+#include "vgi_passes.h"
+
+namespace vgi {
+
+vgi_status DirectSink::submit(int, const PassRec& r) {
+    void* s = (void*)stream;
+    switch (r.kind) {
+        case kPassClear: return fill_zero(r.clear_ptr, r.clear_bytes, stream);
+        case kPassGBuffer: return vgi_pass_gbuffer_primary(&r.fp, r.scene, &r.io.gbuffer, s);
+        case kPassHgDecay: return vgi_pass_hash_grid_decay(&r.io.hg_decay, s);
+        case kPassHgSelect: return vgi_pass_hash_grid_query_select(&r.io.hg_select, &r.pc.frame_hash, s);
+        case kPassHgCompact: return vgi_pass_hash_grid_query_compact(&r.io.hg_compact, s);
+        case kPassHgRaygen: return vgi_pass_hash_grid_raygen(&r.fp, &r.tr, &r.io.hg_raygen, &r.pc.frame_hash, s);
+        case kPassDiffGen: return vgi_pass_ind_diff_sample_gen(&r.fp, &r.tr, &r.io.diff_gen, &r.pc.sample_gen, s);
+        case kPassDiffValidate: return vgi_pass_ind_diff_sample_validate(&r.fp, &r.tr, &r.io.diff_validate, s);
+        case kPassAoGen: return vgi_pass_raytraced_ao_gen(&r.fp, &r.io.ao_gen, &r.pc.ao_gen, s);
+        case kPassAoFilter: return vgi_pass_raytraced_ao_filter(&r.io.ao_filter, s);
+        case kPassDiffTemporal:
+            return vgi_pass_ind_diff_temporal_resample(&r.fp, &r.io.diff_temporal, &r.pc.diff_temporal, s);
+        case kPassDiffSpatial:
+            return vgi_pass_ind_diff_spatial_resample(&r.fp, &r.io.diff_spatial, &r.pc.diff_spatial, s);
+        case kPassSpecGen: return vgi_pass_ind_spec_sample_gen(&r.fp, &r.tr, &r.io.spec_gen, &r.pc.sample_gen, s);
+        case kPassSpecValidate: return vgi_pass_ind_spec_sample_validate(&r.fp, &r.tr, &r.io.spec_validate, s);
+        case kPassSpecTemporal:
+            return vgi_pass_ind_spec_temporal_resample(&r.fp, &r.io.spec_temporal, &r.pc.spec_temporal, s);
+        case kPassSpecSpatial:
+            return vgi_pass_ind_spec_spatial_resample(&r.fp, &r.io.spec_spatial, &r.pc.spec_spatial, s);
+        case kPassDenTemporal: return vgi_pass_denoise_temporal(&r.fp, &r.io.den_temporal, &r.pc.den_temporal, s);
+        case kPassDenDisocclusion: return vgi_pass_denoise_disocclusion_fix(&r.io.den_spatial, s);
+        case kPassDenFastClamp: return vgi_pass_denoise_fast_history_clamp(&r.io.den_fast_clamp, s);
+        case kPassDenFallback: return vgi_pass_denoise_spatial_fallback(&r.io.den_spatial, s);
+        case kPassDenAtrous: return vgi_pass_denoise_atrous(&r.io.den_atrous, &r.pc.den_atrous, s);
+        default: return set_error(VGI_E_INVALID, "internal: unknown pass kind %d", r.kind);
+    }
+}
+
+}  // namespace vgi

@@ -1,0 +1,179 @@
+/*
+ * vgi_passes.h — pass records: the arguments of one pass invocation as the
+ * sequencers (renderer, denoiser) produce them, and the sinks that consume
+ * them.  A DirectSink launches each pass at once through its C entry point
+ * (the single-view path); a recording sink keeps them per sequence slot so a
+ * batch can fuse the same slot of many views into one persistent launch.
+ *
+ * Slots follow restir_renderer.rs:258-829 and denoising.rs:66-301; a view's
+ * frame issues a subsequence of them, in this order.
+ */
+#pragma once
+#include <string.h>
+
+#include "vgi_internal.h"
+
+namespace vgi {
+
+enum PassKind : int {
+    kPassClear,            // fill a device buffer with zeros (util_passes.rs clear_buffer)
+    kPassGBuffer,
+    kPassHgDecay,
+    kPassHgSelect,
+    kPassHgCompact,
+    kPassHgRaygen,
+    kPassDiffGen,
+    kPassDiffValidate,
+    kPassAoGen,
+    kPassAoFilter,
+    kPassDiffTemporal,
+    kPassDiffSpatial,
+    kPassSpecGen,
+    kPassSpecValidate,
+    kPassSpecTemporal,
+    kPassSpecSpatial,
+    kPassDenTemporal,
+    kPassDenDisocclusion,
+    kPassDenFastClamp,
+    kPassDenFallback,
+    kPassDenAtrous,
+    kPassKindCount
+};
+
+enum Slot : int {
+    kSlotGBuffer,
+    kSlotClearHgStorage,
+    kSlotClearHgDecay,
+    kSlotClearHgCounter,
+    kSlotHgDecay,
+    kSlotClearSelection,
+    kSlotHgSelect,
+    kSlotClearCompactCount,
+    kSlotHgCompact,
+    kSlotClearNextCounter,
+    kSlotHgRaygen,
+    kSlotDiffGen,
+    kSlotDiffValidate,
+    kSlotAoGen,
+    kSlotAoFilter,
+    kSlotDiffTemporal,
+    kSlotDiffSpatial,
+    kSlotSpecGen,
+    kSlotSpecValidate,
+    kSlotSpecTemporal,
+    kSlotSpecSpatial,
+    kSlotDenClearFastDiff,
+    kSlotDenClearFastSpec,
+    kSlotDenTemporal,
+    kSlotDenDisocclusion,
+    kSlotDenFastClamp,
+    kSlotDenFallback,
+    kSlotDenAtrous0,  // .. kSlotDenAtrous0 + 4
+    kSlotCount = kSlotDenAtrous0 + 5
+};
+
+// The pass kind each slot carries.
+inline int slot_kind(int slot) {
+    static const int k[kSlotCount] = {
+        kPassGBuffer,      kPassClear,        kPassClear,         kPassClear,       kPassHgDecay,
+        kPassClear,        kPassHgSelect,     kPassClear,         kPassHgCompact,   kPassClear,
+        kPassHgRaygen,     kPassDiffGen,      kPassDiffValidate,  kPassAoGen,       kPassAoFilter,
+        kPassDiffTemporal, kPassDiffSpatial,  kPassSpecGen,       kPassSpecValidate, kPassSpecTemporal,
+        kPassSpecSpatial,  kPassClear,        kPassClear,         kPassDenTemporal, kPassDenDisocclusion,
+        kPassDenFastClamp, kPassDenFallback,  kPassDenAtrous,     kPassDenAtrous,   kPassDenAtrous,
+        kPassDenAtrous,    kPassDenAtrous};
+    return k[slot];
+}
+
+struct PassRec {
+    int kind;
+    vgi_frame_params fp;
+    vgi_trace_resources tr;
+    const vgi_scene* scene;
+    union {
+        vgi_gbuffer_io gbuffer;
+        vgi_hash_grid_decay_io hg_decay;
+        vgi_hash_grid_query_select_io hg_select;
+        vgi_hash_grid_query_compact_io hg_compact;
+        vgi_hash_grid_raygen_io hg_raygen;
+        vgi_ind_diff_sample_gen_io diff_gen;
+        vgi_ind_diff_sample_validate_io diff_validate;
+        vgi_ao_gen_io ao_gen;
+        vgi_ao_filter_io ao_filter;
+        vgi_ind_diff_temporal_io diff_temporal;
+        vgi_ind_diff_spatial_io diff_spatial;
+        vgi_ind_spec_sample_gen_io spec_gen;
+        vgi_ind_spec_sample_validate_io spec_validate;
+        vgi_ind_spec_temporal_io spec_temporal;
+        vgi_ind_spec_spatial_io spec_spatial;
+        vgi_denoise_temporal_io den_temporal;
+        vgi_denoise_spatial_io den_spatial;  // disocclusion fix and spatial fallback
+        vgi_denoise_fast_clamp_io den_fast_clamp;
+        vgi_denoise_atrous_io den_atrous;
+    } io;
+    union {
+        vgi_pc_frame_hash frame_hash;
+        vgi_pc_sample_gen sample_gen;
+        vgi_pc_ao_gen ao_gen;
+        vgi_pc_ind_diff_temporal diff_temporal;
+        vgi_pc_ind_diff_spatial diff_spatial;
+        vgi_pc_ind_spec_temporal spec_temporal;
+        vgi_pc_ind_spec_spatial spec_spatial;
+        vgi_pc_denoise_temporal den_temporal;
+        vgi_pc_denoise_atrous den_atrous;
+    } pc;
+    void* clear_ptr;     // kPassClear
+    size_t clear_bytes;  // kPassClear
+};
+
+inline PassRec make_rec(int kind, const vgi_frame_params* fp = nullptr) {
+    PassRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = kind;
+    if (fp) r.fp = *fp;
+    return r;
+}
+inline PassRec make_clear(void* p, size_t bytes) {
+    PassRec r = make_rec(kPassClear);
+    r.clear_ptr = p;
+    r.clear_bytes = bytes;
+    return r;
+}
+
+struct PassSink {
+    virtual ~PassSink() {}
+    virtual vgi_status submit(int slot, const PassRec& r) = 0;
+};
+
+// Launches each pass immediately on `stream` through its C entry point.
+struct DirectSink : PassSink {
+    explicit DirectSink(hipStream_t s) : stream(s) {}
+    vgi_status submit(int slot, const PassRec& r) override;
+    hipStream_t stream;
+};
+
+// Keeps the records of one view's frame by slot.
+struct RecordSink : PassSink {
+    RecordSink() { clear(); }
+    void clear() {
+        for (int s = 0; s < kSlotCount; ++s) has[s] = false;
+    }
+    vgi_status submit(int slot, const PassRec& r) override {
+        if (slot < 0 || slot >= kSlotCount || slot_kind(slot) != r.kind)
+            return set_error(VGI_E_INVALID, "internal: pass kind %d recorded in slot %d", r.kind, slot);
+        recs[slot] = r;
+        has[slot] = true;
+        return VGI_OK;
+    }
+    PassRec recs[kSlotCount];
+    bool has[kSlotCount];
+};
+
+}  // namespace vgi
+
+// Sequencers over a sink (the C entry points call them with a DirectSink).
+uint32_t vgi_renderer_storage(const vgi_renderer* r);
+vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp, uint32_t frame_index,
+                             vgi_renderer_output* out, vgi::PassSink& sink);
+vgi_status vgi_denoiser_plan(vgi_denoiser* d, const vgi_frame_params* fp, const vgi_denoiser_input* in,
+                             vgi_denoiser_output* out, vgi::PassSink& sink);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "vgi_denoiser_impl.h"
+#include "vgi_passes.h"
 
 using namespace vgi;
 
@@ -116,6 +117,7 @@ extern "C" vgi_status vgi_renderer_default_config(vgi_renderer_config* c) {
     c->denoiser.disocclusion_fix = 1;
     c->denoiser.fast_history_clamp = 0;
     c->denoiser.atrous_iterations = 3;
+    c->denoiser.compensated_moments = 1;
     return VGI_OK;
 }
 
@@ -201,28 +203,35 @@ extern "C" vgi_status vgi_renderer_reset(vgi_renderer* r) {
         if (_s != VGI_OK) return _s;      \
     } while (0)
 
-extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp,
-                                              uint32_t frame_index, vgi_renderer_output* out, void* stream) {
+// One frame of restir_renderer.rs:258-829 issued to `sink` (host state -
+// history flags, ping-pong indices, FrameRand chain - advances as the passes
+// are issued, so a recorded frame must be executed before the next one).
+vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp, uint32_t frame_index,
+                             vgi_renderer_output* out, PassSink& sink) {
     VGI_REQUIRE(r && scene && fp && out, "vgi_renderer_add_passes: null argument");
-    hipStream_t st = (hipStream_t)stream;
     const uint32_t w = r->w, h = r->h, S = r->cfg.storage;
     const int c = r->cur, p = 1 - c;
     uint32_t frame_rand = frame_index;  // restir_renderer.rs:274
 
     // G-buffer (restir_renderer.rs:277-280)
-    vgi_gbuffer_io gio{w, h, r->depth[c].as<float>(), r->gbuffer[c].as<uint32_t>()};
-    CALL(vgi_pass_gbuffer_primary(fp, scene, &gio, stream));
+    {
+        PassRec g = make_rec(kPassGBuffer, fp);
+        g.scene = scene;
+        g.io.gbuffer = vgi_gbuffer_io{w, h, r->depth[c].as<float>(), r->gbuffer[c].as<uint32_t>()};
+        CALL(sink.submit(kSlotGBuffer, g));
+    }
 
     // Hash grid cache: create/clear on first use (:282-319)
     if (!r->hash_grid_ready) {
-        CALL(fill_zero(r->hg_storage.p, r->hg_storage.bytes, st));
-        CALL(fill_zero(r->hg_decay.p, r->hg_decay.bytes, st));
-        CALL(fill_zero(r->hg_counter[r->q].p, 4, st));
+        CALL(sink.submit(kSlotClearHgStorage, make_clear(r->hg_storage.p, r->hg_storage.bytes)));
+        CALL(sink.submit(kSlotClearHgDecay, make_clear(r->hg_decay.p, r->hg_decay.bytes)));
+        CALL(sink.submit(kSlotClearHgCounter, make_clear(r->hg_counter[r->q].p, 4)));
         r->hash_grid_ready = true;
     }
     if (r->cfg.hash_grid_cache_decay) {
-        vgi_hash_grid_decay_io d{r->hg_decay.as<uint8_t>(), r->hg_storage.as<vgi_hash_grid_cell>()};
-        CALL(vgi_pass_hash_grid_decay(&d, stream));
+        PassRec d = make_rec(kPassHgDecay);
+        d.io.hg_decay = vgi_hash_grid_decay_io{r->hg_decay.as<uint8_t>(), r->hg_storage.as<vgi_hash_grid_cell>()};
+        CALL(sink.submit(kSlotHgDecay, d));
     }
     const bool has_prev_depth = r->has_prev_depth, has_prev_gbuffer = r->has_prev_gbuffer;
     const bool has_prev_ind_diff = r->has_prev_indirect_diffuse;
@@ -230,25 +239,29 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
     // Cache update (:361-433)
     bool hg_updated = false;
     if (has_prev_depth && has_prev_ind_diff) {
-        CALL(fill_zero(r->hg_selection.p, r->hg_selection.bytes, st));
-        vgi_hash_grid_query_select_io sio{r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
-                                          r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_selection.as<uint32_t>()};
-        vgi_pc_frame_hash spc{frame_rand_next(frame_rand)};
-        CALL(vgi_pass_hash_grid_query_select(&sio, &spc, stream));
-        CALL(fill_zero(r->hg_ccount.p, 4, st));
-        vgi_hash_grid_query_compact_io cio{r->hg_selection.as<uint32_t>(), r->hg_cidx.as<uint32_t>(),
-                                           r->hg_caddr.as<uint32_t>(), r->hg_ccount.as<uint32_t>()};
-        CALL(vgi_pass_hash_grid_query_compact(&cio, stream));
+        CALL(sink.submit(kSlotClearSelection, make_clear(r->hg_selection.p, r->hg_selection.bytes)));
+        PassRec sel = make_rec(kPassHgSelect);
+        sel.io.hg_select = vgi_hash_grid_query_select_io{
+            r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
+            r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_selection.as<uint32_t>()};
+        sel.pc.frame_hash = vgi_pc_frame_hash{frame_rand_next(frame_rand)};
+        CALL(sink.submit(kSlotHgSelect, sel));
+        CALL(sink.submit(kSlotClearCompactCount, make_clear(r->hg_ccount.p, 4)));
+        PassRec cmp = make_rec(kPassHgCompact);
+        cmp.io.hg_compact = vgi_hash_grid_query_compact_io{r->hg_selection.as<uint32_t>(), r->hg_cidx.as<uint32_t>(),
+                                                           r->hg_caddr.as<uint32_t>(), r->hg_ccount.as<uint32_t>()};
+        CALL(sink.submit(kSlotHgCompact, cmp));
         const int nq = 1 - r->q;
-        CALL(fill_zero(r->hg_counter[nq].p, 4, st));
-        vgi_trace_resources tr = trace_res(r, scene, true);
-        tr.rw_hash_grid_query_buffer = r->hg_query[nq].as<vgi_hash_grid_query>();
-        tr.rw_hash_grid_query_counter_buffer = r->hg_counter[nq].as<uint32_t>();
-        vgi_hash_grid_raygen_io rio{r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_query[r->q].as<vgi_hash_grid_query>(),
-                                    r->hg_ccount.as<uint32_t>(), r->hg_cidx.as<uint32_t>(), r->hg_caddr.as<uint32_t>(),
-                                    r->hg_decay.as<uint8_t>()};
-        vgi_pc_frame_hash rpc{frame_rand_next(frame_rand)};
-        CALL(vgi_pass_hash_grid_raygen(fp, &tr, &rio, &rpc, stream));
+        CALL(sink.submit(kSlotClearNextCounter, make_clear(r->hg_counter[nq].p, 4)));
+        PassRec rg = make_rec(kPassHgRaygen, fp);
+        rg.tr = trace_res(r, scene, true);
+        rg.tr.rw_hash_grid_query_buffer = r->hg_query[nq].as<vgi_hash_grid_query>();
+        rg.tr.rw_hash_grid_query_counter_buffer = r->hg_counter[nq].as<uint32_t>();
+        rg.io.hg_raygen = vgi_hash_grid_raygen_io{
+            r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_query[r->q].as<vgi_hash_grid_query>(),
+            r->hg_ccount.as<uint32_t>(), r->hg_cidx.as<uint32_t>(), r->hg_caddr.as<uint32_t>(), r->hg_decay.as<uint8_t>()};
+        rg.pc.frame_hash = vgi_pc_frame_hash{frame_rand_next(frame_rand)};
+        CALL(sink.submit(kSlotHgRaygen, rg));
         r->q = nq;
         hg_updated = true;
     }
@@ -259,31 +272,36 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
     const uint32_t has_new_sample = validation ? 0u : 1u;
     const bool trace_prev = has_prev_depth && has_prev_ind_diff;
     if (!validation) {
-        vgi_trace_resources tr = trace_res(r, scene, trace_prev);
-        vgi_ind_diff_sample_gen_io io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
-                                      r->dnew_hit.as<uint32_t>(), r->dnew_rad.p};
-        vgi_pc_sample_gen pc{frame_rand_next(frame_rand), trace_prev ? 1u : 0u};
-        CALL(vgi_pass_ind_diff_sample_gen(fp, &tr, &io, &pc, stream));
+        PassRec g = make_rec(kPassDiffGen, fp);
+        g.tr = trace_res(r, scene, trace_prev);
+        g.io.diff_gen = vgi_ind_diff_sample_gen_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
+                                                   r->dnew_hit.as<uint32_t>(), r->dnew_rad.p};
+        g.pc.sample_gen = vgi_pc_sample_gen{frame_rand_next(frame_rand), trace_prev ? 1u : 0u};
+        CALL(sink.submit(kSlotDiffGen, g));
     } else {
-        vgi_trace_resources tr = trace_res(r, scene, true);
-        vgi_ind_diff_sample_validate_io io{w, h, S, r->depth[p].as<float>(), r->dsp_rsv[p].as<uint32_t>(),
-                                           r->dsp_hit[p].as<uint32_t>(), r->dsp_rad[p].p};
-        CALL(vgi_pass_ind_diff_sample_validate(fp, &tr, &io, stream));
+        PassRec v = make_rec(kPassDiffValidate, fp);
+        v.tr = trace_res(r, scene, true);
+        v.io.diff_validate = vgi_ind_diff_sample_validate_io{w, h, S, r->depth[p].as<float>(), r->dsp_rsv[p].as<uint32_t>(),
+                                                             r->dsp_hit[p].as<uint32_t>(), r->dsp_rad[p].p};
+        CALL(sink.submit(kSlotDiffValidate, v));
     }
 
     // AO gen + filter (:515-563)
     {
-        vgi_ao_gen_io io{w, h, S, has_new_sample ? r->dnew_hit.as<uint32_t>() : nullptr, r->depth[c].as<float>(),
-                         r->has_prev_ao ? r->ao[p].p : nullptr, r->ao[c].p};
-        vgi_pc_ao_gen pc{has_new_sample, r->has_prev_ao ? 1u : 0u, r->cfg.ao_radius};
-        CALL(vgi_pass_raytraced_ao_gen(fp, &io, &pc, stream));
-        vgi_ao_filter_io fio{w, h, S, r->depth[c].as<float>(), r->ao[c].p, r->ao_filtered.p};
-        CALL(vgi_pass_raytraced_ao_filter(&fio, stream));
+        PassRec a = make_rec(kPassAoGen, fp);
+        a.io.ao_gen = vgi_ao_gen_io{w, h, S, has_new_sample ? r->dnew_hit.as<uint32_t>() : nullptr, r->depth[c].as<float>(),
+                                    r->has_prev_ao ? r->ao[p].p : nullptr, r->ao[c].p};
+        a.pc.ao_gen = vgi_pc_ao_gen{has_new_sample, r->has_prev_ao ? 1u : 0u, r->cfg.ao_radius};
+        CALL(sink.submit(kSlotAoGen, a));
+        PassRec f = make_rec(kPassAoFilter);
+        f.io.ao_filter = vgi_ao_filter_io{w, h, S, r->depth[c].as<float>(), r->ao[c].p, r->ao_filtered.p};
+        CALL(sink.submit(kSlotAoFilter, f));
     }
 
     // Diffuse temporal (:589-607) and spatial (:632-659)
     {
-        vgi_ind_diff_temporal_io io{w, h, S, has_prev_diff_rsv ? r->depth[p].as<float>() : nullptr, r->depth[c].as<float>(),
+        PassRec t = make_rec(kPassDiffTemporal, fp);
+        t.io.diff_temporal = vgi_ind_diff_temporal_io{w, h, S, has_prev_diff_rsv ? r->depth[p].as<float>() : nullptr, r->depth[c].as<float>(),
                                     has_new_sample ? r->dnew_hit.as<uint32_t>() : nullptr,
                                     has_new_sample ? r->dnew_rad.p : nullptr,
                                     has_prev_diff_rsv ? r->dsp_rsv[p].as<uint32_t>() : nullptr,
@@ -292,46 +310,51 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
                                     r->dtm_rsv.as<uint32_t>(), r->dtm_hit.as<uint32_t>(), r->dtm_rad.p};
         // has_prev_frame = has_prev_diffuse_reservoir (restir_renderer.rs:605); the
         // previous depth exists whenever the previous reservoir does
-        vgi_pc_ind_diff_temporal pc{frame_rand_next(frame_rand), (has_prev_diff_rsv && has_prev_depth) ? 1u : 0u,
+        t.pc.diff_temporal = vgi_pc_ind_diff_temporal{frame_rand_next(frame_rand), (has_prev_diff_rsv && has_prev_depth) ? 1u : 0u,
                                     has_new_sample, r->cfg.temporal_m_cap};
-        CALL(vgi_pass_ind_diff_temporal_resample(fp, &io, &pc, stream));
-        vgi_ind_diff_spatial_io sio{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(), r->ao_filtered.p,
+        CALL(sink.submit(kSlotDiffTemporal, t));
+        PassRec sp = make_rec(kPassDiffSpatial, fp);
+        sp.io.diff_spatial = vgi_ind_diff_spatial_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(), r->ao_filtered.p,
                                     r->dtm_rsv.as<uint32_t>(), r->dtm_hit.as<uint32_t>(), r->dtm_rad.p,
                                     r->dsp_rsv[c].as<uint32_t>(), r->dsp_hit[c].as<uint32_t>(), r->dsp_rad[c].p,
                                     r->diff_light[c].p};
-        vgi_pc_ind_diff_spatial spc{frame_rand_next(frame_rand), r->cfg.diff_spatial_taps, r->cfg.spatial_radius_px};
-        CALL(vgi_pass_ind_diff_spatial_resample(fp, &sio, &spc, stream));
+        sp.pc.diff_spatial = vgi_pc_ind_diff_spatial{frame_rand_next(frame_rand), r->cfg.diff_spatial_taps, r->cfg.spatial_radius_px};
+        CALL(sink.submit(kSlotDiffSpatial, sp));
     }
 
     // Specular sample gen (:685-701), validate (:722-737), temporal (:743-761), spatial (:770-784)
     {
-        vgi_trace_resources tr = trace_res(r, scene, trace_prev);
-        vgi_ind_spec_sample_gen_io io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
-                                      r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
-        vgi_pc_sample_gen pc{frame_rand_next(frame_rand), trace_prev ? 1u : 0u};
-        CALL(vgi_pass_ind_spec_sample_gen(fp, &tr, &io, &pc, stream));
+        PassRec g = make_rec(kPassSpecGen, fp);
+        g.tr = trace_res(r, scene, trace_prev);
+        g.io.spec_gen = vgi_ind_spec_sample_gen_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
+                                                   r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
+        g.pc.sample_gen = vgi_pc_sample_gen{frame_rand_next(frame_rand), trace_prev ? 1u : 0u};
+        CALL(sink.submit(kSlotSpecGen, g));
     }
     const bool has_prev_spec = r->has_prev_ind_spec;
     if (r->cfg.ind_spec_validate && has_prev_depth && has_prev_ind_diff && has_prev_spec) {
-        vgi_trace_resources tr = trace_res(r, scene, true);
-        vgi_ind_spec_sample_validate_io io{w, h, S, r->spec_rsv[p].as<uint32_t>(), r->spec_hitpos[p].as<float>(),
-                                           r->spec_rad[p].p};
-        CALL(vgi_pass_ind_spec_sample_validate(fp, &tr, &io, stream));
+        PassRec v = make_rec(kPassSpecValidate, fp);
+        v.tr = trace_res(r, scene, true);
+        v.io.spec_validate = vgi_ind_spec_sample_validate_io{w, h, S, r->spec_rsv[p].as<uint32_t>(),
+                                                             r->spec_hitpos[p].as<float>(), r->spec_rad[p].p};
+        CALL(sink.submit(kSlotSpecValidate, v));
     }
     {
         const bool hp = has_prev_depth && has_prev_spec && has_prev_gbuffer;
-        vgi_ind_spec_temporal_io io{w, h, S, hp ? r->gbuffer[p].as<uint32_t>() : nullptr,
+        PassRec t = make_rec(kPassSpecTemporal, fp);
+        t.io.spec_temporal = vgi_ind_spec_temporal_io{w, h, S, hp ? r->gbuffer[p].as<uint32_t>() : nullptr,
                                     hp ? r->depth[p].as<float>() : nullptr, r->gbuffer[c].as<uint32_t>(),
                                     r->depth[c].as<float>(), hp ? r->spec_rsv[p].as<uint32_t>() : nullptr,
                                     hp ? r->spec_hitpos[p].as<float>() : nullptr, hp ? r->spec_rad[p].p : nullptr,
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
-        vgi_pc_ind_spec_temporal pc{frame_rand_next(frame_rand), hp ? 1u : 0u, r->cfg.temporal_m_cap};
-        CALL(vgi_pass_ind_spec_temporal_resample(fp, &io, &pc, stream));
-        vgi_ind_spec_spatial_io sio{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
+        t.pc.spec_temporal = vgi_pc_ind_spec_temporal{frame_rand_next(frame_rand), hp ? 1u : 0u, r->cfg.temporal_m_cap};
+        CALL(sink.submit(kSlotSpecTemporal, t));
+        PassRec sp = make_rec(kPassSpecSpatial, fp);
+        sp.io.spec_spatial = vgi_ind_spec_spatial_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p,
                                     r->spec_light.p, r->spec_raylen.p};
-        vgi_pc_ind_spec_spatial spc{frame_rand_next(frame_rand), r->cfg.spatial_radius_px};
-        CALL(vgi_pass_ind_spec_spatial_resample(fp, &sio, &spc, stream));
+        sp.pc.spec_spatial = vgi_pc_ind_spec_spatial{frame_rand_next(frame_rand), r->cfg.spatial_radius_px};
+        CALL(sink.submit(kSlotSpecSpatial, sp));
     }
 
     // Denoiser (:790-817)
@@ -344,7 +367,7 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
         vgi_denoiser_input in{r->depth[c].as<float>(), r->gbuffer[c].as<uint32_t>(), r->depth[p].as<float>(),
                               r->gbuffer[p].as<uint32_t>(), r->diff_light[c].p, r->spec_light.p, r->spec_raylen.p};
         vgi_denoiser_output o{};
-        CALL(vgi_denoiser_add_passes(r->den, fp, &in, &o, stream));
+        CALL(vgi_denoiser_plan(r->den, fp, &in, &o, sink));
         den_diff = o.diffuse;
         den_spec = o.specular;
         slot = (o.diffuse == r->den->pp[0].diff.p) ? 2 : 3;
@@ -372,6 +395,14 @@ extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* 
     out->validation_frame = validation ? 1u : 0u;
     out->hash_grid_updated = hg_updated ? 1u : 0u;
     return VGI_OK;
+}
+
+uint32_t vgi_renderer_storage(const vgi_renderer* r) { return r->cfg.storage; }
+
+extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp,
+                                              uint32_t frame_index, vgi_renderer_output* out, void* stream) {
+    DirectSink sink((hipStream_t)stream);
+    return vgi_renderer_plan(r, scene, fp, frame_index, out, sink);
 }
 
 extern "C" vgi_status vgi_renderer_get_image(const vgi_renderer* rc, const char* name, void** ptr, size_t* bytes) {
@@ -460,6 +491,7 @@ extern "C" vgi_status vgi_renderer_export_state(const vgi_renderer* rc, void* bl
     for (int i = 0; i < 8; ++i) hd.flags[i] = f[i];
     hd.den_flags[0] = r->den->has_history;
     hd.den_flags[1] = r->den->has_fast_history;
+    hd.den_flags[2] = r->den->hist_comp != nullptr;
     uint8_t* o = (uint8_t*)blob;
     memcpy(o, &hd, sizeof(hd));
     o += sizeof(hd);
@@ -510,6 +542,7 @@ extern "C" vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* blo
     d->hist_diff = d->cfg.disocclusion_fix ? d->fix[hs].diff.p : d->temporal[hs].diff.p;
     d->hist_spec = d->cfg.disocclusion_fix ? d->fix[hs].spec.p : d->temporal[hs].spec.p;
     d->hist_mom = d->cfg.disocclusion_fix ? d->fix[hs].pair.p : d->temporal[hs].pair.p;
+    d->hist_comp = !hd.den_flags[2] ? nullptr : d->cfg.disocclusion_fix ? d->fix[hs].comp.p : d->temporal[hs].comp.p;
     d->hist_len = d->temporal_len[hs].as<uint8_t>();
     return VGI_OK;
 }

@@ -16,6 +16,7 @@
  */
 #include <cmath>
 
+#include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
 #include "vgi_storage.h"
 
@@ -58,9 +59,9 @@ __device__ __forceinline__ uint32_t ld_u32(const uint32_t* p, size_t i) { return
 // (x, a) is stored here.
 // ---------------------------------------------------------------------------
 template <class S>
-__global__ void __launch_bounds__(64) k_ao_gen(FrameParams fp, vgi_ao_gen_io io, vgi_pc_ao_gen pc) {
+__device__ __forceinline__ void k_ao_gen_body(const FrameParams& fp, const vgi_ao_gen_io& io, const vgi_pc_ao_gen& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -93,14 +94,19 @@ __global__ void __launch_bounds__(64) k_ao_gen(FrameParams fp, vgi_ao_gen_io io,
     S::st2(io.rw_ao_texture, i, hist);
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_ao_gen(FrameParams fp, vgi_ao_gen_io io, vgi_pc_ao_gen pc) {
+    k_ao_gen_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
+}
+
 __device__ __forceinline__ f2 min2v(f2 a, f2 b) { return mk2(fminf(a.x, b.x), fminf(a.y, b.y)); }
 __device__ __forceinline__ f2 max2v(f2 a, f2 b) { return mk2(fmaxf(a.x, b.x), fmaxf(a.y, b.y)); }
 __device__ __forceinline__ f2 clamp2v(f2 v, f2 lo, f2 hi) { return min2v(max2v(v, lo), hi); }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_ao_filter(vgi_ao_filter_io io) {
+__device__ __forceinline__ void k_ao_filter_body(const vgi_ao_filter_io& io, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -137,14 +143,18 @@ __global__ void __launch_bounds__(64) k_ao_filter(vgi_ao_filter_io io) {
     S::st2(io.rw_filtered_ao_texture, i, mk2(lerp(med.x, mn.x, 0.5f), lerp(med.y, mn.y, 0.5f)));
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_ao_filter(vgi_ao_filter_io io) {
+    k_ao_filter_body<S>(io, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Indirect diffuse temporal resampling
 // ---------------------------------------------------------------------------
 template <class S>
-__global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_diff_temporal_io io,
-                                                      vgi_pc_ind_diff_temporal pc) {
+__device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, const vgi_ind_diff_temporal_io& io, const vgi_pc_ind_diff_temporal& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -208,6 +218,11 @@ __global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_di
     S::st3(io.rw_hit_radiance_texture, i, rad);
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_diff_temporal_io io, vgi_pc_ind_diff_temporal pc) {
+    k_diff_temporal_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Indirect diffuse spatial resampling (golden-spiral taps)
 // ---------------------------------------------------------------------------
@@ -243,10 +258,9 @@ __device__ __forceinline__ DiffWR diff_load(const vgi_ind_diff_spatial_io& io, s
 }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_diff_spatial_io io, uint32_t frame_rand,
-                                                     float radius0, SpiralTable tab) {
+__device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const vgi_ind_diff_spatial_io& io, uint32_t frame_rand, float radius0, const SpiralTable& tab, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -325,6 +339,11 @@ __global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_dif
     S::st3(io.rw_lighting_texture, i, res.rad * ((kOneOverPi * NoL) * res.W));
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_diff_spatial_io io, uint32_t frame_rand, float radius0, SpiralTable tab) {
+    k_diff_spatial_body<S>(fp, io, frame_rand, radius0, tab, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Specular target function (ind_spec_temporal_resample.hlsl:63-144,
 // ind_spec_spatial_resample.hlsl:62-151)
@@ -374,10 +393,9 @@ __device__ __forceinline__ GB load_gb(const uint32_t* g, size_t plane, size_t i)
 }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_spec_temporal_io io,
-                                                      vgi_pc_ind_spec_temporal pc) {
+__device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, const vgi_ind_spec_temporal_io& io, const vgi_pc_ind_spec_temporal& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -467,6 +485,11 @@ __global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_sp
     if (use_prev) S::st3(io.rw_hit_radiance_texture, i, res_rad);
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_spec_temporal_io io, vgi_pc_ind_spec_temporal pc) {
+    k_spec_temporal_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
+}
+
 struct SpecTF {
     f3 view_pos, position, normal, f0;
     float pr;
@@ -477,10 +500,9 @@ struct SpecTF {
 };
 
 template <class S>
-__global__ void __launch_bounds__(64) k_spec_spatial(FrameParams fp, vgi_ind_spec_spatial_io io, uint32_t frame_rand,
-                                                     float radius0, float cos_tol) {
+__device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const vgi_ind_spec_spatial_io& io, uint32_t frame_rand, float radius0, float cos_tol, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -575,6 +597,11 @@ __global__ void __launch_bounds__(64) k_spec_spatial(FrameParams fp, vgi_ind_spe
     S::st1(io.rw_ray_len_texture, i, ray_len);
 }
 
+template <class S>
+__global__ void __launch_bounds__(64) k_spec_spatial(FrameParams fp, vgi_ind_spec_spatial_io io, uint32_t frame_rand, float radius0, float cos_tol) {
+    k_spec_spatial_body<S>(fp, io, frame_rand, radius0, cos_tol, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -616,6 +643,146 @@ static float spatial_radius(float radius_px, uint32_t w, uint32_t h) {
     } while (0)
 
 static bool storage_ok(uint32_t s) { return s == VGI_STORAGE_F32 || s == VGI_STORAGE_F16; }
+
+// ---------------------------------------------------------------------------
+// Fused multi-view launches (vgi_batch.h)
+// ---------------------------------------------------------------------------
+template <class S>
+struct JobAoGen {
+    FrameParams fp;
+    vgi_ao_gen_io io;
+    vgi_pc_ao_gen pc;
+    __device__ static void run(const JobAoGen& j, uint32_t bx, uint32_t by) { k_ao_gen_body<S>(j.fp, j.io, j.pc, bx, by); }
+};
+template <class S>
+struct JobAoFilter {
+    vgi_ao_filter_io io;
+    __device__ static void run(const JobAoFilter& j, uint32_t bx, uint32_t by) { k_ao_filter_body<S>(j.io, bx, by); }
+};
+template <class S>
+struct JobDiffTemporal {
+    FrameParams fp;
+    vgi_ind_diff_temporal_io io;
+    vgi_pc_ind_diff_temporal pc;
+    __device__ static void run(const JobDiffTemporal& j, uint32_t bx, uint32_t by) {
+        k_diff_temporal_body<S>(j.fp, j.io, j.pc, bx, by);
+    }
+};
+template <class S>
+struct JobDiffSpatial {
+    FrameParams fp;
+    vgi_ind_diff_spatial_io io;
+    uint32_t frame_rand;
+    float radius0;
+    SpiralTable tab;
+    __device__ static void run(const JobDiffSpatial& j, uint32_t bx, uint32_t by) {
+        k_diff_spatial_body<S>(j.fp, j.io, j.frame_rand, j.radius0, j.tab, bx, by);
+    }
+};
+template <class S>
+struct JobSpecTemporal {
+    FrameParams fp;
+    vgi_ind_spec_temporal_io io;
+    vgi_pc_ind_spec_temporal pc;
+    __device__ static void run(const JobSpecTemporal& j, uint32_t bx, uint32_t by) {
+        k_spec_temporal_body<S>(j.fp, j.io, j.pc, bx, by);
+    }
+};
+template <class S>
+struct JobSpecSpatial {
+    FrameParams fp;
+    vgi_ind_spec_spatial_io io;
+    uint32_t frame_rand;
+    float radius0;
+    float cos_tol;
+    __device__ static void run(const JobSpecSpatial& j, uint32_t bx, uint32_t by) {
+        k_spec_spatial_body<S>(j.fp, j.io, j.frame_rand, j.radius0, j.cos_tol, bx, by);
+    }
+};
+
+static vgi_status fill_ao_gen(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobAoGen>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        t->jobs[v] = {to_fp(&r.fp), r.io.ao_gen, r.pc.ao_gen};
+        batch_add_pixel_tiles(t->hd, v, r.io.ao_gen.width, r.io.ao_gen.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_ao_filter(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobAoFilter>(host);
+    for (int v = 0; v < n; ++v) {
+        t->jobs[v] = {recs[v]->io.ao_filter};
+        batch_add_pixel_tiles(t->hd, v, recs[v]->io.ao_filter.width, recs[v]->io.ao_filter.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_diff_temporal(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDiffTemporal>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        t->jobs[v] = {to_fp(&r.fp), r.io.diff_temporal, r.pc.diff_temporal};
+        batch_add_pixel_tiles(t->hd, v, r.io.diff_temporal.width, r.io.diff_temporal.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_diff_spatial(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDiffSpatial>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        const vgi_pc_ind_diff_spatial& pc = r.pc.diff_spatial;
+        VGI_REQUIRE(pc.num_taps <= (uint32_t)kMaxSpiralTaps, "batched diffuse spatial: num_taps %u", pc.num_taps);
+        t->jobs[v] = {to_fp(&r.fp), r.io.diff_spatial, pc.frame_rand,
+                      spatial_radius(pc.radius_px, r.io.diff_spatial.width, r.io.diff_spatial.height),
+                      make_spiral_table(pc.num_taps ? (int)pc.num_taps : 9)};
+        batch_add_pixel_tiles(t->hd, v, r.io.diff_spatial.width, r.io.diff_spatial.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_spec_temporal(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobSpecTemporal>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        t->jobs[v] = {to_fp(&r.fp), r.io.spec_temporal, r.pc.spec_temporal};
+        batch_add_pixel_tiles(t->hd, v, r.io.spec_temporal.width, r.io.spec_temporal.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_spec_spatial(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobSpecSpatial>(host);
+    static const SpiralTable tab = make_spiral_table(9);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        t->jobs[v] = {to_fp(&r.fp), r.io.spec_spatial, r.pc.spec_spatial.frame_rand,
+                      spatial_radius(r.pc.spec_spatial.radius_px, r.io.spec_spatial.width, r.io.spec_spatial.height),
+                      tab.cos_tol};
+        batch_add_pixel_tiles(t->hd, v, r.io.spec_spatial.width, r.io.spec_spatial.height);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_resample(int kind) {
+    static const BatchOps ao_gen{sizeof(BatchTable<JobAoGen<StorageF32>>), fill_ao_gen, batch_launch<JobAoGen>, 64, false};
+    static const BatchOps ao_filter{sizeof(BatchTable<JobAoFilter<StorageF32>>), fill_ao_filter,
+                                    batch_launch<JobAoFilter>, 64, false};
+    static const BatchOps diff_temporal{sizeof(BatchTable<JobDiffTemporal<StorageF32>>), fill_diff_temporal,
+                                        batch_launch<JobDiffTemporal>, 64, false};
+    static const BatchOps diff_spatial{sizeof(BatchTable<JobDiffSpatial<StorageF32>>), fill_diff_spatial,
+                                       batch_launch<JobDiffSpatial>, 64, false};
+    static const BatchOps spec_temporal{sizeof(BatchTable<JobSpecTemporal<StorageF32>>), fill_spec_temporal,
+                                        batch_launch<JobSpecTemporal>, 64, false};
+    static const BatchOps spec_spatial{sizeof(BatchTable<JobSpecSpatial<StorageF32>>), fill_spec_spatial,
+                                       batch_launch<JobSpecSpatial>, 64, false};
+    switch (kind) {
+        case kPassAoGen: return &ao_gen;
+        case kPassAoFilter: return &ao_filter;
+        case kPassDiffTemporal: return &diff_temporal;
+        case kPassDiffSpatial: return &diff_spatial;
+        case kPassSpecTemporal: return &spec_temporal;
+        case kPassSpecSpatial: return &spec_spatial;
+        default: return nullptr;
+    }
+}
 
 }  // namespace vgi
 

@@ -69,12 +69,13 @@ namespace vgi {
 // lane&3 = position in the quad), as in the quad remap of
 // relax_disocclusion_fix.hlsl:36-40.  Quad members are lanes 4q..4q+3, so a
 // quad reduction is two xor-shuffles.
-__device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) {
-    uint32_t lane = threadIdx.x;
+__device__ __forceinline__ void tile_pixel_at(uint32_t bx, uint32_t by, uint32_t& px, uint32_t& py) {
+    uint32_t lane = threadIdx.x & 63u;
     uint32_t q = lane >> 2;
     uint32_t qi = lane & 3u;
-    px = blockIdx.x * 8u + ((q & 3u) << 1) + (qi & 1u);
-    py = blockIdx.y * 8u + ((q >> 2) << 1) + (qi >> 1);
+    px = bx * 8u + ((q & 3u) << 1) + (qi & 1u);
+    py = by * 8u + ((q >> 2) << 1) + (qi >> 1);
 }
+__device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) { tile_pixel_at(blockIdx.x, blockIdx.y, px, py); }
 }  // namespace vgi
 #endif

@@ -12,6 +12,7 @@
  * without geometry leave the wave early, so the remaining lanes are the
  * active rays.
  */
+#include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
 #include "vgi_radiance.h"
 
@@ -26,11 +27,10 @@ __device__ __forceinline__ f3 position_biased(const float* m, uint32_t px, uint3
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io,
-                                                        vgi_pc_sample_gen pc) {
+__device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_diff_sample_gen_io& io, const vgi_pc_sample_gen& pc, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -56,11 +56,15 @@ __global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, TraceRes rt,
-                                                             vgi_ind_diff_sample_validate_io io) {
+__global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io, vgi_pc_sample_gen pc) {
+    k_diff_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
+}
+
+template <class S, bool kCount>
+__device__ __forceinline__ void k_diff_sample_validate_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_diff_sample_validate_io& io, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -87,6 +91,11 @@ __global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, Tra
     io.rw_hit_pos_normal_texture[2 * i] = ex;
     io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+}
+
+template <class S, bool kCount>
+__global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_validate_io io) {
+    k_diff_sample_validate_body<S, kCount>(fp, rt, io, blockIdx.x, blockIdx.y);
 }
 
 // GGX VNDF sample [Heitz 2014] of ind_spec_sample_gen.hlsl:67-120; returns the
@@ -121,11 +130,10 @@ __device__ __forceinline__ f3 sample_spec_dir(f3 view_dir, f3 normal, float roug
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io,
-                                                        vgi_pc_sample_gen pc) {
+__device__ __forceinline__ void k_spec_sample_gen_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_spec_sample_gen_io& io, const vgi_pc_sample_gen& pc, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -151,11 +159,15 @@ __global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, TraceRes rt,
-                                                             vgi_ind_spec_sample_validate_io io) {
+__global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io, vgi_pc_sample_gen pc) {
+    k_spec_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
+}
+
+template <class S, bool kCount>
+__device__ __forceinline__ void k_spec_sample_validate_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_spec_sample_validate_io& io, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel(px, py);
+    tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
@@ -188,10 +200,14 @@ __global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, Tra
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io,
-                                                         uint32_t frame_hash) {
+__global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_validate_io io) {
+    k_spec_sample_validate_body<S, kCount>(fp, rt, io, blockIdx.x, blockIdx.y);
+}
+
+template <class S, bool kCount>
+__device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
-    uint32_t id = blockIdx.x * 64u + threadIdx.x;
+    uint32_t id = bx_ * 64u + threadIdx.x;
     if (id >= *io.compact_query_counter_buffer) return;
     uint32_t qi = io.compact_query_index_buffer[id];
     uint32_t cell = io.compact_query_cell_addr_buffer[id];
@@ -211,6 +227,11 @@ __global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRe
     storage[cell].g = blend.y;
     storage[cell].b = blend.z;
     io.rw_hash_grid_decay_buffer[cell] = 10;
+}
+
+template <class S, bool kCount>
+__global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io, uint32_t frame_hash) {
+    k_hash_grid_raygen_body<S, kCount>(fp, rt, io, frame_hash, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,6 +281,149 @@ static FrameParams to_fp(const vgi_frame_params* fp) {
                 hipLaunchKernelGGL((kernel<StorageF16, false>), grid, dim3(64), lds_, st, __VA_ARGS__);    \
         }                                                                                                  \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Fused multi-view launches (vgi_batch.h); the scene (and so the LDS stack
+// size) is shared by every view of a batch.
+// ---------------------------------------------------------------------------
+template <class S>
+struct JobDiffGen {
+    FrameParams fp;
+    TraceRes rt;
+    vgi_ind_diff_sample_gen_io io;
+    vgi_pc_sample_gen pc;
+    __device__ static void run(const JobDiffGen& j, uint32_t bx, uint32_t by) {
+        k_diff_sample_gen_body<S, false>(j.fp, j.rt, j.io, j.pc, bx, by);
+    }
+};
+template <class S>
+struct JobDiffValidate {
+    FrameParams fp;
+    TraceRes rt;
+    vgi_ind_diff_sample_validate_io io;
+    __device__ static void run(const JobDiffValidate& j, uint32_t bx, uint32_t by) {
+        k_diff_sample_validate_body<S, false>(j.fp, j.rt, j.io, bx, by);
+    }
+};
+template <class S>
+struct JobSpecGen {
+    FrameParams fp;
+    TraceRes rt;
+    vgi_ind_spec_sample_gen_io io;
+    vgi_pc_sample_gen pc;
+    __device__ static void run(const JobSpecGen& j, uint32_t bx, uint32_t by) {
+        k_spec_sample_gen_body<S, false>(j.fp, j.rt, j.io, j.pc, bx, by);
+    }
+};
+template <class S>
+struct JobSpecValidate {
+    FrameParams fp;
+    TraceRes rt;
+    vgi_ind_spec_sample_validate_io io;
+    __device__ static void run(const JobSpecValidate& j, uint32_t bx, uint32_t by) {
+        k_spec_sample_validate_body<S, false>(j.fp, j.rt, j.io, bx, by);
+    }
+};
+template <class S>
+struct JobHgRaygen {
+    FrameParams fp;
+    TraceRes rt;
+    vgi_hash_grid_raygen_io io;
+    uint32_t frame_hash;
+    __device__ static void run(const JobHgRaygen& j, uint32_t bx, uint32_t by) {
+        k_hash_grid_raygen_body<S, false>(j.fp, j.rt, j.io, j.frame_hash, bx, by);
+    }
+};
+
+static vgi_status fill_diff_gen(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDiffGen>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, r.pc.sample_gen.has_prev_frame != 0, j.rt);
+        if (s != VGI_OK) return s;
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.diff_gen;
+        j.pc = r.pc.sample_gen;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_diff_validate(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDiffValidate>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, true, j.rt);
+        if (s != VGI_OK) return s;
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.diff_validate;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_spec_gen(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobSpecGen>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, r.pc.sample_gen.has_prev_frame != 0, j.rt);
+        if (s != VGI_OK) return s;
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.spec_gen;
+        j.pc = r.pc.sample_gen;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_spec_validate(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobSpecValidate>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, true, j.rt);
+        if (s != VGI_OK) return s;
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.spec_validate;
+        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+    }
+    return VGI_OK;
+}
+static vgi_status fill_hg_raygen(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobHgRaygen>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, true, j.rt);
+        if (s != VGI_OK) return s;
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.hg_raygen;
+        j.frame_hash = r.pc.frame_hash.frame_hash;
+        batch_add_tiles(t->hd, v, kHashGridMaxQueries / 64, 1);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_sample(int kind) {
+    static const BatchOps diff_gen{sizeof(BatchTable<JobDiffGen<StorageF32>>), fill_diff_gen, batch_launch<JobDiffGen>,
+                                   64, true};
+    static const BatchOps diff_validate{sizeof(BatchTable<JobDiffValidate<StorageF32>>), fill_diff_validate,
+                                        batch_launch<JobDiffValidate>, 64, true};
+    static const BatchOps spec_gen{sizeof(BatchTable<JobSpecGen<StorageF32>>), fill_spec_gen, batch_launch<JobSpecGen>,
+                                   64, true};
+    static const BatchOps spec_validate{sizeof(BatchTable<JobSpecValidate<StorageF32>>), fill_spec_validate,
+                                        batch_launch<JobSpecValidate>, 64, true};
+    static const BatchOps hg_raygen{sizeof(BatchTable<JobHgRaygen<StorageF32>>), fill_hg_raygen,
+                                    batch_launch<JobHgRaygen>, 64, true};
+    switch (kind) {
+        case kPassDiffGen: return &diff_gen;
+        case kPassDiffValidate: return &diff_validate;
+        case kPassSpecGen: return &spec_gen;
+        case kPassSpecValidate: return &spec_validate;
+        case kPassHgRaygen: return &hg_raygen;
+        default: return nullptr;
+    }
+}
 
 }  // namespace vgi
 

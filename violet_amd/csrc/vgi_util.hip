@@ -3,6 +3,9 @@
  * zero-fill used for absent histories).  Kernels rather than
  * hipMemsetAsync so that a frame can be captured into a hipGraph.
  */
+#include <algorithm>
+
+#include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
 
 namespace vgi {
@@ -34,6 +37,48 @@ vgi_status fill_zero(void* p, size_t bytes, hipStream_t st) {
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, st, (uint8_t*)p, bytes, (uint8_t)0);
     return check_launch("k_fill_u8");
+}
+
+// Fused multi-view zero fill (vgi_batch.h): a tile is 1024 words; the byte
+// tail of a length that is not a multiple of 4 goes to the first tile.
+constexpr uint32_t kClearTileWords = 1024;
+template <class>
+struct JobClear {
+    uint32_t* words;
+    size_t nwords;
+    uint8_t* tail;
+    uint32_t tail_bytes;
+    __device__ static void run(const JobClear& j, uint32_t bx, uint32_t) {
+        const size_t base = (size_t)bx * kClearTileWords;
+#pragma unroll
+        for (uint32_t k = 0; k < kClearTileWords / 64; ++k) {
+            const size_t i = base + k * 64u + (threadIdx.x & 63u);
+            if (i < j.nwords) j.words[i] = 0u;
+        }
+        if (bx == 0 && threadIdx.x < j.tail_bytes) j.tail[threadIdx.x] = 0u;
+    }
+};
+
+static vgi_status fill_clear(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobClear>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        VGI_REQUIRE(((uintptr_t)r.clear_ptr & 3u) == 0 || r.clear_bytes == 0, "batched clear: unaligned buffer");
+        auto& j = t->jobs[v];
+        j.words = (uint32_t*)r.clear_ptr;
+        j.nwords = r.clear_bytes / 4;
+        j.tail = (uint8_t*)r.clear_ptr + 4 * j.nwords;
+        j.tail_bytes = (uint32_t)(r.clear_bytes & 3u);
+        const size_t tiles = std::max<size_t>(1, (j.nwords + kClearTileWords - 1) / kClearTileWords);
+        VGI_REQUIRE(tiles < (1u << 30), "batched clear: buffer too large");
+        batch_add_tiles(t->hd, v, (uint32_t)tiles, 1);
+    }
+    return VGI_OK;
+}
+
+const BatchOps* batch_ops_util(int kind) {
+    static const BatchOps clear{sizeof(BatchTable<JobClear<StorageF32>>), fill_clear, batch_launch<JobClear>, 64, false};
+    return kind == kPassClear ? &clear : nullptr;
 }
 
 }  // namespace vgi

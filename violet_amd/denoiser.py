@@ -40,12 +40,14 @@ class Output:
 
 
 class Denoiser:
-    def __init__(self, width: int, height: int, storage: int = _lib.STORAGE_F32):
+    def __init__(self, width: int, height: int, storage: int = _lib.STORAGE_F32, compensated_moments: bool = True):
         gpu.require_gpu()
         self.width, self.height, self.storage = width, height, storage
         self.disocclusion_fix = True
         self.fast_history_clamp = False
         self.atrous_iterations = 3
+        # FP16 storage: second moments kept as hi + lo f16 pairs (ignored for FP32)
+        self.compensated_moments = compensated_moments
         h = C.c_void_p()
         cfg = self._cfg()
         _lib.call("vgi_denoiser_create", width, height, storage, C.byref(cfg), C.byref(h))
@@ -53,7 +55,7 @@ class Denoiser:
 
     def _cfg(self) -> _lib.DenoiserConfig:
         return _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
-                                   int(self.atrous_iterations))
+                                   int(self.atrous_iterations), int(self.compensated_moments))
 
     def reset(self) -> None:
         _lib.call("vgi_denoiser_reset", self._h)

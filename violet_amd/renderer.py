@@ -37,6 +37,7 @@ class RestirConfig:
     disocclusion_fix: bool = True
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
+    compensated_moments: bool = True  # FP16 storage: moments as hi + lo f16 pairs
 
     def to_c(self) -> _abi.RendererConfig:
         c = _abi.RendererConfig()
@@ -51,7 +52,7 @@ class RestirConfig:
         c.diff_spatial_taps = self.diff_spatial_taps
         c.spatial_radius_px = self.spatial_radius_px
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
-                                         int(self.atrous_iterations))
+                                         int(self.atrous_iterations), int(self.compensated_moments))
         return c
 
 
