@@ -8,6 +8,13 @@ already resident in HBM.  Prints ONE JSON line (metric, value in Mpix/s,
 per-step latency percentiles, the dominant kernel's roofline from live
 per-pass HIP-event timers, and the CPU baseline = the oracle's FP32 build
 timed on a bounded sample of the same workload on this host's cores).
+
+Beside the plain 1280x720 frame loop (the `value`), the line carries the
+other forms on a shared 1M-triangle scene: "batched" (16 views of seeded
+resolutions, plain vs one fused persistent launch per pass, graph-replayed,
+autotuned), "serving" (2000 Poisson requests at 600/s over 64 persistent
+views, continuously batched, p50/p99 latency) and "precision" (1920x1080
+with FP16 vs FP32 storage).  --forms '' skips them.
 """
 from __future__ import annotations
 
@@ -228,6 +235,151 @@ def cpu_baseline_frame(args, mesh, budget_s: float = 20.0):
                        f"orbit workload at {w}x{h}, FP32 twin of the oracle, {el:.1f} s")
 
 
+# ---------------------------------------------------------------------------
+# Batched, serving and precision forms (reported beside the plain path)
+# ---------------------------------------------------------------------------
+def big_scene(args):
+    from violet_amd import scene
+
+    log(f"[bench] building the {args.big_triangles}-triangle scene of the batched/serving/precision forms")
+    mesh = scene.make_scene(args.big_triangles, seed=13)
+    sc = scene.Scene(mesh)
+    st = sc.stats()
+    log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
+    return mesh, sc
+
+
+def batch_view_specs(n: int, seed: int):
+    """Seeded resolutions w ~ U{320..1920}, h ~ U{240..1080}, one orbit camera each."""
+    from violet_amd import serve
+
+    rng = np.random.default_rng(seed)
+    ws, hs = rng.integers(320, 1921, n), rng.integers(240, 1081, n)
+    ph = rng.uniform(0.0, 360.0, n)
+    return [serve.ViewSpec(int(w), int(h), float(p)) for w, h, p in zip(ws, hs, ph)]
+
+
+def run_batched_form(args, sc, ntris):
+    """16 views over the shared scene, FP16 storage: the plain path (each
+    view's frame issued alone, one after the other) against one fused
+    persistent launch per pass over all views, graph-replayed, with the
+    group counts autotuned on these views."""
+    import torch
+
+    from violet_amd import _lib, autotune, batch, renderer, serve
+
+    specs = batch_view_specs(args.batch_views, seed=21)
+    px = sum(s.width * s.height for s in specs)
+    cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
+    fps_cache = {}
+
+    def fps_of(i, f):
+        k = (i, f)
+        if k not in fps_cache:
+            fps_cache[k] = serve.view_frame_params(specs[i], f)
+        return fps_cache[k]
+
+    K = args.batch_frames
+    # plain: the same views, one single-view frame after another
+    views = [renderer.RestirRenderer(s.width, s.height, cfg) for s in specs]
+    for f in range(3):
+        for i, r in enumerate(views):
+            r.add_passes(sc, fps_of(i, f), f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(3, 3 + K):
+        for i, r in enumerate(views):
+            r.add_passes(sc, fps_of(i, f), f)
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / K
+    for r in views:
+        r.close()
+    # batched: fresh views, autotune, then the timed frames
+    views = [renderer.RestirRenderer(s.width, s.height, cfg) for s in specs]
+    n = len(views)
+    log(f"[bench] batched form: {n} views, {px / 1e6:.2f} Mpx per frame; autotuning")
+    best, table, f0 = autotune.tune_batch(sc, views, fps_of, frame0=0)
+    b = batch.Batch(best)
+    for f in range(f0, f0 + 3):
+        b.add_passes(sc, views, [fps_of(i, f) for i in range(n)], [f] * n)
+    torch.cuda.synchronize()
+    f0 += 3
+    t0 = time.perf_counter()
+    for f in range(f0, f0 + K):
+        b.add_passes(sc, views, [fps_of(i, f) for i in range(n)], [f] * n)
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t0) / K
+    f0 += K
+    lat = []
+    for f in range(f0, f0 + min(K, 30)):
+        t1 = time.perf_counter()
+        b.add_passes(sc, views, [fps_of(i, f) for i in range(n)], [f] * n)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    b.close()
+    for r in views:
+        r.close()
+    return {"workload": f"batched_{n}_views_f16", "views": n, "pixels_per_frame": px, "triangles": ntris,
+            "resolutions": [[s.width, s.height] for s in specs], "frames": K,
+            "plain_mpix_per_s": round(px / plain_s / 1e6, 2), "batched_mpix_per_s": round(px / batched_s / 1e6, 2),
+            "speedup_vs_plain": round(plain_s / batched_s, 3), "plain_ms_per_frame": round(plain_s * 1e3, 3),
+            "batched_ms_per_frame": round(batched_s * 1e3, 3),
+            "batched_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "batched_p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "graph": True, "autotuned": {k: getattr(best, k) for k in ("pixel_groups", "trace_groups", "atrous_groups")},
+            "autotune_table_ms": [[c["trace_groups"], c["pixel_groups"], c["atrous_groups"], round(ms, 3)]
+                                  for c, ms in table]}
+
+
+def run_serving_form(args, sc, ntris):
+    """Poisson stream of view-render requests over persistent views of mixed
+    resolution, continuously batched (violet_amd.serve)."""
+    from violet_amd import _lib, batch, renderer, serve
+
+    specs = serve.serve_views(args.serve_views, seed=31)
+    cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
+    reqs = serve.poisson_requests(args.serve_requests, args.serve_rate, len(specs), seed=32)
+    log(f"[bench] serving form: {len(reqs)} requests at {args.serve_rate:g}/s over {len(specs)} views")
+    srv = serve.Server(sc, specs, hbm_budget_bytes=args.serve_budget_gb * 1e9, config=cfg,
+                       max_batch=args.serve_max_batch, streams=2, batch_config=batch.BatchConfig())
+    stats = srv.run(reqs, timeout_s=600.0)
+    srv.close()
+    s = stats.summary()
+    return {"workload": f"serving_{len(reqs)}_requests_{args.serve_rate:g}_per_s", "views": len(specs),
+            "triangles": ntris, "hbm_budget_gb": args.serve_budget_gb, "max_batch": args.serve_max_batch,
+            "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "mean_ms": round(s["mean_ms"], 3),
+            "mpix_per_s": round(s["mpix_per_s"], 2), "wall_s": round(s["wall_s"], 3), "batches": s["batches"],
+            "mean_batch": round(s["mean_batch"], 2), "evictions": s["evictions"],
+            "peak_hbm_gb": round(s["peak_hbm_gb"], 2)}
+
+
+def run_precision_form(args, sc, ntris):
+    """1920x1080 frame loop with FP16 storage (compensated moments) on the
+    shared scene: throughput beside the same loop with FP32 storage."""
+    import torch
+
+    from violet_amd import _lib, renderer, serve
+
+    spec = serve.ViewSpec(1920, 1080, 0.0)
+    out = {"workload": "precision_1920x1080_f16", "triangles": ntris, "frames": args.batch_frames}
+    for name, st in (("f16", _lib.STORAGE_F16), ("f32", _lib.STORAGE_F32)):
+        r = renderer.RestirRenderer(1920, 1080, renderer.RestirConfig(storage=st, atrous_iterations=args.iterations,
+                                                                      **RESAMPLE[args.resample]))
+        fps = [serve.view_frame_params(spec, f) for f in range(args.batch_frames + 3)]
+        for f in range(3):
+            r.add_passes(sc, fps[f], f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(3, 3 + args.batch_frames):
+            r.add_passes(sc, fps[f], f)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.batch_frames
+        out[f"{name}_mpix_per_s"] = round(1920 * 1080 / el / 1e6, 2)
+        out[f"{name}_ms_per_frame"] = round(el * 1e3, 3)
+        r.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +397,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--resample", default="frame_loop", choices=sorted(RESAMPLE))
     ap.add_argument("--count-frames", type=int, default=8, help="frames run with traversal statistics")
+    ap.add_argument("--forms", default="batched,serving,precision",
+                    help="comma list of forms reported beside the plain frame path ('' for none)")
+    ap.add_argument("--big-triangles", type=int, default=1_000_000)
+    ap.add_argument("--batch-views", type=int, default=16)
+    ap.add_argument("--batch-frames", type=int, default=30)
+    ap.add_argument("--serve-views", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=600.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-max-batch", type=int, default=16)
     args = ap.parse_args()
     frame = args.workload == "frame"
     # full frame: 1280x720, 60-frame orbit, 3 a-trous iterations (the reference
@@ -329,6 +491,14 @@ def main():
         "trace": trace,
         "cpu_baseline": cpu,
     }
+    forms = [f for f in args.forms.split(",") if f] if frame else []
+    if forms:
+        mesh, sc = big_scene(args)
+        ntris = int(len(mesh.indices))
+        runners = {"batched": run_batched_form, "serving": run_serving_form, "precision": run_precision_form}
+        for f in forms:
+            log(f"[bench] form: {f}")
+            out[f] = runners[f](args, sc, ntris)
     print(json.dumps(out))
 
 

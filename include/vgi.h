@@ -627,6 +627,8 @@ vgi_status vgi_renderer_get_image(const vgi_renderer* r, const char* name, void*
 /* Per-view state snapshot (checkpoint/resume, serving eviction): copies all
  * temporal state to/from a host blob of vgi_renderer_state_size() bytes. */
 size_t vgi_renderer_state_size(const vgi_renderer* r);
+/* Device memory held by the view (its state plus per-frame scratch images). */
+size_t vgi_renderer_device_bytes(const vgi_renderer* r);
 vgi_status vgi_renderer_export_state(const vgi_renderer* r, void* host_blob, size_t bytes, void* stream);
 vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* host_blob, size_t bytes, void* stream);
 

@@ -120,3 +120,4 @@ _SIGS = {
 for _n, _a in _SIGS.items():
     _lib.register(_n, C.c_int32, _a)
 _lib.register("vgi_renderer_state_size", C.c_size_t, [vp])
+_lib.register("vgi_renderer_device_bytes", C.c_size_t, [vp])

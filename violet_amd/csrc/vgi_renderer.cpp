@@ -15,6 +15,7 @@
  * Images that violet keeps as RGTemporal resources are double-buffered here
  * (index `cur` is written this frame, `1 - cur` holds last frame's).
  */
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -469,6 +470,19 @@ extern "C" size_t vgi_renderer_state_size(const vgi_renderer* rc) {
     std::vector<DevBuf*> v;
     all_buffers(r, v);
     size_t n = sizeof(StateHeader);
+    for (auto* b : v) n += b->bytes;
+    return n;
+}
+
+extern "C" size_t vgi_renderer_device_bytes(const vgi_renderer* rc) {
+    if (!rc) return 0;
+    vgi_renderer* r = const_cast<vgi_renderer*>(rc);
+    std::vector<DevBuf*> v;
+    all_buffers(r, v);
+    for (DevBuf* b : {&r->dtm_rsv, &r->dtm_hit, &r->dtm_rad, &r->dnew_hit, &r->dnew_rad, &r->ao_filtered, &r->spec_light,
+                      &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount})
+        if (std::find(v.begin(), v.end(), b) == v.end()) v.push_back(b);
+    size_t n = 0;
     for (auto* b : v) n += b->bytes;
     return n;
 }

@@ -105,6 +105,10 @@ class RestirRenderer:
     def state_size(self) -> int:
         return _lib.lib().vgi_renderer_state_size(self._h)
 
+    def device_bytes(self) -> int:
+        """Device memory the view holds (state plus per-frame scratch images)."""
+        return _lib.lib().vgi_renderer_device_bytes(self._h)
+
     def export_state(self) -> np.ndarray:
         n = self.state_size()
         blob = np.empty(n, np.uint8)
