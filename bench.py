@@ -292,6 +292,15 @@ def run_batched_form(args, sc, ntris):
             r.add_passes(sc, fps_of(i, f), f)
     torch.cuda.synchronize()
     plain_s = (time.perf_counter() - t0) / K
+    # per-pass GPU time per frame (all views), profiled outside the timed region
+    _lib.call("vgi_profiler_reset")
+    _lib.call("vgi_profiler_enable", 1)
+    for f in range(3 + K, 6 + K):
+        for i, r in enumerate(views):
+            r.add_passes(sc, fps_of(i, f), f)
+    torch.cuda.synchronize()
+    _lib.call("vgi_profiler_enable", 0)
+    plain_passes = {k: round(v["ms"] / 3, 3) for k, v in _lib.profiler_read().items()}
     for r in views:
         r.close()
     # batched: fresh views, autotune, then the timed frames
@@ -317,6 +326,18 @@ def run_batched_form(args, sc, ntris):
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t1) * 1e3)
     b.close()
+    # per-pass GPU time per frame of the fused launches (same shapes, no graph)
+    pb = batch.Batch(batch.BatchConfig(max_views=n, pixel_groups=best.pixel_groups, trace_groups=best.trace_groups,
+                                       atrous_groups=best.atrous_groups, use_graph=False))
+    f0 += min(K, 30)
+    _lib.call("vgi_profiler_reset")
+    _lib.call("vgi_profiler_enable", 1)
+    for f in range(f0, f0 + 3):
+        pb.add_passes(sc, views, [fps_of(i, f) for i in range(n)], [f] * n)
+    torch.cuda.synchronize()
+    _lib.call("vgi_profiler_enable", 0)
+    batched_passes = {k: round(v["ms"] / 3, 3) for k, v in _lib.profiler_read().items()}
+    pb.close()
     for r in views:
         r.close()
     return {"workload": f"batched_{n}_views_f16", "views": n, "pixels_per_frame": px, "triangles": ntris,
@@ -328,7 +349,8 @@ def run_batched_form(args, sc, ntris):
             "batched_p99_ms": round(float(np.percentile(lat, 99)), 3),
             "graph": True, "autotuned": {k: getattr(best, k) for k in ("pixel_groups", "trace_groups", "atrous_groups")},
             "autotune_table_ms": [[c["trace_groups"], c["pixel_groups"], c["atrous_groups"], round(ms, 3)]
-                                  for c, ms in table]}
+                                  for c, ms in table],
+            "plain_pass_ms_per_frame": plain_passes, "batched_pass_ms_per_frame": batched_passes}
 
 
 def run_serving_form(args, sc, ntris):

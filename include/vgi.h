@@ -644,14 +644,17 @@ vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* host_blob, siz
  * synchronized like single-view renderers.
  * ------------------------------------------------------------------------ */
 typedef struct vgi_batch vgi_batch;
+/* Group counts: N > 0 launches N persistent groups that stride over the
+ * tiles of all views; 0 launches one group per tile (the grid covers the
+ * largest tile count seen for the pass). */
 typedef struct vgi_batch_config {
     uint32_t max_views;      /* views per call, 1..64 */
-    uint32_t pixel_groups;   /* persistent 64-lane groups of the pixel and cache passes */
-    uint32_t trace_groups;   /* persistent 64-lane groups of the ray-tracing passes */
-    uint32_t atrous_groups;  /* persistent 256-lane groups of the a-trous passes */
+    uint32_t pixel_groups;   /* 64-lane groups of the pixel and cache passes */
+    uint32_t trace_groups;   /* 64-lane groups of the ray-tracing passes */
+    uint32_t atrous_groups;  /* 256-lane groups of the a-trous passes */
     uint32_t use_graph;      /* 1: capture the frame once as a HIP graph and replay it */
 } vgi_batch_config;
-/* Defaults: 64 views, 8 / 16 / 4 groups per compute unit, no graph. */
+/* Defaults: 64 views, one group per tile everywhere, no graph. */
 vgi_status vgi_batch_default_config(vgi_batch_config* cfg);
 vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** out);
 vgi_status vgi_batch_destroy(vgi_batch* b);

@@ -56,14 +56,21 @@ SNAP = {
 }
 
 
+# Images whose element type follows the storage mode (binary16 in FP16 mode).
+STORAGE_IMAGES = {"diff_hit_radiance", "diff_temporal_hit_radiance", "diff_new_hit_radiance", "ao", "ao_filtered",
+                  "indirect_diffuse_raw", "spec_hit_radiance", "indirect_specular_raw", "spec_ray_len"}
+
+
 def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
     """Run the GPU renderer; returns (frame params list, outputs list, snapshots list)."""
     import torch
 
-    from violet_amd import renderer, scene, synth
+    from violet_amd import _lib, renderer, scene, synth
 
     sc = scene.Scene(mesh)
-    r = renderer.RestirRenderer(w, h, config or renderer.RestirConfig())
+    config = config or renderer.RestirConfig()
+    f16 = config.storage == _lib.STORAGE_F16
+    r = renderer.RestirRenderer(w, h, config)
     fps, outs, snaps = [], [], []
     prev_view = None
     for f in range(frames):
@@ -74,9 +81,12 @@ def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
         s = {}
         if snap:
             for name, (dt, shp) in SNAP.items():
+                dt = np.float16 if f16 and name in STORAGE_IMAGES else dt
                 s[name] = r.read(name, dt, shp(w, h)).copy()
-            s["out_diffuse"] = r.read_ptr(o.indirect_diffuse, w * h * 12, F32, (h, w, 3)).copy()
-            s["out_specular"] = r.read_ptr(o.indirect_specular, w * h * 12, F32, (h, w, 3)).copy()
+            rb = w * h * (6 if f16 else 12)
+            dt = np.float16 if f16 else F32
+            s["out_diffuse"] = r.read_ptr(o.indirect_diffuse, rb, dt, (h, w, 3)).copy()
+            s["out_specular"] = r.read_ptr(o.indirect_specular, rb, dt, (h, w, 3)).copy()
             s["flags"] = dict(denoised=o.denoised, validation=o.validation_frame, hg=o.hash_grid_updated)
         fps.append(fp)
         outs.append(o)

@@ -80,7 +80,8 @@ def test_closest_hit_large_scene_vs_oracle_bvh(vgi):
 def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
     """The instrumented traversal (trace statistics) returns the same hits and
     counts what the BVH implies: with a 2-triangle scene the root is a leaf,
-    so every ray tests exactly 2 triangles and visits no interior node."""
+    so (wrapped in one interior node) every ray aimed at it visits exactly 1
+    node and tests exactly 2 triangles."""
     from violet_amd import _lib, scene
 
     tiny = scene.Mesh(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32),
@@ -88,9 +89,15 @@ def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
                       indices=np.array([[0, 1, 2], [0, 1, 3]], np.uint32), tri_material=np.zeros(2, np.uint32),
                       albedo=np.full((1, 3), 0.5), metallic=np.zeros(1), roughness=np.full(1, 0.5))
     big = scene.make_scene(4000, seed=5, extent=8.0)
-    rays = _rays(5000, 9, extent=8.0)
+    rng = np.random.default_rng(9)
+    o = rng.standard_normal((5000, 3))
+    o = 4.0 * o / np.linalg.norm(o, axis=1, keepdims=True) + 0.3
+    d = 0.3 - o  # towards the centre of the tiny mesh's box
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    aimed = np.concatenate([o, np.full((5000, 1), 5e-4), d, np.full((5000, 1), 1000.0)], 1).astype(np.float32)
     for mesh, exact in ((tiny, True), (big, False)):
         sc = scene.Scene(mesh)
+        rays = aimed if exact else _rays(5000, 9, extent=8.0)
         ref = [_gpu_trace(sc, rays, a) for a in (False, True)]
         _lib.call("vgi_profiler_reset")
         _lib.call("vgi_trace_stats_enable", 1)
@@ -107,7 +114,7 @@ def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
         assert c["rays_closest"] == len(rays) and c["rays_any"] == 0
         assert a["rays_any"] == len(rays) and a["rays_closest"] == 0
         if exact:
-            assert c["nodes"] == 0 and c["tris"] == 2 * len(rays)
+            assert c["nodes"] == len(rays) and c["tris"] == 2 * len(rays)
         else:
             assert c["nodes"] > len(rays) and c["tris"] > 0
             assert a["nodes"] <= c["nodes"]  # any-hit stops early

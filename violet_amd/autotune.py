@@ -1,11 +1,13 @@
-"""Autotuner for batched frames: picks the persistent group counts of the
-fused launches (ray-tracing passes, pixel and cache passes, a-trous passes)
-on the actual views and scene, by timing whole batched frames.
+"""Autotuner for batched frames: picks the launch shape of the fused passes
+(ray-tracing passes, pixel and cache passes, a-trous passes) on the actual
+views and scene, by timing whole batched frames.
 
-Coordinate descent over the three counts (each a multiple of the device's
-compute units), trace passes first since they dominate a frame; every
-candidate runs graph-replayed frames and the fastest mean frame time wins.
-Tuning advances the views by the frames it renders.
+Each class either launches one workgroup per tile (multiplier 0: the
+hardware dispatcher balances uneven tiles) or a persistent grid of
+`multiplier x compute units` groups striding over the tiles.  Coordinate
+descent over the three classes, trace passes first since they dominate a
+frame; every candidate runs graph-replayed frames and the fastest mean frame
+time wins.  Tuning advances the views by the frames it renders.
 """
 from __future__ import annotations
 
@@ -31,13 +33,13 @@ def _time_frames(b, scene, views, fps_of, frame0: int, warmup: int, frames: int)
 
 
 def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: int = 4,
-               trace=(8, 16, 32), pixel=(4, 8, 16), atrous=(2, 4, 8), use_graph: bool = True):
+               trace=(0, 16, 32), pixel=(0, 8), atrous=(0, 4), use_graph: bool = True):
     """fps_of(view_index, frame) -> FrameParams.  Returns (best BatchConfig,
     list of (config dict, ms per frame)) and the next free frame index."""
     import torch
 
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    best = dict(trace_groups=16 * cus, pixel_groups=8 * cus, atrous_groups=4 * cus)
+    best = dict(trace_groups=0, pixel_groups=0, atrous_groups=0)
     table = []
     f = frame0
     for key, opts in (("trace_groups", trace), ("pixel_groups", pixel), ("atrous_groups", atrous)):

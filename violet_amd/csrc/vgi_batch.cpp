@@ -63,6 +63,10 @@ struct vgi_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t key_storage[2] = {0, 0};
     size_t key_lds[2] = {0, 0};
+    // One-group-per-tile slots launch as many groups as the largest tile
+    // count seen so far for the slot (a graph is re-captured when it grows).
+    uint32_t grid_bound[kSlotCount];
+    uint32_t key_grid[2][kSlotCount];
 
     ~vgi_batch() {
         for (int a = 0; a < 2; ++a) {
@@ -83,11 +87,9 @@ extern "C" vgi_status vgi_batch_default_config(vgi_batch_config* cfg) {
     VGI_REQUIRE(cfg, "vgi_batch_default_config: null");
     memset(cfg, 0, sizeof(*cfg));
     cfg->max_views = kMaxBatchViews;
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    cfg->pixel_groups = 8u * (uint32_t)cus;
-    cfg->trace_groups = 16u * (uint32_t)cus;
-    cfg->atrous_groups = 4u * (uint32_t)cus;
+    cfg->pixel_groups = 0;  // one group per tile
+    cfg->trace_groups = 0;
+    cfg->atrous_groups = 0;
     cfg->use_graph = 0;
     return VGI_OK;
 }
@@ -104,9 +106,8 @@ extern "C" vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** 
     }
     VGI_REQUIRE(c.max_views >= 1 && c.max_views <= (uint32_t)kMaxBatchViews, "vgi_batch_create: max_views %u not in 1..%d",
                 c.max_views, kMaxBatchViews);
-    VGI_REQUIRE(c.pixel_groups >= 1 && c.trace_groups >= 1 && c.atrous_groups >= 1 && c.pixel_groups <= (1u << 20) &&
-                    c.trace_groups <= (1u << 20) && c.atrous_groups <= (1u << 20),
-                "vgi_batch_create: persistent group counts must be in 1..2^20");
+    VGI_REQUIRE(c.pixel_groups <= (1u << 20) && c.trace_groups <= (1u << 20) && c.atrous_groups <= (1u << 20),
+                "vgi_batch_create: persistent group counts must be at most 2^20");
     vgi_batch* b = new (std::nothrow) vgi_batch();
     if (!b) return set_error(VGI_E_NOMEM, "vgi_batch_create: out of host memory");
     b->cfg = c;
@@ -121,6 +122,8 @@ extern "C" vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** 
         off += (o->table_bytes + 255) & ~(size_t)255;
     }
     b->arena_bytes = off;
+    for (int s = 0; s < kSlotCount; ++s) b->grid_bound[s] = 1;
+    memset(b->key_grid, 0, sizeof(b->key_grid));
     hipError_t e = hipSuccess;
     for (int a = 0; a < 2 && e == hipSuccess; ++a) {
         e = hipHostMalloc(&b->host[a], off, hipHostMallocDefault);
@@ -143,9 +146,14 @@ extern "C" vgi_status vgi_batch_destroy(vgi_batch* b) {
     return VGI_OK;
 }
 
-static uint32_t groups_for(const vgi_batch* b, const BatchOps* o) {
+// Persistent group count of a slot's launch, or 0 for one group per tile.
+static uint32_t persistent_groups(const vgi_batch* b, const BatchOps* o) {
     if (o->threads == 256) return b->cfg.atrous_groups;
     return o->trace ? b->cfg.trace_groups : b->cfg.pixel_groups;
+}
+static uint32_t groups_for(const vgi_batch* b, int slot, const BatchOps* o) {
+    const uint32_t g = persistent_groups(b, o);
+    return g ? g : b->grid_bound[slot];
 }
 
 // Upload of the arena and every slot's launch on `st` (all slots in graph
@@ -161,9 +169,9 @@ static vgi_status issue_frame(vgi_batch* b, int a, uint32_t storage, size_t lds,
         vgi_status r;
         if (profile) {
             ProfScope prof(kind_name(kind), st, 0.0);
-            r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, o), o->trace ? lds : 0, st);
+            r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, s, o), o->trace ? lds : 0, st);
         } else {
-            r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, o), o->trace ? lds : 0, st);
+            r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, s, o), o->trace ? lds : 0, st);
         }
         if (r != VGI_OK) return r;
     }
@@ -200,8 +208,12 @@ extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene,
         int n = 0;
         for (uint32_t v = 0; v < num_views; ++v)
             if (b->recs[v].has[s]) list[n++] = &b->recs[v].recs[s];
-        vgi_status r = batch_ops(slot_kind(s))->fill(list, n, (char*)b->host[a] + b->offsets[s]);
+        const BatchOps* o = batch_ops(slot_kind(s));
+        vgi_status r = o->fill(list, n, (char*)b->host[a] + b->offsets[s]);
         if (r != VGI_OK) return r;
+        const BatchHeader* hd = (const BatchHeader*)((const char*)b->host[a] + b->offsets[s]);
+        const uint32_t tiles = hd->tile_begin[hd->njobs];
+        if (persistent_groups(b, o) == 0 && tiles > b->grid_bound[s]) b->grid_bound[s] = tiles;
     }
     // 3. upload + fused launches
     const size_t lds = traversal_lds_bytes(scene_view(scene));
@@ -210,7 +222,8 @@ extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene,
         if (r != VGI_OK) return r;
         VGI_TRY_HIP(hipEventRecord(b->done[a], st));
     } else {
-        if (!b->exec[a] || b->key_storage[a] != storage || b->key_lds[a] != lds) {
+        if (!b->exec[a] || b->key_storage[a] != storage || b->key_lds[a] != lds ||
+            memcmp(b->key_grid[a], b->grid_bound, sizeof(b->grid_bound)) != 0) {
             if (b->exec[a]) {
                 VGI_TRY_HIP(hipGraphExecDestroy(b->exec[a]));
                 b->exec[a] = nullptr;
@@ -229,6 +242,7 @@ extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene,
             VGI_TRY_HIP(e);
             b->key_storage[a] = storage;
             b->key_lds[a] = lds;
+            memcpy(b->key_grid[a], b->grid_bound, sizeof(b->grid_bound));
         }
         VGI_TRY_HIP(hipEventRecord(b->ev_in, st));
         VGI_TRY_HIP(hipStreamWaitEvent(b->cap, b->ev_in, 0));

@@ -226,13 +226,16 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     Box root_box;
     int32_t root = b.build(0, (int)nt, 0, root_box);
     if (root < 0) {
-        // tiny scene: wrap the single leaf in a node with an empty sibling
+        // tiny scene: wrap the single leaf in a node whose second child box
+        // sits at +inf on every axis, where no slab interval can reach it
+        // (an inverted box would not do: the slab test orders the planes)
         BvhNode nd{};
         Box lb = root_box;
         Builder::pad(lb);
+        const float inf = INFINITY;
         nd.c0xy = float4_{lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]};
-        nd.c1xy = float4_{1.0f, -1.0f, 1.0f, -1.0f};
-        nd.z = float4_{lb.lo[2], lb.hi[2], 1.0f, -1.0f};
+        nd.c1xy = float4_{inf, inf, inf, inf};
+        nd.z = float4_{lb.lo[2], lb.hi[2], inf, inf};
         nd.link[0] = root;
         nd.link[1] = root;
         b.nodes.push_back(nd);

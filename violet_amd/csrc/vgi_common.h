@@ -19,6 +19,7 @@
  *   restir/hash_grid.inc.hlsl   hash-grid keys and probing
  */
 #pragma once
+#include <type_traits>
 #include <stdint.h>
 #include "vgi_mathf.h"
 
@@ -43,6 +44,47 @@ constexpr float kHistoryLenUnit = 1.0f / 255.0f;  // svgf.inc.hlsl:4
 // ---------------------------------------------------------------------------
 // Bit casts and scalar helpers
 // ---------------------------------------------------------------------------
+// Loads and stores of device images.  On the device they go through an
+// explicit global-memory pointer: images often arrive as pointers read from
+// memory (the fused multi-view kernels' job tables), where the compiler
+// cannot infer the address space and would emit flat accesses.
+// Structs (cells, queries, materials) move as 32-bit words: C++ copies of a
+// class object cannot bind an address-space-qualified source.
+template <class T>
+VGI_HD T gld(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_scalar<T>::value) {
+        return *(const __attribute__((address_space(1))) T*)p;
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "word-wise load");
+        T r;
+        const __attribute__((address_space(1))) uint32_t* s = (const __attribute__((address_space(1))) uint32_t*)p;
+        uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+        for (size_t k = 0; k < sizeof(T) / 4; ++k) d[k] = s[k];
+        return r;
+    }
+#else
+    return *p;
+#endif
+}
+template <class T>
+VGI_HD void gst(T* p, T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_scalar<T>::value) {
+        *(__attribute__((address_space(1))) T*)p = v;
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "word-wise store");
+        __attribute__((address_space(1))) uint32_t* d = (__attribute__((address_space(1))) uint32_t*)p;
+        const uint32_t* s = reinterpret_cast<const uint32_t*>(&v);
+#pragma unroll
+        for (size_t k = 0; k < sizeof(T) / 4; ++k) d[k] = s[k];
+    }
+#else
+    *p = v;
+#endif
+}
+
 VGI_HD uint32_t asuint(float f) {
     union { float f; uint32_t u; } c;
     c.f = f;

@@ -76,7 +76,7 @@ __device__ __forceinline__ f3 sample_bilinear3(const void* tex, uint32_t w, uint
 }
 
 __device__ __forceinline__ f3 load_normal(const uint32_t* gb, size_t plane, size_t i) {
-    return decode_gbuffer_normal(gb[plane + i]).normal;
+    return decode_gbuffer_normal(G(gb)[plane + i]).normal;
 }
 
 // Second moments.  With FP16 storage and a residual texture `lo`, a moment
@@ -118,9 +118,9 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
 
-    float depth = io.depth_texture[i];
+    float depth = G(io.depth_texture)[i];
     if (depth == 0.0f) {
-        io.rw_history_len_texture[i] = 0;
+        G(io.rw_history_len_texture)[i] = 0;
         S::st3(io.rw_diff_texture, i, mk3s(0.0f));
         S::st3(io.rw_spec_texture, i, mk3s(0.0f));
         st_moments<S>(io.rw_moments_texture, io.rw_moments_comp_texture, i, mk2(0.0f, 0.0f));
@@ -132,8 +132,8 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
     const float suv_x = ((float)px + 0.5f) / bsx;
     const float suv_y = ((float)py + 0.5f) / bsy;
 
-    GBufferNormal gbn = decode_gbuffer_normal(io.gbuffer_texture[plane + i]);
-    f2 fwidth = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]);
+    GBufferNormal gbn = decode_gbuffer_normal(G(io.gbuffer_texture)[plane + i]);
+    f2 fwidth = decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]);
     const bool has_history = pc.has_history != 0;
     const bool fast = pc.fast_history != 0;
 
@@ -216,7 +216,7 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
                 size_t j = (size_t)cy * w + cx;
                 f3 n = load_normal(io.prev_gbuffer_texture, plane, j);
                 f3 nd = n - gbn.normal;
-                bool ok = (fabsf(io.prev_depth_texture[j] - reproj_depth) < depth_tol) &&
+                bool ok = (fabsf(G(io.prev_depth_texture)[j] - reproj_depth) < depth_tol) &&
                           (dot3(nd, nd) < normal_tol_sq);
                 if (ok) {
                     float wgt = wts[y][x];
@@ -225,7 +225,7 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
                     ssum = ssum + S::ld3(io.spec_history_texture, j) * wgt;
                     f2 mh = ld_moments<S>(io.moments_history_texture, io.moments_comp_history_texture, j);
                     msum = mk2(msum.x + wgt * mh.x, msum.y + wgt * mh.y);
-                    lsum += wgt * history_len_load(io.history_len_texture[j]);
+                    lsum += wgt * history_len_load(G(io.history_len_texture)[j]);
                     if (fast) {
                         dfsum = dfsum + S::ld3(io.diff_fast_history_texture, j) * wgt;
                         sfsum = sfsum + S::ld3(io.spec_fast_history_texture, j) * wgt;
@@ -289,7 +289,7 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
     S::st3(io.rw_diff_texture, i, diff_filtered);
     S::st3(io.rw_spec_texture, i, spec_filtered);
     st_moments<S>(io.rw_moments_texture, io.rw_moments_comp_texture, i, mom);
-    io.rw_history_len_texture[i] = history_len_store(history_len);
+    G(io.rw_history_len_texture)[i] = history_len_store(history_len);
     if (fast) {
         S::st3(io.rw_diff_fast_history_texture, i, diff_filtered_fast);
         S::st3(io.rw_spec_fast_history_texture, i, spec_filtered_fast);
@@ -321,7 +321,7 @@ __device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float len = inside ? history_len_load(io.history_len_texture[i]) : 0.0f;
+    float len = inside ? history_len_load(G(io.history_len_texture)[i]) : 0.0f;
     float qmax = quad_max(len);
     if (!inside) return;
     if (len == 0.0f) {
@@ -334,9 +334,9 @@ __device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io
     if (len < 4.0f * kHistoryLenUnit) {
         // EdgeStoppingFunc with sigma_z = 5, sigma_n = 32 and no luminance
         // term: w = pow(sat(n.n_c), 32) * exp(-|z - z_c| / (5 fwidth_z dist + 1e-8))
-        const float depth_center = io.depth_texture[i];
-        const f3 nc = decode_normal_fast(io.gbuffer_texture[plane + i]);
-        const float fwz = 5.0f * decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x;
+        const float depth_center = G(io.depth_texture)[i];
+        const f3 nc = decode_normal_fast(G(io.gbuffer_texture)[plane + i]);
+        const float fwz = 5.0f * decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]).x;
         f2 wsum = mk2(0.0f, 0.0f);
         f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
         f2 msum = mk2(0.0f, 0.0f);
@@ -350,8 +350,8 @@ __device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io
                 if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
                 size_t j = (size_t)cy * w + cx;
                 const float dist = fs * sqrtf((float)(x * x + y * y));  // folded per tap
-                float dz = fabsf(io.depth_texture[j] - depth_center);
-                f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+                float dz = fabsf(G(io.depth_texture)[j] - depth_center);
+                f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
                 float wn = pow_pow2(saturate(dot3(nc, n)), 5);
                 float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
@@ -392,7 +392,7 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float len = inside ? history_len_load(io.history_len_texture[i]) : 0.0f;
+    float len = inside ? history_len_load(G(io.history_len_texture)[i]) : 0.0f;
     float qmax = quad_max(len);
     if (!inside) return;
     if (len == 0.0f) {
@@ -407,9 +407,9 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
     if (len < 4.0f * kHistoryLenUnit) {
         // EdgeStoppingFunc with fwidth_z * 3, sigma_n = 128 and no luminance
         // term, over a 7x7 window
-        const float depth_center = io.depth_texture[i];
-        const f3 nc = decode_normal_fast(io.gbuffer_texture[plane + i]);
-        const float fwz = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x * 3.0f;
+        const float depth_center = G(io.depth_texture)[i];
+        const f3 nc = decode_normal_fast(G(io.gbuffer_texture)[plane + i]);
+        const float fwz = decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]).x * 3.0f;
         float wsum = 0.0f;
         f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
         f2 msum = mk2(0.0f, 0.0f);
@@ -421,8 +421,8 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
                 if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
                 size_t j = (size_t)cy * w + cx;
                 const float dist = sqrtf((float)(x * x + y * y));  // folded per tap
-                float dz = fabsf(io.depth_texture[j] - depth_center);
-                f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+                float dz = fabsf(G(io.depth_texture)[j] - depth_center);
+                f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
                 float wn = pow_pow2(saturate(dot3(nc, n)), 7);
                 float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 f3 dv = S::ld3(io.diff_texture, j);
@@ -551,11 +551,11 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
         for (int k = 0; k < kFCount; ++k) v[k] = 0.0f;
         if (gx >= 0 && gy >= 0 && cx < w && cy < h) {
             size_t j = (size_t)cy * w + cx;
-            f3 n = decode_normal_fast(io.gbuffer_texture[plane + j]);
+            f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
             f3 dv = S::ld3(io.diff_texture, j);
             f3 sv = S::ld3(io.spec_texture, j);
             f2 vv = S::ld2(io.variance_texture, j);
-            v[kFDepth] = io.depth_texture[j];
+            v[kFDepth] = G(io.depth_texture)[j];
             v[kFNx] = n.x, v[kFNy] = n.y, v[kFNz] = n.z;
             v[kFLd] = luminance(dv), v[kFLs] = luminance(sv);
             v[kFDr] = dv.x, v[kFDg] = dv.y, v[kFDb] = dv.z;
@@ -579,7 +579,7 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     f3 dsum = mk3(lds[kFDr][c], lds[kFDg][c], lds[kFDb][c]);
     f3 ssum = mk3(lds[kFSr][c], lds[kFSg][c], lds[kFSb][c]);
     f2 vsum = mk2(lds[kFVx][c], lds[kFVy][c]);
-    const float fwz = decode_gbuffer_fwidth(io.gbuffer_texture[2 * plane + i]).x;
+    const float fwz = decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]).x;
     // exp(-|dl| / (sigma_l*std + eps) - dz / (fwidth_z*dist + eps)) as exp2 of
     // pre-scaled terms (sigma_l = 4, sigma_z = 1, eps = 1e-8)
     const float kl_d = kLog2e * __builtin_amdgcn_rcpf(4.0f * sqrtf(vsum.x) + 1e-8f);

@@ -51,6 +51,7 @@ extern "C" vgi_status vgi_denoiser_create(uint32_t width, uint32_t height, uint3
         if ((st = d->fast_diff[k].alloc(radiance_bytes(storage) * n)) != VGI_OK) break;
         if ((st = d->fast_spec[k].alloc(radiance_bytes(storage) * n)) != VGI_OK) break;
     }
+    if (st == VGI_OK) st = d->fb_var.alloc(pair_bytes(storage) * n);
     if (st != VGI_OK) {
         delete d;
         return st;
@@ -206,7 +207,7 @@ vgi_status vgi_denoiser_plan(vgi_denoiser* d, const vgi_frame_params* fp, const 
     fb.moments_comp_texture = cur_comp;
     fb.rw_diff_texture = d->pp[0].diff.p;
     fb.rw_spec_texture = d->pp[0].spec.p;
-    fb.rw_moments_or_variance_texture = d->pp[0].pair.p;
+    fb.rw_moments_or_variance_texture = d->fb_var.p;
     {
         PassRec rec = make_rec(kPassDenFallback);
         rec.io.den_spatial = fb;
@@ -224,7 +225,7 @@ vgi_status vgi_denoiser_plan(vgi_denoiser* d, const vgi_frame_params* fp, const 
         a.gbuffer_texture = in->gbuffer;
         a.diff_texture = d->pp[src].diff.p;
         a.spec_texture = d->pp[src].spec.p;
-        a.variance_texture = d->pp[src].pair.p;
+        a.variance_texture = it == 0 ? d->fb_var.p : d->pp[src].pair.p;
         a.rw_diff_texture = d->pp[1 - src].diff.p;
         a.rw_spec_texture = d->pp[1 - src].spec.p;
         a.rw_variance_texture = d->pp[1 - src].pair.p;
@@ -238,7 +239,7 @@ vgi_status vgi_denoiser_plan(vgi_denoiser* d, const vgi_frame_params* fp, const 
     out->diffuse = d->pp[src].diff.p;
     out->specular = d->pp[src].spec.p;
     out->history_len = cur_len;
-    out->variance = d->pp[0].pair.p;  // DebugViews.variance = fallback output
+    out->variance = d->fb_var.p;  // DebugViews.variance = fallback output (denoising.rs:296)
     d->cur = 1 - c;
     return VGI_OK;
 }

@@ -23,7 +23,8 @@ struct vgi_denoiser {
     vgi::DevBuf temporal_len[2];
     vgi::ColorSet fix[2];
     vgi::DevBuf fast_diff[2], fast_spec[2];
-    vgi::ColorSet pp[2];  // fallback output + a-trous ping-pong (pair = variance)
+    vgi::ColorSet pp[2];  // fallback colour output + a-trous ping-pong (pair = variance)
+    vgi::DevBuf fb_var;   // fallback variance (kept intact: the debug view of denoising.rs:296)
     int cur = 0;           // set index written this frame
     bool has_history = false;
     bool has_fast_history = false;
@@ -55,5 +56,6 @@ inline void denoiser_buffers(vgi_denoiser* d, std::vector<DevBuf*>& out) {
         out.push_back(&d->pp[k].spec);
         out.push_back(&d->pp[k].pair);
     }
+    out.push_back(&d->fb_var);
 }
 }  // namespace vgi

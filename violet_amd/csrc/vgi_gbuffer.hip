@@ -54,22 +54,22 @@ __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const Scen
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
     if (!geo) {
-        io.rw_depth[i] = 0.0f;
-        io.rw_gbuffer[i] = 0u;
-        io.rw_gbuffer[plane + i] = 0u;
-        io.rw_gbuffer[2 * plane + i] = 0u;
-        io.rw_gbuffer[3 * plane + i] = 0u;
+        G(io.rw_depth)[i] = 0.0f;
+        G(io.rw_gbuffer)[i] = 0u;
+        G(io.rw_gbuffer)[plane + i] = 0u;
+        G(io.rw_gbuffer)[2 * plane + i] = 0u;
+        G(io.rw_gbuffer)[3 * plane + i] = 0u;
         return;
     }
     float fz = (gx ? fabsf(dx - depth) : 0.0f) + (gy ? fabsf(dy - depth) : 0.0f);
     f3 ax = gx ? abs3(nx - n) : mk3s(0.0f);
     f3 ay = gy ? abs3(ny - n) : mk3s(0.0f);
     float fn = length3(ax + ay);
-    io.rw_depth[i] = depth;
-    io.rw_gbuffer[i] = encode_gbuffer_color(sh.base_color, sh.metallic);
-    io.rw_gbuffer[plane + i] = encode_gbuffer_normal(n, sh.roughness);
-    io.rw_gbuffer[2 * plane + i] = encode_gbuffer_fwidth(fz, fn);
-    io.rw_gbuffer[3 * plane + i] = 1u;
+    G(io.rw_depth)[i] = depth;
+    G(io.rw_gbuffer)[i] = encode_gbuffer_color(sh.base_color, sh.metallic);
+    G(io.rw_gbuffer)[plane + i] = encode_gbuffer_normal(n, sh.roughness);
+    G(io.rw_gbuffer)[2 * plane + i] = encode_gbuffer_fwidth(fz, fn);
+    G(io.rw_gbuffer)[3 * plane + i] = 1u;
 }
 
 template <bool kCount>

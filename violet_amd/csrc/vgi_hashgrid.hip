@@ -27,7 +27,7 @@ __device__ __forceinline__ void k_hg_decay_body(uint32_t* decay_words, HashCell*
     uint32_t set = 0;
     for (; set < 4u;) {
         uint32_t set_addr = addr_offset + set * 4u;
-        uint32_t packed = decay_words[set_addr >> 2];
+        uint32_t packed = G(decay_words)[set_addr >> 2];
         if (packed == 0u) break;
         uint32_t updated = 0u;
         uint32_t decay = 0u;
@@ -40,7 +40,7 @@ __device__ __forceinline__ void k_hg_decay_body(uint32_t* decay_words, HashCell*
                 HashCell e;
                 e.checksum = 0u;
                 e.r = e.g = e.b = 0.0f;
-                storage[set_addr + i] = e;
+                gst(storage + set_addr + i, e);
             } else {
                 updated |= (decay << shift);
                 occ |= (1u << (set * 4u + i));
@@ -61,12 +61,12 @@ __device__ __forceinline__ void k_hg_decay_body(uint32_t* decay_words, HashCell*
                 slice[src_set] ^= (d << last_shift);
                 uint32_t fs_shift = (first_space & 3u) * 8u;
                 slice[dst_set] |= (d << fs_shift);
-                HashCell c = storage[addr_offset + last];
-                storage[addr_offset + first_space] = c;
+                HashCell c = gld(storage + addr_offset + last);
+                gst(storage + addr_offset + first_space, c);
                 HashCell e;
                 e.checksum = 0u;
                 e.r = e.g = e.b = 0.0f;
-                storage[addr_offset + last] = e;
+                gst(storage + addr_offset + last, e);
                 occ |= (1u << first_space);
                 occ ^= (1u << last);
             } else {
@@ -74,7 +74,7 @@ __device__ __forceinline__ void k_hg_decay_body(uint32_t* decay_words, HashCell*
             }
         }
     }
-    for (uint32_t i = 0; i < set; ++i) decay_words[(addr_offset >> 2) + i] = slice[i];
+    for (uint32_t i = 0; i < set; ++i) G(decay_words)[(addr_offset >> 2) + i] = slice[i];
 }
 
 __global__ void __launch_bounds__(64) k_hg_decay(uint32_t* decay_words, HashCell* storage) {
@@ -86,10 +86,10 @@ __device__ __forceinline__ bool hash_grid_find_or_insert(HashCell* buf, uint32_t
                                                          uint32_t& addr) {
     uint32_t beg = hash << kHashGridBucketShift;
     for (addr = beg; addr < beg + kHashGridBucketSize; ++addr) {
-        uint32_t slot = __atomic_load_n(&buf[addr].checksum, __ATOMIC_RELAXED);
+        uint32_t slot = __hip_atomic_load(G(&buf[addr].checksum), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (slot == checksum) return true;
         if (slot == 0u) {
-            uint32_t prev = atomicCAS(&buf[addr].checksum, 0u, checksum);
+            uint32_t prev = g_atomic_cas(&buf[addr].checksum, 0u, checksum);
             if (prev == 0u || prev == checksum) return true;
         }
     }
@@ -99,13 +99,13 @@ __device__ __forceinline__ bool hash_grid_find_or_insert(HashCell* buf, uint32_t
 __device__ __forceinline__ void k_hg_select_body(const HashQuery* queries, const uint32_t* counter, HashCell* storage, uint32_t* selection, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
     uint32_t id = bx_ * 64u + threadIdx.x;
     if (id >= kHashGridMaxQueries || id >= *counter) return;
-    HashQuery q = queries[id];
+    HashQuery q = gld(queries + id);
     uint32_t addr;
     if (!hash_grid_find_or_insert(storage, q.hash, q.checksum, addr)) return;
     uint32_t rng = jenkins_hash(id ^ frame_hash);
     uint32_t r = lcg_xorshift(rng);
     uint32_t word = (r & (0xFFFFFFFFu << (kHashGridMaxQueriesShift + 1u))) | (1u << 31) | id;
-    atomicMax(&selection[addr], word);
+    g_atomic_max(&selection[addr], word);
 }
 
 __global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, const uint32_t* counter, HashCell* storage, uint32_t* selection, uint32_t frame_hash) {
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, cons
 
 __device__ __forceinline__ void k_hg_compact_body(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter, uint32_t bx_, uint32_t by_) {
     uint32_t cell = bx_ * 64u + threadIdx.x;
-    uint32_t word = cell < kHashGridNumCells ? selection[cell] : 0u;
+    uint32_t word = cell < kHashGridNumCells ? G(selection)[cell] : 0u;
     bool active = word != 0u;
     uint64_t mask = __ballot(active);
     uint32_t lane = threadIdx.x & 63u;
@@ -123,12 +123,12 @@ __device__ __forceinline__ void k_hg_compact_body(const uint32_t* selection, uin
     uint32_t base = 0;
     if (total != 0u) {
         int leader = __ffsll((long long)mask) - 1;
-        if ((int)lane == leader) base = atomicAdd(out_counter, total);
+        if ((int)lane == leader) base = g_atomic_add(out_counter, total);
         base = __shfl(base, leader);
     }
     if (active) {
-        out_index[base + below] = word & ((1u << kHashGridMaxQueriesShift) - 1u);
-        out_addr[base + below] = cell;
+        G(out_index)[base + below] = word & ((1u << kHashGridMaxQueriesShift) - 1u);
+        G(out_addr)[base + below] = cell;
     }
 }
 

@@ -46,7 +46,7 @@ __device__ __forceinline__ bool hash_grid_find(const HashCell* buf, uint32_t has
                                                uint32_t& addr) {
     uint32_t beg = hash << kHashGridBucketShift;
     for (addr = beg; addr < beg + kHashGridBucketSize; ++addr) {
-        HashCell c = buf[addr];
+        HashCell c = gld(buf + addr);
         if (c.checksum == checksum) {
             radiance = mk3(c.r, c.g, c.b);
             return true;
@@ -73,20 +73,20 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
     SurfaceHit r;
     r.missed = h.tri == 0xFFFFFFFFu;
     if (r.missed) return r;
-    TriShade sh = s.shade[h.tri];
+    TriShade sh = gld(s.shade + h.tri);
     const float* P = s.positions;
-    f3 p0 = mk3(P[3 * sh.i0], P[3 * sh.i0 + 1], P[3 * sh.i0 + 2]);
-    f3 p1 = mk3(P[3 * sh.i1], P[3 * sh.i1 + 1], P[3 * sh.i1 + 2]);
-    f3 p2 = mk3(P[3 * sh.i2], P[3 * sh.i2 + 1], P[3 * sh.i2 + 2]);
+    f3 p0 = mk3(gld(P + 3 * sh.i0), gld(P + 3 * sh.i0 + 1), gld(P + 3 * sh.i0 + 2));
+    f3 p1 = mk3(gld(P + 3 * sh.i1), gld(P + 3 * sh.i1 + 1), gld(P + 3 * sh.i1 + 2));
+    f3 p2 = mk3(gld(P + 3 * sh.i2), gld(P + 3 * sh.i2 + 1), gld(P + 3 * sh.i2 + 2));
     float w0 = (1.0f - h.u) - h.v;
     r.position = (p0 * w0 + p1 * h.u) + p2 * h.v;
     f3 ng = normalize3(cross3(p1 - p0, p2 - p0));
     f3 n;
     if (s.normals) {
         const float* N = s.normals;
-        f3 n0 = mk3(N[3 * sh.i0], N[3 * sh.i0 + 1], N[3 * sh.i0 + 2]);
-        f3 n1 = mk3(N[3 * sh.i1], N[3 * sh.i1 + 1], N[3 * sh.i1 + 2]);
-        f3 n2 = mk3(N[3 * sh.i2], N[3 * sh.i2 + 1], N[3 * sh.i2 + 2]);
+        f3 n0 = mk3(gld(N + 3 * sh.i0), gld(N + 3 * sh.i0 + 1), gld(N + 3 * sh.i0 + 2));
+        f3 n1 = mk3(gld(N + 3 * sh.i1), gld(N + 3 * sh.i1 + 1), gld(N + 3 * sh.i1 + 2));
+        f3 n2 = mk3(gld(N + 3 * sh.i2), gld(N + 3 * sh.i2 + 1), gld(N + 3 * sh.i2 + 2));
         n = normalize3((n0 * w0 + n1 * h.u) + n2 * h.v);
     } else {
         n = ng;
@@ -97,7 +97,7 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
     }
     r.normal = n;
     r.normal_geo = ng;
-    const Material& m = s.materials[sh.material];
+    const Material m = gld(s.materials + sh.material);
     r.base_color = mk3(m.base_color[0], m.base_color[1], m.base_color[2]);
     r.metallic = m.metallic;
     r.roughness = m.roughness;
@@ -108,7 +108,7 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
 // frame's denoised diffuse (the screen-space radiance cache).
 template <class S, bool kCount = false>
 __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt, const FrameParams& fp, f3 o, f3 d,
-                                                              bool has_prev_frame, int32_t* stack) {
+                                                              bool has_prev_frame, LdsStack stack) {
     RadianceTraceResult ret;
     Hit hit = trace_closest<kCount>(rt.scene, o, d, kRayTMin, kRayTMax, stack);
     SurfaceHit sh = shade_hit(rt.scene, hit, d);
@@ -145,7 +145,7 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
             uint32_t ppx = minu(f2u(floorf(u * (float)rt.pw)), rt.pw - 1u);
             uint32_t ppy = minu(f2u(floorf(v * (float)rt.ph)), rt.ph - 1u);
             size_t pi = (size_t)ppy * rt.pw + ppx;
-            float prev_depth_value = rt.prev_depth[pi];
+            float prev_depth_value = gld(rt.prev_depth + pi);
             if (fabsf(reproj_depth - prev_depth_value) < 0.0001f) {
                 radiance = radiance + diffuse_rho * S::ld3(rt.prev_indirect_diffuse, pi);
                 sample_world_radiance_cache = false;
@@ -159,7 +159,7 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
         uint32_t cell_addr;
         if (hash_grid_find(rt.hg_storage, hsh, chk, cell_radiance, cell_addr))
             radiance = radiance + diffuse_rho * cell_radiance;
-        uint32_t qi = atomicAdd(rt.query_counter, 1u);
+        uint32_t qi = g_atomic_add(rt.query_counter, 1u);
         if (qi < kHashGridMaxQueries) {
             HashQuery q;
             q.px = sh.position.x;
@@ -168,7 +168,7 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
             q.normal = normal_encode_oct_u32(sh.normal_geo);
             q.hash = hsh;
             q.checksum = chk;
-            rt.queries[qi] = q;
+            gst(rt.queries + qi, q);
         }
     }
     ret.radiance = radiance;

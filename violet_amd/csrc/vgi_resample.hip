@@ -30,8 +30,8 @@ __device__ __forceinline__ float sample_bilinear1(const float* tex, uint32_t w, 
     int x0 = (int)x0f, y0 = (int)y0f;
     int xa = clampi(x0, 0, (int)w - 1), xb = clampi(x0 + 1, 0, (int)w - 1);
     int ya = clampi(y0, 0, (int)h - 1), yb = clampi(y0 + 1, 0, (int)h - 1);
-    float c00 = tex[(size_t)ya * w + xa], c10 = tex[(size_t)ya * w + xb];
-    float c01 = tex[(size_t)yb * w + xa], c11 = tex[(size_t)yb * w + xb];
+    float c00 = G(tex)[(size_t)ya * w + xa], c10 = G(tex)[(size_t)ya * w + xb];
+    float c01 = G(tex)[(size_t)yb * w + xa], c11 = G(tex)[(size_t)yb * w + xb];
     float top = c00 * (1.0f - fx) + c10 * fx;
     float bot = c01 * (1.0f - fx) + c11 * fx;
     return top * (1.0f - fy) + bot * fy;
@@ -65,7 +65,7 @@ __device__ __forceinline__ void k_ao_gen_body(const FrameParams& fp, const vgi_a
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    float depth = io.depth_buffer[i];
+    float depth = G(io.depth_buffer)[i];
     if (depth == 0.0f) {
         S::st2(io.rw_ao_texture, i, mk2(0.0f, 0.0f));
         return;
@@ -82,7 +82,7 @@ __device__ __forceinline__ void k_ao_gen_body(const FrameParams& fp, const vgi_a
     if (hu == saturate(hu) && hv == saturate(hv) && pc.has_prev_frame)
         hist = sample_bilinear2<S>(io.prev_ao_texture, w, h, hu, hv);
     if (pc.has_new_sample) {
-        HitPosNormal hit = hit_decode(io.hit_pos_normal_texture[2 * i], io.hit_pos_normal_texture[2 * i + 1]);
+        HitPosNormal hit = hit_decode(G(io.hit_pos_normal_texture)[2 * i], G(io.hit_pos_normal_texture)[2 * i + 1]);
         f3 pos2 = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
         float dist = length3(hit.pos - pos2);
         float ao = dist < pc.radius_ws ? 0.0f : 1.0f;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void k_ao_filter_body(const vgi_ao_filter_io& io, uin
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    float depth = io.depth_buffer[i];
+    float depth = G(io.depth_buffer)[i];
     if (depth == 0.0f) {
         S::st2(io.rw_filtered_ao_texture, i, mk2(0.0f, 0.0f));
         return;
@@ -124,7 +124,7 @@ __device__ __forceinline__ void k_ao_filter_body(const vgi_ao_filter_io& io, uin
             // bounds at the far edges, wrapping at the near edges); clamped here
             int cx = clampi((int)px + x, 0, (int)w - 1), cy = clampi((int)py + y, 0, (int)h - 1);
             size_t j = (size_t)cy * w + cx;
-            float sd = io.depth_buffer[j];
+            float sd = G(io.depth_buffer)[j];
             if (fabsf(depth - sd) > 0.0005f) {
                 row[x + 1] = center;
                 continue;
@@ -158,7 +158,7 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    const float depth = io.gbuffer_depth[i];
+    const float depth = G(io.gbuffer_depth)[i];
     // (a pixel without geometry writes reservoir 0 here but carries on; the
     // final store overwrites it, as in the reference)
     f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
@@ -178,12 +178,12 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
     if (sample_prev) {
         uint32_t ppx = minu(f2u(u * (float)w), w - 1u), ppy = minu(f2u(v * (float)h), h - 1u);
         size_t j = (size_t)ppy * w + ppx;
-        ReservoirSimple rs = reservoir_decode_32b(io.prev_reservoir_texture[j]);
+        ReservoirSimple rs = reservoir_decode_32b(G(io.prev_reservoir_texture)[j]);
         M = rs.M;
         W = rs.W;
         rad = S::ld3(io.prev_hit_radiance_texture, j);
-        ex = io.prev_hit_pos_normal_texture[2 * j];
-        ey = io.prev_hit_pos_normal_texture[2 * j + 1];
+        ex = G(io.prev_hit_pos_normal_texture)[2 * j];
+        ey = G(io.prev_hit_pos_normal_texture)[2 * j + 1];
         if (M == 0u) {
             rad = mk3s(0.0f);
             ex = ey = 0u;
@@ -201,8 +201,8 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
         float w_sum = prev_w + new_w;
         float chance = new_w / w_sum;
         if (lcg_rand(rng) < chance) {
-            ex = io.new_hit_pos_normal_texture[2 * i];
-            ey = io.new_hit_pos_normal_texture[2 * i + 1];
+            ex = G(io.new_hit_pos_normal_texture)[2 * i];
+            ey = G(io.new_hit_pos_normal_texture)[2 * i + 1];
             rad = new_rad;
             target = new_target;
         }
@@ -212,9 +212,9 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
     ReservoirSimple rs;
     rs.M = M;
     rs.W = W;
-    io.rw_reservoir_texture[i] = reservoir_encode_32b(rs);
-    io.rw_hit_pos_normal_texture[2 * i] = ex;
-    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
+    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, rad);
 }
 
@@ -245,10 +245,10 @@ struct DiffWR {
 template <class S>
 __device__ __forceinline__ DiffWR diff_load(const vgi_ind_diff_spatial_io& io, size_t j, f3 pix_pos, f3 pix_n) {
     DiffWR r;
-    ReservoirSimple rs = reservoir_decode_32b(io.temp_reservoir_texture[j]);
+    ReservoirSimple rs = reservoir_decode_32b(G(io.temp_reservoir_texture)[j]);
     r.M = rs.M;
     r.W = rs.W;
-    HitPosNormal hit = hit_decode(io.temp_hit_pos_normal_texture[2 * j], io.temp_hit_pos_normal_texture[2 * j + 1]);
+    HitPosNormal hit = hit_decode(G(io.temp_hit_pos_normal_texture)[2 * j], G(io.temp_hit_pos_normal_texture)[2 * j + 1]);
     r.pos = hit.pos;
     r.nrm = hit.normal;
     r.rad = S::ld3(io.temp_hit_radiance_texture, j);
@@ -265,17 +265,17 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float depth = io.gbuffer_depth[i];
+    float depth = G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
-        io.rw_reservoir_texture[i] = 0u;
-        io.rw_hit_pos_normal_texture[2 * i] = 0u;
-        io.rw_hit_pos_normal_texture[2 * i + 1] = 0u;
+        G(io.rw_reservoir_texture)[i] = 0u;
+        G(io.rw_hit_pos_normal_texture)[2 * i] = 0u;
+        G(io.rw_hit_pos_normal_texture)[2 * i + 1] = 0u;
         S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
         S::st3(io.rw_lighting_texture, i, mk3s(0.0f));
         return;
     }
     const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
-    f3 nrm = decode_gbuffer_normal(io.gbuffer_color[plane + i]).normal;
+    f3 nrm = decode_gbuffer_normal(G(io.gbuffer_color)[plane + i]).normal;
     float ao = S::ld2(io.ao_texture, i).x;
     DiffWR res = diff_load<S>(io, i, position, nrm);
     float w_sum = (res.W * res.target) * (float)res.M;
@@ -290,8 +290,8 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
         int ox = (int)(x * radius), oy = (int)(y * radius);
         int cx = clampi((int)px + ox, 0, (int)w - 1), cy = clampi((int)py + oy, 0, (int)h - 1);
         size_t j = (size_t)cy * w + cx;
-        f3 nn = decode_gbuffer_normal(io.gbuffer_color[plane + j]).normal;
-        float depth_n = io.gbuffer_depth[j];
+        f3 nn = decode_gbuffer_normal(G(io.gbuffer_color)[plane + j]).normal;
+        float depth_n = G(io.gbuffer_depth)[j];
         bool geometrical_diff = dot3(nrm, nn) < tab.cos_tol;
         f3 position_n = depth_to_position(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, depth_n);
         float dist_to_plane = dot3(position_n - position, nrm);
@@ -328,11 +328,11 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
     ReservoirSimple rs;
     rs.M = res.M;
     rs.W = res.W;
-    io.rw_reservoir_texture[i] = reservoir_encode_32b(rs);
+    G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
     uint32_t ex, ey;
     hit_encode(res.pos, res.nrm, ex, ey);
-    io.rw_hit_pos_normal_texture[2 * i] = ex;
-    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, res.rad);
     f3 dir = normalize3(res.pos - position);
     float NoL = saturate(dot3(nrm, dir));
@@ -400,10 +400,10 @@ __device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, cons
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float depth = io.gbuffer_depth[i];
+    float depth = G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
-        io.rw_reservoir_texture[2 * i] = 0u;
-        io.rw_reservoir_texture[2 * i + 1] = 0u;
+        G(io.rw_reservoir_texture)[2 * i] = 0u;
+        G(io.rw_reservoir_texture)[2 * i + 1] = 0u;
         return;
     }
     const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
@@ -427,8 +427,8 @@ __device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, cons
     if (sample_prev) {
         uint32_t ppx = minu(f2u(u * (float)w), w - 1u), ppy = minu(f2u(v * (float)h), h - 1u);
         size_t j = (size_t)ppy * w + ppx;
-        M = io.prev_reservoir_texture[2 * j];
-        W = asfloat(io.prev_reservoir_texture[2 * j + 1]);
+        M = G(io.prev_reservoir_texture)[2 * j];
+        W = asfloat(G(io.prev_reservoir_texture)[2 * j + 1]);
         const float* hp = io.prev_hit_pos_texture + 4 * j;
         res_pos = mk3(hp[0], hp[1], hp[2]);
         res_tf = hp[3];
@@ -475,8 +475,8 @@ __device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, cons
         W = w_sum / res_tf;  // the 1/M of the MIS weights is already in w_sum
     else
         W = 0.0f;
-    io.rw_reservoir_texture[2 * i] = M;
-    io.rw_reservoir_texture[2 * i + 1] = asuint(W);
+    G(io.rw_reservoir_texture)[2 * i] = M;
+    G(io.rw_reservoir_texture)[2 * i + 1] = asuint(W);
     float* o = io.rw_hit_pos_texture + 4 * i;
     o[0] = res_pos.x;
     o[1] = res_pos.y;
@@ -507,7 +507,7 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float depth = io.gbuffer_depth[i];
+    float depth = G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
         S::st3(io.rw_lighting_texture, i, mk3s(0.0f));
         S::st1(io.rw_ray_len_texture, i, 0.0f);
@@ -515,8 +515,8 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
     }
     const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
     GB g = load_gb(io.gbuffer_color, plane, i);
-    uint32_t M = io.reservoir_texture[2 * i];
-    float W = asfloat(io.reservoir_texture[2 * i + 1]);
+    uint32_t M = G(io.reservoir_texture)[2 * i];
+    float W = asfloat(G(io.reservoir_texture)[2 * i + 1]);
     const float* hp = io.hit_pos_texture + 4 * i;
     f3 hit_pos = mk3(hp[0], hp[1], hp[2]);
     f3 hit_rad = S::ld3(io.hit_radiance_texture, i);
@@ -539,7 +539,7 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
         if (cx == (int)px && cy == (int)py) continue;
         size_t j = (size_t)cy * w + cx;
         GB gn = load_gb(io.gbuffer_color, plane, j);
-        float depth_n = io.gbuffer_depth[j];
+        float depth_n = G(io.gbuffer_depth)[j];
         bool diff = dot3(g.normal, gn.normal) < cos_tol;
         f3 position_n = depth_to_position(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, depth_n);
         float dist_to_plane = dot3(position_n - position, g.normal);
@@ -548,8 +548,8 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
             radius = fmaxf(radius * 0.5f, 1.5f);
             continue;
         }
-        uint32_t Mn = io.reservoir_texture[2 * j];
-        float Wn = asfloat(io.reservoir_texture[2 * j + 1]);
+        uint32_t Mn = G(io.reservoir_texture)[2 * j];
+        float Wn = asfloat(G(io.reservoir_texture)[2 * j + 1]);
         const float* hpn = io.hit_pos_texture + 4 * j;
         f3 hit_pos_n = mk3(hpn[0], hpn[1], hpn[2]);
         f3 hit_rad_n = S::ld3(io.hit_radiance_texture, j);

@@ -77,5 +77,29 @@ __device__ __forceinline__ void tile_pixel_at(uint32_t bx, uint32_t by, uint32_t
     py = by * 8u + ((q >> 2) << 1) + (qi >> 1);
 }
 __device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) { tile_pixel_at(blockIdx.x, blockIdx.y, px, py); }
+
+// An image pointer as a global-memory pointer (see gld in vgi_common.h):
+// G(io.image)[i] reads and writes global memory even when the pointer was
+// loaded from a job table rather than passed as a kernel argument.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// Device-scope relaxed atomics on global memory.  Through a global pointer
+// the compiler's atomic optimizer folds a wave's same-address adds into one
+// atomic; through a flat pointer it issues one per lane (a 64-fold
+// contention on hot counters such as the cache's query counter).
+__device__ __forceinline__ uint32_t g_atomic_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t g_atomic_max(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_max(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t g_atomic_cas(uint32_t* p, uint32_t expected, uint32_t desired) {
+    __hip_atomic_compare_exchange_strong(G(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;
+}
 }  // namespace vgi
 #endif

@@ -34,14 +34,14 @@ __device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, co
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    float depth = io.gbuffer_depth[i];
+    float depth = G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
-        io.rw_hit_pos_normal_texture[2 * i] = 0u;
-        io.rw_hit_pos_normal_texture[2 * i + 1] = 0u;
+        G(io.rw_hit_pos_normal_texture)[2 * i] = 0u;
+        G(io.rw_hit_pos_normal_texture)[2 * i + 1] = 0u;
         S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
         return;
     }
-    GBufferNormal g = decode_gbuffer_normal(io.gbuffer_color[(size_t)w * h + i]);
+    GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[(size_t)w * h + i]);
     f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
     uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
     float u0 = lcg_rand(rng);
@@ -50,8 +50,8 @@ __device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, co
     RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
     uint32_t ex, ey;
     hit_encode(r.position_ws, r.normal_ws, ex, ey);
-    io.rw_hit_pos_normal_texture[2 * i] = ex;
-    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, r.radiance);
 }
 
@@ -68,9 +68,9 @@ __device__ __forceinline__ void k_diff_sample_validate_body(const FrameParams& f
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    float depth = io.depth_texture[i];
+    float depth = G(io.depth_texture)[i];
     if (depth == 0.0f) return;
-    HitPosNormal hit = hit_decode(io.rw_hit_pos_normal_texture[2 * i], io.rw_hit_pos_normal_texture[2 * i + 1]);
+    HitPosNormal hit = hit_decode(G(io.rw_hit_pos_normal_texture)[2 * i], G(io.rw_hit_pos_normal_texture)[2 * i + 1]);
     f3 prev_rad = S::ld3(io.rw_hit_radiance_texture, i);
     f3 origin = depth_to_position(fp.prev_inv_view_proj, px, py, w, h, depth);
     f3 dir = normalize3(hit.pos - origin);
@@ -82,14 +82,14 @@ __device__ __forceinline__ void k_diff_sample_validate_body(const FrameParams& f
     float m_scale = saturate(1.0f - (1.0f * lumi_diff) / lumi_prev);
     m_scale = pos_changed ? 0.0f : m_scale;
     if ((lumi_prev > 0.0f) && (m_scale <= (1.0f - 0.5f / 32.0f))) {
-        ReservoirSimple rs = reservoir_decode_32b(io.rw_reservoir_texture[i]);
+        ReservoirSimple rs = reservoir_decode_32b(G(io.rw_reservoir_texture)[i]);
         rs.M = f2u(floorf((float)rs.M * m_scale));
-        io.rw_reservoir_texture[i] = reservoir_encode_32b(rs);
+        G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
     }
     uint32_t ex, ey;
     hit_encode(r.position_ws, r.normal_ws, ex, ey);
-    io.rw_hit_pos_normal_texture[2 * i] = ex;
-    io.rw_hit_pos_normal_texture[2 * i + 1] = ey;
+    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, r.radiance);
 }
 
@@ -138,12 +138,12 @@ __device__ __forceinline__ void k_spec_sample_gen_body(const FrameParams& fp, co
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float depth = io.gbuffer_depth[i];
+    float depth = G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
         S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
         return;
     }
-    GBufferNormal g = decode_gbuffer_normal(io.gbuffer_color[plane + i]);
+    GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[plane + i]);
     f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
     uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
     f3 view_dir = normalize3(xyz(fp.view_pos) - position);
@@ -171,11 +171,11 @@ __device__ __forceinline__ void k_spec_sample_validate_body(const FrameParams& f
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    float prev_depth = rt.prev_depth[i];
+    float prev_depth = G(rt.prev_depth)[i];
     if (prev_depth == 0.0f) return;
     f3 position = position_biased(fp.prev_inv_view_proj, px, py, w, h, prev_depth);
-    uint32_t M = io.rw_prev_reservoir_texture[2 * i];
-    uint32_t Wbits = io.rw_prev_reservoir_texture[2 * i + 1];
+    uint32_t M = G(io.rw_prev_reservoir_texture)[2 * i];
+    uint32_t Wbits = G(io.rw_prev_reservoir_texture)[2 * i + 1];
     const float* hp = io.rw_prev_hit_pos_texture + 4 * i;
     f3 prev_hit_pos = mk3(hp[0], hp[1], hp[2]);
     float prev_target = hp[3];
@@ -188,8 +188,8 @@ __device__ __forceinline__ void k_spec_sample_validate_body(const FrameParams& f
     bool changed = dr.x > lim || dr.y > lim || dr.z > lim;
     if (changed) {
         (void)M;
-        io.rw_prev_reservoir_texture[2 * i] = 1u;
-        io.rw_prev_reservoir_texture[2 * i + 1] = Wbits;
+        G(io.rw_prev_reservoir_texture)[2 * i] = 1u;
+        G(io.rw_prev_reservoir_texture)[2 * i + 1] = Wbits;
         float* o = io.rw_prev_hit_pos_texture + 4 * i;
         o[0] = r.position_ws.x;
         o[1] = r.position_ws.y;
@@ -209,8 +209,8 @@ __device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, c
     VGI_TRAVERSAL_STACK(stack);
     uint32_t id = bx_ * 64u + threadIdx.x;
     if (id >= *io.compact_query_counter_buffer) return;
-    uint32_t qi = io.compact_query_index_buffer[id];
-    uint32_t cell = io.compact_query_cell_addr_buffer[id];
+    uint32_t qi = G(io.compact_query_index_buffer)[id];
+    uint32_t cell = G(io.compact_query_cell_addr_buffer)[id];
     uint32_t rng = jenkins_hash(id ^ frame_hash);
     const HashQuery q = reinterpret_cast<const HashQuery*>(io.hash_grid_query_buffer)[qi];
     f3 normal = normal_decode_oct_u32(q.normal);
@@ -226,7 +226,7 @@ __device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, c
     storage[cell].r = blend.x;
     storage[cell].g = blend.y;
     storage[cell].b = blend.z;
-    io.rw_hash_grid_decay_buffer[cell] = 10;
+    G(io.rw_hash_grid_decay_buffer)[cell] = 10;
 }
 
 template <class S, bool kCount>

@@ -27,25 +27,25 @@ struct StorageF32 {
     static constexpr int kScalarBytes = 4;
     VGI_HD static f3 ld3(const void* p, size_t i) {
         const float* f = (const float*)p + 3 * i;
-        return mk3(f[0], f[1], f[2]);
+        return mk3(gld(f), gld(f + 1), gld(f + 2));
     }
     VGI_HD static void st3(void* p, size_t i, f3 v) {
         float* f = (float*)p + 3 * i;
-        f[0] = v.x;
-        f[1] = v.y;
-        f[2] = v.z;
+        gst(f, v.x);
+        gst(f + 1, v.y);
+        gst(f + 2, v.z);
     }
     VGI_HD static f2 ld2(const void* p, size_t i) {
         const float* f = (const float*)p + 2 * i;
-        return mk2(f[0], f[1]);
+        return mk2(gld(f), gld(f + 1));
     }
     VGI_HD static void st2(void* p, size_t i, f2 v) {
         float* f = (float*)p + 2 * i;
-        f[0] = v.x;
-        f[1] = v.y;
+        gst(f, v.x);
+        gst(f + 1, v.y);
     }
-    VGI_HD static float ld1(const void* p, size_t i) { return ((const float*)p)[i]; }
-    VGI_HD static void st1(void* p, size_t i, float v) { ((float*)p)[i] = v; }
+    VGI_HD static float ld1(const void* p, size_t i) { return gld((const float*)p + i); }
+    VGI_HD static void st1(void* p, size_t i, float v) { gst((float*)p + i, v); }
 };
 
 struct StorageF16 {
@@ -56,23 +56,23 @@ struct StorageF16 {
     static constexpr int kScalarBytes = 2;
     VGI_HD static f3 ld3(const void* p, size_t i) {
         const uint16_t* h = (const uint16_t*)p + 3 * i;
-        return mk3(f16tof32(h[0]), f16tof32(h[1]), f16tof32(h[2]));
+        return mk3(f16tof32(gld(h)), f16tof32(gld(h + 1)), f16tof32(gld(h + 2)));
     }
     VGI_HD static void st3(void* p, size_t i, f3 v) {
         uint16_t* h = (uint16_t*)p + 3 * i;
-        h[0] = (uint16_t)f32tof16(v.x);
-        h[1] = (uint16_t)f32tof16(v.y);
-        h[2] = (uint16_t)f32tof16(v.z);
+        gst(h, (uint16_t)f32tof16(v.x));
+        gst(h + 1, (uint16_t)f32tof16(v.y));
+        gst(h + 2, (uint16_t)f32tof16(v.z));
     }
     VGI_HD static f2 ld2(const void* p, size_t i) {
-        uint32_t u = ((const uint32_t*)p)[i];
+        uint32_t u = gld((const uint32_t*)p + i);
         return mk2(f16tof32(u & 0xFFFFu), f16tof32(u >> 16));
     }
     VGI_HD static void st2(void* p, size_t i, f2 v) {
-        ((uint32_t*)p)[i] = f32tof16(v.x) | (f32tof16(v.y) << 16);
+        gst((uint32_t*)p + i, f32tof16(v.x) | (f32tof16(v.y) << 16));
     }
-    VGI_HD static float ld1(const void* p, size_t i) { return f16tof32(((const uint16_t*)p)[i]); }
-    VGI_HD static void st1(void* p, size_t i, float v) { ((uint16_t*)p)[i] = (uint16_t)f32tof16(v); }
+    VGI_HD static float ld1(const void* p, size_t i) { return f16tof32(gld((const uint16_t*)p + i)); }
+    VGI_HD static void st1(void* p, size_t i, float v) { gst((uint16_t*)p + i, (uint16_t)f32tof16(v)); }
 };
 
 // History length: R8_UNORM in the reference (denoising.rs:122-125).  Stored

@@ -13,6 +13,18 @@
 
 namespace vgi {
 
+// Explicit address spaces for the traversal's memory traffic: the scene
+// arrays are global memory and the stack is LDS, also when the scene arrives
+// through a pointer loaded from memory (the fused multi-view kernels read it
+// from a job table, where the compiler cannot infer the address space and
+// would otherwise emit flat accesses that also count against lgkmcnt).
+typedef __attribute__((address_space(3))) int32_t* LdsStack;
+typedef float Vf4 __attribute__((ext_vector_type(4)));
+// 16-byte load of scene data (element k of a float4 array) from global memory.
+__device__ __forceinline__ Vf4 ld_global16(const void* p, int k) {
+    return ((const __attribute__((address_space(1))) Vf4*)p)[k];
+}
+
 struct RayInv {
     f3 o, d, idir;
 };
@@ -37,17 +49,17 @@ __device__ __forceinline__ float slab(const RayInv& r, float lox, float hix, flo
 }
 
 __device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, int32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    float4 a = p[0];
-    float4 b = p[1];
-    float4 c = p[2];
-    int4 d = *reinterpret_cast<const int4*>(p + 3);
+    const void* p = nodes + i;
+    Vf4 a = ld_global16(p, 0);
+    Vf4 b = ld_global16(p, 1);
+    Vf4 c = ld_global16(p, 2);
+    Vf4 d = ld_global16(p, 3);
     BvhNode n;
     n.c0xy = float4_{a.x, a.y, a.z, a.w};
     n.c1xy = float4_{b.x, b.y, b.z, b.w};
     n.z = float4_{c.x, c.y, c.z, c.w};
-    n.link[0] = d.x;
-    n.link[1] = d.y;
+    n.link[0] = __float_as_int(d.x);
+    n.link[1] = __float_as_int(d.y);
     return n;
 }
 
@@ -63,7 +75,7 @@ __device__ __forceinline__ void count_traversal(const SceneView& s, bool any_hit
 // kCount: the instrumented build (traversal statistics) that adds this ray's
 // node and triangle counts to s.counters.
 template <bool kAnyHit, bool kCount = false>
-__device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+__device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
     RayInv r = make_ray(o, d);
     Hit h;
     h.t = tmax;
@@ -104,8 +116,8 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             uint32_t first = enc >> 4, cnt = enc & 15u;
             for (uint32_t k = 0; k < cnt; ++k) {
                 if (kCount) ++n_tris;
-                const float4* tp = reinterpret_cast<const float4*>(s.tris + first + k);
-                float4 a = tp[0], e1 = tp[1], e2 = tp[2];
+                const void* tp = s.tris + first + k;
+                Vf4 a = ld_global16(tp, 0), e1 = ld_global16(tp, 1), e2 = ld_global16(tp, 2);
                 float t, u, v;
                 if (intersect_tri(o, d, mk3(a.x, a.y, a.z), mk3(e1.x, e1.y, e1.z), mk3(e2.x, e2.y, e2.z), tmin, t, u,
                                   v)) {
@@ -132,12 +144,12 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
 }
 
 template <bool kCount = false>
-__device__ __forceinline__ Hit trace_closest(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+__device__ __forceinline__ Hit trace_closest(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
     return traverse<false, kCount>(s, o, d, tmin, tmax, stack);
 }
 // Shadow ray: true if something is hit on [tmin, tmax] (trace_shadow returns !this).
 template <bool kCount = false>
-__device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float tmin, float tmax, int32_t* stack) {
+__device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
     return traverse<true, kCount>(s, o, d, tmin, tmax, stack).tri != 0xFFFFFFFFu;
 }
 
@@ -147,6 +159,6 @@ __device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float 
 // at most d stacked far children, so stack_depth entries always suffice.
 #define VGI_TRAVERSAL_STACK(name)                        \
     extern __shared__ int32_t name##_mem[];              \
-    int32_t* name = name##_mem + (threadIdx.x & 63u)
+    LdsStack name = (LdsStack)(name##_mem) + (threadIdx.x & 63u)
 
 }  // namespace vgi
