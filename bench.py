@@ -44,6 +44,43 @@ RESAMPLE_DESC = {
 }
 
 
+# Pass name -> kernel symbol (for counter summaries of that kernel).
+PASS_KERNEL = {
+    "denoise_atrous": "k_atrous", "denoise_temporal": "k_temporal", "denoise_disocclusion_fix": "k_disocclusion",
+    "denoise_spatial_fallback": "k_fallback", "ind_diff_spatial_resample": "k_diff_spatial",
+    "ind_diff_temporal_resample": "k_diff_temporal", "ind_spec_temporal_resample": "k_spec_temporal",
+    "ind_spec_spatial_resample": "k_spec_spatial", "raytraced_ao_gen": "k_ao_gen", "raytraced_ao_filter": "k_ao_filter",
+    "hash_grid_query_compact": "k_hg_compact", "hash_grid_decay": "k_hg_decay", "hash_grid_query_select": "k_hg_select",
+}
+
+
+def calibration_copy(n: int = 4):
+    """Plain device copies of a known size (256 MiB read + 256 MiB written
+    each), for scaling HBM counters in a rocprofv3 --pmc run of this script."""
+    import torch
+
+    src = torch.ones((256 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(n):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(workload: str, pass_name: str):
+    """HBM bytes per launch of a pass from the committed counter summary of
+    this workload (profiles/traffic_<workload>.json), or None."""
+    path = ROOT / "profiles" / f"traffic_{workload}.json"
+    sym = PASS_KERNEL.get(pass_name)
+    if not sym or not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    for k, v in data["kernels"].items():
+        if k.split("<")[0].split("::")[-1] == sym:
+            return round(v["fetch_bytes"] + v["write_bytes"])
+    return None
+
+
 def _lib_const(name: str):
     from violet_amd import _lib
 
@@ -180,6 +217,8 @@ def run_frame(args):
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
+    if args.calibration_copy:
+        calibration_copy()
     lat = []
     base = args.warmup
     for k in range(min(args.steps, 120)):
@@ -419,6 +458,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--resample", default="frame_loop", choices=sorted(RESAMPLE))
     ap.add_argument("--count-frames", type=int, default=8, help="frames run with traversal statistics")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="run plain 256 MiB device copies after warmup (scale for rocprofv3 --pmc counters)")
     ap.add_argument("--forms", default="batched,serving,precision",
                     help="comma list of forms reported beside the plain frame path ('' for none)")
     ap.add_argument("--big-triangles", type=int, default=1_000_000)
@@ -445,6 +486,8 @@ def main():
         sys.exit(2)
 
     r = run_frame(args) if frame else run_denoise(args)
+    workload_name = (f"restir_gi_frame_{args.width}x{args.height}" if frame else
+                     f"denoise_only_{args.width}x{args.height}")
     mpix = args.steps * r["pixels"] / r["elapsed"] / 1e6
     prof = r["prof"]
     counts = r.get("counts", {})
@@ -507,7 +550,8 @@ def main():
                     "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
                     "storage": args.storage, "parallelism": "single-gpu"}),
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(workload_name, dname),
                      "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": bytes_per_launch},
         "passes": passes,
         "trace": trace,

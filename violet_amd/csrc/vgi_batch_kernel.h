@@ -12,12 +12,15 @@
 namespace vgi {
 
 // kBarrier: the job body shares LDS across its workgroup, so the next tile
-// must wait until every thread is done with this one.
-template <class Job, int kThreads, bool kBarrier>
+// must wait until every thread is done with this one.  kXcdOrder: walk the
+// tiles in XCD-aware order (xcd_remap), for bodies whose consecutive tiles
+// share cache lines.
+template <class Job, int kThreads, bool kBarrier, bool kXcdOrder>
 __global__ void __launch_bounds__(kThreads) k_batch(const BatchTable<Job>* __restrict__ t) {
     const uint32_t n = t->hd.njobs;
     const uint32_t total = t->hd.tile_begin[n];
-    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t first = kXcdOrder ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (uint32_t b = first; b < total; b += gridDim.x) {
         uint32_t lo = 0, hi = n;
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -33,14 +36,14 @@ __global__ void __launch_bounds__(kThreads) k_batch(const BatchTable<Job>* __res
     }
 }
 
-template <template <class> class J, int kThreads = 64, bool kBarrier = false>
+template <template <class> class J, int kThreads = 64, bool kBarrier = false, bool kXcdOrder = false>
 vgi_status batch_launch(const void* dev, uint32_t storage, uint32_t grid, size_t lds, hipStream_t st) {
     if (storage == VGI_STORAGE_F32)
-        hipLaunchKernelGGL((k_batch<J<StorageF32>, kThreads, kBarrier>), dim3(grid), dim3(kThreads), lds, st,
-                           (const BatchTable<J<StorageF32>>*)dev);
+        hipLaunchKernelGGL((k_batch<J<StorageF32>, kThreads, kBarrier, kXcdOrder>), dim3(grid), dim3(kThreads), lds,
+                           st, (const BatchTable<J<StorageF32>>*)dev);
     else
-        hipLaunchKernelGGL((k_batch<J<StorageF16>, kThreads, kBarrier>), dim3(grid), dim3(kThreads), lds, st,
-                           (const BatchTable<J<StorageF16>>*)dev);
+        hipLaunchKernelGGL((k_batch<J<StorageF16>, kThreads, kBarrier, kXcdOrder>), dim3(grid), dim3(kThreads), lds,
+                           st, (const BatchTable<J<StorageF16>>*)dev);
     return check_launch("k_batch");
 }
 

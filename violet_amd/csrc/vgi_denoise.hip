@@ -539,8 +539,10 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     __shared__ float lds[kFCount][kAtrousN];
     const int w = (int)io.width, h = (int)io.height;
     const size_t plane = (size_t)w * h;
-    const int rx = (int)(bx_ / nbx), bx = (int)(bx_ % nbx);
-    const int ry = (int)(by_ / nby), by = (int)(by_ % nby);
+    // residue innermost: the s x s residue classes of one block region are
+    // consecutive tiles (the same XCD's L2 serves their shared cache lines)
+    const int rx = (int)(bx_ % (uint32_t)step), bx = (int)(bx_ / (uint32_t)step);
+    const int ry = (int)(by_ % (uint32_t)step), by = (int)(by_ / (uint32_t)step);
     const int t = (int)threadIdx.x;
 
     for (int e = t; e < kAtrousN; e += 256) {
@@ -620,7 +622,9 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
 
 template <class S>
 __global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
-    k_atrous_body<S>(io, step, nbx, nby, blockIdx.x, blockIdx.y);
+    const uint32_t gx = gridDim.x;
+    const uint32_t l = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+    k_atrous_body<S>(io, step, nbx, nby, l % gx, l / gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,7 +799,7 @@ const BatchOps* batch_ops_denoise(int kind) {
     static const BatchOps fast_clamp{sizeof(BatchTable<JobDenFastClamp<StorageF32>>), fill_den_fast_clamp,
                                      batch_launch<JobDenFastClamp>, 64, false};
     static const BatchOps atrous{sizeof(BatchTable<JobDenAtrous<StorageF32>>), fill_den_atrous,
-                                 batch_launch<JobDenAtrous, 256, true>, 256, false};
+                                 batch_launch<JobDenAtrous, 256, true, true>, 256, false};
     switch (kind) {
         case kPassDenTemporal: return &temporal;
         case kPassDenDisocclusion: return &disocclusion;

@@ -78,6 +78,20 @@ __device__ __forceinline__ void tile_pixel_at(uint32_t bx, uint32_t by, uint32_t
 }
 __device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) { tile_pixel_at(blockIdx.x, blockIdx.y, px, py); }
 
+// XCD-aware workgroup order.  MI355X dispatches consecutive workgroup ids
+// round-robin over its 8 XCDs, each with its own L2.  xcd_remap maps the
+// dispatch id p in [0, n) to a logical id such that every XCD gets one
+// contiguous range of logical ids (the first n % 8 XCDs one more).  The
+// a-trous pass orders its tiles so that the residue classes of one region
+// are consecutive, and this keeps them in one L2 (measured: the plain tile
+// passes, whose neighbours are contiguous already, get slower with it).
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t p, uint32_t n) {
+    const uint32_t q = n / kNumXcd, r = n % kNumXcd;
+    const uint32_t xcd = p % kNumXcd, k = p / kNumXcd;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
 // An image pointer as a global-memory pointer (see gld in vgi_common.h):
 // G(io.image)[i] reads and writes global memory even when the pointer was
 // loaded from a job table rather than passed as a kernel argument.
