@@ -77,6 +77,52 @@ def test_closest_hit_large_scene_vs_oracle_bvh(vgi):
     assert np.array_equal(g_thv[hit].view(np.uint32), o_thv[hit].astype(np.float32).view(np.uint32))
 
 
+def _onion(n_quads, seed):
+    """n_quads unit quads stacked along z (slightly shifted in x and y), so a
+    ray along z overlaps every box of the BVH."""
+    from violet_amd import scene
+
+    rng = np.random.default_rng(seed)
+    z = np.sort(rng.uniform(0.0, 10.0, n_quads))
+    j = rng.uniform(-0.2, 0.2, (n_quads, 2))
+    c = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float64)
+    xy = c[None, :, :] + j[:, None, :]
+    pos = np.concatenate([xy, np.broadcast_to(z[:, None, None], (n_quads, 4, 1))], 2).reshape(-1, 3)
+    base = 4 * np.arange(n_quads, dtype=np.uint32)[:, None]
+    idx = np.concatenate([base + np.array([0, 1, 2], np.uint32), base + np.array([0, 2, 3], np.uint32)], 1)
+    n_t = 2 * n_quads
+    return scene.Mesh(positions=pos.astype(np.float32), normals=np.tile(np.array([0, 0, 1], np.float32), (4 * n_quads, 1)),
+                      indices=idx.reshape(-1, 3), tri_material=np.zeros(n_t, np.uint32), albedo=np.full((1, 3), 0.5),
+                      metallic=np.zeros(1), roughness=np.full(1, 0.5))
+
+
+def test_deep_traversal_stack_spills_exactly(vgi):
+    """Rays through 25000 stacked quads hit every box on their way down, so
+    their traversal stacks grow past the 16 entries kept in LDS (to 21 on this
+    scene, by a host simulation of the traversal) into private memory; hits
+    must still equal the brute-force loop bit for bit, also for oblique rays."""
+    import oracle
+    from violet_amd import scene
+
+    mesh = _onion(25000, 4)
+    sc = scene.Scene(mesh)
+    rng = np.random.default_rng(4)
+    m = 4000
+    below = rng.random(m) < 0.5
+    o = np.column_stack([rng.uniform(-0.8, 0.8, m), rng.uniform(-0.8, 0.8, m), np.where(below, -1.0, 11.0)])
+    d = np.column_stack([rng.normal(0, 0.05, m), rng.normal(0, 0.05, m), np.where(below, 1.0, -1.0)])
+    d[: m // 4] = rng.standard_normal((m // 4, 3))  # a quarter in random directions
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.full((m, 1), 5e-4), d, np.full((m, 1), 1000.0)], 1).astype(np.float32)
+    g_thv, g_id = _gpu_trace(sc, rays, False)
+    o_thv, o_id = oracle.trace(32, mesh.positions, mesh.indices, rays, brute=True)
+    assert np.array_equal(g_id, o_id), f"hit ids differ on {np.mean(g_id != o_id):.3%} of rays"
+    hit = g_id != 0xFFFFFFFF
+    assert hit.mean() > 0.7
+    assert np.array_equal(g_thv[hit].view(np.uint32), o_thv[hit].astype(np.float32).view(np.uint32))
+    assert np.array_equal(_gpu_trace(sc, rays, True)[1] != 0xFFFFFFFF, hit)
+
+
 def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
     """The instrumented traversal (trace statistics) returns the same hits and
     counts what the BVH implies: with a 2-triangle scene the root is a leaf,

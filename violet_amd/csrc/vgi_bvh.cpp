@@ -3,13 +3,17 @@
  *
  * Replaces violet's driver-built BLAS/TLAS (render_scene.rs:1170-1210 BLAS
  * per geometry group, :1557-1712 TLAS over instances).  Instances are baked
- * to world space by the caller, so one binary BVH over all triangles serves
- * every ray.  Build: 3-axis binned SAH (32 bins); leaves hold at most 15
- * triangles; below depth 40 the split switches to the object median so the
- * depth (and the traversal stack) stays bounded by kBvhMaxDepth.
+ * to world space by the caller, so one BVH over all triangles serves every
+ * ray.  Build: a binary tree by 3-axis binned SAH (32 bins; leaves hold at
+ * most 15 triangles; from depth 36 the split is the object median, so the
+ * depth stays bounded by kBvhMaxDepth), then collapsed into a 4-wide tree:
+ * each 4-wide node takes its binary node's two children and repeatedly
+ * opens the inner child with the largest surface area until it has four
+ * children (or only leaves are left).
  */
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <new>
 #include <vector>
 
@@ -42,6 +46,12 @@ struct Box {
     }
 };
 
+// Binary build node: both child boxes (padded) and links.
+struct Node2 {
+    Box box[2];
+    int32_t link[2];
+};
+
 struct Prim {
     Box box;
     float c[3];
@@ -54,7 +64,7 @@ constexpr int kMedianDepth = 36;
 
 struct Builder {
     std::vector<Prim> prims;
-    std::vector<BvhNode> nodes;
+    std::vector<Node2> nodes;
     std::vector<uint32_t> order;
     int max_depth = 0;
 
@@ -161,22 +171,72 @@ struct Builder {
                              [a](const Prim& x, const Prim& y) { return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.id < y.id); });
         }
         int32_t idx = (int32_t)nodes.size();
-        nodes.push_back(BvhNode{});
+        nodes.push_back(Node2{});
         Box lb, rb;
         int32_t l = build(begin, mid, depth + 1, lb);
         int32_t r = build(mid, end, depth + 1, rb);
         pad(lb);
         pad(rb);
-        BvhNode& nd = nodes[idx];
-        nd.c0xy = float4_{lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]};
-        nd.c1xy = float4_{rb.lo[0], rb.hi[0], rb.lo[1], rb.hi[1]};
-        nd.z = float4_{lb.lo[2], lb.hi[2], rb.lo[2], rb.hi[2]};
+        Node2& nd = nodes[idx];
+        nd.box[0] = lb;
+        nd.box[1] = rb;
         nd.link[0] = l;
         nd.link[1] = r;
-        nd.pad[0] = nd.pad[1] = 0;
         return idx;
     }
 };
+
+// Collapse of the binary tree into 4-wide nodes (depth-first, so a parent
+// precedes its children in memory).
+struct Collapser {
+    const std::vector<Node2>& n2;
+    std::vector<BvhNode> out;
+    int max_level = 0;
+
+    int32_t emit(int32_t bin, int level) {
+        max_level = std::max(max_level, level);
+        Box box[4];
+        int32_t link[4];
+        int cnt = 2;
+        for (int k = 0; k < 2; ++k) {
+            box[k] = n2[bin].box[k];
+            link[k] = n2[bin].link[k];
+        }
+        while (cnt < 4) {
+            int best = -1;
+            double best_area = -1.0;
+            for (int k = 0; k < cnt; ++k)
+                if (link[k] >= 0 && box[k].area() > best_area) {
+                    best_area = box[k].area();
+                    best = k;
+                }
+            if (best < 0) break;
+            const Node2& c = n2[link[best]];
+            box[best] = c.box[0];
+            link[best] = c.link[0];
+            box[cnt] = c.box[1];
+            link[cnt] = c.link[1];
+            ++cnt;
+        }
+        const int32_t idx = (int32_t)out.size();
+        out.push_back(BvhNode{});
+        int32_t child[4];
+        for (int k = 0; k < 4; ++k) child[k] = k < cnt && link[k] >= 0 ? emit(link[k], level + 1) : (k < cnt ? link[k] : ~0);
+        BvhNode& nd = out[idx];
+        float* f[6] = {&nd.lox.x, &nd.hix.x, &nd.loy.x, &nd.hiy.x, &nd.loz.x, &nd.hiz.x};
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < cnt;
+            for (int a = 0; a < 3; ++a) {
+                f[2 * a][k] = used ? box[k].lo[a] : INFINITY;
+                f[2 * a + 1][k] = used ? box[k].hi[a] : INFINITY;
+            }
+            nd.link[k] = child[k];
+            nd.pad[k] = 0;
+        }
+        return idx;
+    }
+};
+static_assert(offsetof(float4_, y) == 4 && offsetof(float4_, w) == 12, "float4_ component layout");
 
 }  // namespace
 
@@ -226,23 +286,24 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     Box root_box;
     int32_t root = b.build(0, (int)nt, 0, root_box);
     if (root < 0) {
-        // tiny scene: wrap the single leaf in a node whose second child box
-        // sits at +inf on every axis, where no slab interval can reach it
-        // (an inverted box would not do: the slab test orders the planes)
-        BvhNode nd{};
-        Box lb = root_box;
-        Builder::pad(lb);
-        const float inf = INFINITY;
-        nd.c0xy = float4_{lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]};
-        nd.c1xy = float4_{inf, inf, inf, inf};
-        nd.z = float4_{lb.lo[2], lb.hi[2], inf, inf};
+        // tiny scene: wrap the single leaf in a node whose other child is an
+        // empty leaf with a box at +inf on every axis, where no slab interval
+        // can reach it (an inverted box would not do: the slab test orders
+        // the planes)
+        Node2 nd{};
+        nd.box[0] = root_box;
+        Builder::pad(nd.box[0]);
+        for (int a = 0; a < 3; ++a) nd.box[1].lo[a] = nd.box[1].hi[a] = INFINITY;
         nd.link[0] = root;
-        nd.link[1] = root;
+        nd.link[1] = ~0;
         b.nodes.push_back(nd);
         root = 0;
     }
     if (b.max_depth + 1 > kBvhMaxDepth)
         return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: BVH depth %d exceeds %d", b.max_depth + 1, kBvhMaxDepth);
+    Collapser c4{b.nodes, {}, 0};
+    c4.out.reserve(b.nodes.size() / 2 + 1);
+    root = c4.emit(root, 0);
 
     std::vector<BvhTri> tris(nt);
     std::vector<TriShade> shade(nt);
@@ -279,7 +340,7 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     vgi_scene* sc = new (std::nothrow) vgi_scene();
     if (!sc) return set_error(VGI_E_NOMEM, "vgi_scene_create: out of host memory");
     vgi_status st;
-    if ((st = upload(sc->nodes, b.nodes)) != VGI_OK || (st = upload(sc->tris, tris)) != VGI_OK ||
+    if ((st = upload(sc->nodes, c4.out)) != VGI_OK || (st = upload(sc->tris, tris)) != VGI_OK ||
         (st = upload(sc->shade, shade)) != VGI_OK || (st = upload(sc->positions, pos)) != VGI_OK ||
         (st = upload(sc->normals, nrm)) != VGI_OK || (st = upload(sc->materials, mats)) != VGI_OK) {
         delete sc;
@@ -287,17 +348,22 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     }
     sc->num_vertices = nv;
     sc->num_triangles = nt;
-    sc->depth = (uint32_t)b.max_depth + 1;
+    sc->depth = (uint32_t)c4.max_level + 1;  // levels of 4-wide nodes
     sc->view.nodes = sc->nodes.as<BvhNode>();
     sc->view.tris = sc->tris.as<BvhTri>();
     sc->view.shade = sc->shade.as<TriShade>();
     sc->view.positions = sc->positions.as<float>();
     sc->view.normals = desc->normals ? sc->normals.as<float>() : nullptr;
     sc->view.materials = sc->materials.as<Material>();
-    sc->view.num_nodes = (uint32_t)b.nodes.size();
+    sc->view.num_nodes = (uint32_t)c4.out.size();
     sc->view.num_tris = nt;
     sc->view.root = root;
-    sc->view.stack_depth = sc->depth;
+    sc->view.stack_depth = 3u * sc->depth;  // at most 3 far children pushed per level
+    if (sc->view.stack_depth > kStackMaxEntries) {
+        delete sc;
+        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: BVH of %u levels exceeds the traversal stack",
+                         (unsigned)c4.max_level + 1);
+    }
     sc->view.counters = nullptr;
     *out = sc;
     return VGI_OK;

@@ -5,10 +5,12 @@
  * (restir/raytrace.inc.hlsl:31-56,67-82; render_scene.rs:1170-1712).
  *
  * Layout (built on the host by vgi_bvh.cpp, uploaded once per scene):
- *   nodes : binary BVH, one 64-byte node holds both children's boxes
- *           (x/y of child 0, x/y of child 1, z of both, child links), so a
- *           wave fetches a node with four 16-byte loads.  A child link >= 0
- *           is an inner node index; < 0 encodes a leaf ~((first << 4) | n).
+ *   nodes : 4-wide BVH, one 128-byte node (one L2 line) holds its four
+ *           children's boxes as six float4 planes (lo.x of children 0..3,
+ *           hi.x, lo.y, hi.y, lo.z, hi.z) and four child links, fetched with
+ *           eight 16-byte loads.  A child link >= 0 is an inner node index;
+ *           < 0 encodes a leaf ~((first << 4) | n).  Unused children have a
+ *           box at +inf on every axis, which no slab interval reaches.
  *   tris  : leaf-ordered triangles as {v0, e1 = v1-v0, e2 = v2-v0} (3 x
  *           float4, 48 bytes); v0.w carries the scene triangle id.
  * Child boxes are padded by a small relative margin so the slab test is
@@ -25,13 +27,11 @@
 namespace vgi {
 
 struct BvhNode {
-    float4_ c0xy;  // c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y
-    float4_ c1xy;  // c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y
-    float4_ z;     // c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z
-    int32_t link[2];
-    int32_t pad[2];
+    float4_ lox, hix, loy, hiy, loz, hiz;  // component c = child c
+    int32_t link[4];
+    int32_t pad[4];
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 
 struct BvhTri {
     float4_ v0;  // w = triangle id (bits)
@@ -63,18 +63,28 @@ struct SceneView {
     const Material* materials;
     uint32_t num_nodes;
     uint32_t num_tris;
-    int32_t root;           // link of the root (node 0, or a leaf for tiny scenes)
-    uint32_t stack_depth;   // levels of the tree = most traversal-stack entries a ray can hold
+    int32_t root;           // link of the root (node 0)
+    uint32_t stack_depth;   // most traversal-stack entries a ray can hold (3 per node level)
     unsigned long long* counters;  // traversal statistics slots (instrumented kernels only)
 };
 
 
-// Dynamic LDS bytes of a 64-lane traversal stack for this scene.
-inline size_t traversal_lds_bytes(const SceneView& s) { return (size_t)s.stack_depth * 64u * sizeof(int32_t); }
-
 constexpr float kRayTMin = 0.0005f;  // raytrace.inc.hlsl:39,70
 constexpr float kRayTMax = 1000.0f;  // raytrace.inc.hlsl:40,71
-constexpr int kBvhMaxDepth = 64;     // builder guarantees depth <= this (= stack size)
+constexpr int kBvhMaxDepth = 64;     // binary build depth bound (the 4-wide tree is shallower)
+
+// Traversal stack: the bottom kLdsStackEntries entries of each lane live in
+// LDS, deeper ones in private (scratch) memory.  A one-wave workgroup with a
+// 4 KiB stack leaves LDS for more resident waves than the registers allow,
+// and the scratch part is touched only by rays whose stack grows past 16.
+constexpr uint32_t kLdsStackEntries = 16;
+constexpr uint32_t kStackMaxEntries = 3 * kBvhMaxDepth;  // 3 far children per 4-wide level
+
+// Dynamic LDS bytes of a 64-lane traversal stack for this scene.
+inline size_t traversal_lds_bytes(const SceneView& s) {
+    const uint32_t e = s.stack_depth < kLdsStackEntries ? s.stack_depth : kLdsStackEntries;
+    return (size_t)e * 64u * sizeof(int32_t);
+}
 
 struct Hit {
     float t;

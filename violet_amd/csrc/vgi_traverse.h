@@ -1,10 +1,12 @@
 /*
  * vgi_traverse.h — device-side BVH traversal (closest hit and any hit).
  *
- * One lane per ray.  The traversal stack lives in LDS, interleaved across
- * the 64 lanes of the wave (entry k of lane l at [k*64 + l]) so that a push
- * or pop by all lanes touches 64 consecutive dwords (no bank conflicts).
- * Near child first; the far child is pushed.  Boxes are tested with the
+ * One lane per ray over the 4-wide BVH.  The traversal stack lives in LDS,
+ * interleaved across the 64 lanes of the wave (entry k of lane l at
+ * [k*64 + l]) so that a push or pop by all lanes touches 64 consecutive
+ * dwords (no bank conflicts).  At a node the hit children are ordered by
+ * entry distance; the nearest is visited next and the others are pushed
+ * farthest first.  Boxes are tested with the
  * slab method in exact IEEE single precision; NaNs from 0*inf drop out of
  * fminf/fmaxf, which only widens the interval (conservative).
  */
@@ -48,26 +50,22 @@ __device__ __forceinline__ float slab(const RayInv& r, float lox, float hix, flo
     return tn <= tf ? tn : INFINITY;
 }
 
-__device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, int32_t i) {
-    const void* p = nodes + i;
-    Vf4 a = ld_global16(p, 0);
-    Vf4 b = ld_global16(p, 1);
-    Vf4 c = ld_global16(p, 2);
-    Vf4 d = ld_global16(p, 3);
-    BvhNode n;
-    n.c0xy = float4_{a.x, a.y, a.z, a.w};
-    n.c1xy = float4_{b.x, b.y, b.z, b.w};
-    n.z = float4_{c.x, c.y, c.z, c.w};
-    n.link[0] = __float_as_int(d.x);
-    n.link[1] = __float_as_int(d.y);
-    return n;
-}
-
 __device__ __forceinline__ void count_traversal(const SceneView& s, bool any_hit, uint32_t nodes, uint32_t tris) {
     unsigned long long* c = s.counters + 4u * ((blockIdx.x + 7u * blockIdx.y) % (unsigned)kTraceCounterSlots);
     atomicAdd(c + (any_hit ? 1 : 0), 1ull);
     atomicAdd(c + 2, (unsigned long long)nodes);
     atomicAdd(c + 3, (unsigned long long)tris);
+}
+
+__device__ __forceinline__ void sort_pair(float& ta, int32_t& la, float& tb, int32_t& lb) {
+    if (tb < ta) {
+        const float t = ta;
+        ta = tb;
+        tb = t;
+        const int32_t l = la;
+        la = lb;
+        lb = l;
+    }
 }
 
 // Closest hit over [tmin, tmax]; ties in t resolve to the lowest triangle id.
@@ -83,32 +81,46 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
     h.tri = 0xFFFFFFFFu;
     int sp = 0;
     int32_t link = s.root;
-    // Each node and leaf of a tree is reached at most once; the budget only
-    // guards against a corrupted BVH turning into an endless loop.
-    uint32_t budget = 2u * s.num_nodes + 8u;
+    // Each node and leaf of a tree is reached at most once (a node has at
+    // most 4 children); the budget only guards against a corrupted BVH
+    // turning into an endless loop.
+    uint32_t budget = 5u * s.num_nodes + 8u;
     uint32_t n_nodes = 0, n_tris = 0;
+    int32_t spill[kStackMaxEntries - kLdsStackEntries];
+    const int depth = (int)s.stack_depth;  // <= kStackMaxEntries for every scene vgi_scene_create builds
+    auto push = [&](int32_t x) {
+        if (sp < (int)kLdsStackEntries)
+            stack[sp * 64] = x;
+        else
+            spill[sp - (int)kLdsStackEntries] = x;
+        ++sp;
+    };
     while (budget-- != 0u) {
         if (link >= 0) {
             if (kCount) ++n_nodes;
-            BvhNode n = load_node(s.nodes, link);
-            float t0 = slab(r, n.c0xy.x, n.c0xy.y, n.c0xy.z, n.c0xy.w, n.z.x, n.z.y, tmin, h.t);
-            float t1 = slab(r, n.c1xy.x, n.c1xy.y, n.c1xy.z, n.c1xy.w, n.z.z, n.z.w, tmin, h.t);
-            bool b0 = t0 != INFINITY, b1 = t1 != INFINITY;
-            if (b0 && b1) {
-                int32_t nearl = n.link[0], farl = n.link[1];
-                if (t1 < t0) {
-                    nearl = n.link[1];
-                    farl = n.link[0];
-                }
-                if (sp < (int)s.stack_depth) {  // always true: the stack never outgrows the tree depth
-                    stack[sp * 64] = farl;
-                    ++sp;
-                }
-                link = nearl;
-                continue;
-            }
-            if (b0 || b1) {
-                link = b0 ? n.link[0] : n.link[1];
+            const void* p = s.nodes + link;
+            const Vf4 lox = ld_global16(p, 0), hix = ld_global16(p, 1), loy = ld_global16(p, 2);
+            const Vf4 hiy = ld_global16(p, 3), loz = ld_global16(p, 4), hiz = ld_global16(p, 5);
+            const Vf4 lk = ld_global16(p, 6);
+            float t0 = slab(r, lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, tmin, h.t);
+            float t1 = slab(r, lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, tmin, h.t);
+            float t2 = slab(r, lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, tmin, h.t);
+            float t3 = slab(r, lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, tmin, h.t);
+            int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
+            int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
+            // ascending entry distance (misses are +inf): a 5-exchange network
+            sort_pair(t0, l0, t1, l1);
+            sort_pair(t2, l2, t3, l3);
+            sort_pair(t0, l0, t2, l2);
+            sort_pair(t1, l1, t3, l3);
+            sort_pair(t1, l1, t2, l2);
+            if (t0 != INFINITY) {
+                // always true for a BVH from vgi_scene_create: the stack never
+                // outgrows 3 far children per level
+                if (t3 != INFINITY && sp < depth) push(l3);
+                if (t2 != INFINITY && sp < depth) push(l2);
+                if (t1 != INFINITY && sp < depth) push(l1);
+                link = l0;
                 continue;
             }
         } else {
@@ -137,7 +149,7 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
         }
         if (sp == 0) break;
         --sp;
-        link = stack[sp * 64];
+        link = sp < (int)kLdsStackEntries ? stack[sp * 64] : spill[sp - (int)kLdsStackEntries];
     }
     if (kCount) count_traversal(s, kAnyHit, n_nodes, n_tris);
     return h;
@@ -154,9 +166,10 @@ __device__ __forceinline__ bool trace_any(const SceneView& s, f3 o, f3 d, float 
 }
 
 // Per-wave LDS traversal stack: SceneView::stack_depth entries x 64 lanes in
-// dynamic LDS (launch with traversal_lds_bytes(scene) bytes), so a shallow
-// tree leaves LDS for more resident waves.  A ray at a node of depth d holds
-// at most d stacked far children, so stack_depth entries always suffice.
+// dynamic LDS (launch with traversal_lds_bytes(scene) bytes: at most
+// kLdsStackEntries entries per lane; deeper entries go to private memory).
+// A ray at a 4-wide node of level d holds at most 3(d+1) stacked children, so
+// stack_depth entries always suffice.
 #define VGI_TRAVERSAL_STACK(name)                        \
     extern __shared__ int32_t name##_mem[];              \
     LdsStack name = (LdsStack)(name##_mem) + (threadIdx.x & 63u)
