@@ -207,7 +207,8 @@ def run_frame(args):
     st = sc.stats()
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16 if args.storage == "f16" else _lib.STORAGE_F32,
-                                atrous_iterations=args.iterations, **RESAMPLE[args.resample])
+                                atrous_iterations=args.iterations, concurrent_lanes=not args.no_lanes,
+                                **RESAMPLE[args.resample])
     r = renderer.RestirRenderer(w, h, cfg)
     fps = frame_params_seq(w, h, args.frames)
 
@@ -458,6 +459,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--resample", default="frame_loop", choices=sorted(RESAMPLE))
     ap.add_argument("--count-frames", type=int, default=8, help="frames run with traversal statistics")
+    ap.add_argument("--no-lanes", action="store_true",
+                    help="every pass on one stream (per-pass times without overlap between pass chains)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run plain 256 MiB device copies after warmup (scale for rocprofv3 --pmc counters)")
     ap.add_argument("--forms", default="batched,serving,precision",
@@ -545,7 +548,8 @@ def main():
                     "height": args.height, "triangles": int(len(r["mesh"].indices)), "frames_cycled": args.frames,
                     "gi_rays_per_px": "1 diffuse + 1 specular (+1 shadow each)", "atrous_iterations": args.iterations,
                     "restir_params": RESAMPLE_DESC[args.resample],
-                    "storage": args.storage, "parallelism": "single-gpu"} if frame else
+                    "storage": args.storage, "parallelism": "single-gpu",
+                    "concurrent_lanes": not args.no_lanes} if frame else
                    {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
                     "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
                     "storage": args.storage, "parallelism": "single-gpu"}),
@@ -554,6 +558,10 @@ def main():
                      "traffic": measured_traffic(workload_name, dname),
                      "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": bytes_per_launch},
         "passes": passes,
+        "passes_note": ("per-pass HIP-event times; with concurrent lanes the hash-grid update overlaps the "
+                        "G-buffer and specular overlaps diffuse resampling, so overlapped passes read longer "
+                        "and shares do not add up to the frame" if frame and not args.no_lanes else
+                        "per-pass HIP-event times, one stream"),
         "trace": trace,
         "cpu_baseline": cpu,
     }

@@ -590,6 +590,11 @@ typedef struct vgi_renderer_config {
     uint32_t temporal_m_cap;         /* diffuse M clamp; specular uint(cap * roughness); 0 = 16 */
     uint32_t diff_spatial_taps;      /* diffuse spatial golden-spiral taps; 0 = 9, at most 16 */
     float spatial_radius_px;         /* spatial search radius; 0 = 0.05 * min(w, h) */
+    /* default 1: add_passes runs independent pass chains of a frame on a
+     * second stream owned by the renderer (hash-grid cache update beside the
+     * G-buffer, specular beside diffuse resampling), forked from and joined
+     * back into the caller's stream; 0 = every pass on the caller's stream */
+    uint32_t concurrent_lanes;
     vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
 } vgi_renderer_config;
 
@@ -653,8 +658,10 @@ typedef struct vgi_batch_config {
     uint32_t trace_groups;   /* 64-lane groups of the ray-tracing passes */
     uint32_t atrous_groups;  /* 256-lane groups of the a-trous passes */
     uint32_t use_graph;      /* 1: capture the frame once as a HIP graph and replay it */
+    uint32_t concurrent_lanes; /* 1: independent pass chains (hash-grid update beside the
+                                * G-buffer, specular beside diffuse) on a second stream */
 } vgi_batch_config;
-/* Defaults: 64 views, one group per tile everywhere, no graph. */
+/* Defaults: 64 views, one group per tile everywhere, no graph, two lanes. */
 vgi_status vgi_batch_default_config(vgi_batch_config* cfg);
 vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** out);
 vgi_status vgi_batch_destroy(vgi_batch* b);

@@ -1,0 +1,72 @@
+"""Every ctypes mirror of a struct in include/vgi.h has the C layout: a C
+program compiled with gcc against the header prints sizeof and the offset of
+each field the mirror names, and they must equal ctypes' (a field added on
+one side only, or in another order, fails here instead of corrupting
+arguments at run time)."""
+from __future__ import annotations
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _mirrors():
+    from violet_amd import _abi, _lib, scene
+
+    m = {
+        "vgi_profile_entry": _lib.ProfileEntry, "vgi_frame_params": _lib.FrameParams,
+        "vgi_denoise_temporal_io": _lib.DenoiseTemporalIO, "vgi_pc_denoise_temporal": _lib.PCDenoiseTemporal,
+        "vgi_denoise_spatial_io": _lib.DenoiseSpatialIO, "vgi_denoise_fast_clamp_io": _lib.DenoiseFastClampIO,
+        "vgi_denoise_atrous_io": _lib.DenoiseAtrousIO, "vgi_pc_denoise_atrous": _lib.PCDenoiseAtrous,
+        "vgi_denoiser_config": _lib.DenoiserConfig, "vgi_denoiser_input": _lib.DenoiserInput,
+        "vgi_denoiser_output": _lib.DenoiserOutput, "vgi_material": scene.Material,
+        "vgi_scene_desc": scene.SceneDesc, "vgi_scene_stats": scene.SceneStats,
+        "vgi_trace_resources": _abi.TraceResources, "vgi_ind_diff_sample_gen_io": _abi.DiffSampleGenIO,
+        "vgi_pc_sample_gen": _abi.PCSampleGen, "vgi_ind_diff_sample_validate_io": _abi.DiffSampleValidateIO,
+        "vgi_ind_spec_sample_gen_io": _abi.SpecSampleGenIO, "vgi_ind_spec_sample_validate_io": _abi.SpecSampleValidateIO,
+        "vgi_ao_gen_io": _abi.AOGenIO, "vgi_pc_ao_gen": _abi.PCAOGen, "vgi_ao_filter_io": _abi.AOFilterIO,
+        "vgi_ind_diff_temporal_io": _abi.DiffTemporalIO, "vgi_pc_ind_diff_temporal": _abi.PCDiffTemporal,
+        "vgi_ind_diff_spatial_io": _abi.DiffSpatialIO, "vgi_pc_frame_rand": _abi.PCFrameRand,
+        "vgi_pc_ind_diff_spatial": _abi.PCDiffSpatial, "vgi_pc_ind_spec_spatial": _abi.PCSpecSpatial,
+        "vgi_ind_spec_temporal_io": _abi.SpecTemporalIO, "vgi_pc_ind_spec_temporal": _abi.PCSpecTemporal,
+        "vgi_ind_spec_spatial_io": _abi.SpecSpatialIO, "vgi_hash_grid_decay_io": _abi.HGDecayIO,
+        "vgi_hash_grid_query_select_io": _abi.HGSelectIO, "vgi_pc_frame_hash": _abi.PCFrameHash,
+        "vgi_hash_grid_query_compact_io": _abi.HGCompactIO, "vgi_hash_grid_raygen_io": _abi.HGRaygenIO,
+        "vgi_gbuffer_io": _abi.GBufferIO, "vgi_renderer_config": _abi.RendererConfig,
+        "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
+    }
+    return m
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_ctypes_mirrors_match_c_layout(tmp_path):
+    import ctypes as C
+
+    mirrors = _mirrors()
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "vgi.h"', "int main(void) {"]
+    expect = []
+    for cname, py in mirrors.items():
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        expect.append(f"{cname} {C.sizeof(py)}")
+        for f in py._fields_:
+            lines.append(f'    printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+            expect.append(f"{cname}.{f[0]} {getattr(py, f[0]).offset}")
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True,
+                   capture_output=True, text=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")[:-1]
+    bad = [(e, g) for e, g in zip(expect, got) if e != g]
+    assert len(got) == len(expect) and not bad, f"layout mismatches (ctypes, C): {bad[:10]}"
+    # every struct the header declares has a mirror
+    import re
+
+    declared = set(re.findall(r"typedef struct (vgi_\w+) \{", (ROOT / "include" / "vgi.h").read_text()))
+    unmirrored = declared - set(mirrors) - {"vgi_hash_grid_cell", "vgi_hash_grid_query"}
+    assert not unmirrored, f"structs without a ctypes mirror: {sorted(unmirrored)}"

@@ -76,14 +76,14 @@ GBufferIO = _struct("GBufferIO", [("width", u32), ("height", u32), ("rw_depth", 
 RendererConfig = _struct("RendererConfig", [
     ("storage", u32), ("denoise", u32), ("ind_diff_validate", u32), ("ind_spec_validate", u32),
     ("hash_grid_cache_decay", u32), ("ao_radius", f32), ("sky_radiance", f32 * 3),
-    ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32),
+    ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32), ("concurrent_lanes", u32),
     ("denoiser", _lib.DenoiserConfig)])
 RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
     ("validation_frame", u32), ("hash_grid_updated", u32)])
 
 BatchConfig = _struct("BatchConfig", [("max_views", u32), ("pixel_groups", u32), ("trace_groups", u32),
-                                     ("atrous_groups", u32), ("use_graph", u32)])
+                                     ("atrous_groups", u32), ("use_graph", u32), ("concurrent_lanes", u32)])
 
 FPp = P(FrameParams)
 TRp = P(TraceResources)

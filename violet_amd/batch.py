@@ -17,14 +17,16 @@ from .scene import Scene
 
 @dataclass
 class BatchConfig:
-    """Persistent group counts (0 = the library default for this device) and
-    whether the frame is captured once as a HIP graph and replayed."""
+    """Persistent group counts (0 = the library default for this device),
+    whether the frame is captured once as a HIP graph and replayed, and
+    whether independent pass chains run on a second stream."""
 
     max_views: int = 64
     pixel_groups: int = 0
     trace_groups: int = 0
     atrous_groups: int = 0
     use_graph: bool = False
+    concurrent_lanes: bool = True
 
     def to_c(self) -> _abi.BatchConfig:
         c = _abi.BatchConfig()
@@ -37,6 +39,7 @@ class BatchConfig:
         if self.atrous_groups:
             c.atrous_groups = self.atrous_groups
         c.use_graph = int(self.use_graph)
+        c.concurrent_lanes = int(self.concurrent_lanes)
         return c
 
 

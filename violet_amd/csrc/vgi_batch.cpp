@@ -60,6 +60,9 @@ struct vgi_batch {
     std::vector<RecordSink> recs;
     hipStream_t cap = nullptr;  // capture/replay stream (graph mode)
     hipEvent_t ev_in = nullptr;
+    // second lane (concurrent_lanes): independent slot chains beside the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t key_storage[2] = {0, 0};
     size_t key_lds[2] = {0, 0};
@@ -79,6 +82,9 @@ struct vgi_batch {
             if (dev[a]) (void)hipFree(dev[a]);
         }
         if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (cap) (void)hipStreamDestroy(cap);
     }
 };
@@ -91,6 +97,7 @@ extern "C" vgi_status vgi_batch_default_config(vgi_batch_config* cfg) {
     cfg->trace_groups = 0;
     cfg->atrous_groups = 0;
     cfg->use_graph = 0;
+    cfg->concurrent_lanes = 1;
     return VGI_OK;
 }
 
@@ -132,6 +139,9 @@ extern "C" vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** 
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
     if (e == hipSuccess && c.use_graph) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete b;
         return check_hip(e, "vgi_batch_create");
@@ -156,17 +166,20 @@ static uint32_t groups_for(const vgi_batch* b, int slot, const BatchOps* o) {
     return g ? g : b->grid_bound[slot];
 }
 
-// Upload of the arena and every slot's launch on `st` (all slots in graph
-// mode, so the captured topology never changes; only occupied ones otherwise).
-static vgi_status issue_frame(vgi_batch* b, int a, uint32_t storage, size_t lds, hipStream_t st, bool all_slots,
+// Upload of the arena and every slot's launch on `main` and the side lane
+// (all slots in graph mode, so the captured topology never changes; only
+// occupied ones otherwise), joined back into `main` at the end.
+static vgi_status issue_slots(vgi_batch* b, int a, uint32_t storage, size_t lds, Lanes& lanes, bool all_slots,
                               bool profile) {
-    VGI_TRY_HIP(hipMemcpyAsync(b->dev[a], b->host[a], b->arena_bytes, hipMemcpyHostToDevice, st));
+    VGI_TRY_HIP(hipMemcpyAsync(b->dev[a], b->host[a], b->arena_bytes, hipMemcpyHostToDevice, lanes.main));
     for (int s = 0; s < kSlotCount; ++s) {
         const int kind = slot_kind(s);
         const BatchOps* o = batch_ops(kind);
         const BatchHeader* hd = (const BatchHeader*)((const char*)b->host[a] + b->offsets[s]);
         if (!all_slots && hd->njobs == 0) continue;
-        vgi_status r;
+        hipStream_t st;
+        vgi_status r = lanes.enter(s, st);
+        if (r != VGI_OK) return r;
         if (profile) {
             ProfScope prof(kind_name(kind), st, 0.0);
             r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, s, o), o->trace ? lds : 0, st);
@@ -176,6 +189,18 @@ static vgi_status issue_frame(vgi_batch* b, int a, uint32_t storage, size_t lds,
         if (r != VGI_OK) return r;
     }
     return VGI_OK;
+}
+
+static vgi_status issue_frame(vgi_batch* b, int a, uint32_t storage, size_t lds, hipStream_t st, bool all_slots,
+                              bool profile) {
+    Lanes lanes;
+    lanes.main = st;
+    lanes.side = b->side;
+    lanes.ev_fork = b->ev_fork;
+    lanes.ev_join = b->ev_join;
+    vgi_status r = issue_slots(b, a, storage, lds, lanes, all_slots, profile);
+    vgi_status j = lanes.finish();
+    return r != VGI_OK ? r : j;
 }
 
 extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene, uint32_t num_views,

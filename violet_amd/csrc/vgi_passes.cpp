@@ -5,10 +5,39 @@
 
 namespace vgi {
 
-vgi_status DirectSink::submit(int, const PassRec& r) {
-    void* s = (void*)stream;
+vgi_status Lanes::finish() {
+    if (!side || !side_work) return VGI_OK;
+    side_work = false;
+    VGI_TRY_HIP(hipEventRecord(ev_join, side));
+    VGI_TRY_HIP(hipStreamWaitEvent(main, ev_join, 0));
+    return VGI_OK;
+}
+
+vgi_status Lanes::enter(int slot, hipStream_t& st) {
+    st = main;
+    if (!side) return VGI_OK;
+    const int ph = slot_phase(slot);
+    if (ph != phase) {
+        vgi_status j = finish();
+        if (j != VGI_OK) return j;
+        VGI_TRY_HIP(hipEventRecord(ev_fork, main));
+        VGI_TRY_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        phase = ph;
+    }
+    if (slot_lane(slot)) {
+        st = side;
+        side_work = true;
+    }
+    return VGI_OK;
+}
+
+vgi_status DirectSink::submit(int slot, const PassRec& r) {
+    hipStream_t st;
+    vgi_status e = lanes.enter(slot, st);
+    if (e != VGI_OK) return e;
+    void* s = (void*)st;
     switch (r.kind) {
-        case kPassClear: return fill_zero(r.clear_ptr, r.clear_bytes, stream);
+        case kPassClear: return fill_zero(r.clear_ptr, r.clear_bytes, st);
         case kPassGBuffer: return vgi_pass_gbuffer_primary(&r.fp, r.scene, &r.io.gbuffer, s);
         case kPassHgDecay: return vgi_pass_hash_grid_decay(&r.io.hg_decay, s);
         case kPassHgSelect: return vgi_pass_hash_grid_query_select(&r.io.hg_select, &r.pc.frame_hash, s);

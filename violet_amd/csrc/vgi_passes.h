@@ -145,11 +145,44 @@ struct PassSink {
     virtual vgi_status submit(int slot, const PassRec& r) = 0;
 };
 
-// Launches each pass immediately on `stream` through its C entry point.
+// Phases of a frame: the slots of one phase depend only on earlier phases,
+// so the two lanes of a phase may run concurrently.  Phase 0: G-buffer
+// (lane 0) beside the hash-grid cache update (lane 1: it reads the previous
+// frame's depth and queries only); phase 1: diffuse (lane 0) beside specular
+// sampling and resampling (lane 1; both read the G-buffer and the updated
+// cache, and only append to the shared query buffer); phase 2: denoiser.
+inline int slot_phase(int slot) { return slot < kSlotDiffGen ? 0 : slot < kSlotDenClearFastDiff ? 1 : 2; }
+inline int slot_lane(int slot) {
+    return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygen) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial);
+}
+
+// The streams of a frame's slots, issued in slot order.  Without a side
+// stream every slot runs on `main`.  With one, lane-1 slots go to `side`:
+// entering a phase joins the side stream into the main one (if it has work)
+// and forks it again from the main stream, so a lane-1 slot waits for every
+// earlier phase and for nothing else; finish() joins it back at the end.
+// Works the same under stream capture (the side stream joins the capture).
+struct Lanes {
+    hipStream_t main = nullptr, side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int phase = -1;
+    bool side_work = false;
+    vgi_status enter(int slot, hipStream_t& st);
+    vgi_status finish();
+};
+
+// Launches each record at once through its C entry point.
 struct DirectSink : PassSink {
-    explicit DirectSink(hipStream_t s) : stream(s) {}
+    explicit DirectSink(hipStream_t s) { lanes.main = s; }
+    DirectSink(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+        lanes.main = s;
+        lanes.side = side;
+        lanes.ev_fork = fork;
+        lanes.ev_join = join;
+    }
     vgi_status submit(int slot, const PassRec& r) override;
-    hipStream_t stream;
+    vgi_status finish() { return lanes.finish(); }
+    Lanes lanes;
 };
 
 // Keeps the records of one view's frame by slot.

@@ -51,8 +51,14 @@ struct vgi_renderer {
     const void* prev_indirect_diffuse = nullptr;  // last frame's denoised (or raw) diffuse
     int prev_ind_diff_slot = -1;                  // 0/1 = diff_light[k]; 2/3 = denoiser pp[k-2]
 
+    // second lane of add_passes (concurrent_lanes): stream + fork/join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     ~vgi_renderer() {
         if (den) vgi_denoiser_destroy(den);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
     }
     std::vector<std::pair<std::string, DevBuf*>> images() {
         return {{"depth0", &depth[0]},         {"depth1", &depth[1]},
@@ -113,6 +119,7 @@ extern "C" vgi_status vgi_renderer_default_config(vgi_renderer_config* c) {
     c->ind_diff_validate = 1;
     c->ind_spec_validate = 1;
     c->hash_grid_cache_decay = 1;
+    c->concurrent_lanes = 1;
     c->ao_radius = 2.0f;
     c->sky_radiance[0] = c->sky_radiance[1] = c->sky_radiance[2] = 1.0f;
     c->denoiser.disocclusion_fix = 1;
@@ -175,6 +182,14 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     A(r->hg_caddr, 4 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
     A(r->hg_ccount, 4);
     if (st == VGI_OK) st = vgi_denoiser_create(width, height, c.storage, &c.denoiser, &r->den);
+    if (st == VGI_OK && c.concurrent_lanes) {
+        // non-blocking: no implicit ordering against the legacy null stream,
+        // which would serialise the two lanes when the caller passes it
+        hipError_t e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) st = check_hip(e, "vgi_renderer_create: side stream");
+    }
     if (st != VGI_OK) {
         delete r;
         return st;
@@ -402,8 +417,11 @@ uint32_t vgi_renderer_storage(const vgi_renderer* r) { return r->cfg.storage; }
 
 extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp,
                                               uint32_t frame_index, vgi_renderer_output* out, void* stream) {
-    DirectSink sink((hipStream_t)stream);
-    return vgi_renderer_plan(r, scene, fp, frame_index, out, sink);
+    VGI_REQUIRE(r, "vgi_renderer_add_passes: null renderer");
+    DirectSink sink((hipStream_t)stream, r->side, r->ev_fork, r->ev_join);
+    vgi_status s = vgi_renderer_plan(r, scene, fp, frame_index, out, sink);
+    vgi_status j = sink.finish();
+    return s != VGI_OK ? s : j;
 }
 
 extern "C" vgi_status vgi_renderer_get_image(const vgi_renderer* rc, const char* name, void** ptr, size_t* bytes) {

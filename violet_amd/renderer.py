@@ -34,6 +34,9 @@ class RestirConfig:
     temporal_m_cap: int = 0
     diff_spatial_taps: int = 0
     spatial_radius_px: float = 0.0
+    # independent pass chains on a second stream (hash-grid update beside the
+    # G-buffer, specular beside diffuse), joined into the caller's stream
+    concurrent_lanes: bool = True
     disocclusion_fix: bool = True
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
@@ -51,6 +54,7 @@ class RestirConfig:
         c.temporal_m_cap = self.temporal_m_cap
         c.diff_spatial_taps = self.diff_spatial_taps
         c.spatial_radius_px = self.spatial_radius_px
+        c.concurrent_lanes = int(self.concurrent_lanes)
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
                                          int(self.atrous_iterations), int(self.compensated_moments))
         return c
