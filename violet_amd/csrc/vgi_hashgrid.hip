@@ -9,8 +9,8 @@
  * decay 1 byte per cell (read four cells per 32-bit word).  Insertion uses a
  * CAS on the checksum and the per-cell query choice an atomicMax, as in the
  * reference; which query index / which slot wins depends on scheduling, so
- * only the resulting sets are deterministic.  Compaction is a wave-ballot
- * stream compaction: one global atomic per wave of 64 cells.
+ * only the resulting sets are deterministic.  Compaction is a wave-level
+ * stream compaction: one global atomic per wave of 4096 cells.
  */
 #include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
@@ -112,23 +112,48 @@ __global__ void __launch_bounds__(64) k_hg_select(const HashQuery* queries, cons
     k_hg_select_body(queries, counter, storage, selection, frame_hash, blockIdx.x, blockIdx.y);
 }
 
+// One wave compacts 4096 cells: each lane scans 64 consecutive selection
+// words (16-byte loads) into a bit mask, a wave prefix sum over the lanes'
+// counts places them, and one atomic per wave reserves the output range
+// (a few percent of the cells are selected, so one atomic per 64 cells
+// serialises on the counter instead).  Output is in cell order within a wave.
+constexpr uint32_t kCompactCellsPerLane = 64;
+constexpr uint32_t kCompactCellsPerWave = 64 * kCompactCellsPerLane;
+constexpr uint32_t kCompactWaves = kHashGridNumCells / kCompactCellsPerWave;
+static_assert(kHashGridNumCells % kCompactCellsPerWave == 0, "compaction covers the cells exactly");
+
 __device__ __forceinline__ void k_hg_compact_body(const uint32_t* selection, uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter, uint32_t bx_, uint32_t by_) {
-    uint32_t cell = bx_ * 64u + threadIdx.x;
-    uint32_t word = cell < kHashGridNumCells ? G(selection)[cell] : 0u;
-    bool active = word != 0u;
-    uint64_t mask = __ballot(active);
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t total = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (total != 0u) {
-        int leader = __ffsll((long long)mask) - 1;
-        if ((int)lane == leader) base = g_atomic_add(out_counter, total);
-        base = __shfl(base, leader);
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t first = bx_ * kCompactCellsPerWave + lane * kCompactCellsPerLane;
+    const __attribute__((address_space(1))) U4* src = (const __attribute__((address_space(1))) U4*)(selection + first);
+    uint64_t act = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCompactCellsPerLane / 4; ++k) {
+        const U4 v = src[k];
+        act |= (uint64_t)((v.x != 0u ? 1u : 0u) | (v.y != 0u ? 2u : 0u) | (v.z != 0u ? 4u : 0u) | (v.w != 0u ? 8u : 0u))
+               << (4u * k);
     }
-    if (active) {
-        G(out_index)[base + below] = word & ((1u << kHashGridMaxQueriesShift) - 1u);
-        G(out_addr)[base + below] = cell;
+    const uint32_t cnt = (uint32_t)__popcll(act);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0u) return;
+    uint32_t base = 0;
+    if (lane == 0) base = g_atomic_add(out_counter, total);
+    base = __shfl(base, 0);
+    uint32_t o = base + incl - cnt;
+    while (act != 0u) {
+        const uint32_t k = (uint32_t)__ffsll((long long)act) - 1u;
+        act &= act - 1u;
+        const uint32_t cell = first + k;
+        G(out_index)[o] = G(selection)[cell] & ((1u << kHashGridMaxQueriesShift) - 1u);
+        G(out_addr)[o] = cell;
+        ++o;
     }
 }
 
@@ -194,7 +219,9 @@ static vgi_status fill_hg_compact(const PassRec* const* recs, int n, void* host)
         const vgi_hash_grid_query_compact_io& io = recs[v]->io.hg_compact;
         t->jobs[v] = {io.hash_grid_query_selection_buffer, io.rw_compact_query_index_buffer,
                       io.rw_compact_query_cell_addr_buffer, io.rw_compact_query_counter_buffer};
-        batch_add_tiles(t->hd, v, kHashGridNumCells / 64, 1);
+        VGI_REQUIRE(((uintptr_t)io.hash_grid_query_selection_buffer & 15u) == 0,
+                    "batched hash-grid compaction: selection buffer not 16-byte aligned");
+        batch_add_tiles(t->hd, v, kCompactWaves, 1);
     }
     return VGI_OK;
 }
@@ -247,9 +274,11 @@ extern "C" vgi_status vgi_pass_hash_grid_query_compact(const vgi_hash_grid_query
     VGI_REQUIRE(io && io->hash_grid_query_selection_buffer && io->rw_compact_query_index_buffer &&
                     io->rw_compact_query_cell_addr_buffer && io->rw_compact_query_counter_buffer,
                 "vgi_pass_hash_grid_query_compact: missing binding");
+    VGI_REQUIRE(((uintptr_t)io->hash_grid_query_selection_buffer & 15u) == 0,
+                "vgi_pass_hash_grid_query_compact: selection buffer not 16-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("hash_grid_query_compact", st, (double)kHashGridNumCells * 4.0);
-    hipLaunchKernelGGL(k_hg_compact, dim3(kHashGridNumCells / 64), dim3(64), 0, st, io->hash_grid_query_selection_buffer,
+    hipLaunchKernelGGL(k_hg_compact, dim3(kCompactWaves), dim3(64), 0, st, io->hash_grid_query_selection_buffer,
                        io->rw_compact_query_index_buffer, io->rw_compact_query_cell_addr_buffer,
                        io->rw_compact_query_counter_buffer);
     return check_launch("k_hg_compact");

@@ -7,7 +7,7 @@
  *   k_spec_sample_validate restir/ind_spec_sample_validate.hlsl:144-191
  *   k_hash_grid_raygen     restir/hash_grid_raygen.hlsl:141-204
  *
- * One lane per pixel (or per compacted cache query).  Each lane traces one
+ * One lane per pixel (or per compacted cache query, 16 per wave).  Each lane traces one
  * GI ray plus one sun shadow ray through the LDS-stack traversal.  Pixels
  * without geometry leave the wave early, so the remaining lanes are the
  * active rays.
@@ -204,11 +204,10 @@ __global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, Tra
     k_spec_sample_validate_body<S, kCount>(fp, rt, io, blockIdx.x, blockIdx.y);
 }
 
+// One cache ray: hash_grid_raygen.hlsl:141-204 for compacted query `id`.
 template <class S, bool kCount>
-__device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
-    VGI_TRAVERSAL_STACK(stack);
-    uint32_t id = bx_ * 64u + threadIdx.x;
-    if (id >= *io.compact_query_counter_buffer) return;
+__device__ __forceinline__ void hash_grid_ray(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io,
+                                              uint32_t frame_hash, uint32_t id, LdsStack stack) {
     uint32_t qi = G(io.compact_query_index_buffer)[id];
     uint32_t cell = G(io.compact_query_cell_addr_buffer)[id];
     uint32_t rng = jenkins_hash(id ^ frame_hash);
@@ -227,6 +226,24 @@ __device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, c
     storage[cell].g = blend.y;
     storage[cell].b = blend.z;
     G(io.rw_hash_grid_decay_buffer)[cell] = 10;
+}
+
+// Cache rays start at scattered surface points in random directions, so a
+// wave's lanes share no traversal path and a wave lasts as long as the union
+// of its rays' paths.  Few rays per wave (in more waves) shortens that union;
+// there are only ~1e4 such rays per frame, against ~1e6 sample rays.  The
+// groups stride over the compacted queries (at most kHashGridMaxQueries).
+constexpr uint32_t kRaygenRaysPerWave = 16;
+constexpr uint32_t kRaygenGroups = kHashGridMaxQueries / 64;  // groups per launch (per view in a batch)
+
+template <class S, bool kCount>
+__device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
+    VGI_TRAVERSAL_STACK(stack);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (lane >= kRaygenRaysPerWave) return;
+    const uint32_t count = G(io.compact_query_counter_buffer)[0];
+    for (uint32_t id = bx_ * kRaygenRaysPerWave + lane; id < count; id += kRaygenGroups * kRaygenRaysPerWave)
+        hash_grid_ray<S, kCount>(fp, rt, io, frame_hash, id, stack);
 }
 
 template <class S, bool kCount>
@@ -399,7 +416,7 @@ static vgi_status fill_hg_raygen(const PassRec* const* recs, int n, void* host) 
         j.fp = to_fp(&r.fp);
         j.io = r.io.hg_raygen;
         j.frame_hash = r.pc.frame_hash.frame_hash;
-        batch_add_tiles(t->hd, v, kHashGridMaxQueries / 64, 1);
+        batch_add_tiles(t->hd, v, kRaygenGroups, 1);
     }
     return VGI_OK;
 }
@@ -515,7 +532,7 @@ extern "C" vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, cons
     hipStream_t st = (hipStream_t)stream;
     // ray per compacted query: query 24 B + index 4 + addr 4 + cell rmw 32 + decay 1
     ProfScope prof("hash_grid_raygen", st, 0.0);
-    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kHashGridMaxQueries / 64), to_fp(fp), t, *io,
+    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kRaygenGroups), to_fp(fp), t, *io,
                          pc->frame_hash);
     trace_stats_collect("hash_grid_raygen", st);
     return check_launch("k_hash_grid_raygen");
