@@ -6,9 +6,10 @@
  * [k*64 + l]) so that a push or pop by all lanes touches 64 consecutive
  * dwords (no bank conflicts).  At a node the hit children are ordered by
  * entry distance; the nearest is visited next and the others are pushed
- * farthest first.  Boxes are tested with the
- * slab method in exact IEEE single precision; NaNs from 0*inf drop out of
- * fminf/fmaxf, which only widens the interval (conservative).
+ * farthest first.  Boxes are tested with the slab method in exact IEEE
+ * single precision (no fused multiply-add, so every box test is
+ * conservative up to its own rounding); NaNs from 0*inf drop out of the
+ * max/min, which only widens the interval.
  */
 #pragma once
 #include "vgi_bvh.h"
@@ -39,14 +40,17 @@ __device__ __forceinline__ RayInv make_ray(f3 o, f3 d) {
     return r;
 }
 
-// Returns the entry distance of the box on [tmin, tmax], or +inf if missed.
-__device__ __forceinline__ float slab(const RayInv& r, float lox, float hix, float loy, float hiy, float loz, float hiz,
-                                      float tmin, float tmax) {
-    float tx0 = (lox - r.o.x) * r.idir.x, tx1 = (hix - r.o.x) * r.idir.x;
-    float ty0 = (loy - r.o.y) * r.idir.y, ty1 = (hiy - r.o.y) * r.idir.y;
-    float tz0 = (loz - r.o.z) * r.idir.z, tz1 = (hiz - r.o.z) * r.idir.z;
-    float tn = fmaxf(fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1)), tmin);
-    float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), tmax);
+// Entry distance of a box given each axis' near and far plane for this ray
+// (near = lo where the direction component is >= 0, else hi), or +inf if
+// missed.  A NaN from 0 * inf (origin on a plane of a box the ray runs
+// parallel to) drops out of the max/min, which only widens the interval.
+__device__ __forceinline__ float slab_nf(const RayInv& r, float nx, float fx, float ny, float fy, float nz, float fz,
+                                         float tmin, float tmax) {
+    const float tnx = (nx - r.o.x) * r.idir.x, tfx = (fx - r.o.x) * r.idir.x;
+    const float tny = (ny - r.o.y) * r.idir.y, tfy = (fy - r.o.y) * r.idir.y;
+    const float tnz = (nz - r.o.z) * r.idir.z, tfz = (fz - r.o.z) * r.idir.z;
+    const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+    const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tmax);
     return tn <= tf ? tn : INFINITY;
 }
 
@@ -57,30 +61,46 @@ __device__ __forceinline__ void count_traversal(const SceneView& s, bool any_hit
     atomicAdd(c + 3, (unsigned long long)tris);
 }
 
+// Compare-exchange of (entry distance, link) pairs without branches.
 __device__ __forceinline__ void sort_pair(float& ta, int32_t& la, float& tb, int32_t& lb) {
-    if (tb < ta) {
-        const float t = ta;
-        ta = tb;
-        tb = t;
-        const int32_t l = la;
-        la = lb;
-        lb = l;
-    }
+    const bool sw = tb < ta;
+    const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+    const int32_t l0 = sw ? lb : la, l1 = sw ? la : lb;
+    ta = t0;
+    tb = t1;
+    la = l0;
+    lb = l1;
 }
+
+// Traversal finished.  Never a leaf link: leaf links are ~(first << 4 | count)
+// with first + count <= number of triangles < 2^27.
+constexpr int32_t kTraversalDone = INT32_MIN;
 
 // Closest hit over [tmin, tmax]; ties in t resolve to the lowest triangle id.
 // `stack` points at this lane's slot 0 (stride 64 between entries).
 // kCount: the instrumented build (traversal statistics) that adds this ray's
 // node and triangle counts to s.counters.
+//
+// Loop structure (while-while with postponed leaves): lanes step through
+// interior nodes, each setting aside the first leaf it reaches and going on
+// down the tree, until every lane of the wave holds a leaf; then all lanes
+// intersect their leaves together.  Interior-node and triangle code thus run
+// with most lanes active instead of alternating per iteration.  Boxes are
+// tested against the near and far plane of each axis, chosen once per ray by
+// the sign of the direction.  The hits do not depend on the visiting order
+// (closest t, then lowest id), only the work does.
 template <bool kAnyHit, bool kCount = false>
 __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
-    RayInv r = make_ray(o, d);
+    const RayInv r = make_ray(o, d);
+    // float4 slot of each axis' near plane in a node (lo = 2 axis, hi = 2 axis + 1)
+    const int nx = r.idir.x >= 0.0f ? 0 : 1, ny = r.idir.y >= 0.0f ? 2 : 3, nz = r.idir.z >= 0.0f ? 4 : 5;
     Hit h;
     h.t = tmax;
     h.u = h.v = 0.0f;
     h.tri = 0xFFFFFFFFu;
     int sp = 0;
-    int32_t link = s.root;
+    int32_t node = s.root;  // next interior node, a leaf link, or kTraversalDone
+    int32_t leaf = 0;       // postponed leaf link (< 0), 0 for none
     // Each node and leaf of a tree is reached at most once (a node has at
     // most 4 children); the budget only guards against a corrupted BVH
     // turning into an endless loop.
@@ -95,17 +115,28 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             spill[sp - (int)kLdsStackEntries] = x;
         ++sp;
     };
-    while (budget-- != 0u) {
-        if (link >= 0) {
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kTraversalDone;
+        --sp;
+        return sp < (int)kLdsStackEntries ? stack[sp * 64] : spill[sp - (int)kLdsStackEntries];
+    };
+    while (node != kTraversalDone || leaf != 0) {
+        while (node >= 0) {
+            if (budget-- == 0u) {
+                node = kTraversalDone;
+                leaf = 0;
+                break;
+            }
             if (kCount) ++n_nodes;
-            const void* p = s.nodes + link;
-            const Vf4 lox = ld_global16(p, 0), hix = ld_global16(p, 1), loy = ld_global16(p, 2);
-            const Vf4 hiy = ld_global16(p, 3), loz = ld_global16(p, 4), hiz = ld_global16(p, 5);
+            const void* p = s.nodes + node;
+            const Vf4 nrx = ld_global16(p, nx), frx = ld_global16(p, nx ^ 1);
+            const Vf4 nry = ld_global16(p, ny), fry = ld_global16(p, ny ^ 1);
+            const Vf4 nrz = ld_global16(p, nz), frz = ld_global16(p, nz ^ 1);
             const Vf4 lk = ld_global16(p, 6);
-            float t0 = slab(r, lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, tmin, h.t);
-            float t1 = slab(r, lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, tmin, h.t);
-            float t2 = slab(r, lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, tmin, h.t);
-            float t3 = slab(r, lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, tmin, h.t);
+            float t0 = slab_nf(r, nrx.x, frx.x, nry.x, fry.x, nrz.x, frz.x, tmin, h.t);
+            float t1 = slab_nf(r, nrx.y, frx.y, nry.y, fry.y, nrz.y, frz.y, tmin, h.t);
+            float t2 = slab_nf(r, nrx.z, frx.z, nry.z, fry.z, nrz.z, frz.z, tmin, h.t);
+            float t3 = slab_nf(r, nrx.w, frx.w, nry.w, fry.w, nrz.w, frz.w, tmin, h.t);
             int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
             int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
             // ascending entry distance (misses are +inf): a 5-exchange network
@@ -115,17 +146,29 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             sort_pair(t1, l1, t3, l3);
             sort_pair(t1, l1, t2, l2);
             if (t0 != INFINITY) {
-                // always true for a BVH from vgi_scene_create: the stack never
-                // outgrows 3 far children per level
+                // the depth test always passes for a BVH from vgi_scene_create:
+                // the stack never outgrows 3 far children per level
                 if (t3 != INFINITY && sp < depth) push(l3);
                 if (t2 != INFINITY && sp < depth) push(l2);
                 if (t1 != INFINITY && sp < depth) push(l1);
-                link = l0;
-                continue;
+                node = l0;
+            } else {
+                node = pop();
             }
-        } else {
-            uint32_t enc = (uint32_t)(~link);
-            uint32_t first = enc >> 4, cnt = enc & 15u;
+            if (node < 0 && node != kTraversalDone && leaf == 0) {
+                leaf = node;
+                node = pop();
+            }
+            if (__all(leaf != 0)) break;
+        }
+        while (leaf != 0) {
+            if (budget-- == 0u) {
+                node = kTraversalDone;
+                leaf = 0;
+                break;
+            }
+            const uint32_t enc = (uint32_t)(~leaf);
+            const uint32_t first = enc >> 4, cnt = enc & 15u;
             for (uint32_t k = 0; k < cnt; ++k) {
                 if (kCount) ++n_tris;
                 const void* tp = s.tris + first + k;
@@ -146,10 +189,12 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
                     }
                 }
             }
+            leaf = 0;
+            if (node < 0 && node != kTraversalDone) {
+                leaf = node;
+                node = pop();
+            }
         }
-        if (sp == 0) break;
-        --sp;
-        link = sp < (int)kLdsStackEntries ? stack[sp * 64] : spill[sp - (int)kLdsStackEntries];
     }
     if (kCount) count_traversal(s, kAnyHit, n_nodes, n_tris);
     return h;
