@@ -11,12 +11,19 @@
 
 namespace vgi {
 
+// Waves per SIMD a job's fused kernel is compiled for (a register budget;
+// 1 = no constraint).  Specialised next to the job where it matters.
+template <class Job>
+struct BatchMinWaves {
+    static constexpr int value = 1;
+};
+
 // kBarrier: the job body shares LDS across its workgroup, so the next tile
 // must wait until every thread is done with this one.  kXcdOrder: walk the
 // tiles in XCD-aware order (xcd_remap), for bodies whose consecutive tiles
 // share cache lines.
 template <class Job, int kThreads, bool kBarrier, bool kXcdOrder>
-__global__ void __launch_bounds__(kThreads) k_batch(const BatchTable<Job>* __restrict__ t) {
+__global__ void __launch_bounds__(kThreads, BatchMinWaves<Job>::value) k_batch(const BatchTable<Job>* __restrict__ t) {
     const uint32_t n = t->hd.njobs;
     const uint32_t total = t->hd.tile_begin[n];
     const uint32_t first = kXcdOrder ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;

@@ -207,9 +207,12 @@ struct XXHash32 {
 VGI_HD uint32_t f32tof16(float f) {
 #if defined(__HIP_DEVICE_COMPILE__)
     // v_cvt_f16_f32 rounds to nearest even like the host code below; NaNs are
-    // canonicalised to the host code's quiet NaN.
-    if (f != f) return ((asuint(f) >> 16) & 0x8000u) | 0x7E00u;
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
+    // canonicalised to the host code's quiet NaN (a select, not a branch: a
+    // branch per conversion splits the caller into blocks and costs it
+    // registers where several conversions follow each other)
+    const uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
+    const uint32_t qnan = ((asuint(f) >> 16) & 0x8000u) | 0x7E00u;
+    return f != f ? qnan : h;
 #endif
     uint32_t x = asuint(f);
     uint32_t sign = (x >> 16) & 0x8000u;

@@ -531,12 +531,18 @@ __global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io)
 constexpr int kAtrousB = 16;                   // block edge in lattice points
 constexpr int kAtrousF = kAtrousB + 4;         // footprint edge (2 each side)
 constexpr int kAtrousN = kAtrousF * kAtrousF;  // footprint entries
+// LDS layout: each wave filters an 8x8 quarter of the block and footprint
+// rows are kAtrousP = 24 dwords apart, so the 8 rows of a wave's reads start
+// at 24 r mod 64 = 0, 24, 48, 8, 32, 56, 16, 40: the 64 lanes of one LDS
+// read hit 64 distinct banks (and any 32 consecutive lanes 32 distinct ones).
+constexpr int kAtrousP = 24;                   // footprint row pitch
+constexpr int kAtrousRow = kAtrousF * kAtrousP;  // dwords per field
 
 enum AtrousField { kFDepth, kFNx, kFNy, kFNz, kFLd, kFLs, kFDr, kFDg, kFDb, kFSr, kFSg, kFSb, kFVx, kFVy, kFCount };
 
 template <class S>
 __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, int step, uint32_t nbx, uint32_t nby, uint32_t bx_, uint32_t by_) {
-    __shared__ float lds[kFCount][kAtrousN];
+    __shared__ float lds[kFCount][kAtrousRow];
     const int w = (int)io.width, h = (int)io.height;
     const size_t plane = (size_t)w * h;
     // residue innermost: the s x s residue classes of one block region are
@@ -545,6 +551,7 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     const int ry = (int)(by_ % (uint32_t)step), by = (int)(by_ / (uint32_t)step);
     const int t = (int)threadIdx.x;
 
+#pragma unroll 1
     for (int e = t; e < kAtrousN; e += 256) {
         int gx = bx * kAtrousB + e % kAtrousF - 2, gy = by * kAtrousB + e / kAtrousF - 2;
         int cx = rx + step * gx, cy = ry + step * gy;
@@ -564,16 +571,18 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
             v[kFSr] = sv.x, v[kFSg] = sv.y, v[kFSb] = sv.z;
             v[kFVx] = vv.x, v[kFVy] = vv.y;
         }
+        const int le = (e / kAtrousF) * kAtrousP + e % kAtrousF;
 #pragma unroll
-        for (int k = 0; k < kFCount; ++k) lds[k][e] = v[k];
+        for (int k = 0; k < kFCount; ++k) lds[k][le] = v[k];
     }
     __syncthreads();
 
-    const int li = t & (kAtrousB - 1), lj = t / kAtrousB;
+    const int lane = t & 63, wv = t >> 6;
+    const int li = (wv & 1) * 8 + (lane & 7), lj = (wv >> 1) * 8 + (lane >> 3);
     const int px = rx + step * (bx * kAtrousB + li), py = ry + step * (by * kAtrousB + lj);
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
-    const int c = (lj + 2) * kAtrousF + (li + 2);
+    const int c = (lj + 2) * kAtrousP + (li + 2);
 
     const float depth_center = lds[kFDepth][c];
     const f3 nc = mk3(lds[kFNx][c], lds[kFNy][c], lds[kFNz][c]);
@@ -592,36 +601,75 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
         float a = (float)((q == 2 ? 2 : 1) * step), b = (float)((q == 0 ? 1 : 2) * step);
         kz[q] = kLog2e * __builtin_amdgcn_rcpf(fwz * sqrtf(a * a + b * b) + 1e-8f);
     }
-    const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    // Tap weight w = kw[|x|] kw[|y|] * ndot^128 * exp(-dl/(4 std) - dz/(fwz dist))
+    // evaluated as one exp2 per colour of an exponent that carries every
+    // factor: log2 of the kernel weight (a constant per tap), 128 log2(ndot)
+    // (ndot = 0 gives -inf, hence w = 0), the depth term and the luminance
+    // term.  Contracted to fused multiply-adds.
+    const float lkw[3] = {0.0f, -0.5849625007211562f, -2.584962500721156f};  // log2 of 1, 2/3, 1/6
     f2 wsum = mk2(1.0f, 1.0f);
+    // every tap read is at a constant offset from the footprint corner of
+    // this pixel's 5x5 window, so all LDS reads share one address register
+    const float* L = &lds[0][0] + (c - 2 * kAtrousP - 2);
+    // one base register per field, kept opaque so the compiler addresses the
+    // tap pairs of a field as immediate offsets from it instead of adding a
+    // field offset per pair (the empty asm emits nothing)
+    typedef __attribute__((address_space(3))) const float* LdsF;
+    LdsF F[kFCount];
+#pragma unroll
+    for (int k = 0; k < kFCount; ++k) {
+        F[k] = (LdsF)(L + k * kAtrousRow);
+        asm volatile("" : "+v"(F[k]));
+    }
 #pragma unroll
     for (int y = -2; y <= 2; y++) {
 #pragma unroll
         for (int x = -2; x <= 2; x++) {
             if (x == 0 || y == 0) continue;
             const int ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
-            const float kwxy = kw[ay] * kw[ax];
-            const int j = c + y * kAtrousF + x;
-            float dz = fabsf(lds[kFDepth][j] - depth_center);
-            float ndot = saturate(nc.x * lds[kFNx][j] + nc.y * lds[kFNy][j] + nc.z * lds[kFNz][j]);
-            float wn = pow_pow2(ndot, 7);  // sigma_n = 128
-            float ez = dz * kz[ax + ay - 2];
-            float wx = kwxy * (wn * __builtin_amdgcn_exp2f(-(fabsf(lds[kFLd][j] - lc.x) * kl_d) - ez));
-            float wy = kwxy * (wn * __builtin_amdgcn_exp2f(-(fabsf(lds[kFLs][j] - lc.y) * kl_s) - ez));
-            dsum = dsum + mk3(lds[kFDr][j], lds[kFDg][j], lds[kFDb][j]) * wx;
-            ssum = ssum + mk3(lds[kFSr][j], lds[kFSg][j], lds[kFSb][j]) * wy;
-            vsum = mk2(vsum.x + (wx * wx) * lds[kFVx][j], vsum.y + (wy * wy) * lds[kFVy][j]);
+            const int j = (y + 2) * kAtrousP + (x + 2);
+#define AT(f) F[f][j]
+            const float dz = fabsf(AT(kFDepth) - depth_center);
+            const float ndot = saturate(__builtin_fmaf(nc.z, AT(kFNz), __builtin_fmaf(nc.y, AT(kFNy), nc.x * AT(kFNx))));
+            // sigma_n = 128
+            const float e = __builtin_fmaf(__builtin_amdgcn_logf(ndot), 128.0f,
+                                           __builtin_fmaf(dz, -kz[ax + ay - 2], lkw[ay] + lkw[ax]));
+            const float wx = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(AT(kFLd) - lc.x), kl_d, e));
+            const float wy = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(AT(kFLs) - lc.y), kl_s, e));
+            dsum = mk3(__builtin_fmaf(AT(kFDr), wx, dsum.x), __builtin_fmaf(AT(kFDg), wx, dsum.y),
+                       __builtin_fmaf(AT(kFDb), wx, dsum.z));
+            ssum = mk3(__builtin_fmaf(AT(kFSr), wy, ssum.x), __builtin_fmaf(AT(kFSg), wy, ssum.y),
+                       __builtin_fmaf(AT(kFSb), wy, ssum.z));
+            vsum = mk2(__builtin_fmaf(wx * wx, AT(kFVx), vsum.x), __builtin_fmaf(wy * wy, AT(kFVy), vsum.y));
             wsum = mk2(wsum.x + wx, wsum.y + wy);
+#undef AT
         }
     }
+    // the output addresses are formed after the taps: redefining the pixel
+    // index here (the empty asm emits nothing) keeps the compiler from
+    // computing them up front and holding them through the loop, which
+    // spills in FP16 mode (three 64-bit addresses per image)
+    uint32_t io_px = (uint32_t)i;
+    asm volatile("" : "+v"(io_px));
     const float id = __builtin_amdgcn_rcpf(wsum.x), is = __builtin_amdgcn_rcpf(wsum.y);
-    S::st3(io.rw_diff_texture, i, dsum * id);
-    S::st3(io.rw_spec_texture, i, ssum * is);
-    S::st2(io.rw_variance_texture, i, mk2(vsum.x * (id * id), vsum.y * (is * is)));
+    S::st3(io.rw_diff_texture, io_px, dsum * id);
+    S::st3(io.rw_spec_texture, io_px, ssum * is);
+    S::st2(io.rw_variance_texture, io_px, mk2(vsum.x * (id * id), vsum.y * (is * is)));
 }
 
+// Register budget of the a-trous kernels: 6 waves per SIMD, the LDS limit
+// of 26.9 KB workgroups.
 template <class S>
-__global__ void __launch_bounds__(256) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
+struct AtrousWaves {
+    static constexpr int value = 6;
+};
+template <>
+struct AtrousWaves<StorageF16> {
+    static constexpr int value = 6;
+};
+
+template <class S>
+__global__ void __launch_bounds__(256, AtrousWaves<S>::value) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
     const uint32_t gx = gridDim.x;
     const uint32_t l = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
     k_atrous_body<S>(io, step, nbx, nby, l % gx, l / gx);
@@ -745,6 +793,10 @@ struct JobDenAtrous {
     __device__ static void run(const JobDenAtrous& j, uint32_t bx, uint32_t by) {
         k_atrous_body<S>(j.io, j.step, j.nbx, j.nby, bx, by);
     }
+};
+template <class S>
+struct BatchMinWaves<JobDenAtrous<S>> {
+    static constexpr int value = AtrousWaves<S>::value;
 };
 
 static vgi_status fill_den_temporal(const PassRec* const* recs, int n, void* host) {
