@@ -80,6 +80,12 @@ constexpr int kBvhMaxDepth = 64;     // binary build depth bound (the 4-wide tre
 constexpr uint32_t kLdsStackEntries = 16;
 constexpr uint32_t kStackMaxEntries = 3 * kBvhMaxDepth;  // 3 far children per 4-wide level
 
+// Register budget of the per-pixel ray-tracing kernels, as waves per SIMD:
+// the traversal hides node-fetch latency with resident waves, and 4 KiB of
+// LDS stack per wave allows more of them than ~80 registers would.  (The
+// cache-ray pass runs too few waves to gain from it and keeps its registers.)
+constexpr int kTraceWaves = 8;
+
 // Dynamic LDS bytes of a 64-lane traversal stack for this scene.
 inline size_t traversal_lds_bytes(const SceneView& s) {
     const uint32_t e = s.stack_depth < kLdsStackEntries ? s.stack_depth : kLdsStackEntries;

@@ -73,7 +73,7 @@ __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const Scen
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(64) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
+__global__ void __launch_bounds__(64, kTraceWaves) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
     k_gbuffer_body<kCount>(fp, s, io, blockIdx.x, blockIdx.y);
 }
 
@@ -86,6 +86,10 @@ struct JobGBuffer {
     __device__ static void run(const JobGBuffer& j, uint32_t bx, uint32_t by) {
         k_gbuffer_body<false>(j.fp, j.s, j.io, bx, by);
     }
+};
+template <class S>
+struct BatchMinWaves<JobGBuffer<S>> {
+    static constexpr int value = kTraceWaves;
 };
 
 static vgi_status fill_gbuffer(const PassRec* const* recs, int n, void* host) {

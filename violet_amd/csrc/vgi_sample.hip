@@ -56,7 +56,7 @@ __device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, co
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io, vgi_pc_sample_gen pc) {
+__global__ void __launch_bounds__(64, kTraceWaves) k_diff_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_gen_io io, vgi_pc_sample_gen pc) {
     k_diff_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
 }
 
@@ -94,7 +94,7 @@ __device__ __forceinline__ void k_diff_sample_validate_body(const FrameParams& f
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_diff_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_validate_io io) {
+__global__ void __launch_bounds__(64, kTraceWaves) k_diff_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_diff_sample_validate_io io) {
     k_diff_sample_validate_body<S, kCount>(fp, rt, io, blockIdx.x, blockIdx.y);
 }
 
@@ -159,7 +159,7 @@ __device__ __forceinline__ void k_spec_sample_gen_body(const FrameParams& fp, co
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io, vgi_pc_sample_gen pc) {
+__global__ void __launch_bounds__(64, kTraceWaves) k_spec_sample_gen(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_gen_io io, vgi_pc_sample_gen pc) {
     k_spec_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
 }
 
@@ -200,7 +200,7 @@ __device__ __forceinline__ void k_spec_sample_validate_body(const FrameParams& f
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_spec_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_validate_io io) {
+__global__ void __launch_bounds__(64, kTraceWaves) k_spec_sample_validate(FrameParams fp, TraceRes rt, vgi_ind_spec_sample_validate_io io) {
     k_spec_sample_validate_body<S, kCount>(fp, rt, io, blockIdx.x, blockIdx.y);
 }
 
@@ -350,6 +350,22 @@ struct JobHgRaygen {
     __device__ static void run(const JobHgRaygen& j, uint32_t bx, uint32_t by) {
         k_hash_grid_raygen_body<S, false>(j.fp, j.rt, j.io, j.frame_hash, bx, by);
     }
+};
+template <class S>
+struct BatchMinWaves<JobDiffGen<S>> {
+    static constexpr int value = kTraceWaves;
+};
+template <class S>
+struct BatchMinWaves<JobDiffValidate<S>> {
+    static constexpr int value = kTraceWaves;
+};
+template <class S>
+struct BatchMinWaves<JobSpecGen<S>> {
+    static constexpr int value = kTraceWaves;
+};
+template <class S>
+struct BatchMinWaves<JobSpecValidate<S>> {
+    static constexpr int value = kTraceWaves;
 };
 
 static vgi_status fill_diff_gen(const PassRec* const* recs, int n, void* host) {

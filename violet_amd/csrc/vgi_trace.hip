@@ -9,7 +9,7 @@ namespace vgi {
 const SceneView& scene_view(const vgi_scene* s);
 
 template <bool kCount>
-__global__ void __launch_bounds__(64) k_trace_rays(SceneView s, const float* rays, uint32_t n, uint32_t any_hit,
+__global__ void __launch_bounds__(64, kTraceWaves) k_trace_rays(SceneView s, const float* rays, uint32_t n, uint32_t any_hit,
                                                    uint32_t* hits) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t i = blockIdx.x * 64u + threadIdx.x;
