@@ -104,14 +104,15 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
     return r;
 }
 
-// raytrace.inc.hlsl:65-208.  `S` is the storage policy of the previous
-// frame's denoised diffuse (the screen-space radiance cache).
-template <class S, bool kCount = false>
-__device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt, const FrameParams& fp, f3 o, f3 d,
-                                                              bool has_prev_frame, LdsStack stack) {
+// raytrace.inc.hlsl:65-208 after its two TraceRay calls: the radiance of
+// the ray (o, d) given its closest hit `sh` (shade_hit) and whether the sun
+// shadow ray from the hit was unoccluded.  `S` is the storage policy of the
+// previous frame's denoised diffuse (the screen-space radiance cache).
+template <class S>
+__device__ __forceinline__ RadianceTraceResult radiance_at(const TraceRes& rt, const FrameParams& fp, f3 o, f3 d,
+                                                           const SurfaceHit& sh, bool sun_visible,
+                                                           bool has_prev_frame) {
     RadianceTraceResult ret;
-    Hit hit = trace_closest<kCount>(rt.scene, o, d, kRayTMin, kRayTMax, stack);
-    SurfaceHit sh = shade_hit(rt.scene, hit, d);
     if (sh.missed) {
         ret.radiance = rt.sky;
         ret.position_ws = o + d * kRayTMax;
@@ -124,8 +125,7 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
     f3 sun_dir = xyz(fp.sun_dir);
     f3 sun_inten = xyz(fp.sun_inten);
     f3 radiance = mk3s(0.0f);
-    bool missed = !trace_any<kCount>(rt.scene, sh.position, sun_dir, kRayTMin, kRayTMax, stack);
-    if (missed) {
+    if (sun_visible) {
         float pr = fmaxf(sh.roughness, 0.045f);
         pr = fmaxf(pr, 0.5f);  // RAYTRACE_INC_FIREFLY_SUPPRESS
         float NoL = saturate(dot3(sh.normal, sun_dir));
@@ -173,6 +173,18 @@ __device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt
     }
     ret.radiance = radiance;
     return ret;
+}
+
+// raytrace.inc.hlsl:65-208: closest hit of (o, d), then the sun shadow ray
+// from the hit (any hit), then radiance_at.
+template <class S, bool kCount = false>
+__device__ __forceinline__ RadianceTraceResult trace_radiance(const TraceRes& rt, const FrameParams& fp, f3 o, f3 d,
+                                                              bool has_prev_frame, LdsStack stack) {
+    Hit hit = trace_closest<kCount>(rt.scene, o, d, kRayTMin, kRayTMax, stack);
+    SurfaceHit sh = shade_hit(rt.scene, hit, d);
+    bool sun_visible = false;
+    if (!sh.missed) sun_visible = !trace_any<kCount>(rt.scene, sh.position, xyz(fp.sun_dir), kRayTMin, kRayTMax, stack);
+    return radiance_at<S>(rt, fp, o, d, sh, sun_visible, has_prev_frame);
 }
 
 }  // namespace vgi

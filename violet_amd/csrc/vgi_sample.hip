@@ -26,33 +26,61 @@ __device__ __forceinline__ f3 position_biased(const float* m, uint32_t px, uint3
     return depth_to_position(m, px, py, w, h, depth + kDepthError);
 }
 
-template <class S, bool kCount>
-__device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_diff_sample_gen_io& io, const vgi_pc_sample_gen& pc, uint32_t bx_, uint32_t by_) {
+// Each sample pass is a pair of functions around trace_radiance: gen_px()
+// turns a pixel into its GI ray (or writes the outputs of a pixel without
+// one and returns false) and finish_px() writes the outputs of the traced
+// sample; sample_pixel() runs the pair for one lane per pixel.
+template <class S>
+struct DiffGenPass {
+    const FrameParams& fp;
+    const vgi_ind_diff_sample_gen_io& io;
+    const vgi_pc_sample_gen& pc;
+    __device__ bool gen_px(uint32_t px, uint32_t py, f3& o, f3& d) const {
+        const uint32_t w = io.width, h = io.height;
+        if (px >= w || py >= h) return false;
+        const size_t i = (size_t)py * w + px;
+        float depth = G(io.gbuffer_depth)[i];
+        if (depth == 0.0f) {
+            G(io.rw_hit_pos_normal_texture)[2 * i] = 0u;
+            G(io.rw_hit_pos_normal_texture)[2 * i + 1] = 0u;
+            S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
+            return false;
+        }
+        GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[(size_t)w * h + i]);
+        o = position_biased(fp.inv_view_proj, px, py, w, h, depth);
+        uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
+        float u0 = lcg_rand(rng);
+        float u1 = lcg_rand(rng);
+        d = sample_hemisphere_uniform_with_normal(u0, u1, g.normal);
+        return true;
+    }
+    __device__ void finish_px(uint32_t px, uint32_t py, f3, f3, const RadianceTraceResult& r) const {
+        const size_t i = (size_t)py * io.width + px;
+        uint32_t ex, ey;
+        hit_encode(r.position_ws, r.normal_ws, ex, ey);
+        G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+        G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
+        S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    }
+};
+
+// One lane per pixel of tile (bx_, by_).
+template <class P, class S, bool kCount>
+__device__ __forceinline__ void sample_pixel(const P& p, const TraceRes& rt, const FrameParams& fp, bool has_prev,
+                                             uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
     tile_pixel_at(bx_, by_, px, py);
-    const uint32_t w = io.width, h = io.height;
-    if (px >= w || py >= h) return;
-    const size_t i = (size_t)py * w + px;
-    float depth = G(io.gbuffer_depth)[i];
-    if (depth == 0.0f) {
-        G(io.rw_hit_pos_normal_texture)[2 * i] = 0u;
-        G(io.rw_hit_pos_normal_texture)[2 * i + 1] = 0u;
-        S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
-        return;
-    }
-    GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[(size_t)w * h + i]);
-    f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
-    uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
-    float u0 = lcg_rand(rng);
-    float u1 = lcg_rand(rng);
-    f3 dir = sample_hemisphere_uniform_with_normal(u0, u1, g.normal);
-    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
-    uint32_t ex, ey;
-    hit_encode(r.position_ws, r.normal_ws, ex, ey);
-    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
-    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
-    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    f3 o, d;
+    if (!p.gen_px(px, py, o, d)) return;
+    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, o, d, has_prev, stack);
+    p.finish_px(px, py, o, d, r);
+}
+
+template <class S, bool kCount>
+__device__ __forceinline__ void k_diff_sample_gen_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_diff_sample_gen_io& io, const vgi_pc_sample_gen& pc, uint32_t bx_, uint32_t by_) {
+    const DiffGenPass<S> p{fp, io, pc};
+    sample_pixel<DiffGenPass<S>, S, kCount>(p, rt, fp, pc.has_prev_frame != 0, bx_, by_);
 }
 
 template <class S, bool kCount>
@@ -60,37 +88,50 @@ __global__ void __launch_bounds__(64, kTraceWaves) k_diff_sample_gen(FrameParams
     k_diff_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
 }
 
+template <class S>
+struct DiffValidatePass {
+    const FrameParams& fp;
+    const vgi_ind_diff_sample_validate_io& io;
+    __device__ bool gen_px(uint32_t px, uint32_t py, f3& o, f3& d) const {
+        const uint32_t w = io.width, h = io.height;
+        if (px >= w || py >= h) return false;
+        const size_t i = (size_t)py * w + px;
+        float depth = G(io.depth_texture)[i];
+        if (depth == 0.0f) return false;
+        HitPosNormal hit =
+            hit_decode(G(io.rw_hit_pos_normal_texture)[2 * i], G(io.rw_hit_pos_normal_texture)[2 * i + 1]);
+        o = depth_to_position(fp.prev_inv_view_proj, px, py, w, h, depth);
+        d = normalize3(hit.pos - o);
+        return true;
+    }
+    __device__ void finish_px(uint32_t px, uint32_t py, f3 origin, f3, const RadianceTraceResult& r) const {
+        const size_t i = (size_t)py * io.width + px;
+        HitPosNormal hit =
+            hit_decode(G(io.rw_hit_pos_normal_texture)[2 * i], G(io.rw_hit_pos_normal_texture)[2 * i + 1]);
+        f3 prev_rad = S::ld3(io.rw_hit_radiance_texture, i);
+        float lumi_prev = luminance(prev_rad);
+        float lumi_diff = luminance(abs3(r.radiance - prev_rad));
+        float prev_ray_len = length3(hit.pos - origin);
+        bool pos_changed = component_max(abs3(hit.pos - r.position_ws)) > fmaxf(prev_ray_len / 128.0f, 0.01f);
+        float m_scale = saturate(1.0f - (1.0f * lumi_diff) / lumi_prev);
+        m_scale = pos_changed ? 0.0f : m_scale;
+        if ((lumi_prev > 0.0f) && (m_scale <= (1.0f - 0.5f / 32.0f))) {
+            ReservoirSimple rs = reservoir_decode_32b(G(io.rw_reservoir_texture)[i]);
+            rs.M = f2u(floorf((float)rs.M * m_scale));
+            G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
+        }
+        uint32_t ex, ey;
+        hit_encode(r.position_ws, r.normal_ws, ex, ey);
+        G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
+        G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
+        S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    }
+};
+
 template <class S, bool kCount>
 __device__ __forceinline__ void k_diff_sample_validate_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_diff_sample_validate_io& io, uint32_t bx_, uint32_t by_) {
-    VGI_TRAVERSAL_STACK(stack);
-    uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
-    const uint32_t w = io.width, h = io.height;
-    if (px >= w || py >= h) return;
-    const size_t i = (size_t)py * w + px;
-    float depth = G(io.depth_texture)[i];
-    if (depth == 0.0f) return;
-    HitPosNormal hit = hit_decode(G(io.rw_hit_pos_normal_texture)[2 * i], G(io.rw_hit_pos_normal_texture)[2 * i + 1]);
-    f3 prev_rad = S::ld3(io.rw_hit_radiance_texture, i);
-    f3 origin = depth_to_position(fp.prev_inv_view_proj, px, py, w, h, depth);
-    f3 dir = normalize3(hit.pos - origin);
-    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, origin, dir, true, stack);
-    float lumi_prev = luminance(prev_rad);
-    float lumi_diff = luminance(abs3(r.radiance - prev_rad));
-    float prev_ray_len = length3(hit.pos - origin);
-    bool pos_changed = component_max(abs3(hit.pos - r.position_ws)) > fmaxf(prev_ray_len / 128.0f, 0.01f);
-    float m_scale = saturate(1.0f - (1.0f * lumi_diff) / lumi_prev);
-    m_scale = pos_changed ? 0.0f : m_scale;
-    if ((lumi_prev > 0.0f) && (m_scale <= (1.0f - 0.5f / 32.0f))) {
-        ReservoirSimple rs = reservoir_decode_32b(G(io.rw_reservoir_texture)[i]);
-        rs.M = f2u(floorf((float)rs.M * m_scale));
-        G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
-    }
-    uint32_t ex, ey;
-    hit_encode(r.position_ws, r.normal_ws, ex, ey);
-    G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
-    G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
-    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    const DiffValidatePass<S> p{fp, io};
+    sample_pixel<DiffValidatePass<S>, S, kCount>(p, rt, fp, true, bx_, by_);
 }
 
 template <class S, bool kCount>
@@ -129,33 +170,44 @@ __device__ __forceinline__ f3 sample_spec_dir(f3 view_dir, f3 normal, float roug
     return rotate_point(invert_rotation(rot), L);
 }
 
+template <class S>
+struct SpecGenPass {
+    const FrameParams& fp;
+    const vgi_ind_spec_sample_gen_io& io;
+    const vgi_pc_sample_gen& pc;
+    __device__ bool gen_px(uint32_t px, uint32_t py, f3& o, f3& d) const {
+        const uint32_t w = io.width, h = io.height;
+        if (px >= w || py >= h) return false;
+        const size_t i = (size_t)py * w + px;
+        const size_t plane = (size_t)w * h;
+        float depth = G(io.gbuffer_depth)[i];
+        if (depth == 0.0f) {
+            S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
+            return false;
+        }
+        GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[plane + i]);
+        o = position_biased(fp.inv_view_proj, px, py, w, h, depth);
+        uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
+        f3 view_dir = normalize3(xyz(fp.view_pos) - o);
+        float roughness = fmaxf(g.perceptual_roughness * g.perceptual_roughness, 0.002f);
+        d = sample_spec_dir(view_dir, g.normal, roughness, rng);
+        return true;
+    }
+    __device__ void finish_px(uint32_t px, uint32_t py, f3, f3, const RadianceTraceResult& r) const {
+        const size_t i = (size_t)py * io.width + px;
+        float* hp = io.rw_hit_pos_texture + 4 * i;
+        hp[0] = r.position_ws.x;
+        hp[1] = r.position_ws.y;
+        hp[2] = r.position_ws.z;
+        hp[3] = 1.0f;
+        S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    }
+};
+
 template <class S, bool kCount>
 __device__ __forceinline__ void k_spec_sample_gen_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_spec_sample_gen_io& io, const vgi_pc_sample_gen& pc, uint32_t bx_, uint32_t by_) {
-    VGI_TRAVERSAL_STACK(stack);
-    uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
-    const uint32_t w = io.width, h = io.height;
-    if (px >= w || py >= h) return;
-    const size_t i = (size_t)py * w + px;
-    const size_t plane = (size_t)w * h;
-    float depth = G(io.gbuffer_depth)[i];
-    if (depth == 0.0f) {
-        S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
-        return;
-    }
-    GBufferNormal g = decode_gbuffer_normal(G(io.gbuffer_color)[plane + i]);
-    f3 position = position_biased(fp.inv_view_proj, px, py, w, h, depth);
-    uint32_t rng = lcg_init_with_seed(px, py, pc.frame_rand);
-    f3 view_dir = normalize3(xyz(fp.view_pos) - position);
-    float roughness = fmaxf(g.perceptual_roughness * g.perceptual_roughness, 0.002f);
-    f3 dir = sample_spec_dir(view_dir, g.normal, roughness, rng);
-    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, pc.has_prev_frame != 0, stack);
-    float* hp = io.rw_hit_pos_texture + 4 * i;
-    hp[0] = r.position_ws.x;
-    hp[1] = r.position_ws.y;
-    hp[2] = r.position_ws.z;
-    hp[3] = 1.0f;
-    S::st3(io.rw_hit_radiance_texture, i, r.radiance);
+    const SpecGenPass<S> p{fp, io, pc};
+    sample_pixel<SpecGenPass<S>, S, kCount>(p, rt, fp, pc.has_prev_frame != 0, bx_, by_);
 }
 
 template <class S, bool kCount>
@@ -163,40 +215,49 @@ __global__ void __launch_bounds__(64, kTraceWaves) k_spec_sample_gen(FrameParams
     k_spec_sample_gen_body<S, kCount>(fp, rt, io, pc, blockIdx.x, blockIdx.y);
 }
 
+template <class S>
+struct SpecValidatePass {
+    const FrameParams& fp;
+    const TraceRes& rt;
+    const vgi_ind_spec_sample_validate_io& io;
+    __device__ bool gen_px(uint32_t px, uint32_t py, f3& o, f3& d) const {
+        const uint32_t w = io.width, h = io.height;
+        if (px >= w || py >= h) return false;
+        const size_t i = (size_t)py * w + px;
+        float prev_depth = G(rt.prev_depth)[i];
+        if (prev_depth == 0.0f) return false;
+        o = position_biased(fp.prev_inv_view_proj, px, py, w, h, prev_depth);
+        const float* hp = io.rw_prev_hit_pos_texture + 4 * i;
+        f3 prev_hit_pos = mk3(hp[0], hp[1], hp[2]);
+        d = normalize3(prev_hit_pos - o);
+        return true;
+    }
+    __device__ void finish_px(uint32_t px, uint32_t py, f3, f3, const RadianceTraceResult& r) const {
+        const size_t i = (size_t)py * io.width + px;
+        uint32_t Wbits = G(io.rw_prev_reservoir_texture)[2 * i + 1];
+        float prev_target = io.rw_prev_hit_pos_texture[4 * i + 3];
+        f3 prev_rad = S::ld3(io.rw_prev_hit_radiance_texture, i);
+        float max_rad = fmaxf(component_max(r.radiance), component_max(prev_rad));
+        f3 dr = abs3(r.radiance - prev_rad);
+        float lim = 0.5f * max_rad;
+        bool changed = dr.x > lim || dr.y > lim || dr.z > lim;
+        if (changed) {
+            G(io.rw_prev_reservoir_texture)[2 * i] = 1u;
+            G(io.rw_prev_reservoir_texture)[2 * i + 1] = Wbits;
+            float* out = io.rw_prev_hit_pos_texture + 4 * i;
+            out[0] = r.position_ws.x;
+            out[1] = r.position_ws.y;
+            out[2] = r.position_ws.z;
+            out[3] = prev_target;
+            S::st3(io.rw_prev_hit_radiance_texture, i, r.radiance);
+        }
+    }
+};
+
 template <class S, bool kCount>
 __device__ __forceinline__ void k_spec_sample_validate_body(const FrameParams& fp, const TraceRes& rt, const vgi_ind_spec_sample_validate_io& io, uint32_t bx_, uint32_t by_) {
-    VGI_TRAVERSAL_STACK(stack);
-    uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
-    const uint32_t w = io.width, h = io.height;
-    if (px >= w || py >= h) return;
-    const size_t i = (size_t)py * w + px;
-    float prev_depth = G(rt.prev_depth)[i];
-    if (prev_depth == 0.0f) return;
-    f3 position = position_biased(fp.prev_inv_view_proj, px, py, w, h, prev_depth);
-    uint32_t M = G(io.rw_prev_reservoir_texture)[2 * i];
-    uint32_t Wbits = G(io.rw_prev_reservoir_texture)[2 * i + 1];
-    const float* hp = io.rw_prev_hit_pos_texture + 4 * i;
-    f3 prev_hit_pos = mk3(hp[0], hp[1], hp[2]);
-    float prev_target = hp[3];
-    f3 prev_rad = S::ld3(io.rw_prev_hit_radiance_texture, i);
-    f3 dir = normalize3(prev_hit_pos - position);
-    RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, position, dir, true, stack);
-    float max_rad = fmaxf(component_max(r.radiance), component_max(prev_rad));
-    f3 dr = abs3(r.radiance - prev_rad);
-    float lim = 0.5f * max_rad;
-    bool changed = dr.x > lim || dr.y > lim || dr.z > lim;
-    if (changed) {
-        (void)M;
-        G(io.rw_prev_reservoir_texture)[2 * i] = 1u;
-        G(io.rw_prev_reservoir_texture)[2 * i + 1] = Wbits;
-        float* o = io.rw_prev_hit_pos_texture + 4 * i;
-        o[0] = r.position_ws.x;
-        o[1] = r.position_ws.y;
-        o[2] = r.position_ws.z;
-        o[3] = prev_target;
-        S::st3(io.rw_prev_hit_radiance_texture, i, r.radiance);
-    }
+    const SpecValidatePass<S> p{fp, rt, io};
+    sample_pixel<SpecValidatePass<S>, S, kCount>(p, rt, fp, true, bx_, by_);
 }
 
 template <class S, bool kCount>
