@@ -139,6 +139,32 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             float t3 = slab_nf(r, nrx.w, frx.w, nry.w, fry.w, nrz.w, frz.w, tmin, h.t);
             int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
             int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
+            if (kAnyHit) {
+                // an occlusion test may visit the hit children in any order:
+                // descend into the first one, push the others unsorted
+                bool have = false;
+                int32_t next = kTraversalDone;
+                auto take = [&](float t, int32_t l) {
+                    if (t == INFINITY) return;
+                    if (!have) {
+                        next = l;
+                        have = true;
+                    } else if (sp < depth) {
+                        push(l);
+                    }
+                };
+                take(t0, l0);
+                take(t1, l1);
+                take(t2, l2);
+                take(t3, l3);
+                node = have ? next : pop();
+                if (node < 0 && node != kTraversalDone && leaf == 0) {
+                    leaf = node;
+                    node = pop();
+                }
+                if (__all(leaf != 0)) break;
+                continue;
+            }
             // ascending entry distance (misses are +inf): a 5-exchange network
             sort_pair(t0, l0, t1, l1);
             sort_pair(t2, l2, t3, l3);
