@@ -404,6 +404,7 @@ def run_serving_form(args, sc, ntris):
     log(f"[bench] serving form: {len(reqs)} requests at {args.serve_rate:g}/s over {len(specs)} views")
     srv = serve.Server(sc, specs, hbm_budget_bytes=args.serve_budget_gb * 1e9, config=cfg,
                        max_batch=args.serve_max_batch, streams=2, batch_config=batch.BatchConfig())
+    admitted = srv.admit_all()
     stats = srv.run(reqs, timeout_s=600.0)
     srv.close()
     s = stats.summary()
@@ -412,7 +413,7 @@ def run_serving_form(args, sc, ntris):
             "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "mean_ms": round(s["mean_ms"], 3),
             "mpix_per_s": round(s["mpix_per_s"], 2), "wall_s": round(s["wall_s"], 3), "batches": s["batches"],
             "mean_batch": round(s["mean_batch"], 2), "evictions": s["evictions"],
-            "peak_hbm_gb": round(s["peak_hbm_gb"], 2)}
+            "peak_hbm_gb": round(s["peak_hbm_gb"], 2), "admitted_up_front": admitted}
 
 
 def run_precision_form(args, sc, ntris):

@@ -7,7 +7,8 @@ asks for the next frame of one of many persistent views.  A view's state
 (reservoirs, denoiser history, radiance cache) stays resident in HBM between
 its requests; a view is allocated on its first request if the HBM budget
 allows (least-recently-used idle views are evicted otherwise) and reset on
-re-admission after an eviction.
+re-admission after an eviction; admit_all() makes the views resident up
+front instead, as far as the budget allows.
 
 Scheduling (continuous batching): requests join a FIFO queue when they
 arrive.  Whenever one of `streams` batch slots is free, the scheduler takes
@@ -183,8 +184,24 @@ class Server:
         self.busy[k] = (ev, list(zip(picked, outs)))
         stats.batches += 1
 
+    def admit_all(self, stats: ServeStats | None = None) -> int:
+        """Make every view resident up front, in view order, while the HBM
+        budget allows (the rest are admitted on their first request).  A
+        view's device state is allocated once and kept, so this moves the
+        allocations out of the request path.  Returns the resident count."""
+        stats = stats or ServeStats()
+        for v in range(len(self.views)):
+            if v in self.resident:
+                continue
+            if self.bytes_resident + self._estimate_bytes(self.views[v]) > self.budget:
+                break
+            self._ensure_resident(v, stats, set())
+        self.torch.cuda.synchronize()
+        return len(self.resident)
+
     def run(self, requests: list[Request], timeout_s: float = 120.0) -> ServeStats:
         stats = ServeStats()
+        stats.peak_bytes = self.bytes_resident
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
         queue: collections.deque = collections.deque()
         self.t0 = time.perf_counter()
