@@ -10,9 +10,10 @@ sequencer) against the oracle.
   be bit-identical, float outputs bit-identical (they use IEEE operations
   only), and within the FP64 oracle's tolerance.
 
-Every test runs twice: with the reference's resampling constants (M cap 16,
-9 spiral taps, radius 0.05*min(w,h)) and with the 1280x720 frame-loop
-workload's (M cap 30, 5 taps, 30 px).
+Every test runs with the reference's resampling constants (M cap 16, 9
+spiral taps, radius 0.05*min(w,h)), with the 1280x720 frame-loop workload's
+(M cap 30, 5 taps, 30 px), and with the latter on a single stream (no
+concurrent lanes: the renderer's default overlaps independent pass chains).
 """
 from __future__ import annotations
 
@@ -30,6 +31,8 @@ W, H = 96, 64
 PARAMS = {
     "reference": dict(temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0),
     "frame_loop": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0),
+    "frame_loop_one_stream": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0,
+                                  concurrent_lanes=False),
 }
 
 

@@ -86,3 +86,21 @@ def test_admission_evicts_idle_views_under_a_small_budget(vgi):
     assert one > 0 and s["requests"] == len(reqs)
     assert s["evictions"] > 0
     assert s["peak_hbm_gb"] * 1e9 <= srv.budget
+
+
+def test_up_front_admission_stops_at_the_budget(vgi):
+    """admit_all makes views resident in view order while they fit; the run
+    then admits and evicts the rest on demand, and every request completes."""
+    from violet_amd import serve
+
+    sc, views, cfg = _setup()
+    srv = serve.Server(sc, views, hbm_budget_bytes=0, config=cfg, max_batch=4, streams=2)
+    sizes = [srv._estimate_bytes(v) for v in views]
+    srv.budget = sizes[0] + sizes[1] + 0.5 * min(sizes[2:])  # the first two views fit
+    assert srv.admit_all() == 2 and sorted(srv.resident) == [0, 1]
+    assert srv.bytes_resident <= srv.budget
+    reqs = serve.poisson_requests(12, 500.0, len(views), seed=7)
+    stats = srv.run(reqs, timeout_s=60.0)
+    srv.close()
+    s = stats.summary()
+    assert s["requests"] == len(reqs) and s["peak_hbm_gb"] * 1e9 <= srv.budget
