@@ -208,7 +208,7 @@ def run_frame(args):
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16 if args.storage == "f16" else _lib.STORAGE_F32,
                                 atrous_iterations=args.iterations, concurrent_lanes=not args.no_lanes,
-                                **RESAMPLE[args.resample])
+                                compose=args.compose, **RESAMPLE[args.resample])
     r = renderer.RestirRenderer(w, h, cfg)
     fps = frame_params_seq(w, h, args.frames)
 
@@ -466,6 +466,8 @@ def main():
                     help="run plain 256 MiB device copies after warmup (scale for rocprofv3 --pmc counters)")
     ap.add_argument("--forms", default="batched,serving,precision",
                     help="comma list of forms reported beside the plain frame path ('' for none)")
+    ap.add_argument("--compose", action="store_true",
+                    help="also run the sun shadow mask and the final lighting in every frame")
     ap.add_argument("--big-triangles", type=int, default=1_000_000)
     ap.add_argument("--batch-views", type=int, default=16)
     ap.add_argument("--batch-frames", type=int, default=30)
@@ -550,7 +552,7 @@ def main():
                     "gi_rays_per_px": "1 diffuse + 1 specular (+1 shadow each)", "atrous_iterations": args.iterations,
                     "restir_params": RESAMPLE_DESC[args.resample],
                     "storage": args.storage, "parallelism": "single-gpu",
-                    "concurrent_lanes": not args.no_lanes} if frame else
+                    "concurrent_lanes": not args.no_lanes, "compose": args.compose} if frame else
                    {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
                     "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
                     "storage": args.storage, "parallelism": "single-gpu"}),

@@ -570,6 +570,41 @@ vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const vgi_scene*
                                     void* stream);
 
 /* ------------------------------------------------------------------------
+ * Direct light and composition, the passes right after the GI path
+ * (restir_renderer.rs:831-911).
+ * ---------------------------------------------------------------------- */
+/* raytraced_shadow.hlsl:9-53 (restir_renderer.rs:831-849): one sun shadow
+ * ray per pixel from the depth-reconstructed position (+2^-24 depth bias),
+ * t in [0.0005, 1000], any hit.  R8_UNORM semantics: 255 = lit, 0 =
+ * shadowed.  Pixels without geometry (depth 0) get 0; the reference leaves
+ * them unwritten. */
+typedef struct vgi_raytraced_shadow_io {
+    uint32_t width, height;
+    const float* gbuffer_depth;
+    uint8_t* rw_shadow;
+} vgi_raytraced_shadow_io;
+vgi_status vgi_pass_raytraced_shadow(const vgi_frame_params* fp, const vgi_scene* scene,
+                                     const vgi_raytraced_shadow_io* io, void* stream);
+
+/* restir/final_lighting.hlsl:16-62 (restir_renderer.rs:893-911): sun light
+ * through the shadow mask plus the demodulated indirect lighting
+ * re-modulated by the diffuse albedo and the GGX integral
+ * (DEMODULATE_INDIRECT_SPECULAR_FOR_DENOISER, IND_SPEC_R_MIN = 0.002, floor
+ * 1/65536; restir/config.inc.hlsl).  Pixels without geometry get
+ * sky_radiance (the sky pass's output, constant here).  Colours in the
+ * storage mode's radiance layout. */
+typedef struct vgi_final_lighting_io {
+    uint32_t width, height, storage;
+    const uint32_t* gbuffer_color;          /* 4 layers */
+    const uint8_t* shadow_mask_buffer;
+    const void* indirect_diffuse_texture;   /* radiance */
+    const void* indirect_specular_texture;  /* radiance */
+    void* rw_color_buffer;                  /* radiance */
+    float sky_radiance[3];
+} vgi_final_lighting_io;
+vgi_status vgi_pass_final_lighting(const vgi_frame_params* fp, const vgi_final_lighting_io* io, void* stream);
+
+/* ------------------------------------------------------------------------
  * Renderer sequencer: crates/violet-restir/src/restir_renderer.rs
  * `RestirRenderer::add_passes` (:258-1028).  Records the whole GI pass chain
  * of one frame in violet's order, with violet's FrameRand seed consumption
@@ -595,6 +630,9 @@ typedef struct vgi_renderer_config {
      * G-buffer, specular beside diffuse resampling), forked from and joined
      * back into the caller's stream; 0 = every pass on the caller's stream */
     uint32_t concurrent_lanes;
+    /* default 0: 1 = after the denoiser also run the sun shadow mask and the
+     * final lighting (restir_renderer.rs:831-911), the lit image in `color` */
+    uint32_t compose;
     vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
 } vgi_renderer_config;
 
@@ -608,6 +646,7 @@ typedef struct vgi_renderer_output {
     uint32_t denoised;              /* 1 if the denoiser ran this frame */
     uint32_t validation_frame;      /* 1 if this was a diffuse validation frame */
     uint32_t hash_grid_updated;     /* 1 if the cache update passes ran */
+    const void* color;              /* final lighting (radiance layout) when compose = 1, else NULL */
 } vgi_renderer_output;
 
 vgi_status vgi_renderer_default_config(vgi_renderer_config* cfg);

@@ -247,6 +247,8 @@ def _restir_sigs(lib):
     lib.oracle_hg_compact.restype = U32
     lib.oracle_hg_compact.argtypes = [P, P, P]
     lib.oracle_hg_raygen.argtypes = [P, TR, P, P, U32, P, P, P, U32]
+    lib.oracle_raytraced_shadow.argtypes = [P, P, I, I, P, P]
+    lib.oracle_final_lighting.argtypes = [P, I, I, P, P, P, P, P, P]
 
 
 def _fp_arr(fp):
@@ -320,6 +322,21 @@ class OracleRestir:
         gb = np.zeros((4, h, w), np.uint32)
         self.lib.oracle_gbuffer(_np_ptr(_fp_arr(fp)), self.scene.h, w, h, _np_ptr(depth), _np_ptr(gb))
         return depth, gb
+
+    def raytraced_shadow(self, fp, depth):
+        h, w = depth.shape
+        mask = np.zeros((h, w), np.uint8)
+        self.lib.oracle_raytraced_shadow(_np_ptr(_fp_arr(fp)), self.scene.h, w, h, _np_ptr(self.r(depth)), _np_ptr(mask))
+        return mask
+
+    def final_lighting(self, fp, gb, mask, ind_diff, ind_spec):
+        h, w = mask.shape
+        color = np.zeros((h, w, 3), self.rt_dtype)
+        sky = np.asarray(self.sky, np.float32)
+        self.lib.oracle_final_lighting(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(np.ascontiguousarray(gb, np.uint32)),
+                                       _np_ptr(np.ascontiguousarray(mask, np.uint8)), _np_ptr(self.r(ind_diff)),
+                                       _np_ptr(self.r(ind_spec)), _np_ptr(sky), _np_ptr(color))
+        return color
 
     def diff_sample_gen(self, fp, tr, gb, depth, frame_rand, has_prev):
         h, w = depth.shape

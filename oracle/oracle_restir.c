@@ -15,6 +15,8 @@
  *   trace_radiance / shading    restir/raytrace.inc.hlsl:65-208,
  *                               raytrace/geometry_rchit.inc.hlsl:376-472
  *   hash-grid passes            restir/hash_grid_{decay,query_select,query_compact,raygen}.hlsl
+ *   oracle_raytraced_shadow     raytraced_shadow.hlsl:9-53
+ *   oracle_final_lighting       restir/final_lighting.hlsl:16-62
  *
  * Scene: world-space f32 positions/normals, u32 indices, per-triangle
  * material {base_color rgba, metallic, roughness, pad, pad}; queried through
@@ -1075,4 +1077,66 @@ void oracle_hg_raygen(const float* fpf, const otrace_res* rt, uint32_t* cells, c
         c[3] = f_bits((float)b.z);
         decay[cell] = 10;
     }
+}
+
+/* raytraced_shadow.hlsl:9-53: one any-hit ray toward the sun per pixel with
+ * geometry; 255 (R8_UNORM 1.0) where it misses, 0 where it hits.  Pixels
+ * without geometry hold 0 (the reference leaves them unwritten; the final
+ * lighting never reads them). */
+void oracle_raytraced_shadow(const float* fpf, const oscene* s, int w, int h, const real* depth, uint8_t* o_mask) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            real d = depth[i];
+            if (d == RC(0.0)) {
+                o_mask[i] = 0;
+                continue;
+            }
+            v3 pos = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, d + DEPTH_ERR);
+            ohit sh = obvh_trace(s->bvh, pos, o_xyz(fp.sun_dir), RC(0.0005), RC(1000.0), 1);
+            o_mask[i] = sh.id == 0xFFFFFFFFu ? 255 : 0;
+        }
+}
+
+/* restir/final_lighting.hlsl:16-62 (DIRECT_SPECULAR_SUPRESSION 1,
+ * DEMODULATE_INDIRECT_SPECULAR_FOR_DENOISER 1, IND_SPEC_R_MIN 0.002,
+ * integral floor 1/65536: restir/config.inc.hlsl:5-10).  Pixels without
+ * geometry take the constant sky radiance (the sky pass's colour). */
+void oracle_final_lighting(const float* fpf, int w, int h, const uint32_t* gb, const uint8_t* mask,
+                           const real* ind_diff, const real* ind_spec, const float* sky, real* o_color) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    size_t plane = (size_t)w * h;
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            if (gb[3 * plane + i] == 0u) {
+                st3r(o_color, i, V3(sky[0], sky[1], sky[2]));
+                continue;
+            }
+            v3 col = o_gb_color(gb[i]);
+            real metal = o_gb_metal(gb[i]);
+            v3 n = o_gb_normal(gb[plane + i]);
+            real perc = o_gb_rough(gb[plane + i]);
+            real atten = (real)mask[i] / RC(255.0);
+            /* cs_view_ray_direction with zero jitter: the pixel corner */
+            real ndx = (((real)px + RC(0.5)) + RC(-0.5)) / (real)w * RC(2.0) - RC(1.0);
+            real ndy = (((real)py + RC(0.5)) + RC(-0.5)) / (real)h * RC(2.0) - RC(1.0);
+            v4 e = o_mul44(fp.inv_view_proj, V4(ndx, ndy, RC(1.0), RC(1.0)));
+            v3 vd = neg3(norm3(sub3(V3(e.x / e.w, e.y / e.w, e.z / e.w), o_xyz(fp.view_pos))));
+            real pr = R_FMAX(perc, RC(0.045));
+            v3 rho = scale3(col, RC(1.0) - metal), f0 = o_f0(col, metal);
+            v3 sun = o_xyz(fp.sun_dir);
+            real NoL = sat(dot3(n, sun));
+            v3 direct = mul3(scale3(o_eval_ggx_lambert(vd, sun, n, pr, rho, f0), NoL * atten), o_xyz(fp.sun_inten));
+            real NoV = sat(dot3(n, vd));
+            v3 ri = o_ggx_integral(NoV, R_FMAX(perc * perc, RC(0.002)), f0);
+            real rmin = RC(1.0) / RC(65536.0);
+            v3 sr = V3(R_FMAX(ri.x, rmin), R_FMAX(ri.y, rmin), R_FMAX(ri.z, rmin));
+            v3 ind = add3(mul3(ld3r(ind_diff, i), rho), mul3(ld3r(ind_spec, i), sr));
+            st3r(o_color, i, add3(direct, ind));
+        }
 }

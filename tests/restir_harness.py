@@ -88,6 +88,9 @@ def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
             s["out_diffuse"] = r.read_ptr(o.indirect_diffuse, rb, dt, (h, w, 3)).copy()
             s["out_specular"] = r.read_ptr(o.indirect_specular, rb, dt, (h, w, 3)).copy()
             s["flags"] = dict(denoised=o.denoised, validation=o.validation_frame, hg=o.hash_grid_updated)
+            if o.color:
+                s["shadow_mask"] = r.read("shadow_mask", np.uint8, (h, w)).copy()
+                s["out_color"] = r.read_ptr(o.color, rb, dt, (h, w, 3)).copy()
         fps.append(fp)
         outs.append(o)
         snaps.append(s)

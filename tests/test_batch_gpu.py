@@ -3,14 +3,15 @@
 same view rendered alone.
 
 * Frame 0 (no cache, no history): every image bit-identical.
-* Every frame: depth, G-buffer and the new diffuse hit positions (functions
-  of the camera, the scene and the per-pixel random streams only) are
-  bit-identical.
+* Every frame: depth, G-buffer, the sun shadow mask and the new diffuse hit
+  positions (functions of the camera, the scene and the per-pixel random
+  streams only) are bit-identical.  The views also run the composition
+  passes (shadow mask and final lighting).
 * Later frames: the hash-grid cache update is scheduling dependent (query
   slots come from an atomic counter, reference restir/raytrace.inc.hlsl:
   190-197), alone as well as in the mix, so radiance that went through the
   cache may differ: the bar there is rel_mean <= 2e-2 between the two runs'
-  raw and denoised lighting.
+  raw, denoised and final lighting.
 """
 from __future__ import annotations
 
@@ -24,11 +25,12 @@ pytestmark = pytest.mark.gpu
 
 SIZES = [(96, 64), (72, 40), (44, 57)]
 FRAMES = 4
-EXACT_ALWAYS = ["depth", "gbuffer"]
+EXACT_ALWAYS = ["depth", "gbuffer", "shadow_mask"]
 EXACT_FRAME0 = ["diff_reservoir", "diff_hit_pos_normal", "diff_hit_radiance", "indirect_diffuse_raw",
                 "spec_reservoir", "indirect_specular_raw", "spec_ray_len", "ao", "ao_filtered"]
 SHAPES = {
     "depth": (F32, lambda w, h: (h, w)), "gbuffer": (np.uint32, lambda w, h: (4, h, w)),
+    "shadow_mask": (np.uint8, lambda w, h: (h, w)),
     "diff_reservoir": (np.uint32, lambda w, h: (h, w)), "diff_hit_pos_normal": (np.uint32, lambda w, h: (h, w, 2)),
     "diff_hit_radiance": (F32, lambda w, h: (h, w, 3)), "indirect_diffuse_raw": (F32, lambda w, h: (h, w, 3)),
     "spec_reservoir": (np.uint32, lambda w, h: (h, w, 2)), "indirect_specular_raw": (F32, lambda w, h: (h, w, 3)),
@@ -48,6 +50,7 @@ def _views():
 def _snap(r, w, h, o):
     s = {n: r.read(n, dt, shp(w, h)).copy() for n, (dt, shp) in SHAPES.items()}
     s["out_diffuse"] = r.read_ptr(o.indirect_diffuse, w * h * 12, F32, (h, w, 3)).copy()
+    s["out_color"] = r.read_ptr(o.color, w * h * 12, F32, (h, w, 3)).copy()
     s["validation"] = o.validation_frame
     return s
 
@@ -60,7 +63,7 @@ def test_views_same_alone_and_in_the_mix(vgi, graph):
 
     mesh = make_mesh()
     sc = scene.Scene(mesh)
-    cfg = renderer.RestirConfig(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0)
+    cfg = renderer.RestirConfig(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0, compose=True)
     views = _views()
     fps = [[synth.make_frame_params(vs[f], vs[f - 1] if f else None) for f in range(FRAMES)] for _, _, vs in views]
     alone = [renderer.RestirRenderer(w, h, cfg) for w, h, _ in views]
@@ -77,13 +80,13 @@ def test_views_same_alone_and_in_the_mix(vgi, graph):
         for k, (sa, sb) in enumerate(zip(snap_a, snap_b)):
             where = f"view {k} {SIZES[k]} frame {f}"
             assert outs_a[k].denoised == outs_b[k].denoised and sa["validation"] == sb["validation"], where
-            names = EXACT_ALWAYS + (EXACT_FRAME0 + ["out_diffuse"] if f == 0 else [])
+            names = EXACT_ALWAYS + (EXACT_FRAME0 + ["out_diffuse", "out_color"] if f == 0 else [])
             if not sa["validation"]:
                 names = names + ["diff_new_hit_pos_normal"]
             for n in names:
                 assert np.array_equal(sa[n].view(np.uint8), sb[n].view(np.uint8)), f"{n} differs: {where}"
             if f > 0:
-                for n in ("indirect_diffuse_raw", "indirect_specular_raw", "out_diffuse"):
+                for n in ("indirect_diffuse_raw", "indirect_specular_raw", "out_diffuse", "out_color"):
                     e = rel_mean(sb[n], sa[n])
                     assert e <= 2e-2, f"{n} rel_mean {e:.3g}: {where}"
 

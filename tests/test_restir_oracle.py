@@ -8,7 +8,12 @@
 * resampling parameters: the explicit reference constants (M cap 16, 9
   spiral taps, radius 0.05*min(w,h)) give bit-identical results to the
   defaults; a cap bounds the temporal M exactly; a different tap count or
-  radius changes the spatial result.
+  radius changes the spatial result;
+* sun shadow mask (raytraced_shadow.hlsl:9-53): against brute-force any-hit
+  rays built here from the depth buffer;
+* final lighting (restir/final_lighting.hlsl:16-62): the direct term against
+  a textbook GGX + Lambert evaluation, the indirect terms for linearity and
+  the specular reflectance against quadrature of the GGX albedo.
 """
 from __future__ import annotations
 
@@ -200,3 +205,198 @@ def test_temporal_m_cap_bounds_history(R32, frame_inputs, cap):
     geo = f["d1"] > 0
     assert M_out[geo].max() <= cap + 1
     assert (M_out[geo] == cap + 1).any()
+
+
+# ---------------------------------------------------------------------------
+# direct light and composition
+# ---------------------------------------------------------------------------
+def _inv_view_proj(fp):
+    return np.array(fp.inv_view_proj, np.float64).reshape(4, 4).T  # stored column-major
+
+
+def test_raytraced_shadow_matches_brute_force_any_hit():
+    """The shadow mask equals a brute-force any-hit test (every triangle) of a
+    ray toward the sun from the position reconstructed at depth + 2^-24,
+    with the reference's t range [0.0005, 1000]."""
+    import oracle
+    from restir_harness import orbit_view
+    from violet_amd import synth
+
+    from restir_harness import make_mesh
+
+    mesh = make_mesh()
+    w, h = 48, 32
+    R = oracle.OracleRestir(64, oracle.OracleScene(64, mesh))
+    fp = synth.make_frame_params(orbit_view(0, w, h), sun_dir=(0.6, 0.3, 0.35))  # low sun: long shadows
+    depth, _ = R.gbuffer(fp, w, h)
+    mask = R.raytraced_shadow(fp, depth)
+    assert (mask[depth == 0] == 0).all()
+    py, px = np.nonzero(depth > 0)
+    ndc = np.stack([(px + 0.5) / w * 2 - 1, (py + 0.5) / h * 2 - 1, depth[py, px] + 2.0 ** -24, np.ones(len(px))], 1)
+    hp = ndc @ _inv_view_proj(fp).T
+    rays = np.zeros((len(px), 8), np.float32)
+    rays[:, :3] = hp[:, :3] / hp[:, 3:]
+    rays[:, 3] = 0.0005
+    rays[:, 4:7] = np.array(fp.sun_dir[:3])
+    rays[:, 7] = 1000.0
+    _, ids = oracle.trace(64, mesh.positions, mesh.indices, rays, any_hit=True, brute=True)
+    expect = np.where(ids == 0xFFFFFFFF, 255, 0)
+    assert 0.05 < np.mean(expect == 0) < 0.95, "scene must have lit and shadowed pixels"
+    # origins rounded to f32 for the brute-force rays may flip grazing pixels
+    assert np.mean(mask[py, px] != expect) <= 0.005
+
+
+def make_mesh_small():
+    from restir_harness import make_mesh
+
+    return make_mesh(400)
+
+
+def _oct_encode12(n):
+    p = n[:, :2] / np.abs(n).sum(1, keepdims=True)
+    lower = n[:, 2] < 0
+    p[lower] = (1 - np.abs(p[lower][:, ::-1])) * np.where(p[lower] >= 0, 1.0, -1.0)
+    e = np.rint((p * 0.5 + 0.5) * 4095).astype(np.uint32)
+    return e[:, 0] | (e[:, 1] << 12)
+
+
+def _oct_decode12(l1):
+    f = np.stack([(l1 & 0xFFF) / 4095.0, ((l1 >> 12) & 0xFFF) / 4095.0], 1) * 2 - 1
+    nz = 1 - np.abs(f).sum(1)
+    t = np.clip(-nz, 0, 1)
+    n = np.stack([f[:, 0] + np.where(f[:, 0] >= 0, -t, t), f[:, 1] + np.where(f[:, 1] >= 0, -t, t), nz], 1)
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+@pytest.fixture(scope="module")
+def lighting_case():
+    """A synthetic G-buffer over a 16x8 view: random colours, metalness and
+    roughness, normals tilted toward the viewer, three sky pixels."""
+    from restir_harness import orbit_view
+    from violet_amd import synth
+
+    w, h = 16, 8
+    rng = np.random.default_rng(11)
+    fp = synth.make_frame_params(orbit_view(3, w, h))
+    n_px = w * h
+    py, px = np.divmod(np.arange(n_px), w)
+    ndc = np.stack([px / w * 2 - 1, py / h * 2 - 1, np.ones(n_px), np.ones(n_px)], 1)  # pixel corners
+    e = ndc @ _inv_view_proj(fp).T
+    d = e[:, :3] / e[:, 3:] - np.array(fp.view_pos[:3])
+    v = -d / np.linalg.norm(d, axis=1, keepdims=True)
+    n = v + 0.7 * rng.normal(size=(n_px, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    col = rng.integers(0, 256, (n_px, 3)).astype(np.uint32)
+    metal = rng.choice([0, 0, 255, 128], n_px).astype(np.uint32)
+    rough = rng.integers(0, 256, n_px).astype(np.uint32)
+    rough[:4] = [0, 5, 11, 12]  # below and at the 0.045 clamp
+    gb = np.zeros((4, h, w), np.uint32)
+    gb[0] = (col[:, 0] | (col[:, 1] << 8) | (col[:, 2] << 16) | (metal << 24)).reshape(h, w)
+    gb[1] = (_oct_encode12(n) | (rough << 24)).reshape(h, w)
+    gb[3] = 1
+    gb[3].reshape(-1)[[20, 57, 100]] = 0
+    mask = rng.choice([0, 255, 128], n_px).astype(np.uint8).reshape(h, w)
+    return dict(fp=fp, w=w, h=h, v=v, n=_oct_decode12(gb[1].reshape(-1)), col=col / 255.0, metal=metal / 255.0,
+                perc=rough / 255.0, gb=gb, mask=mask, geo=gb[3].reshape(-1) != 0)
+
+
+def _textbook_ggx_lambert(v, l, n, pr, rho, f0):
+    """Lambert + GGX (Walter 2007 D, height-correlated Smith visibility,
+    Heitz 2014, Schlick Fresnel), alpha = perceptual roughness squared."""
+    hv = v + l
+    hv /= np.linalg.norm(hv, axis=1, keepdims=True)
+    NoL = np.clip((n * l).sum(1), 0, 1)
+    NoV = np.clip((n * v).sum(1), 0, 1)
+    NoH = np.clip((n * hv).sum(1), 0, 1)
+    LoH = np.clip((l * hv).sum(1), 0, 1)
+    a2 = (pr * pr) ** 2
+    D = a2 / (np.pi * (NoH ** 2 * (a2 - 1) + 1) ** 2)
+    Vis = 0.5 / (NoL * np.sqrt(NoV ** 2 * (1 - a2) + a2) + NoV * np.sqrt(NoL ** 2 * (1 - a2) + a2))
+    F = f0 + (1 - f0) * ((1 - LoH) ** 5)[:, None]
+    return rho / np.pi + F * (D * Vis)[:, None]
+
+
+def test_final_lighting_direct_term_matches_textbook_ggx(lighting_case):
+    """Indirect inputs zero: the output is the sun's direct light,
+    (Lambert + GGX) * NoL * mask * sun_intensity, with the view direction
+    through the pixel corner and perceptual roughness clamped to 0.045; sky
+    pixels take the sky radiance."""
+    import oracle
+
+    c = lighting_case
+    R = oracle.OracleRestir(64, oracle.OracleScene(64, make_mesh_small()), sky=(0.25, 0.5, 2.0))
+    z = np.zeros((c["h"], c["w"], 3))
+    out = R.final_lighting(c["fp"], c["gb"], c["mask"], z, z).reshape(-1, 3)
+    g = c["geo"]
+    assert np.array_equal(out[~g], np.tile([0.25, 0.5, 2.0], (int((~g).sum()), 1)))
+    l = np.tile(np.array(c["fp"].sun_dir[:3], np.float64), (len(g), 1))
+    rho = c["col"] * (1 - c["metal"])[:, None]
+    f0 = 0.04 + (c["col"] - 0.04) * c["metal"][:, None]
+    pr = np.maximum(c["perc"], 0.045)
+    NoL = np.clip((c["n"] * l).sum(1), 0, 1)
+    atten = c["mask"].reshape(-1) / 255.0
+    lit = NoL * atten > 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        brdf = _textbook_ggx_lambert(c["v"], l, c["n"], pr, rho, f0)
+        ref = np.where(lit[:, None], brdf * (NoL * atten)[:, None], 0.0) * np.array(c["fp"].sun_inten[:3])
+    assert lit[g].sum() > 30
+    np.testing.assert_allclose(out[g], ref[g], rtol=1e-6, atol=1e-9)
+
+
+def _ggx_albedo(mu, alpha, n=400):
+    """Directional albedo of the GGX lobe (height-correlated Smith), split as
+    E = f0 * A + B for Schlick Fresnel, by midpoint quadrature (the mirror
+    limit for a lobe narrower than the grid)."""
+    if alpha < 1e-3:
+        fc = (1 - mu) ** 5
+        return 1 - fc, fc
+    v = np.array([np.sqrt(max(0.0, 1 - mu * mu)), 0.0, mu])
+    th = (np.arange(n) + 0.5) / n * np.pi / 2
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    l = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    hv = l + v
+    hv /= np.linalg.norm(hv, axis=-1, keepdims=True)
+    NoL, NoH, VoH = l[..., 2], hv[..., 2], (hv * v).sum(-1)
+    a2 = alpha * alpha
+    D = a2 / (np.pi * (NoH ** 2 * (a2 - 1) + 1) ** 2)
+    Vis = 0.5 / (NoL * np.sqrt(mu * mu * (1 - a2) + a2) + mu * np.sqrt(NoL ** 2 * (1 - a2) + a2))
+    base = D * Vis * NoL * np.sin(T) * (np.pi / 2 / n) * (np.pi / n)
+    Fc = (1 - VoH) ** 5
+    return (base * (1 - Fc)).sum(), (base * Fc).sum()
+
+
+def test_final_lighting_indirect_terms(lighting_case):
+    """Sun masked out: the output is ind_diff * diffuse_rho + ind_spec * R,
+    linear in both inputs.  R is the rational fit of the GGX directional
+    albedo (Hirvonen 2019) evaluated at roughness = max(perceptual^2, 0.002);
+    the fit's own argument is perceptual roughness (alpha = argument^2), so R
+    is checked against quadrature of the GGX albedo at alpha = roughness^2,
+    to the fit's accuracy."""
+    import oracle
+
+    c = lighting_case
+    R = oracle.OracleRestir(64, oracle.OracleScene(64, make_mesh_small()))
+    g = c["geo"]
+    shape = (c["h"], c["w"], 3)
+    m0 = np.zeros_like(c["mask"])
+    rng = np.random.default_rng(2)
+    dif, spc = rng.random(shape), rng.random(shape)
+    z = np.zeros(shape)
+    base = R.final_lighting(c["fp"], c["gb"], m0, dif, spc).reshape(-1, 3)
+    only_d = R.final_lighting(c["fp"], c["gb"], m0, dif, z).reshape(-1, 3)
+    only_s = R.final_lighting(c["fp"], c["gb"], m0, z, spc).reshape(-1, 3)
+    np.testing.assert_allclose(base[g], only_d[g] + only_s[g], rtol=1e-12)
+    rho = c["col"] * (1 - c["metal"])[:, None]
+    np.testing.assert_allclose(only_d[g], dif.reshape(-1, 3)[g] * rho[g], rtol=1e-12)
+    refl = R.final_lighting(c["fp"], c["gb"], m0, z, np.ones(shape)).reshape(-1, 3)
+    assert (refl[g] >= 1.0 / 65536.0).all()
+    f0 = 0.04 + (c["col"] - 0.04) * c["metal"][:, None]
+    NoV = np.clip((c["n"] * c["v"]).sum(1), 0, 1)
+    rough = np.maximum(c["perc"] ** 2, 0.002)
+    errs = []
+    for i in np.nonzero(g & (NoV > 0.15))[0][::6]:
+        A, B = _ggx_albedo(NoV[i], rough[i] ** 2)
+        errs.append(np.abs(refl[i] - (f0[i] * A + B)).max())
+    assert len(errs) >= 10
+    assert max(errs) < 0.08 and np.mean(errs) < 0.03, errs

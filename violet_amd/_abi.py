@@ -72,15 +72,20 @@ HGRaygenIO = _struct("HGRaygenIO", [
     ("rw_hash_grid_storage_buffer", vp), ("hash_grid_query_buffer", vp), ("compact_query_counter_buffer", vp),
     ("compact_query_index_buffer", vp), ("compact_query_cell_addr_buffer", vp), ("rw_hash_grid_decay_buffer", vp)])
 GBufferIO = _struct("GBufferIO", [("width", u32), ("height", u32), ("rw_depth", vp), ("rw_gbuffer", vp)])
+RaytracedShadowIO = _struct("RaytracedShadowIO", [("width", u32), ("height", u32), ("gbuffer_depth", vp),
+                                                  ("rw_shadow", vp)])
+FinalLightingIO = _struct("FinalLightingIO", WH + [
+    ("gbuffer_color", vp), ("shadow_mask_buffer", vp), ("indirect_diffuse_texture", vp),
+    ("indirect_specular_texture", vp), ("rw_color_buffer", vp), ("sky_radiance", f32 * 3)])
 
 RendererConfig = _struct("RendererConfig", [
     ("storage", u32), ("denoise", u32), ("ind_diff_validate", u32), ("ind_spec_validate", u32),
     ("hash_grid_cache_decay", u32), ("ao_radius", f32), ("sky_radiance", f32 * 3),
     ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32), ("concurrent_lanes", u32),
-    ("denoiser", _lib.DenoiserConfig)])
+    ("compose", u32), ("denoiser", _lib.DenoiserConfig)])
 RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
-    ("validation_frame", u32), ("hash_grid_updated", u32)])
+    ("validation_frame", u32), ("hash_grid_updated", u32), ("color", vp)])
 
 BatchConfig = _struct("BatchConfig", [("max_views", u32), ("pixel_groups", u32), ("trace_groups", u32),
                                      ("atrous_groups", u32), ("use_graph", u32), ("concurrent_lanes", u32)])
@@ -104,6 +109,8 @@ _SIGS = {
     "vgi_pass_hash_grid_raygen": [FPp, TRp, P(HGRaygenIO), P(PCFrameHash), vp],
     "vgi_pass_clear_buffer": [vp, u32, u32, vp],
     "vgi_pass_gbuffer_primary": [FPp, vp, P(GBufferIO), vp],
+    "vgi_pass_raytraced_shadow": [FPp, vp, P(RaytracedShadowIO), vp],
+    "vgi_pass_final_lighting": [FPp, P(FinalLightingIO), vp],
     "vgi_renderer_default_config": [P(RendererConfig)],
     "vgi_renderer_create": [u32, u32, P(RendererConfig), P(vp)],
     "vgi_renderer_destroy": [vp],

@@ -63,7 +63,7 @@ class Batch:
         _lib.call("vgi_batch_add_passes", self._h, scene.handle, n, hv, fa, ia, oa,
                   gpu.stream() if stream is None else stream)
         return [FrameOutput(o.indirect_diffuse, o.indirect_specular, o.depth, o.gbuffer, bool(o.denoised),
-                            bool(o.validation_frame), bool(o.hash_grid_updated)) for o in oa]
+                            bool(o.validation_frame), bool(o.hash_grid_updated), o.color) for o in oa]
 
     def close(self):
         if getattr(self, "_h", None):

@@ -32,6 +32,7 @@ static const BatchOps* batch_ops(int kind) {
     if (!o) o = batch_ops_sample(kind);
     if (!o) o = batch_ops_resample(kind);
     if (!o) o = batch_ops_denoise(kind);
+    if (!o) o = batch_ops_compose(kind);
     return o;
 }
 
@@ -43,7 +44,8 @@ static const char* kind_name(int kind) {
         "batch_raytraced_ao_filter", "batch_ind_diff_temporal_resample", "batch_ind_diff_spatial_resample",
         "batch_ind_spec_sample_gen", "batch_ind_spec_sample_validate", "batch_ind_spec_temporal_resample",
         "batch_ind_spec_spatial_resample", "batch_denoise_temporal", "batch_denoise_disocclusion_fix",
-        "batch_denoise_fast_history_clamp", "batch_denoise_spatial_fallback", "batch_denoise_atrous"};
+        "batch_denoise_fast_history_clamp", "batch_denoise_spatial_fallback", "batch_denoise_atrous",
+        "batch_raytraced_shadow", "batch_final_lighting"};
     return kind >= 0 && kind < kPassKindCount ? n[kind] : "batch_unknown";
 }
 }  // namespace vgi

@@ -53,6 +53,7 @@ const BatchOps* batch_ops_denoise(int kind);
 const BatchOps* batch_ops_resample(int kind);
 const BatchOps* batch_ops_sample(int kind);
 const BatchOps* batch_ops_hashgrid(int kind);
+const BatchOps* batch_ops_compose(int kind);
 const BatchOps* batch_ops_gbuffer(int kind);
 const BatchOps* batch_ops_util(int kind);
 

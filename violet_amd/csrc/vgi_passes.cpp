@@ -62,6 +62,8 @@ vgi_status DirectSink::submit(int slot, const PassRec& r) {
         case kPassDenFastClamp: return vgi_pass_denoise_fast_history_clamp(&r.io.den_fast_clamp, s);
         case kPassDenFallback: return vgi_pass_denoise_spatial_fallback(&r.io.den_spatial, s);
         case kPassDenAtrous: return vgi_pass_denoise_atrous(&r.io.den_atrous, &r.pc.den_atrous, s);
+        case kPassShadow: return vgi_pass_raytraced_shadow(&r.fp, r.scene, &r.io.shadow, s);
+        case kPassFinalLighting: return vgi_pass_final_lighting(&r.fp, &r.io.final_lighting, s);
         default: return set_error(VGI_E_INVALID, "internal: unknown pass kind %d", r.kind);
     }
 }

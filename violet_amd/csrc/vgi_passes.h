@@ -37,6 +37,8 @@ enum PassKind : int {
     kPassDenFastClamp,
     kPassDenFallback,
     kPassDenAtrous,
+    kPassShadow,          // raytraced_shadow.hlsl
+    kPassFinalLighting,   // restir/final_lighting.hlsl
     kPassKindCount
 };
 
@@ -69,7 +71,9 @@ enum Slot : int {
     kSlotDenFastClamp,
     kSlotDenFallback,
     kSlotDenAtrous0,  // .. kSlotDenAtrous0 + 4
-    kSlotCount = kSlotDenAtrous0 + 5
+    kSlotShadow = kSlotDenAtrous0 + 5,
+    kSlotFinalLighting,
+    kSlotCount
 };
 
 // The pass kind each slot carries.
@@ -81,7 +85,7 @@ inline int slot_kind(int slot) {
         kPassDiffTemporal, kPassDiffSpatial,  kPassSpecGen,       kPassSpecValidate, kPassSpecTemporal,
         kPassSpecSpatial,  kPassClear,        kPassClear,         kPassDenTemporal, kPassDenDisocclusion,
         kPassDenFastClamp, kPassDenFallback,  kPassDenAtrous,     kPassDenAtrous,   kPassDenAtrous,
-        kPassDenAtrous,    kPassDenAtrous};
+        kPassDenAtrous,    kPassDenAtrous,    kPassShadow,        kPassFinalLighting};
     return k[slot];
 }
 
@@ -110,6 +114,8 @@ struct PassRec {
         vgi_denoise_spatial_io den_spatial;  // disocclusion fix and spatial fallback
         vgi_denoise_fast_clamp_io den_fast_clamp;
         vgi_denoise_atrous_io den_atrous;
+        vgi_raytraced_shadow_io shadow;
+        vgi_final_lighting_io final_lighting;
     } io;
     union {
         vgi_pc_frame_hash frame_hash;
@@ -150,10 +156,15 @@ struct PassSink {
 // (lane 0) beside the hash-grid cache update (lane 1: it reads the previous
 // frame's depth and queries only); phase 1: diffuse (lane 0) beside specular
 // sampling and resampling (lane 1; both read the G-buffer and the updated
-// cache, and only append to the shared query buffer); phase 2: denoiser.
-inline int slot_phase(int slot) { return slot < kSlotDiffGen ? 0 : slot < kSlotDenClearFastDiff ? 1 : 2; }
+// cache, and only append to the shared query buffer); phase 2: denoiser
+// (lane 0) beside the sun shadow mask (lane 1: it reads the depth only);
+// phase 3: final lighting.
+inline int slot_phase(int slot) {
+    return slot < kSlotDiffGen ? 0 : slot < kSlotDenClearFastDiff ? 1 : slot < kSlotFinalLighting ? 2 : 3;
+}
 inline int slot_lane(int slot) {
-    return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygen) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial);
+    return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygen) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial) ||
+           slot == kSlotShadow;
 }
 
 // The streams of a frame's slots, issued in slot order.  Without a side

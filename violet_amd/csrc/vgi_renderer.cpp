@@ -39,6 +39,7 @@ struct vgi_renderer {
     // indirect specular: new sample / temporal result (in place) = next frame's prev, 2 sets
     DevBuf spec_rsv[2], spec_hitpos[2], spec_rad[2];
     DevBuf spec_light, spec_raylen;
+    DevBuf shadow_mask, color;  // compose: sun shadow mask (u8) and final lighting
     // hash grid
     DevBuf hg_storage, hg_decay, hg_query[2], hg_counter[2], hg_selection, hg_cidx, hg_caddr, hg_ccount;
     int q = 0;  // query buffer holding pending queries
@@ -181,6 +182,10 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     A(r->hg_cidx, 4 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
     A(r->hg_caddr, 4 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
     A(r->hg_ccount, 4);
+    if (c.compose) {
+        A(r->shadow_mask, n);
+        A(r->color, R * n);
+    }
     if (st == VGI_OK) st = vgi_denoiser_create(width, height, c.storage, &c.denoiser, &r->den);
     if (st == VGI_OK && c.concurrent_lanes) {
         // non-blocking: no implicit ordering against the legacy null stream,
@@ -392,6 +397,20 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         CALL(vgi_denoiser_reset(r->den));
     }
 
+    // Direct light + composition (:831-911)
+    if (r->cfg.compose) {
+        PassRec sh = make_rec(kPassShadow, fp);
+        sh.scene = scene;
+        sh.io.shadow = vgi_raytraced_shadow_io{w, h, r->depth[c].as<float>(), r->shadow_mask.as<uint8_t>()};
+        CALL(sink.submit(kSlotShadow, sh));
+        PassRec fl = make_rec(kPassFinalLighting, fp);
+        fl.io.final_lighting = vgi_final_lighting_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->shadow_mask.as<uint8_t>(),
+                                                     den_diff, den_spec, r->color.p,
+                                                     {r->cfg.sky_radiance[0], r->cfg.sky_radiance[1],
+                                                      r->cfg.sky_radiance[2]}};
+        CALL(sink.submit(kSlotFinalLighting, fl));
+    }
+
     // Feedback (:819-829, :1012-1025)
     r->prev_indirect_diffuse = den_diff;
     r->prev_ind_diff_slot = slot;
@@ -410,6 +429,7 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
     out->denoised = denoised ? 1u : 0u;
     out->validation_frame = validation ? 1u : 0u;
     out->hash_grid_updated = hg_updated ? 1u : 0u;
+    out->color = r->cfg.compose ? r->color.p : nullptr;
     return VGI_OK;
 }
 
@@ -456,6 +476,8 @@ extern "C" vgi_status vgi_renderer_get_image(const vgi_renderer* rc, const char*
         {"hash_grid_decay", &r->hg_decay},
         {"hash_grid_queries", &r->hg_query[r->q]},
         {"hash_grid_query_counter", &r->hg_counter[r->q]},
+        {"shadow_mask", &r->shadow_mask},
+        {"color", &r->color},
     };
     for (auto& e : table)
         if (strcmp(e.n, name) == 0) {
@@ -498,7 +520,8 @@ extern "C" size_t vgi_renderer_device_bytes(const vgi_renderer* rc) {
     std::vector<DevBuf*> v;
     all_buffers(r, v);
     for (DevBuf* b : {&r->dtm_rsv, &r->dtm_hit, &r->dtm_rad, &r->dnew_hit, &r->dnew_rad, &r->ao_filtered, &r->spec_light,
-                      &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount})
+                      &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount, &r->shadow_mask,
+                      &r->color})
         if (std::find(v.begin(), v.end(), b) == v.end()) v.push_back(b);
     size_t n = 0;
     for (auto* b : v) n += b->bytes;

@@ -37,6 +37,8 @@ class RestirConfig:
     # independent pass chains on a second stream (hash-grid update beside the
     # G-buffer, specular beside diffuse), joined into the caller's stream
     concurrent_lanes: bool = True
+    # also run the sun shadow mask and the final lighting (the lit image)
+    compose: bool = False
     disocclusion_fix: bool = True
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
@@ -55,6 +57,7 @@ class RestirConfig:
         c.diff_spatial_taps = self.diff_spatial_taps
         c.spatial_radius_px = self.spatial_radius_px
         c.concurrent_lanes = int(self.concurrent_lanes)
+        c.compose = int(self.compose)
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
                                          int(self.atrous_iterations), int(self.compensated_moments))
         return c
@@ -69,6 +72,7 @@ class FrameOutput:
     denoised: bool
     validation_frame: bool
     hash_grid_updated: bool
+    color: int | None = None  # final lighting when the config composes
 
 
 class RestirRenderer:
@@ -89,7 +93,7 @@ class RestirRenderer:
         _lib.call("vgi_renderer_add_passes", self._h, scene.handle, C.byref(fp), frame_index & 0xFFFFFFFF, C.byref(o),
                   gpu.stream() if stream is None else stream)
         return FrameOutput(o.indirect_diffuse, o.indirect_specular, o.depth, o.gbuffer, bool(o.denoised),
-                           bool(o.validation_frame), bool(o.hash_grid_updated))
+                           bool(o.validation_frame), bool(o.hash_grid_updated), o.color)
 
     def image(self, name: str) -> tuple[int, int]:
         p = C.c_void_p()
