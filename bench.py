@@ -174,8 +174,9 @@ def cpu_baseline_denoise(args, budget_s: float = 20.0):
                        f"FP32 twin of the oracle, {el:.1f} s")
 
 
-def orbit_view(frame: int, w: int, h: int, radius=30.0, height=9.0, deg_per_frame=6.0):
-    """Camera orbit around the scene centre (60 frames = one turn)."""
+def orbit_view(frame: int, w: int, h: int, radius=30.0, height=9.0, deg_per_frame=6.0, jitter_cycle=0):
+    """Camera orbit around the scene centre (60 frames = one turn); with a
+    jitter cycle, Halton(2,3) sub-pixel jitter of frame mod cycle."""
     import math
 
     from violet_amd import synth
@@ -185,13 +186,14 @@ def orbit_view(frame: int, w: int, h: int, radius=30.0, height=9.0, deg_per_fram
     d = -pos / np.linalg.norm(pos)
     yaw = math.degrees(math.atan2(d[0], d[1]))
     pitch = math.degrees(math.asin(d[2])) + 8.0
-    return synth.View(synth.Camera(pos, yaw, pitch), w, h)
+    j = synth.jitter_ndc(frame % jitter_cycle, w, h) if jitter_cycle else (0.0, 0.0)
+    return synth.View(synth.Camera(pos, yaw, pitch), w, h, jitter=j)
 
 
-def frame_params_seq(w, h, n):
+def frame_params_seq(w, h, n, jitter_cycle=0):
     from violet_amd import synth
 
-    views = [orbit_view(f, w, h) for f in range(n + 1)]
+    views = [orbit_view(f, w, h, jitter_cycle=jitter_cycle) for f in range(n + 1)]
     return [synth.make_frame_params(views[f], views[f - 1] if f else views[-1]) for f in range(n)]
 
 
@@ -208,9 +210,10 @@ def run_frame(args):
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16 if args.storage == "f16" else _lib.STORAGE_F32,
                                 atrous_iterations=args.iterations, concurrent_lanes=not args.no_lanes,
-                                compose=args.compose, **RESAMPLE[args.resample])
+                                compose=args.compose or args.taa, taa=args.taa, **RESAMPLE[args.resample])
     r = renderer.RestirRenderer(w, h, cfg)
-    fps = frame_params_seq(w, h, args.frames)
+    # TAA: Halton(2,3) jitter with violet's cycle of 8 (restir_renderer.rs:65)
+    fps = frame_params_seq(w, h, args.frames, jitter_cycle=8 if args.taa else 0)
 
     def step(k: int):
         r.add_passes(sc, fps[k % len(fps)], k)
@@ -468,6 +471,8 @@ def main():
                     help="comma list of forms reported beside the plain frame path ('' for none)")
     ap.add_argument("--compose", action="store_true",
                     help="also run the sun shadow mask and the final lighting in every frame")
+    ap.add_argument("--taa", action="store_true",
+                    help="composition plus temporal AA over a Halton-jittered orbit")
     ap.add_argument("--big-triangles", type=int, default=1_000_000)
     ap.add_argument("--batch-views", type=int, default=16)
     ap.add_argument("--batch-frames", type=int, default=30)
@@ -552,7 +557,8 @@ def main():
                     "gi_rays_per_px": "1 diffuse + 1 specular (+1 shadow each)", "atrous_iterations": args.iterations,
                     "restir_params": RESAMPLE_DESC[args.resample],
                     "storage": args.storage, "parallelism": "single-gpu",
-                    "concurrent_lanes": not args.no_lanes, "compose": args.compose} if frame else
+                    "concurrent_lanes": not args.no_lanes, "compose": args.compose or args.taa,
+                    "taa": args.taa} if frame else
                    {"workload": f"denoise_only_{args.width}x{args.height}", "width": args.width,
                     "height": args.height, "frames_cycled": args.frames, "atrous_iterations": args.iterations,
                     "storage": args.storage, "parallelism": "single-gpu"}),

@@ -604,6 +604,27 @@ typedef struct vgi_final_lighting_io {
 } vgi_final_lighting_io;
 vgi_status vgi_pass_final_lighting(const vgi_frame_params* fp, const vgi_final_lighting_io* io, void* stream);
 
+/* temporal_aa.hlsl:112-264 (restir_renderer.rs:914-948, "Temporal AA"):
+ * Mitchell-filtered 3x3 source around the un-jittered pixel centre,
+ * reprojection of the pixel's own depth with FrameParams' current and
+ * previous jitter, bilinear clamp-to-edge history fetch, neighbourhood
+ * min/max clamp, variance clip (gamma 1) and the anti-flicker weighted blend
+ * (1/32).  Without history, or where the history position leaves the
+ * previous view, the output is the filtered source.  Images in the storage
+ * mode's radiance layout. */
+typedef struct vgi_temporal_aa_io {
+    uint32_t width, height, storage;
+    const float* gbuffer_depth;
+    const void* source_texture;   /* radiance */
+    const void* history_texture;  /* radiance; may be NULL when has_history = 0 */
+    void* rw_target;              /* radiance */
+} vgi_temporal_aa_io;
+typedef struct vgi_pc_temporal_aa {
+    uint32_t has_history;
+} vgi_pc_temporal_aa;
+vgi_status vgi_pass_temporal_aa(const vgi_frame_params* fp, const vgi_temporal_aa_io* io,
+                                const vgi_pc_temporal_aa* pc, void* stream);
+
 /* ------------------------------------------------------------------------
  * Renderer sequencer: crates/violet-restir/src/restir_renderer.rs
  * `RestirRenderer::add_passes` (:258-1028).  Records the whole GI pass chain
@@ -633,6 +654,11 @@ typedef struct vgi_renderer_config {
     /* default 0: 1 = after the denoiser also run the sun shadow mask and the
      * final lighting (restir_renderer.rs:831-911), the lit image in `color` */
     uint32_t compose;
+    /* default 0: 1 = temporal AA of the lit image (needs compose = 1;
+     * violet's RestirConfig::taa, restir_renderer.rs:51-74).  The caller
+     * jitters FrameParams (jitter.xy this frame, .zw the previous one) by
+     * Halton(2,3) of (frame_index mod jitter_cycle) + 1, render_loop.rs:89-106. */
+    uint32_t taa;
     vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
 } vgi_renderer_config;
 
@@ -646,7 +672,8 @@ typedef struct vgi_renderer_output {
     uint32_t denoised;              /* 1 if the denoiser ran this frame */
     uint32_t validation_frame;      /* 1 if this was a diffuse validation frame */
     uint32_t hash_grid_updated;     /* 1 if the cache update passes ran */
-    const void* color;              /* final lighting (radiance layout) when compose = 1, else NULL */
+    const void* color;              /* the lit image (radiance layout) when compose = 1 (after temporal
+                                     * AA when taa = 1), else NULL */
 } vgi_renderer_output;
 
 vgi_status vgi_renderer_default_config(vgi_renderer_config* cfg);

@@ -249,6 +249,7 @@ def _restir_sigs(lib):
     lib.oracle_hg_raygen.argtypes = [P, TR, P, P, U32, P, P, P, U32]
     lib.oracle_raytraced_shadow.argtypes = [P, P, I, I, P, P]
     lib.oracle_final_lighting.argtypes = [P, I, I, P, P, P, P, P, P]
+    lib.oracle_temporal_aa.argtypes = [P, I, I, P, P, P, I, P]
 
 
 def _fp_arr(fp):
@@ -337,6 +338,14 @@ class OracleRestir:
                                        _np_ptr(np.ascontiguousarray(mask, np.uint8)), _np_ptr(self.r(ind_diff)),
                                        _np_ptr(self.r(ind_spec)), _np_ptr(sky), _np_ptr(color))
         return color
+
+    def temporal_aa(self, fp, depth, source, history=None):
+        h, w = depth.shape
+        out = np.zeros((h, w, 3), self.rt_dtype)
+        hist = self.r(history)
+        self.lib.oracle_temporal_aa(_np_ptr(_fp_arr(fp)), w, h, _np_ptr(self.r(depth)), _np_ptr(self.r(source)),
+                                    _np_ptr(hist), int(history is not None), _np_ptr(out))
+        return out
 
     def diff_sample_gen(self, fp, tr, gb, depth, frame_rand, has_prev):
         h, w = depth.shape

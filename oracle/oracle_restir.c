@@ -17,6 +17,7 @@
  *   hash-grid passes            restir/hash_grid_{decay,query_select,query_compact,raygen}.hlsl
  *   oracle_raytraced_shadow     raytraced_shadow.hlsl:9-53
  *   oracle_final_lighting       restir/final_lighting.hlsl:16-62
+ *   oracle_temporal_aa          temporal_aa.hlsl:112-264
  *
  * Scene: world-space f32 positions/normals, u32 indices, per-triangle
  * material {base_color rgba, metallic, roughness, pad, pad}; queried through
@@ -1138,5 +1139,92 @@ void oracle_final_lighting(const float* fpf, int w, int h, const uint32_t* gb, c
             v3 sr = V3(R_FMAX(ri.x, rmin), R_FMAX(ri.y, rmin), R_FMAX(ri.z, rmin));
             v3 ind = add3(mul3(ld3r(ind_diff, i), rho), mul3(ld3r(ind_spec, i), sr));
             st3r(o_color, i, add3(direct, ind));
+        }
+}
+
+/* temporal_aa.hlsl:17-29 */
+static real o_mitchell(real x) {
+    const real B = RC(1.0) / RC(3.0), C = RC(1.0) / RC(3.0);
+    real y = RC(0.0), x2 = x * x, x3 = x * x * x;
+    if (x < RC(1.0))
+        y = (RC(12.0) - RC(9.0) * B - RC(6.0) * C) * x3 + (RC(-18.0) + RC(12.0) * B + RC(6.0) * C) * x2 +
+            (RC(6.0) - RC(2.0) * B);
+    else if (x <= RC(2.0))
+        y = (-B - RC(6.0) * C) * x3 + (RC(6.0) * B + RC(30.0) * C) * x2 + (RC(-12.0) * B - RC(48.0) * C) * x +
+            (RC(8.0) * B + RC(24.0) * C);
+    return y / RC(6.0);
+}
+
+/* sampler_linear_clamp: texel = uv*size - 0.5, clamp-to-edge */
+static v3 o_bilinear3r(const real* tex, int w, int h, real u, real v) {
+    real tx = u * (real)w - RC(0.5), ty = v * (real)h - RC(0.5);
+    real x0f = R_FLOOR(tx), y0f = R_FLOOR(ty);
+    real fx = tx - x0f, fy = ty - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
+    int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
+    v3 top = add3(scale3(ld3r(tex, (size_t)ya * w + xa), RC(1.0) - fx), scale3(ld3r(tex, (size_t)ya * w + xb), fx));
+    v3 bot = add3(scale3(ld3r(tex, (size_t)yb * w + xa), RC(1.0) - fx), scale3(ld3r(tex, (size_t)yb * w + xb), fx));
+    return add3(scale3(top, RC(1.0) - fy), scale3(bot, fy));
+}
+
+/* NaN-safe saturate: fmin/fmax return the other operand for a NaN */
+static inline real o_sat_nan0(real x) { return R_FMIN(R_FMAX(x, RC(0.0)), RC(1.0)); }
+
+/* temporal_aa.hlsl:112-264.  The nearest-depth neighbour the reference
+ * tracks is overwritten before use (:186-187), so it is not computed. */
+void oracle_temporal_aa(const float* fpf, int w, int h, const real* depth, const real* source, const real* history,
+                        int has_history, real* o_target) {
+    ofp fp;
+    o_load_fp(fpf, &fp);
+    real sox = -fp.jitter[0] * RC(0.5) * (real)w, soy = -fp.jitter[1] * RC(0.5) * (real)h;
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) {
+            size_t i = (size_t)py * w + px;
+            v3 accu = V3s(0), nmin = V3s(RC(1e5)), nmax = V3s(RC(-1e5)), m1 = V3s(0), m2 = V3s(0);
+            real wsum = RC(0.0);
+            for (int x = -1; x <= 1; ++x)
+                for (int y = -1; y <= 1; ++y) {
+                    int cx = clampi(px + x, 0, w - 1), cy = clampi(py + y, 0, h - 1);
+                    real ox = (real)(cx - px) + sox, oy = (real)(cy - py) + soy;
+                    v3 nb = ld3r(source, (size_t)cy * w + cx);
+                    real wt = o_mitchell(R_SQRT(ox * ox + oy * oy));
+                    accu = add3(accu, scale3(nb, wt));
+                    wsum += wt;
+                    nmin = V3(R_FMIN(nmin.x, nb.x), R_FMIN(nmin.y, nb.y), R_FMIN(nmin.z, nb.z));
+                    nmax = V3(R_FMAX(nmax.x, nb.x), R_FMAX(nmax.y, nb.y), R_FMAX(nmax.z, nb.z));
+                    m1 = add3(m1, nb);
+                    m2 = add3(m2, mul3(nb, nb));
+                }
+            v3 src = V3(accu.x / wsum, accu.y / wsum, accu.z / wsum);
+            v3 pos = o_depth_to_pos(fp.inv_view_proj, px, py, w, h, depth[i]);
+            v4 hc = o_mul44(fp.view_proj, V4(pos.x, pos.y, pos.z, RC(1.0)));
+            v4 hr = o_mul44(fp.prev_view_proj, V4(pos.x, pos.y, pos.z, RC(1.0)));
+            real mvx = ((hc.x / hc.w - fp.jitter[0]) - (hr.x / hr.w - fp.jitter[2])) * RC(0.5);
+            real mvy = ((hc.y / hc.w - fp.jitter[1]) - (hr.y / hr.w - fp.jitter[3])) * RC(0.5);
+            real hu = ((real)px + RC(0.5)) / (real)w - mvx, hv = ((real)py + RC(0.5)) / (real)h - mvy;
+            int out_of_view = hu != o_sat_nan0(hu) || hv != o_sat_nan0(hv);
+            if (out_of_view || !has_history) {
+                st3r(o_target, i, src);
+                continue;
+            }
+            v3 hs = o_bilinear3r(history, w, h, hu, hv);
+            hs = V3(R_FMIN(R_FMAX(hs.x, nmin.x), nmax.x), R_FMIN(R_FMAX(hs.y, nmin.y), nmax.y),
+                    R_FMIN(R_FMAX(hs.z, nmin.z), nmax.z));
+            real inv_n = RC(1.0) / RC(9.0);
+            v3 m1u = scale3(m1, inv_n), m2u = scale3(m2, inv_n);
+            v3 d2 = sub3(m2u, mul3(m1u, m1u));
+            v3 e = V3(R_SQRT(R_FABS(d2.x)) + RC(1e-8), R_SQRT(R_FABS(d2.y)) + RC(1e-8), R_SQRT(R_FABS(d2.z)) + RC(1e-8));
+            v3 v = sub3(hs, m1u);
+            real ma = R_FMAX(R_FABS(v.x / e.x), R_FMAX(R_FABS(v.y / e.y), R_FABS(v.z / e.z)));
+            if (ma > RC(1.0)) hs = add3(m1u, V3(v.x / ma, v.y / ma, v.z / ma));
+            real sw = RC(1.0) / RC(32.0), hw = RC(1.0) - RC(1.0) / RC(32.0);
+            real rs = RC(1.0) / (R_FMAX(R_FMAX(src.x, src.y), src.z) + RC(1.0));
+            real rh = RC(1.0) / (R_FMAX(R_FMAX(hs.x, hs.y), hs.z) + RC(1.0));
+            sw *= RC(1.0) / (RC(1.0) + lum(scale3(src, rs)));
+            hw *= RC(1.0) / (RC(1.0) + lum(scale3(hs, rh)));
+            real den = R_FMAX(sw + hw, RC(1e-5));
+            v3 num = add3(scale3(src, sw), scale3(hs, hw));
+            st3r(o_target, i, V3(num.x / den, num.y / den, num.z / den));
         }
 }

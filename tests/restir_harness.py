@@ -61,7 +61,7 @@ STORAGE_IMAGES = {"diff_hit_radiance", "diff_temporal_hit_radiance", "diff_new_h
                   "indirect_diffuse_raw", "spec_hit_radiance", "indirect_specular_raw", "spec_ray_len"}
 
 
-def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
+def run_gpu_frames(mesh, w, h, frames, config=None, snap=True, jitter=False):
     """Run the GPU renderer; returns (frame params list, outputs list, snapshots list)."""
     import torch
 
@@ -74,7 +74,7 @@ def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
     fps, outs, snaps = [], [], []
     prev_view = None
     for f in range(frames):
-        view = orbit_view(f, w, h)
+        view = orbit_view(f, w, h, jitter=jitter)
         fp = synth.make_frame_params(view, prev_view)
         o = r.add_passes(sc, fp, f)
         torch.cuda.synchronize()
@@ -91,6 +91,7 @@ def run_gpu_frames(mesh, w, h, frames, config=None, snap=True):
             if o.color:
                 s["shadow_mask"] = r.read("shadow_mask", np.uint8, (h, w)).copy()
                 s["out_color"] = r.read_ptr(o.color, rb, dt, (h, w, 3)).copy()
+                s["scene_color"] = r.read("color", dt, (h, w, 3)).copy()
         fps.append(fp)
         outs.append(o)
         snaps.append(s)

@@ -38,6 +38,7 @@ def _mirrors():
         "vgi_hash_grid_query_compact_io": _abi.HGCompactIO, "vgi_hash_grid_raygen_io": _abi.HGRaygenIO,
         "vgi_gbuffer_io": _abi.GBufferIO, "vgi_renderer_config": _abi.RendererConfig,
         "vgi_raytraced_shadow_io": _abi.RaytracedShadowIO, "vgi_final_lighting_io": _abi.FinalLightingIO,
+        "vgi_temporal_aa_io": _abi.TemporalAAIO, "vgi_pc_temporal_aa": _abi.PCTemporalAA,
         "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
     }
     return m

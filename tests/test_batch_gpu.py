@@ -6,7 +6,7 @@ same view rendered alone.
 * Every frame: depth, G-buffer, the sun shadow mask and the new diffuse hit
   positions (functions of the camera, the scene and the per-pixel random
   streams only) are bit-identical.  The views also run the composition
-  passes (shadow mask and final lighting).
+  passes (shadow mask, final lighting and temporal AA).
 * Later frames: the hash-grid cache update is scheduling dependent (query
   slots come from an atomic counter, reference restir/raytrace.inc.hlsl:
   190-197), alone as well as in the mix, so radiance that went through the
@@ -63,7 +63,8 @@ def test_views_same_alone_and_in_the_mix(vgi, graph):
 
     mesh = make_mesh()
     sc = scene.Scene(mesh)
-    cfg = renderer.RestirConfig(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0, compose=True)
+    cfg = renderer.RestirConfig(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0, compose=True,
+                                taa=True)
     views = _views()
     fps = [[synth.make_frame_params(vs[f], vs[f - 1] if f else None) for f in range(FRAMES)] for _, _, vs in views]
     alone = [renderer.RestirRenderer(w, h, cfg) for w, h, _ in views]

@@ -77,12 +77,15 @@ RaytracedShadowIO = _struct("RaytracedShadowIO", [("width", u32), ("height", u32
 FinalLightingIO = _struct("FinalLightingIO", WH + [
     ("gbuffer_color", vp), ("shadow_mask_buffer", vp), ("indirect_diffuse_texture", vp),
     ("indirect_specular_texture", vp), ("rw_color_buffer", vp), ("sky_radiance", f32 * 3)])
+TemporalAAIO = _struct("TemporalAAIO", WH + [
+    ("gbuffer_depth", vp), ("source_texture", vp), ("history_texture", vp), ("rw_target", vp)])
+PCTemporalAA = _struct("PCTemporalAA", [("has_history", u32)])
 
 RendererConfig = _struct("RendererConfig", [
     ("storage", u32), ("denoise", u32), ("ind_diff_validate", u32), ("ind_spec_validate", u32),
     ("hash_grid_cache_decay", u32), ("ao_radius", f32), ("sky_radiance", f32 * 3),
     ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32), ("concurrent_lanes", u32),
-    ("compose", u32), ("denoiser", _lib.DenoiserConfig)])
+    ("compose", u32), ("taa", u32), ("denoiser", _lib.DenoiserConfig)])
 RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
     ("validation_frame", u32), ("hash_grid_updated", u32), ("color", vp)])
@@ -111,6 +114,7 @@ _SIGS = {
     "vgi_pass_gbuffer_primary": [FPp, vp, P(GBufferIO), vp],
     "vgi_pass_raytraced_shadow": [FPp, vp, P(RaytracedShadowIO), vp],
     "vgi_pass_final_lighting": [FPp, P(FinalLightingIO), vp],
+    "vgi_pass_temporal_aa": [FPp, P(TemporalAAIO), P(PCTemporalAA), vp],
     "vgi_renderer_default_config": [P(RendererConfig)],
     "vgi_renderer_create": [u32, u32, P(RendererConfig), P(vp)],
     "vgi_renderer_destroy": [vp],

@@ -55,26 +55,6 @@ __device__ __forceinline__ f3 decode_normal_fast(uint32_t enc) {
 // dist + 1e-8) - |l - l_c| / (sigma_l std + 1e-8)), with exp(a) as
 // exp2(a log2 e) and the divisions as reciprocal multiplies.
 
-// Bilinear fetch with clamp-to-edge addressing (sampler_linear_clamp), in
-// exact f32: texel t = uv*size - 0.5, weights from t - floor(t).
-template <class S>
-__device__ __forceinline__ f3 sample_bilinear3(const void* tex, uint32_t w, uint32_t h, float u, float v) {
-    float tx = u * (float)w - 0.5f;
-    float ty = v * (float)h - 0.5f;
-    float x0f = floorf(tx), y0f = floorf(ty);
-    float fx = tx - x0f, fy = ty - y0f;
-    int x0 = (int)x0f, y0 = (int)y0f;
-    int xa = clampi(x0, 0, (int)w - 1), xb = clampi(x0 + 1, 0, (int)w - 1);
-    int ya = clampi(y0, 0, (int)h - 1), yb = clampi(y0 + 1, 0, (int)h - 1);
-    f3 c00 = S::ld3(tex, (size_t)ya * w + xa);
-    f3 c10 = S::ld3(tex, (size_t)ya * w + xb);
-    f3 c01 = S::ld3(tex, (size_t)yb * w + xa);
-    f3 c11 = S::ld3(tex, (size_t)yb * w + xb);
-    f3 top = c00 * (1.0f - fx) + c10 * fx;
-    f3 bot = c01 * (1.0f - fx) + c11 * fx;
-    return top * (1.0f - fy) + bot * fy;
-}
-
 __device__ __forceinline__ f3 load_normal(const uint32_t* gb, size_t plane, size_t i) {
     return decode_gbuffer_normal(G(gb)[plane + i]).normal;
 }

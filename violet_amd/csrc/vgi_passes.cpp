@@ -64,6 +64,7 @@ vgi_status DirectSink::submit(int slot, const PassRec& r) {
         case kPassDenAtrous: return vgi_pass_denoise_atrous(&r.io.den_atrous, &r.pc.den_atrous, s);
         case kPassShadow: return vgi_pass_raytraced_shadow(&r.fp, r.scene, &r.io.shadow, s);
         case kPassFinalLighting: return vgi_pass_final_lighting(&r.fp, &r.io.final_lighting, s);
+        case kPassTemporalAA: return vgi_pass_temporal_aa(&r.fp, &r.io.taa, &r.pc.taa, s);
         default: return set_error(VGI_E_INVALID, "internal: unknown pass kind %d", r.kind);
     }
 }

@@ -39,6 +39,7 @@ enum PassKind : int {
     kPassDenAtrous,
     kPassShadow,          // raytraced_shadow.hlsl
     kPassFinalLighting,   // restir/final_lighting.hlsl
+    kPassTemporalAA,      // temporal_aa.hlsl
     kPassKindCount
 };
 
@@ -73,6 +74,7 @@ enum Slot : int {
     kSlotDenAtrous0,  // .. kSlotDenAtrous0 + 4
     kSlotShadow = kSlotDenAtrous0 + 5,
     kSlotFinalLighting,
+    kSlotTemporalAA,
     kSlotCount
 };
 
@@ -85,7 +87,7 @@ inline int slot_kind(int slot) {
         kPassDiffTemporal, kPassDiffSpatial,  kPassSpecGen,       kPassSpecValidate, kPassSpecTemporal,
         kPassSpecSpatial,  kPassClear,        kPassClear,         kPassDenTemporal, kPassDenDisocclusion,
         kPassDenFastClamp, kPassDenFallback,  kPassDenAtrous,     kPassDenAtrous,   kPassDenAtrous,
-        kPassDenAtrous,    kPassDenAtrous,    kPassShadow,        kPassFinalLighting};
+        kPassDenAtrous,    kPassDenAtrous,    kPassShadow,        kPassFinalLighting, kPassTemporalAA};
     return k[slot];
 }
 
@@ -116,6 +118,7 @@ struct PassRec {
         vgi_denoise_atrous_io den_atrous;
         vgi_raytraced_shadow_io shadow;
         vgi_final_lighting_io final_lighting;
+        vgi_temporal_aa_io taa;
     } io;
     union {
         vgi_pc_frame_hash frame_hash;
@@ -127,6 +130,7 @@ struct PassRec {
         vgi_pc_ind_spec_spatial spec_spatial;
         vgi_pc_denoise_temporal den_temporal;
         vgi_pc_denoise_atrous den_atrous;
+        vgi_pc_temporal_aa taa;
     } pc;
     void* clear_ptr;     // kPassClear
     size_t clear_bytes;  // kPassClear
@@ -158,7 +162,7 @@ struct PassSink {
 // sampling and resampling (lane 1; both read the G-buffer and the updated
 // cache, and only append to the shared query buffer); phase 2: denoiser
 // (lane 0) beside the sun shadow mask (lane 1: it reads the depth only);
-// phase 3: final lighting.
+// phase 3: final lighting, then temporal AA.
 inline int slot_phase(int slot) {
     return slot < kSlotDiffGen ? 0 : slot < kSlotDenClearFastDiff ? 1 : slot < kSlotFinalLighting ? 2 : 3;
 }

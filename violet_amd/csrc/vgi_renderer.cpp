@@ -40,6 +40,7 @@ struct vgi_renderer {
     DevBuf spec_rsv[2], spec_hitpos[2], spec_rad[2];
     DevBuf spec_light, spec_raylen;
     DevBuf shadow_mask, color;  // compose: sun shadow mask (u8) and final lighting
+    DevBuf taa[2];              // taa: resolved colour, double-buffered (= next frame's history)
     // hash grid
     DevBuf hg_storage, hg_decay, hg_query[2], hg_counter[2], hg_selection, hg_cidx, hg_caddr, hg_ccount;
     int q = 0;  // query buffer holding pending queries
@@ -49,6 +50,7 @@ struct vgi_renderer {
     bool has_prev_depth = false, has_prev_gbuffer = false, has_prev_indirect_diffuse = false;
     bool has_prev_diffuse_reservoir = false, has_prev_ind_spec = false, has_prev_ao = false;
     bool hash_grid_ready = false;
+    bool has_prev_color = false;  // TAA history (restir_renderer.rs:914-918)
     const void* prev_indirect_diffuse = nullptr;  // last frame's denoised (or raw) diffuse
     int prev_ind_diff_slot = -1;                  // 0/1 = diff_light[k]; 2/3 = denoiser pp[k-2]
 
@@ -74,7 +76,8 @@ struct vgi_renderer {
                 {"spec_rad0", &spec_rad[0]},   {"spec_rad1", &spec_rad[1]},
                 {"hg_storage", &hg_storage},   {"hg_decay", &hg_decay},
                 {"hg_query0", &hg_query[0]},   {"hg_query1", &hg_query[1]},
-                {"hg_counter0", &hg_counter[0]}, {"hg_counter1", &hg_counter[1]}};
+                {"hg_counter0", &hg_counter[0]}, {"hg_counter1", &hg_counter[1]},
+                {"taa0", &taa[0]},             {"taa1", &taa[1]}};
     }
 };
 
@@ -143,6 +146,7 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     VGI_REQUIRE(c.storage == VGI_STORAGE_F32 || c.storage == VGI_STORAGE_F16, "vgi_renderer_create: bad storage");
     VGI_REQUIRE(c.diff_spatial_taps <= 16u, "vgi_renderer_create: diff_spatial_taps %u > 16", c.diff_spatial_taps);
     VGI_REQUIRE(c.spatial_radius_px >= 0.0f && c.spatial_radius_px <= 16384.0f, "vgi_renderer_create: bad spatial_radius_px");
+    VGI_REQUIRE(!c.taa || c.compose, "vgi_renderer_create: taa needs compose");
     vgi_renderer* r = new (std::nothrow) vgi_renderer();
     if (!r) return set_error(VGI_E_NOMEM, "vgi_renderer_create: out of host memory");
     r->w = width;
@@ -186,6 +190,10 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
         A(r->shadow_mask, n);
         A(r->color, R * n);
     }
+    if (c.taa) {
+        A(r->taa[0], R * n);
+        A(r->taa[1], R * n);
+    }
     if (st == VGI_OK) st = vgi_denoiser_create(width, height, c.storage, &c.denoiser, &r->den);
     if (st == VGI_OK && c.concurrent_lanes) {
         // non-blocking: no implicit ordering against the legacy null stream,
@@ -213,6 +221,7 @@ extern "C" vgi_status vgi_renderer_reset(vgi_renderer* r) {
     r->has_prev_depth = r->has_prev_gbuffer = r->has_prev_indirect_diffuse = false;
     r->has_prev_diffuse_reservoir = r->has_prev_ind_spec = r->has_prev_ao = false;
     r->hash_grid_ready = false;
+    r->has_prev_color = false;
     r->prev_indirect_diffuse = nullptr;
     r->prev_ind_diff_slot = -1;
     return vgi_denoiser_reset(r->den);
@@ -409,6 +418,13 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
                                                      {r->cfg.sky_radiance[0], r->cfg.sky_radiance[1],
                                                       r->cfg.sky_radiance[2]}};
         CALL(sink.submit(kSlotFinalLighting, fl));
+        if (r->cfg.taa) {  // :914-944
+            PassRec t = make_rec(kPassTemporalAA, fp);
+            t.io.taa = vgi_temporal_aa_io{w, h, S, r->depth[c].as<float>(), r->color.p,
+                                          r->has_prev_color ? r->taa[p].p : nullptr, r->taa[c].p};
+            t.pc.taa = vgi_pc_temporal_aa{r->has_prev_color ? 1u : 0u};
+            CALL(sink.submit(kSlotTemporalAA, t));
+        }
     }
 
     // Feedback (:819-829, :1012-1025)
@@ -420,6 +436,7 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
     r->has_prev_gbuffer = true;
     r->has_prev_depth = true;
     r->has_prev_ao = true;
+    r->has_prev_color = r->cfg.taa != 0;
     r->cur = p;
 
     out->indirect_diffuse = den_diff;
@@ -429,7 +446,7 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
     out->denoised = denoised ? 1u : 0u;
     out->validation_frame = validation ? 1u : 0u;
     out->hash_grid_updated = hg_updated ? 1u : 0u;
-    out->color = r->cfg.compose ? r->color.p : nullptr;
+    out->color = r->cfg.taa ? r->taa[c].p : r->cfg.compose ? r->color.p : nullptr;
     return VGI_OK;
 }
 
@@ -478,6 +495,7 @@ extern "C" vgi_status vgi_renderer_get_image(const vgi_renderer* rc, const char*
         {"hash_grid_query_counter", &r->hg_counter[r->q]},
         {"shadow_mask", &r->shadow_mask},
         {"color", &r->color},
+        {"taa_color", &r->taa[last]},
     };
     for (auto& e : table)
         if (strcmp(e.n, name) == 0) {
@@ -542,7 +560,7 @@ extern "C" vgi_status vgi_renderer_export_state(const vgi_renderer* rc, void* bl
     hd.slot = r->prev_ind_diff_slot;
     hd.den_cur = r->den->cur;
     bool f[8] = {r->has_prev_depth, r->has_prev_gbuffer, r->has_prev_indirect_diffuse, r->has_prev_diffuse_reservoir,
-                 r->has_prev_ind_spec, r->has_prev_ao, r->hash_grid_ready, false};
+                 r->has_prev_ind_spec, r->has_prev_ao, r->hash_grid_ready, r->has_prev_color};
     for (int i = 0; i < 8; ++i) hd.flags[i] = f[i];
     hd.den_flags[0] = r->den->has_history;
     hd.den_flags[1] = r->den->has_fast_history;
@@ -588,6 +606,7 @@ extern "C" vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* blo
     r->has_prev_ind_spec = hd.flags[4];
     r->has_prev_ao = hd.flags[5];
     r->hash_grid_ready = hd.flags[6];
+    r->has_prev_color = hd.flags[7] && r->cfg.taa;
     r->prev_indirect_diffuse = slot_ptr(r, hd.slot);
     vgi_denoiser* d = r->den;
     d->cur = hd.den_cur;

@@ -75,6 +75,26 @@ struct StorageF16 {
     VGI_HD static void st1(void* p, size_t i, float v) { gst((uint16_t*)p + i, (uint16_t)f32tof16(v)); }
 };
 
+// Bilinear fetch with clamp-to-edge addressing (sampler_linear_clamp), in
+// exact f32: texel t = uv*size - 0.5, weights from t - floor(t).
+template <class S>
+VGI_HD f3 sample_bilinear3(const void* tex, uint32_t w, uint32_t h, float u, float v) {
+    float tx = u * (float)w - 0.5f;
+    float ty = v * (float)h - 0.5f;
+    float x0f = floorf(tx), y0f = floorf(ty);
+    float fx = tx - x0f, fy = ty - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int xa = clampi(x0, 0, (int)w - 1), xb = clampi(x0 + 1, 0, (int)w - 1);
+    int ya = clampi(y0, 0, (int)h - 1), yb = clampi(y0 + 1, 0, (int)h - 1);
+    f3 c00 = S::ld3(tex, (size_t)ya * w + xa);
+    f3 c10 = S::ld3(tex, (size_t)ya * w + xb);
+    f3 c01 = S::ld3(tex, (size_t)yb * w + xa);
+    f3 c11 = S::ld3(tex, (size_t)yb * w + xb);
+    f3 top = c00 * (1.0f - fx) + c10 * fx;
+    f3 bot = c01 * (1.0f - fx) + c11 * fx;
+    return top * (1.0f - fy) + bot * fy;
+}
+
 // History length: R8_UNORM in the reference (denoising.rs:122-125).  Stored
 // as the 8-bit count round(len * 255) (round half to even), read as k/255.
 VGI_HD uint8_t history_len_store(float len) {

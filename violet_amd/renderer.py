@@ -39,6 +39,9 @@ class RestirConfig:
     concurrent_lanes: bool = True
     # also run the sun shadow mask and the final lighting (the lit image)
     compose: bool = False
+    # temporal AA of the lit image (needs compose; jitter the frame params with
+    # synth.jitter_ndc(frame_index % jitter_cycle, w, h))
+    taa: bool = False
     disocclusion_fix: bool = True
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
@@ -58,6 +61,7 @@ class RestirConfig:
         c.spatial_radius_px = self.spatial_radius_px
         c.concurrent_lanes = int(self.concurrent_lanes)
         c.compose = int(self.compose)
+        c.taa = int(self.taa)
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
                                          int(self.atrous_iterations), int(self.compensated_moments))
         return c
