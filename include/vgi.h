@@ -295,6 +295,36 @@ vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** out);
 vgi_status vgi_scene_destroy(vgi_scene* s);
 vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out);
 
+/* glTF 2.0 import (crates/violet/src/model.rs:389-720 import_gltf_uncached;
+ * the scene's BLAS/TLAS of render_scene.rs:1170-1712 are replaced by one BVH
+ * over the baked triangles).  Kept primitives: TRIANGLES mode with a
+ * material, indices and POSITION; indices read as u16 (u32 indices are
+ * truncated, as violet's importer truncates them).  The default scene
+ * (`scene` property; none is an error) is flattened depth first with each
+ * node's matrix or T*R*S after its parent's, under the root transform
+ * rotation(+90 deg about X) * scale (glTF Y-up -> violet Z-up).  Instances
+ * are baked to world space: positions through the affine transform, normals
+ * through the inverse transpose without normalising, winding reversed under
+ * a mirroring transform; a primitive without NORMAL gets per-corner vertices
+ * with the geometric normal.  Materials keep the PBR factors (glTF defaults
+ * 1); textures are not sampled (stats.textured_materials counts materials
+ * that reference one).  Sources: .gltf with relative-file or base64 data-URI
+ * buffers, or .glb. */
+typedef struct vgi_gltf_model vgi_gltf_model;
+typedef struct vgi_gltf_config {
+    float scale; /* LoadConfig::scale (model.rs:20-33), default 1 */
+} vgi_gltf_config;
+typedef struct vgi_gltf_stats {
+    uint32_t meshes, primitives, primitives_skipped, instances, materials, textured_materials;
+    uint32_t triangles, vertices, truncated_indices;
+} vgi_gltf_stats;
+vgi_status vgi_gltf_default_config(vgi_gltf_config* cfg);
+vgi_status vgi_gltf_import(const char* path, const vgi_gltf_config* cfg, vgi_gltf_model** out);
+/* Fills desc with the model's arrays (valid until vgi_gltf_destroy); pass it to vgi_scene_create. */
+vgi_status vgi_gltf_model_desc(const vgi_gltf_model* m, vgi_scene_desc* desc);
+vgi_status vgi_gltf_model_stats(const vgi_gltf_model* m, vgi_gltf_stats* out);
+vgi_status vgi_gltf_destroy(vgi_gltf_model* m);
+
 /* Batch ray query (TraceRay without shading).  rays: 8 floats per ray
  * {o.x,o.y,o.z,tmin, d.x,d.y,d.z,tmax} in device memory; hits: 4 words per
  * ray {t (f32), u (f32), v (f32), triangle id (u32, 0xFFFFFFFF = miss)}.

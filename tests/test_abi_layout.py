@@ -25,6 +25,7 @@ def _mirrors():
         "vgi_denoiser_config": _lib.DenoiserConfig, "vgi_denoiser_input": _lib.DenoiserInput,
         "vgi_denoiser_output": _lib.DenoiserOutput, "vgi_material": scene.Material,
         "vgi_scene_desc": scene.SceneDesc, "vgi_scene_stats": scene.SceneStats,
+        "vgi_gltf_config": scene.GltfConfig, "vgi_gltf_stats": scene.GltfStats,
         "vgi_trace_resources": _abi.TraceResources, "vgi_ind_diff_sample_gen_io": _abi.DiffSampleGenIO,
         "vgi_pc_sample_gen": _abi.PCSampleGen, "vgi_ind_diff_sample_validate_io": _abi.DiffSampleValidateIO,
         "vgi_ind_spec_sample_gen_io": _abi.SpecSampleGenIO, "vgi_ind_spec_sample_validate_io": _abi.SpecSampleValidateIO,

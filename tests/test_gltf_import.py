@@ -1,0 +1,196 @@
+"""glTF import (vgi_gltf_import; crates/violet/src/model.rs:389-720) against
+an independent numpy statement of the same rules, in double precision:
+
+* default scene walked depth first from its root nodes, node matrix or
+  T*R*S (rotation from scipy's quaternion conversion) after the parent's,
+  under rotation(+90 deg about X) * scale;
+* kept primitives: TRIANGLES with a material, indices and POSITION (a LINES
+  primitive and one without material are skipped); u8/u16/u32 indices read
+  as u16 (u32 values above 65535 truncated);
+* instances baked to world space: normals through the inverse transpose
+  (not normalised), a mirrored instance with reversed winding, a primitive
+  without normals as per-corner vertices carrying the geometric normal;
+* materials: PBR factors with glTF defaults; a textured material counted.
+
+The .gltf + .bin, data-URI and .glb forms of the document import to the
+same arrays, and malformed inputs are rejected with VGI_E_INVALID.
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from gltf_fixture import Doc
+
+ANG = np.float32(1.57079637)
+
+
+def _root(scale):
+    c, s = float(np.cos(ANG)), float(np.sin(ANG))
+    rx = np.array([[1, 0, 0, 0], [0, c, -s, 0], [0, s, c, 0], [0, 0, 0, 1.0]])
+    return rx @ np.diag([scale, scale, scale, 1.0])
+
+
+def _trs(t=(0, 0, 0), q=(0, 0, 0, 1), s=(1, 1, 1)):
+    m = np.eye(4)
+    m[:3, :3] = Rotation.from_quat(q).as_matrix() @ np.diag(s)
+    m[:3, 3] = t
+    return m
+
+
+def _build():
+    rng = np.random.default_rng(21)
+    d = Doc()
+    m_red = d.material({"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.2, 0.1, 1.0], "metallicFactor": 0.0,
+                                                 "roughnessFactor": 0.5}})
+    m_def = d.material({})
+    m_tex = d.material({"pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.6, 0.9, 1.0],
+                                                 "baseColorTexture": {"index": 0}, "metallicFactor": 0.25}})
+    d.j["textures"] = [{"source": 0}]
+    d.j["images"] = [{"uri": "unused.png"}]
+    # mesh 0: a quad with normals, interleaved POSITION/NORMAL, u16 indices
+    qp = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0]], np.float32)
+    qn = rng.normal(size=(4, 3)).astype(np.float32)
+    pa, na = d.interleaved(qp, qn)
+    mesh_quad = d.mesh([{"attributes": {"POSITION": pa, "NORMAL": na},
+                         "indices": d.accessor(np.array([0, 1, 2, 0, 2, 3], np.uint16)), "material": m_red}])
+    # mesh 1: a tetrahedron without normals, u8 indices; a LINES primitive
+    # and one without material are skipped
+    tp = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    ti = np.array([0, 2, 1, 0, 1, 3, 0, 3, 2, 1, 2, 3], np.uint8)
+    tpa = d.accessor(tp)
+    mesh_tet = d.mesh([{"attributes": {"POSITION": tpa}, "indices": d.accessor(ti), "material": m_def},
+                       {"attributes": {"POSITION": tpa}, "indices": d.accessor(np.array([0, 1], np.uint8)),
+                        "material": m_def, "mode": 1},
+                       {"attributes": {"POSITION": tpa}, "indices": d.accessor(ti)}])
+    # mesh 2: 70000 vertices, u32 indices above 65535 (truncated to u16)
+    bp = rng.normal(size=(70000, 3)).astype(np.float32)
+    bi = np.array([0, 1, 2, 65537, 65538, 65539], np.uint32)
+    mesh_big = d.mesh([{"attributes": {"POSITION": d.accessor(bp)}, "indices": d.accessor(bi), "material": m_tex}])
+    q30 = Rotation.from_rotvec(np.radians(30) * np.array([1, 1, 0]) / np.sqrt(2)).as_quat()
+    q70 = Rotation.from_rotvec(np.radians(70) * np.array([0, 0.6, 0.8])).as_quat()
+    mat1 = _trs((0.5, -1, 2), q70, (1, 3, 1))
+    n1 = d.node(mesh=mesh_quad, matrix=mat1.T.reshape(-1).tolist())  # column-major
+    n2 = d.node(mesh=mesh_quad, translation=[3, 0, 0], scale=[-1, 1, 1])  # mirrored instance
+    n4 = d.node(mesh=mesh_big, translation=[0, 0, -4])
+    n3 = d.node(mesh=mesh_tet, translation=[0, 2, 0], rotation=q30.tolist(), children=[n4])
+    n0 = d.node(translation=[1, 2, 3], rotation=q30.tolist(), scale=[2, 2, 2], children=[n1, n2, n3])
+    d.node(mesh=mesh_quad)  # not part of the scene
+    d.j["scenes"] = [{"nodes": [n0]}]
+    d.j["scene"] = 0
+    xf = {
+        "root": _trs((1, 2, 3), q30, (2, 2, 2)), "n1": mat1, "n2": _trs((3, 0, 0), (0, 0, 0, 1), (-1, 1, 1)),
+        "n3": _trs((0, 2, 0), q30), "n4": _trs((0, 0, -4)),
+    }
+    parts = {"quad": (qp, qn, np.array([0, 1, 2, 0, 2, 3]), m_red), "tet": (tp, None, ti.astype(np.int64), m_def),
+             "big": (bp, None, bi.astype(np.int64) & 0xFFFF, m_tex)}
+    return d, xf, parts
+
+
+def _expected(xf, parts, scale):
+    """World-space triangles (T,3,3), their materials and (for primitives
+    with normals) the per-corner baked normals, in instance order."""
+    R = _root(scale)
+    order = [(R @ xf["root"] @ xf["n1"], "quad"), (R @ xf["root"] @ xf["n2"], "quad"),
+             (R @ xf["root"] @ xf["n3"], "tet"), (R @ xf["root"] @ xf["n3"] @ xf["n4"], "big")]
+    tris, mats, nrms = [], [], []
+    for M, name in order:
+        P, N, I, m = parts[name]
+        A = M[:3, :3]
+        W = P.astype(np.float64) @ A.T + M[:3, 3]
+        NT = np.linalg.inv(A).T
+        flip = np.linalg.det(A) < 0
+        for t in I.reshape(-1, 3):
+            c = [t[0], t[2], t[1]] if flip else list(t)
+            tris.append(W[c])
+            mats.append(m)
+            nrms.append(None if N is None else N[c].astype(np.float64) @ NT.T)
+    return np.array(tris), np.array(mats), nrms
+
+
+@pytest.fixture(scope="module")
+def doc():
+    return _build()
+
+
+def _imported(path, scale=1.0):
+    from violet_amd import scene
+
+    return scene.load_gltf(path, scale)
+
+
+@pytest.mark.parametrize("form", ["gltf", "data_uri", "glb"])
+@pytest.mark.parametrize("scale", [1.0, 0.5])
+def test_import_matches_model_rules(doc, tmp_path, form, scale):
+    d, xf, parts = doc
+    path = {"gltf": lambda: d.save_gltf(tmp_path / "s.gltf"), "data_uri": lambda: d.save_data_uri(tmp_path / "s.gltf"),
+            "glb": lambda: d.save_glb(tmp_path / "s.glb")}[form]()
+    mesh, st = _imported(path, scale)
+    tris, mats, nrms = _expected(xf, parts, scale)
+    assert (st.meshes, st.primitives, st.primitives_skipped, st.instances) == (3, 3, 2, 4)
+    assert (st.materials, st.textured_materials, st.triangles, st.truncated_indices) == (3, 1, len(tris), 3)
+    got = mesh.positions[mesh.indices]
+    np.testing.assert_allclose(got, tris, rtol=1e-5, atol=1e-5 * scale * 10)
+    assert np.array_equal(mesh.tri_material, mats)
+    # baked normals: inverse transpose without normalisation; geometric
+    # normals (per-corner vertices) point where the transformed local
+    # cross product points, also under the mirror
+    for k, n in enumerate(nrms):
+        gn = mesh.normals[mesh.indices[k]]
+        if n is not None:
+            np.testing.assert_allclose(gn, n, rtol=1e-4, atol=1e-5 / scale)
+        else:
+            e = np.cross(tris[k][1] - tris[k][0], tris[k][2] - tris[k][0])
+            assert all(np.dot(g, e) > 0 for g in gn)
+            np.testing.assert_allclose(gn / np.linalg.norm(gn, axis=1, keepdims=True),
+                                       np.tile(e / np.linalg.norm(e), (3, 1)), atol=1e-4)
+    # mirrored quad: the baked geometric normal is the transformed local one
+    A = (_root(scale) @ xf["root"] @ xf["n2"])[:3, :3]
+    qp = parts["quad"][0].astype(np.float64)
+    local = np.cross(qp[1] - qp[0], qp[2] - qp[0])
+    t = tris[2]
+    assert np.dot(np.cross(t[1] - t[0], t[2] - t[0]), np.linalg.inv(A).T @ local) > 0
+    # materials: factors, glTF defaults for the empty material
+    assert np.allclose(mesh.albedo, [[0.8, 0.2, 0.1], [1, 1, 1], [0.3, 0.6, 0.9]])
+    assert np.allclose(mesh.metallic, [0.0, 1.0, 0.25]) and np.allclose(mesh.roughness, [0.5, 1.0, 1.0])
+
+
+def test_forms_import_identically(doc, tmp_path):
+    d, _, _ = doc
+    a, _ = _imported(d.save_gltf(tmp_path / "a.gltf"))
+    b, _ = _imported(d.save_data_uri(tmp_path / "b.gltf"))
+    c, _ = _imported(d.save_glb(tmp_path / "c.glb"))
+    for x in (b, c):
+        for f in ("positions", "normals", "indices", "tri_material"):
+            assert np.array_equal(getattr(a, f), getattr(x, f)), f
+
+
+def test_malformed_inputs_are_rejected(doc, tmp_path):
+    from violet_amd import _lib
+
+    d, _, _ = doc
+    good = d.save_gltf(tmp_path / "good.gltf")
+    j = json.loads(good.read_text())
+
+    def bad(mutate, name):
+        jj = json.loads(json.dumps(j))
+        mutate(jj)
+        p = tmp_path / name
+        p.write_text(json.dumps(jj))
+        with pytest.raises(_lib.VgiError) as e:
+            _imported(p)
+        return str(e.value)
+
+    assert "default scene" in bad(lambda x: x.pop("scene"), "noscene.gltf")
+    assert "exceeds" in bad(lambda x: x["accessors"][0].update(count=10 ** 6), "range.gltf")
+    assert "version" in bad(lambda x: x["asset"].update(version="1.0"), "v1.gltf")
+    assert "node" in bad(lambda x: x["scenes"][0].update(nodes=[99]), "node.gltf")
+    p = tmp_path / "broken.gltf"
+    p.write_text(good.read_text()[:-20])
+    with pytest.raises(_lib.VgiError):
+        _imported(p)
+    with pytest.raises(_lib.VgiError):
+        _imported(tmp_path / "missing.gltf")

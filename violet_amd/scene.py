@@ -42,6 +42,22 @@ _lib.register("vgi_scene_info", C.c_int32, [C.c_void_p, C.POINTER(SceneStats)])
 _lib.register("vgi_trace_rays", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p])
 
 
+class GltfConfig(C.Structure):
+    _fields_ = [("scale", C.c_float)]
+
+
+class GltfStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("meshes", "primitives", "primitives_skipped", "instances", "materials",
+                                           "textured_materials", "triangles", "vertices", "truncated_indices")]
+
+
+_lib.register("vgi_gltf_default_config", C.c_int32, [C.POINTER(GltfConfig)])
+_lib.register("vgi_gltf_import", C.c_int32, [C.c_char_p, C.POINTER(GltfConfig), C.POINTER(C.c_void_p)])
+_lib.register("vgi_gltf_model_desc", C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)])
+_lib.register("vgi_gltf_model_stats", C.c_int32, [C.c_void_p, C.POINTER(GltfStats)])
+_lib.register("vgi_gltf_destroy", C.c_int32, [C.c_void_p])
+
+
 @dataclass
 class Mesh:
     positions: np.ndarray  # (V,3) f32
@@ -197,3 +213,35 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def load_gltf(path, scale: float = 1.0) -> tuple[Mesh, GltfStats]:
+    """Import a .gltf / .glb file through the library's importer
+    (vgi_gltf_import: model.rs import semantics, instances baked to world
+    space, Z up) into a Mesh for Scene(); also returns the import statistics."""
+    cfg = GltfConfig(scale)
+    h = C.c_void_p()
+    _lib.call("vgi_gltf_import", str(path).encode(), C.byref(cfg), C.byref(h))
+    try:
+        d = SceneDesc()
+        _lib.call("vgi_gltf_model_desc", h, C.byref(d))
+        st = GltfStats()
+        _lib.call("vgi_gltf_model_stats", h, C.byref(st))
+
+        def arr(ptr, n, dt, shape):
+            if n == 0:
+                return np.zeros(shape, dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), (n,)).reshape(
+                shape).copy()
+
+        nv, nt, nm = d.num_vertices, d.num_triangles, d.num_materials
+        pos = arr(d.positions, 3 * nv, np.float32, (nv, 3))
+        nrm = arr(d.normals, 3 * nv, np.float32, (nv, 3))
+        idx = arr(d.indices, 3 * nt, np.uint32, (nt, 3))
+        tm = arr(d.triangle_material, nt, np.uint32, (nt,))
+        mats = [d.materials[k] for k in range(nm)]
+        mesh = Mesh(pos, nrm, idx, tm, np.array([list(m.base_color)[:3] for m in mats], np.float32).reshape(nm, 3),
+                    np.array([m.metallic for m in mats], np.float32), np.array([m.roughness for m in mats], np.float32))
+        return mesh, st
+    finally:
+        _lib.lib().vgi_gltf_destroy(h)
