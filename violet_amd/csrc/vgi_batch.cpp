@@ -64,7 +64,7 @@ struct vgi_batch {
     hipEvent_t ev_in = nullptr;
     // second lane (concurrent_lanes): independent slot chains beside the main stream
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cross = nullptr;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t key_storage[2] = {0, 0};
     size_t key_lds[2] = {0, 0};
@@ -86,6 +86,7 @@ struct vgi_batch {
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_cross) (void)hipEventDestroy(ev_cross);
         if (side) (void)hipStreamDestroy(side);
         if (cap) (void)hipStreamDestroy(cap);
     }
@@ -144,6 +145,7 @@ extern "C" vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** 
     if (e == hipSuccess && c.concurrent_lanes) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
     if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_cross, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete b;
         return check_hip(e, "vgi_batch_create");
@@ -188,6 +190,7 @@ static vgi_status issue_slots(vgi_batch* b, int a, uint32_t storage, size_t lds,
         } else {
             r = o->launch((const char*)b->dev[a] + b->offsets[s], storage, groups_for(b, s, o), o->trace ? lds : 0, st);
         }
+        if (r == VGI_OK) r = lanes.leave(s, st);
         if (r != VGI_OK) return r;
     }
     return VGI_OK;
@@ -200,6 +203,7 @@ static vgi_status issue_frame(vgi_batch* b, int a, uint32_t storage, size_t lds,
     lanes.side = b->side;
     lanes.ev_fork = b->ev_fork;
     lanes.ev_join = b->ev_join;
+    lanes.ev_cross = b->ev_cross;
     vgi_status r = issue_slots(b, a, storage, lds, lanes, all_slots, profile);
     vgi_status j = lanes.finish();
     return r != VGI_OK ? r : j;

@@ -23,18 +23,35 @@ vgi_status Lanes::enter(int slot, hipStream_t& st) {
         VGI_TRY_HIP(hipEventRecord(ev_fork, main));
         VGI_TRY_HIP(hipStreamWaitEvent(side, ev_fork, 0));
         phase = ph;
+        cross_from = nullptr;
     }
     if (slot_lane(slot)) {
         st = side;
         side_work = true;
     }
+    if (slot_waits(slot) && cross_from && cross_from != st) {
+        VGI_TRY_HIP(hipStreamWaitEvent(st, ev_cross, 0));
+        cross_from = nullptr;
+    }
+    return VGI_OK;
+}
+
+vgi_status Lanes::leave(int slot, hipStream_t st) {
+    if (!side || !ev_cross || !slot_signals(slot)) return VGI_OK;
+    VGI_TRY_HIP(hipEventRecord(ev_cross, st));
+    cross_from = st;
     return VGI_OK;
 }
 
 vgi_status DirectSink::submit(int slot, const PassRec& r) {
     hipStream_t st;
     vgi_status e = lanes.enter(slot, st);
-    if (e != VGI_OK) return e;
+    if (e == VGI_OK) e = launch(r, st);
+    if (e == VGI_OK) e = lanes.leave(slot, st);
+    return e;
+}
+
+vgi_status DirectSink::launch(const PassRec& r, hipStream_t st) {
     void* s = (void*)st;
     switch (r.kind) {
         case kPassClear: return fill_zero(r.clear_ptr, r.clear_bytes, st);

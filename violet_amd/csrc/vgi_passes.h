@@ -55,6 +55,7 @@ enum Slot : int {
     kSlotHgCompact,
     kSlotClearNextCounter,
     kSlotHgRaygen,
+    kSlotSpecValidate,  // issued ahead of the diffuse passes (see slot_lane)
     kSlotDiffGen,
     kSlotDiffValidate,
     kSlotAoGen,
@@ -62,7 +63,6 @@ enum Slot : int {
     kSlotDiffTemporal,
     kSlotDiffSpatial,
     kSlotSpecGen,
-    kSlotSpecValidate,
     kSlotSpecTemporal,
     kSlotSpecSpatial,
     kSlotDenClearFastDiff,
@@ -83,8 +83,8 @@ inline int slot_kind(int slot) {
     static const int k[kSlotCount] = {
         kPassGBuffer,      kPassClear,        kPassClear,         kPassClear,       kPassHgDecay,
         kPassClear,        kPassHgSelect,     kPassClear,         kPassHgCompact,   kPassClear,
-        kPassHgRaygen,     kPassDiffGen,      kPassDiffValidate,  kPassAoGen,       kPassAoFilter,
-        kPassDiffTemporal, kPassDiffSpatial,  kPassSpecGen,       kPassSpecValidate, kPassSpecTemporal,
+        kPassHgRaygen,     kPassSpecValidate, kPassDiffGen,       kPassDiffValidate, kPassAoGen,
+        kPassAoFilter,     kPassDiffTemporal, kPassDiffSpatial,   kPassSpecGen,     kPassSpecTemporal,
         kPassSpecSpatial,  kPassClear,        kPassClear,         kPassDenTemporal, kPassDenDisocclusion,
         kPassDenFastClamp, kPassDenFallback,  kPassDenAtrous,     kPassDenAtrous,   kPassDenAtrous,
         kPassDenAtrous,    kPassDenAtrous,    kPassShadow,        kPassFinalLighting, kPassTemporalAA};
@@ -162,42 +162,56 @@ struct PassSink {
 // sampling and resampling (lane 1; both read the G-buffer and the updated
 // cache, and only append to the shared query buffer); phase 2: denoiser
 // (lane 0) beside the sun shadow mask (lane 1: it reads the depth only);
-// phase 3: final lighting, then temporal AA.
+// phase 3: final lighting, then temporal AA.  The specular validation re-
+// traces last frame's samples and needs nothing of this frame's specular
+// generation, so it runs at the head of lane 0, which balances the two
+// lanes' ray tracing; the specular temporal pass waits for it (a cross-lane
+// event, slot_signals / slot_waits).
 inline int slot_phase(int slot) {
-    return slot < kSlotDiffGen ? 0 : slot < kSlotDenClearFastDiff ? 1 : slot < kSlotFinalLighting ? 2 : 3;
+    return slot < kSlotSpecValidate ? 0 : slot < kSlotDenClearFastDiff ? 1 : slot < kSlotFinalLighting ? 2 : 3;
 }
 inline int slot_lane(int slot) {
     return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygen) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial) ||
            slot == kSlotShadow;
 }
+inline bool slot_signals(int slot) { return slot == kSlotSpecValidate; }
+inline bool slot_waits(int slot) { return slot == kSlotSpecTemporal; }
 
 // The streams of a frame's slots, issued in slot order.  Without a side
 // stream every slot runs on `main`.  With one, lane-1 slots go to `side`:
 // entering a phase joins the side stream into the main one (if it has work)
 // and forks it again from the main stream, so a lane-1 slot waits for every
 // earlier phase and for nothing else; finish() joins it back at the end.
-// Works the same under stream capture (the side stream joins the capture).
+// leave() after a signalling slot records ev_cross on its stream, and a
+// waiting slot of the same phase on the other lane waits for it.  Works the
+// same under stream capture (the side stream joins the capture).
 struct Lanes {
     hipStream_t main = nullptr, side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cross = nullptr;
     int phase = -1;
     bool side_work = false;
+    hipStream_t cross_from = nullptr;  // stream ev_cross was recorded on this phase, or null
     vgi_status enter(int slot, hipStream_t& st);
+    vgi_status leave(int slot, hipStream_t st);
     vgi_status finish();
 };
 
 // Launches each record at once through its C entry point.
 struct DirectSink : PassSink {
     explicit DirectSink(hipStream_t s) { lanes.main = s; }
-    DirectSink(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+    DirectSink(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join, hipEvent_t cross) {
         lanes.main = s;
         lanes.side = side;
         lanes.ev_fork = fork;
         lanes.ev_join = join;
+        lanes.ev_cross = cross;
     }
     vgi_status submit(int slot, const PassRec& r) override;
     vgi_status finish() { return lanes.finish(); }
     Lanes lanes;
+
+   private:
+    static vgi_status launch(const PassRec& r, hipStream_t st);
 };
 
 // Keeps the records of one view's frame by slot.

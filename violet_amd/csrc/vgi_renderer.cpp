@@ -56,11 +56,12 @@ struct vgi_renderer {
 
     // second lane of add_passes (concurrent_lanes): stream + fork/join events
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cross = nullptr;
     ~vgi_renderer() {
         if (den) vgi_denoiser_destroy(den);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_cross) (void)hipEventDestroy(ev_cross);
         if (side) (void)hipStreamDestroy(side);
     }
     std::vector<std::pair<std::string, DevBuf*>> images() {
@@ -201,6 +202,7 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
         hipError_t e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_cross, hipEventDisableTiming);
         if (e != hipSuccess) st = check_hip(e, "vgi_renderer_create: side stream");
     }
     if (st != VGI_OK) {
@@ -296,6 +298,18 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         hg_updated = true;
     }
 
+    // Specular sample validate (:722-737): issued first, it re-traces last
+    // frame's specular samples and is independent of this frame's passes
+    // up to the specular temporal resample
+    const bool has_prev_spec = r->has_prev_ind_spec;
+    if (r->cfg.ind_spec_validate && has_prev_depth && has_prev_ind_diff && has_prev_spec) {
+        PassRec v = make_rec(kPassSpecValidate, fp);
+        v.tr = trace_res(r, scene, true);
+        v.io.spec_validate = vgi_ind_spec_sample_validate_io{w, h, S, r->spec_rsv[p].as<uint32_t>(),
+                                                             r->spec_hitpos[p].as<float>(), r->spec_rad[p].p};
+        CALL(sink.submit(kSlotSpecValidate, v));
+    }
+
     // Indirect diffuse: sample gen or validation (:436-513)
     const bool has_prev_diff_rsv = r->has_prev_diffuse_reservoir;
     const bool validation = has_prev_depth && has_prev_diff_rsv && ((frame_index & 3u) == 0u) && r->cfg.ind_diff_validate;
@@ -352,7 +366,7 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         CALL(sink.submit(kSlotDiffSpatial, sp));
     }
 
-    // Specular sample gen (:685-701), validate (:722-737), temporal (:743-761), spatial (:770-784)
+    // Specular sample gen (:685-701), temporal (:743-761), spatial (:770-784)
     {
         PassRec g = make_rec(kPassSpecGen, fp);
         g.tr = trace_res(r, scene, trace_prev);
@@ -360,14 +374,6 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
                                                    r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
         g.pc.sample_gen = vgi_pc_sample_gen{frame_rand_next(frame_rand), trace_prev ? 1u : 0u};
         CALL(sink.submit(kSlotSpecGen, g));
-    }
-    const bool has_prev_spec = r->has_prev_ind_spec;
-    if (r->cfg.ind_spec_validate && has_prev_depth && has_prev_ind_diff && has_prev_spec) {
-        PassRec v = make_rec(kPassSpecValidate, fp);
-        v.tr = trace_res(r, scene, true);
-        v.io.spec_validate = vgi_ind_spec_sample_validate_io{w, h, S, r->spec_rsv[p].as<uint32_t>(),
-                                                             r->spec_hitpos[p].as<float>(), r->spec_rad[p].p};
-        CALL(sink.submit(kSlotSpecValidate, v));
     }
     {
         const bool hp = has_prev_depth && has_prev_spec && has_prev_gbuffer;
@@ -455,7 +461,7 @@ uint32_t vgi_renderer_storage(const vgi_renderer* r) { return r->cfg.storage; }
 extern "C" vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, const vgi_frame_params* fp,
                                               uint32_t frame_index, vgi_renderer_output* out, void* stream) {
     VGI_REQUIRE(r, "vgi_renderer_add_passes: null renderer");
-    DirectSink sink((hipStream_t)stream, r->side, r->ev_fork, r->ev_join);
+    DirectSink sink((hipStream_t)stream, r->side, r->ev_fork, r->ev_join, r->ev_cross);
     vgi_status s = vgi_renderer_plan(r, scene, fp, frame_index, out, sink);
     vgi_status j = sink.finish();
     return s != VGI_OK ? s : j;
