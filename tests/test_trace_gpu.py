@@ -164,3 +164,46 @@ def test_traversal_statistics_count_exactly_and_change_nothing(vgi):
         else:
             assert c["nodes"] > len(rays) and c["tris"] > 0
             assert a["nodes"] <= c["nodes"]  # any-hit stops early
+
+
+def test_zero_direction_components_keep_culling(vgi):
+    """Rays along the axes (two direction components exactly 0) hit exactly
+    what the brute-force loop hits, and the box tests keep culling on every
+    axis: they visit no more nodes than the same rays tilted by 1e-6 (an
+    infinite inverse direction would make a fused plane distance
+    inf - inf = NaN, which drops an axis out of the slab test)."""
+    import oracle
+    from violet_amd import _lib, scene
+
+    mesh = scene.make_scene(4000, seed=7, extent=8.0)
+    sc = scene.Scene(mesh)
+    rng = np.random.default_rng(17)
+    n = 6000
+    o = rng.uniform(-8.0, 8.0, (n, 3))
+    o[:, 2] = rng.uniform(0.05, 6.0, n)
+    axis = rng.integers(0, 3, n)
+    d = np.zeros((n, 3))
+    d[np.arange(n), axis] = rng.choice([-1.0, 1.0], n)
+    tilt = d + 1e-6 * (d == 0)
+    tilt /= np.linalg.norm(tilt, axis=1, keepdims=True)
+
+    def rays(dd):
+        return np.concatenate([o, np.full((n, 1), 5e-4), dd, np.full((n, 1), 1000.0)], 1).astype(np.float32)
+
+    counts = []
+    for dd in (d, tilt):
+        rr = rays(dd)
+        _lib.call("vgi_profiler_reset")
+        _lib.call("vgi_trace_stats_enable", 1)
+        _lib.call("vgi_profiler_enable", 1)
+        try:
+            thv, ids = _gpu_trace(sc, rr, False)
+        finally:
+            _lib.call("vgi_profiler_enable", 0)
+            _lib.call("vgi_trace_stats_enable", 0)
+        counts.append(_lib.profiler_read()["trace_rays_closest"]["nodes"])
+        o_thv, o_id = oracle.trace(32, mesh.positions, mesh.indices, rr, brute=True)
+        assert np.array_equal(ids, o_id)
+        hit = ids != 0xFFFFFFFF
+        assert np.array_equal(thv[hit].view(np.uint32), o_thv[hit].astype(np.float32).view(np.uint32))
+    assert counts[0] <= 1.2 * counts[1], f"axis-aligned rays visit {counts[0]} nodes vs {counts[1]} tilted"

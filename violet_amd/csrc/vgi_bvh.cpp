@@ -365,6 +365,12 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
                          (unsigned)c4.max_level + 1);
     }
     sc->view.counters = nullptr;
+    // traversal reads both arrays through 32-bit buffer offsets
+    if ((uint64_t)sc->view.num_nodes * sizeof(BvhNode) > 0x7FFFFFFFull ||
+        (uint64_t)nt * sizeof(BvhTri) > 0x7FFFFFFFull) {
+        delete sc;
+        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: %u triangles exceed the 2 GiB node/triangle arrays", nt);
+    }
     *out = sc;
     return VGI_OK;
 }

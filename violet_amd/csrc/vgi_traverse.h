@@ -6,10 +6,10 @@
  * [k*64 + l]) so that a push or pop by all lanes touches 64 consecutive
  * dwords (no bank conflicts).  At a node the hit children are ordered by
  * entry distance; the nearest is visited next and the others are pushed
- * farthest first.  Boxes are tested with the slab method in exact IEEE
- * single precision (no fused multiply-add, so every box test is
- * conservative up to its own rounding); NaNs from 0*inf drop out of the
- * max/min, which only widens the interval.
+ * farthest first.  Boxes are tested with the slab method, one fused
+ * multiply-add per plane (conservative within the boxes' padding, see
+ * slab_nf); NaNs from 0*inf drop out of the max/min, which only widens the
+ * interval.
  */
 #pragma once
 #include "vgi_bvh.h"
@@ -28,27 +28,55 @@ __device__ __forceinline__ Vf4 ld_global16(const void* p, int k) {
     return ((const __attribute__((address_space(1))) Vf4*)p)[k];
 }
 
+// The node and triangle arrays are read through buffer descriptors (wave-
+// uniform base and size in scalar registers, a 32-bit byte offset per
+// lane): the addresses of a node's seven 16-byte loads then cost one 32-bit
+// add each instead of 64-bit address arithmetic, and fewer registers.
+// Descriptor word 3 of the gfx9 family for untyped dword access; loads
+// beyond `bytes` return zeros.  vgi_scene_create keeps both arrays below
+// 2^31 bytes.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+__device__ __forceinline__ BufRsrc scene_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ Vf4 ld_buf16(BufRsrc r, uint32_t byte_off) {
+    return __builtin_bit_cast(Vf4, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+
 struct RayInv {
     f3 o, d, idir;
 };
+
+// A direction component of magnitude below 1e-20 gives a finite inverse of
+// +-1e20 (slab_nf evaluates P * idir - o * idir, where an infinite inverse
+// would turn every plane of that axis into inf - inf = NaN and stop the
+// axis from culling anything).  The slab interval of such an axis is then
+// (-huge, +huge) for an origin between the planes and empty otherwise, as
+// for an exact zero.
+__device__ __forceinline__ float safe_dir(float v) { return fabsf(v) < 1e-20f ? copysignf(1e-20f, v) : v; }
 
 __device__ __forceinline__ RayInv make_ray(f3 o, f3 d) {
     RayInv r;
     r.o = o;
     r.d = d;
-    r.idir = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    r.idir = mk3(1.0f / safe_dir(d.x), 1.0f / safe_dir(d.y), 1.0f / safe_dir(d.z));
     return r;
 }
 
 // Entry distance of a box given each axis' near and far plane for this ray
 // (near = lo where the direction component is >= 0, else hi), or +inf if
-// missed.  A NaN from 0 * inf (origin on a plane of a box the ray runs
-// parallel to) drops out of the max/min, which only widens the interval.
-__device__ __forceinline__ float slab_nf(const RayInv& r, float nx, float fx, float ny, float fy, float nz, float fz,
-                                         float tmin, float tmax) {
-    const float tnx = (nx - r.o.x) * r.idir.x, tfx = (fx - r.o.x) * r.idir.x;
-    const float tny = (ny - r.o.y) * r.idir.y, tfy = (fy - r.o.y) * r.idir.y;
-    const float tnz = (nz - r.o.z) * r.idir.z, tfz = (fz - r.o.z) * r.idir.z;
+// missed.  A plane's distance is one fused multiply-add, P * idir + oid with
+// oid = -o * idir: against the exact (P - o) * idir it errs by at most an
+// ulp of o * idir plus the final rounding, i.e. a world distance of about
+// ulp(|o|), far inside the 2^-16 relative padding of every child box, so
+// the test stays conservative.  A NaN from 0 * inf (origin on a plane of a
+// box the ray runs parallel to) drops out of the max/min, which only widens
+// the interval.
+__device__ __forceinline__ float slab_nf(const RayInv& r, f3 oid, float nx, float fx, float ny, float fy, float nz,
+                                         float fz, float tmin, float tmax) {
+    const float tnx = __builtin_fmaf(nx, r.idir.x, oid.x), tfx = __builtin_fmaf(fx, r.idir.x, oid.x);
+    const float tny = __builtin_fmaf(ny, r.idir.y, oid.y), tfy = __builtin_fmaf(fy, r.idir.y, oid.y);
+    const float tnz = __builtin_fmaf(nz, r.idir.z, oid.z), tfz = __builtin_fmaf(fz, r.idir.z, oid.z);
     const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
     const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tmax);
     return tn <= tf ? tn : INFINITY;
@@ -92,8 +120,11 @@ constexpr int32_t kTraversalDone = INT32_MIN;
 template <bool kAnyHit, bool kCount = false>
 __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
     const RayInv r = make_ray(o, d);
-    // float4 slot of each axis' near plane in a node (lo = 2 axis, hi = 2 axis + 1)
-    const int nx = r.idir.x >= 0.0f ? 0 : 1, ny = r.idir.y >= 0.0f ? 2 : 3, nz = r.idir.z >= 0.0f ? 4 : 5;
+    // byte offset of each axis' near plane in a node (float4 slot lo = 2 axis,
+    // hi = 2 axis + 1; the far plane is the other one, offset ^ 16)
+    const uint32_t nx = r.idir.x >= 0.0f ? 0u : 16u, ny = r.idir.y >= 0.0f ? 32u : 48u, nz = r.idir.z >= 0.0f ? 64u : 80u;
+    const BufRsrc nodes = scene_rsrc(s.nodes, s.num_nodes * (uint32_t)sizeof(BvhNode));
+    const BufRsrc tris = scene_rsrc(s.tris, s.num_tris * (uint32_t)sizeof(BvhTri));
     Hit h;
     h.t = tmax;
     h.u = h.v = 0.0f;
@@ -128,15 +159,21 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
                 break;
             }
             if (kCount) ++n_nodes;
-            const void* p = s.nodes + node;
-            const Vf4 nrx = ld_global16(p, nx), frx = ld_global16(p, nx ^ 1);
-            const Vf4 nry = ld_global16(p, ny), fry = ld_global16(p, ny ^ 1);
-            const Vf4 nrz = ld_global16(p, nz), frz = ld_global16(p, nz ^ 1);
-            const Vf4 lk = ld_global16(p, 6);
-            float t0 = slab_nf(r, nrx.x, frx.x, nry.x, fry.x, nrz.x, frz.x, tmin, h.t);
-            float t1 = slab_nf(r, nrx.y, frx.y, nry.y, fry.y, nrz.y, frz.y, tmin, h.t);
-            float t2 = slab_nf(r, nrx.z, frx.z, nry.z, fry.z, nrz.z, frz.z, tmin, h.t);
-            float t3 = slab_nf(r, nrx.w, frx.w, nry.w, fry.w, nrz.w, frz.w, tmin, h.t);
+            const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(BvhNode);
+            const Vf4 nrx = ld_buf16(nodes, nb + nx), frx = ld_buf16(nodes, nb + (nx ^ 16u));
+            const Vf4 nry = ld_buf16(nodes, nb + ny), fry = ld_buf16(nodes, nb + (ny ^ 16u));
+            const Vf4 nrz = ld_buf16(nodes, nb + nz), frz = ld_buf16(nodes, nb + (nz ^ 16u));
+            const Vf4 lk = ld_buf16(nodes, nb + 96u);
+            // -o * idir per node: kept out of loop-invariant hoisting (three
+            // more live registers across the traversal cost more than three
+            // multiplies per node)
+            f3 oo = r.o;
+            asm volatile("" : "+v"(oo.x), "+v"(oo.y), "+v"(oo.z));
+            const f3 oid = mk3(-(oo.x * r.idir.x), -(oo.y * r.idir.y), -(oo.z * r.idir.z));
+            float t0 = slab_nf(r, oid, nrx.x, frx.x, nry.x, fry.x, nrz.x, frz.x, tmin, h.t);
+            float t1 = slab_nf(r, oid, nrx.y, frx.y, nry.y, fry.y, nrz.y, frz.y, tmin, h.t);
+            float t2 = slab_nf(r, oid, nrx.z, frx.z, nry.z, fry.z, nrz.z, frz.z, tmin, h.t);
+            float t3 = slab_nf(r, oid, nrx.w, frx.w, nry.w, fry.w, nrz.w, frz.w, tmin, h.t);
             int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
             int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
             if (kAnyHit) {
@@ -197,8 +234,8 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             const uint32_t first = enc >> 4, cnt = enc & 15u;
             for (uint32_t k = 0; k < cnt; ++k) {
                 if (kCount) ++n_tris;
-                const void* tp = s.tris + first + k;
-                Vf4 a = ld_global16(tp, 0), e1 = ld_global16(tp, 1), e2 = ld_global16(tp, 2);
+                const uint32_t tb = (first + k) * (uint32_t)sizeof(BvhTri);
+                Vf4 a = ld_buf16(tris, tb), e1 = ld_buf16(tris, tb + 16u), e2 = ld_buf16(tris, tb + 32u);
                 float t, u, v;
                 if (intersect_tri(o, d, mk3(a.x, a.y, a.z), mk3(e1.x, e1.y, e1.z), mk3(e2.x, e2.y, e2.z), tmin, t, u,
                                   v)) {
