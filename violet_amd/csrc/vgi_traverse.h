@@ -209,11 +209,24 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             sort_pair(t1, l1, t3, l3);
             sort_pair(t1, l1, t2, l2);
             if (t0 != INFINITY) {
-                // the depth test always passes for a BVH from vgi_scene_create:
-                // the stack never outgrows 3 far children per level
-                if (t3 != INFINITY && sp < depth) push(l3);
-                if (t2 != INFINITY && sp < depth) push(l2);
-                if (t1 != INFINITY && sp < depth) push(l1);
+                if (sp + 3 <= (int)kLdsStackEntries) {
+                    // all three fit in the LDS part: write every slot and
+                    // advance past the hit ones (no branch per child; a
+                    // slot written for a missed child lies above the top)
+                    stack[sp * 64] = l3;
+                    sp += t3 != INFINITY;
+                    stack[sp * 64] = l2;
+                    sp += t2 != INFINITY;
+                    stack[sp * 64] = l1;
+                    sp += t1 != INFINITY;
+                } else {
+                    // the depth test always passes for a BVH from
+                    // vgi_scene_create: the stack never outgrows 3 far
+                    // children per level
+                    if (t3 != INFINITY && sp < depth) push(l3);
+                    if (t2 != INFINITY && sp < depth) push(l2);
+                    if (t1 != INFINITY && sp < depth) push(l1);
+                }
                 node = l0;
             } else {
                 node = pop();
