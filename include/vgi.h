@@ -75,6 +75,9 @@ typedef struct vgi_profile_entry {
     double tris;         /* triangles tested (VGI_BVH_TRI_BYTES each) */
 } vgi_profile_entry;
 vgi_status vgi_profiler_enable(uint32_t enable);
+/* Restrict the timers to the passes named `name` (NULL or "" = every pass):
+ * a timed region can then carry the events of one kernel only. */
+vgi_status vgi_profiler_only(const char* name);
 vgi_status vgi_profiler_reset(void);
 int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap);
 

@@ -139,6 +139,7 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_profiler_reset": (C.c_int32, []),
     "vgi_profiler_read": (C.c_int32, [C.POINTER(ProfileEntry), C.c_int32]),
     "vgi_trace_stats_enable": (C.c_int32, [u32]),
+    "vgi_profiler_only": (C.c_int32, [C.c_char_p]),
     "vgi_pass_denoise_temporal": (C.c_int32, [C.POINTER(FrameParams), C.POINTER(DenoiseTemporalIO),
                                               C.POINTER(PCDenoiseTemporal), vp]),
     "vgi_pass_denoise_disocclusion_fix": (C.c_int32, [C.POINTER(DenoiseSpatialIO), vp]),

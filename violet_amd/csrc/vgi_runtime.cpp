@@ -48,6 +48,7 @@ struct Accum {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+std::string g_prof_only;  // record only this pass name (empty = all)
 std::vector<Accum> g_prof_acc;
 std::vector<Pending> g_prof_pending;
 std::vector<hipEvent_t> g_prof_pool;
@@ -86,6 +87,7 @@ void drain_locked() {
 ProfScope::ProfScope(const char* name, hipStream_t s, double bytes) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof_on) return;
+    if (!g_prof_only.empty() && g_prof_only != name) return;
     a_ = take_event();
     if (!a_) return;
     if (hipEventRecord(a_, s) != hipSuccess) {
@@ -160,6 +162,12 @@ extern "C" vgi_status vgi_trace_stats_enable(uint32_t enable) {
 extern "C" vgi_status vgi_profiler_enable(uint32_t enable) {
     std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
     vgi::g_prof_on = enable != 0;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_profiler_only(const char* name) {
+    std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
+    vgi::g_prof_only = name ? name : "";
     return VGI_OK;
 }
 
