@@ -755,7 +755,7 @@ typedef struct vgi_batch_config {
     uint32_t max_views;      /* views per call, 1..64 */
     uint32_t pixel_groups;   /* 64-lane groups of the pixel and cache passes */
     uint32_t trace_groups;   /* 64-lane groups of the ray-tracing passes */
-    uint32_t atrous_groups;  /* 256-lane groups of the a-trous passes */
+    uint32_t atrous_groups;  /* 256-lane groups of the a-trous and spatial-fallback passes */
     uint32_t use_graph;      /* 1: capture the frame once as a HIP graph and replay it */
     uint32_t concurrent_lanes; /* 1: independent pass chains (hash-grid update beside the
                                 * G-buffer, specular beside diffuse) on a second stream */

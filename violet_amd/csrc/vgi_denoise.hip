@@ -363,17 +363,72 @@ __global__ void __launch_bounds__(64) k_disocclusion(vgi_denoise_spatial_io io) 
 
 // ---------------------------------------------------------------------------
 // Spatial fallback / variance estimation (svgf_spatial_fallback_filter.hlsl)
+//
+// A 256-thread workgroup handles a 16x16 block (each wave an 8x8 quarter).
+// Pixels whose quad history length is below 4 frames take a 7x7 edge-
+// stopping filter; when a block holds any, its 22x22 footprint (depth,
+// decoded normal, colours, moments) is staged in LDS once and the 49 taps
+// read LDS.  Out-of-image footprint entries are zero: a zero normal gives
+// weight 0 and zero colours, so such a tap adds exactly nothing, as the
+// skipped tap of the reference's bounds test does.
 // ---------------------------------------------------------------------------
+constexpr int kFbB = 16;               // block edge
+constexpr int kFbF = kFbB + 6;         // footprint edge (3 each side)
+constexpr int kFbN = kFbF * kFbF;      // footprint entries
+constexpr int kFbP = 24;               // footprint row pitch (conflict-free reads, as the a-trous pass)
+constexpr int kFbRow = kFbF * kFbP;    // dwords per field
+enum FallbackField { kGDepth, kGNx, kGNy, kGNz, kGDr, kGDg, kGDb, kGSr, kGSg, kGSb, kGMx, kGMy, kGCount };
+
 template <class S>
 __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io, uint32_t bx_, uint32_t by_) {
-    uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
-    const uint32_t w = io.width, h = io.height;
+    __shared__ float lds[kGCount][kFbRow];
+    const int w = (int)io.width, h = (int)io.height;
+    const size_t plane = (size_t)w * h;
+    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int li = (wv & 1) * 8 + (lane & 7), lj = (wv >> 1) * 8 + (lane >> 3);
+    const int bx = (int)bx_, by = (int)by_;
+    const int px = bx * kFbB + li, py = by * kFbB + lj;
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
-    const size_t plane = (size_t)w * h;
-    float len = inside ? history_len_load(G(io.history_len_texture)[i]) : 0.0f;
-    float qmax = quad_max(len);
+    // this pixel's history length and the max over its 2x2 quad
+    // (QuadReadAcrossX/Y; a partner outside the image counts 0)
+    float len = 0.0f, qmax = 0.0f;
+    if (inside) {
+        len = history_len_load(G(io.history_len_texture)[i]);
+        const int qx = px & ~1, qy = py & ~1;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+                if (qx + dx < w && qy + dy < h)
+                    qmax = fmaxf(qmax, history_len_load(G(io.history_len_texture)[(size_t)(qy + dy) * w + qx + dx]));
+    }
+    const bool filtered = inside && len != 0.0f && qmax < 4.0f * kHistoryLenUnit;
+    if (__syncthreads_or(filtered)) {
+#pragma unroll 1
+        for (int e = t; e < kFbN; e += 256) {
+            const int gx = bx * kFbB + e % kFbF - 3, gy = by * kFbB + e / kFbF - 3;
+            float v[kGCount];
+#pragma unroll
+            for (int k = 0; k < kGCount; ++k) v[k] = 0.0f;
+            if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
+                const size_t j = (size_t)gy * w + gx;
+                const f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
+                const f3 dv = S::ld3(io.diff_texture, j);
+                const f3 sv = S::ld3(io.spec_texture, j);
+                const f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
+                v[kGDepth] = G(io.depth_texture)[j];
+                v[kGNx] = n.x, v[kGNy] = n.y, v[kGNz] = n.z;
+                v[kGDr] = dv.x, v[kGDg] = dv.y, v[kGDb] = dv.z;
+                v[kGSr] = sv.x, v[kGSg] = sv.y, v[kGSb] = sv.z;
+                v[kGMx] = mv.x, v[kGMy] = mv.y;
+            }
+            const int le = (e / kFbF) * kFbP + e % kFbF;
+#pragma unroll
+            for (int k = 0; k < kGCount; ++k) lds[k][le] = v[k];
+        }
+        __syncthreads();
+    }
     if (!inside) return;
     if (len == 0.0f) {
         S::st3(io.rw_diff_texture, i, mk3s(0.0f));
@@ -384,11 +439,12 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
     len = qmax;
     f3 diff, spec;
     f2 sm;
-    if (len < 4.0f * kHistoryLenUnit) {
+    if (filtered) {
         // EdgeStoppingFunc with fwidth_z * 3, sigma_n = 128 and no luminance
         // term, over a 7x7 window
-        const float depth_center = G(io.depth_texture)[i];
-        const f3 nc = decode_normal_fast(G(io.gbuffer_texture)[plane + i]);
+        const int c = (lj + 3) * kFbP + (li + 3);
+        const float depth_center = lds[kGDepth][c];
+        const f3 nc = mk3(lds[kGNx][c], lds[kGNy][c], lds[kGNz][c]);
         const float fwz = decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]).x * 3.0f;
         float wsum = 0.0f;
         f3 dsum = mk3s(0.0f), ssum = mk3s(0.0f);
@@ -397,21 +453,16 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
         for (int y = -3; y <= 3; y++) {
 #pragma unroll
             for (int x = -3; x <= 3; x++) {
-                int cx = (int)px + x, cy = (int)py + y;
-                if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
-                size_t j = (size_t)cy * w + cx;
+                const int j = c + y * kFbP + x;
                 const float dist = sqrtf((float)(x * x + y * y));  // folded per tap
-                float dz = fabsf(G(io.depth_texture)[j] - depth_center);
-                f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
-                float wn = pow_pow2(saturate(dot3(nc, n)), 7);
-                float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
-                f3 dv = S::ld3(io.diff_texture, j);
-                f3 sv = S::ld3(io.spec_texture, j);
-                f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
+                const float dz = fabsf(lds[kGDepth][j] - depth_center);
+                const f3 n = mk3(lds[kGNx][j], lds[kGNy][j], lds[kGNz][j]);
+                const float wn = pow_pow2(saturate(dot3(nc, n)), 7);
+                const float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
                 wsum += wt;
-                dsum = dsum + dv * wt;
-                ssum = ssum + sv * wt;
-                msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
+                dsum = dsum + mk3(lds[kGDr][j], lds[kGDg][j], lds[kGDb][j]) * wt;
+                ssum = ssum + mk3(lds[kGSr][j], lds[kGSg][j], lds[kGSb][j]) * wt;
+                msum = mk2(msum.x + lds[kGMx][j] * wt, msum.y + lds[kGMy][j] * wt);
             }
         }
         const float iw = __builtin_amdgcn_rcpf(wsum);
@@ -432,8 +483,11 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
     S::st2(io.rw_moments_or_variance_texture, i, var);
 }
 
+// 5 waves per SIMD (a register budget under the LDS limit of six 25.6 KB
+// workgroups per CU; at 6 the unrolled 7x7 window spills).
+constexpr int kFallbackWaves = 5;
 template <class S>
-__global__ void __launch_bounds__(64) k_fallback(vgi_denoise_spatial_io io) {
+__global__ void __launch_bounds__(256, kFallbackWaves) k_fallback(vgi_denoise_spatial_io io) {
     k_fallback_body<S>(io, blockIdx.x, blockIdx.y);
 }
 
@@ -691,11 +745,11 @@ vgi_status launch_denoise_disocclusion(const vgi_denoise_spatial_io& io, hipStre
 }
 vgi_status launch_denoise_fallback(const vgi_denoise_spatial_io& io, hipStream_t st) {
     ProfScope prof("denoise_spatial_fallback", st, px_bytes(io.storage, 4, 2, 0, 1) * io.width * io.height);
-    dim3 g = tile_grid(io.width, io.height);
+    const dim3 g((io.width + kFbB - 1) / kFbB, (io.height + kFbB - 1) / kFbB);
     if (io.storage == VGI_STORAGE_F32)
-        hipLaunchKernelGGL(k_fallback<StorageF32>, g, dim3(64), 0, st, io);
+        hipLaunchKernelGGL(k_fallback<StorageF32>, g, dim3(256), 0, st, io);
     else
-        hipLaunchKernelGGL(k_fallback<StorageF16>, g, dim3(64), 0, st, io);
+        hipLaunchKernelGGL(k_fallback<StorageF16>, g, dim3(256), 0, st, io);
     return check_launch("k_fallback");
 }
 vgi_status launch_denoise_fast_clamp(const vgi_denoise_fast_clamp_io& io, hipStream_t st) {
@@ -759,6 +813,10 @@ struct JobDenFallback {
     __device__ static void run(const JobDenFallback& j, uint32_t bx, uint32_t by) { k_fallback_body<S>(j.io, bx, by); }
 };
 template <class S>
+struct BatchMinWaves<JobDenFallback<S>> {
+    static constexpr int value = kFallbackWaves;
+};
+template <class S>
 struct JobDenFastClamp {
     vgi_denoise_fast_clamp_io io;
     __device__ static void run(const JobDenFastClamp& j, uint32_t bx, uint32_t by) {
@@ -800,6 +858,15 @@ static vgi_status fill_den_spatial(const PassRec* const* recs, int n, void* host
     }
     return VGI_OK;
 }
+static vgi_status fill_den_fallback(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobDenFallback>(host);
+    for (int v = 0; v < n; ++v) {
+        const vgi_denoise_spatial_io& io = recs[v]->io.den_spatial;
+        t->jobs[v].io = io;
+        batch_add_tiles(t->hd, v, (io.width + kFbB - 1) / kFbB, (io.height + kFbB - 1) / kFbB);
+    }
+    return VGI_OK;
+}
 static vgi_status fill_den_fast_clamp(const PassRec* const* recs, int n, void* host) {
     auto* t = batch_table<JobDenFastClamp>(host);
     for (int v = 0; v < n; ++v) {
@@ -826,8 +893,8 @@ const BatchOps* batch_ops_denoise(int kind) {
                                    batch_launch<JobDenTemporal>, 64, false};
     static const BatchOps disocclusion{sizeof(BatchTable<JobDenDisocclusion<StorageF32>>),
                                        fill_den_spatial<JobDenDisocclusion>, batch_launch<JobDenDisocclusion>, 64, false};
-    static const BatchOps fallback{sizeof(BatchTable<JobDenFallback<StorageF32>>), fill_den_spatial<JobDenFallback>,
-                                   batch_launch<JobDenFallback>, 64, false};
+    static const BatchOps fallback{sizeof(BatchTable<JobDenFallback<StorageF32>>), fill_den_fallback,
+                                   batch_launch<JobDenFallback, 256, true>, 256, false};
     static const BatchOps fast_clamp{sizeof(BatchTable<JobDenFastClamp<StorageF32>>), fill_den_fast_clamp,
                                      batch_launch<JobDenFastClamp>, 64, false};
     static const BatchOps atrous{sizeof(BatchTable<JobDenAtrous<StorageF32>>), fill_den_atrous,
