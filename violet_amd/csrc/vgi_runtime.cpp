@@ -59,8 +59,11 @@ hipEvent_t take_event() {
         g_prof_pool.pop_back();
         return e;
     }
+    // timing-only events: no system-scope fence (cache writeback and
+    // invalidation) on record, which would otherwise add to the measured
+    // interval and slow the work that follows
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 int name_id(const char* name) {
