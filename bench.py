@@ -438,8 +438,9 @@ def run_serving_form(args, sc, ntris):
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
     reqs = serve.poisson_requests(args.serve_requests, args.serve_rate, len(specs), seed=32)
     log(f"[bench] serving form: {len(reqs)} requests at {args.serve_rate:g}/s over {len(specs)} views")
+    bcfg = batch.BatchConfig(trace_groups=args.serve_trace_groups, use_graph=args.serve_graph)
     srv = serve.Server(sc, specs, hbm_budget_bytes=args.serve_budget_gb * 1e9, config=cfg,
-                       max_batch=args.serve_max_batch, streams=2, batch_config=batch.BatchConfig())
+                       max_batch=args.serve_max_batch, streams=2, batch_config=bcfg)
     admitted = srv.admit_all()
     stats = srv.run(reqs, timeout_s=600.0)
     srv.close()
@@ -449,6 +450,7 @@ def run_serving_form(args, sc, ntris):
             "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "mean_ms": round(s["mean_ms"], 3),
             "mpix_per_s": round(s["mpix_per_s"], 2), "wall_s": round(s["wall_s"], 3), "batches": s["batches"],
             "mean_batch": round(s["mean_batch"], 2), "evictions": s["evictions"],
+            "graph": args.serve_graph, "trace_groups": args.serve_trace_groups,
             "peak_hbm_gb": round(s["peak_hbm_gb"], 2), "admitted_up_front": admitted}
 
 
@@ -460,19 +462,19 @@ def run_precision_form(args, sc, ntris):
     from violet_amd import _lib, renderer, serve
 
     spec = serve.ViewSpec(1920, 1080, 0.0)
-    out = {"workload": "precision_1920x1080_f16", "triangles": ntris, "frames": args.batch_frames}
+    out = {"workload": "precision_1920x1080_f16", "triangles": ntris, "frames": args.precision_frames}
     for name, st in (("f16", _lib.STORAGE_F16), ("f32", _lib.STORAGE_F32)):
         r = renderer.RestirRenderer(1920, 1080, renderer.RestirConfig(storage=st, atrous_iterations=args.iterations,
                                                                       **RESAMPLE[args.resample]))
-        fps = [serve.view_frame_params(spec, f) for f in range(args.batch_frames + 3)]
+        fps = [serve.view_frame_params(spec, f) for f in range(args.precision_frames + 3)]
         for f in range(3):
             r.add_passes(sc, fps[f], f)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for f in range(3, 3 + args.batch_frames):
+        for f in range(3, 3 + args.precision_frames):
             r.add_passes(sc, fps[f], f)
         torch.cuda.synchronize()
-        el = (time.perf_counter() - t0) / args.batch_frames
+        el = (time.perf_counter() - t0) / args.precision_frames
         out[f"{name}_mpix_per_s"] = round(1920 * 1080 / el / 1e6, 2)
         out[f"{name}_ms_per_frame"] = round(el * 1e3, 3)
         r.close()
@@ -509,11 +511,15 @@ def main():
     ap.add_argument("--big-triangles", type=int, default=1_000_000)
     ap.add_argument("--batch-views", type=int, default=16)
     ap.add_argument("--batch-frames", type=int, default=30)
+    ap.add_argument("--precision-frames", type=int, default=120)
     ap.add_argument("--serve-views", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=600.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-max-batch", type=int, default=16)
+    ap.add_argument("--serve-graph", type=int, default=0, help="1: replay each batch slot's frame as a HIP graph")
+    ap.add_argument("--serve-trace-groups", type=int, default=0,
+                    help="persistent groups of the fused trace passes (0 = one per tile)")
     args = ap.parse_args()
     frame = args.workload == "frame"
     # full frame: 1280x720, 60-frame orbit, 3 a-trous iterations (the reference
