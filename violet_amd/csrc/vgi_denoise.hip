@@ -334,13 +334,17 @@ __device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io
                 f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
                 float wn = pow_pow2(saturate(dot3(nc, n)), 5);
                 float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
-                f3 dv = S::ld3(io.diff_texture, j);
-                f3 sv = S::ld3(io.spec_texture, j);
-                f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
-                wsum = mk2(wsum.x + wt, wsum.y + wt);
-                dsum = dsum + dv * wt;
-                ssum = ssum + sv * wt;
-                msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
+                // a zero-weight tap adds exactly nothing to the (finite)
+                // sums: its colours and moments are not fetched
+                if (wt > 0.0f) {
+                    f3 dv = S::ld3(io.diff_texture, j);
+                    f3 sv = S::ld3(io.spec_texture, j);
+                    f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
+                    wsum = mk2(wsum.x + wt, wsum.y + wt);
+                    dsum = dsum + dv * wt;
+                    ssum = ssum + sv * wt;
+                    msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
+                }
             }
         }
         const float iw = __builtin_amdgcn_rcpf(wsum.x);
