@@ -532,7 +532,7 @@ def main():
     args.cpu_height = args.cpu_height or (90 if frame else 360)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
-              "path is described in DESIGN.md and is not built", file=sys.stderr)
+              "path is designed but not built", file=sys.stderr)
         sys.exit(2)
 
     r = run_frame(args) if frame else run_denoise(args)

@@ -1,7 +1,7 @@
 """Shared pytest configuration.
 
 Markers: `gpu` = needs a ROCm GPU and the built HIP library (libvgi.so).
-GPU tests fail loudly (not skip) on a GPU box whose library is missing, so a
+GPU tests fail loudly (not skip) on a machine with a GPU whose library is missing, so a
 silent fallback is impossible.
 """
 from __future__ import annotations
