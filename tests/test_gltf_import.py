@@ -194,3 +194,79 @@ def test_malformed_inputs_are_rejected(doc, tmp_path):
         _imported(p)
     with pytest.raises(_lib.VgiError):
         _imported(tmp_path / "missing.gltf")
+
+
+def test_corrupted_documents_never_crash(doc, tmp_path):
+    """Seeded truncations, byte flips and deleted or retyped fields of the
+    JSON and GLB forms: every import returns a model or a VgiError (a crash
+    would take the test process down)."""
+    from violet_amd import _lib
+
+    d, _, _ = doc
+    rng = np.random.default_rng(99)
+    gl = d.save_gltf(tmp_path / "base.gltf")
+    glb = d.save_glb(tmp_path / "base.glb").read_bytes()
+    text = gl.read_text()
+    j = json.loads(text)
+    outcomes = {"ok": 0, "error": 0}
+
+    def attempt(path):
+        try:
+            _imported(path)
+            outcomes["ok"] += 1
+        except _lib.VgiError:
+            outcomes["error"] += 1
+
+    for k in range(60):
+        p = tmp_path / f"t{k}.gltf"
+        p.write_text(text[: int(rng.integers(0, len(text)))])
+        attempt(p)
+    for k in range(60):
+        b = bytearray(glb)
+        for _ in range(int(rng.integers(1, 8))):
+            b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        p = tmp_path / f"f{k}.glb"
+        p.write_bytes(bytes(b))
+        attempt(p)
+    keys = [("accessors", "count"), ("accessors", "componentType"), ("bufferViews", "byteOffset"),
+            ("bufferViews", "byteLength"), ("bufferViews", "buffer"), ("meshes", "primitives"), ("nodes", "mesh"),
+            ("nodes", "children"), ("nodes", "matrix"), ("accessors", "bufferView")]
+    for k in range(80):
+        jj = json.loads(json.dumps(j))
+        arr, key = keys[int(rng.integers(0, len(keys)))]
+        items = [x for x in jj[arr] if isinstance(x, dict)]
+        tgt = items[int(rng.integers(0, len(items)))]
+        choice = int(rng.integers(0, 4))
+        if choice == 0:
+            tgt.pop(key, None)
+        elif choice == 1:
+            tgt[key] = -1
+        elif choice == 2:
+            tgt[key] = "x"
+        else:
+            tgt[key] = 2 ** 40
+        p = tmp_path / f"m{k}.gltf"
+        p.write_text(json.dumps(jj))
+        attempt(p)
+    assert outcomes["error"] > 100 and sum(outcomes.values()) == 200
+
+
+def test_exponential_node_reuse_is_capped(doc, tmp_path):
+    """A chain of nodes that each list the next one twice as children would
+    instance the leaf 2^40 times: the importer stops at its cap with an
+    error instead of exhausting host memory."""
+    from violet_amd import _lib
+
+    d, _, _ = doc
+    j = json.loads(d.save_gltf(tmp_path / "base.gltf").read_text())
+    base = len(j["nodes"])
+    depth = 40
+    for k in range(depth):
+        j["nodes"].append({"children": [base + k + 1, base + k + 1]})
+    j["nodes"].append({"mesh": 0})
+    j["scenes"] = [{"nodes": [base]}]
+    p = tmp_path / "bomb.gltf"
+    p.write_text(json.dumps(j))
+    with pytest.raises(_lib.VgiError) as e:
+        _imported(p)
+    assert "exceeds" in str(e.value)

@@ -368,6 +368,9 @@ struct vgi_gltf_model {
 
 namespace {
 
+constexpr uint32_t kMaxInstances = 1u << 20;
+constexpr uint32_t kMaxTriangles = 0x7FFFFFFFu / 48u;
+
 struct Importer {
     JVal doc;
     std::vector<std::vector<uint8_t>> buffers;
@@ -748,6 +751,14 @@ struct Importer {
         uint32_t mesh = 0;
         if (as_u32(n.get("mesh"), mesh)) {
             if (mesh >= meshes.size()) return set_error(VGI_E_INVALID, "vgi_gltf_import: node %u: mesh %u", node, mesh);
+            // a hierarchy that reuses nodes can multiply instances without
+            // bound: cap what one scene may bake (the BVH's own limit is
+            // 2^31 / 48 triangles)
+            size_t add = 0;
+            for (const Prim& p : meshes[mesh]) add += p.idx.size() / 3;
+            if (m.stats.instances >= kMaxInstances || m.stats.triangles + add > kMaxTriangles)
+                return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: the scene exceeds %u instances or %u triangles",
+                                 kMaxInstances, kMaxTriangles);
             ++m.stats.instances;
             for (const Prim& p : meshes[mesh]) bake(m, p, x);
         }
