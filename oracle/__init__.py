@@ -302,7 +302,14 @@ class OracleRestir:
         self.sky = sky
 
     def r(self, a):
-        return None if a is None else np.ascontiguousarray(a, dtype=self.rt_dtype)
+        # Texels a pass leaves unwritten (pixels without geometry in the
+        # specular hit-position texture, as in ind_spec_sample_gen.hlsl:37-42)
+        # may hold any bits, signalling NaNs included, whose widening to f64
+        # raises the invalid flag; no pass reads those texels.
+        if a is None:
+            return None
+        with np.errstate(invalid="ignore"):
+            return np.ascontiguousarray(a, dtype=self.rt_dtype)
 
     def trace_res(self, hg_cells, queries, counter, prev_ind_diff=None, prev_depth=None):
         tr = _TraceRes()
