@@ -465,14 +465,33 @@ typedef struct vgi_ind_diff_temporal_io {
     uint32_t* rw_hit_pos_normal_texture;
     void* rw_hit_radiance_texture;
 } vgi_ind_diff_temporal_io;
+/* PushConstants of ind_diff_temporal_resample.hlsl:31-36, field for field (12 B). */
 typedef struct vgi_pc_ind_diff_temporal {
     uint32_t frame_rand;
     uint32_t has_prev_frame;
     uint32_t has_new_sample;
-    uint32_t m_cap;     /* history M clamp; 0 = the reference's M_MAX 16 (:136) */
 } vgi_pc_ind_diff_temporal;
+
+/* Resampling constants the HLSL compiles in, exposed as one optional
+ * argument of the four resampling passes (it is no part of any HLSL
+ * PushConstants).  NULL, or a zero field, keeps the reference constant:
+ *   temporal_m_cap     diffuse history M clamp M_MAX = 16
+ *                      (ind_diff_temporal_resample.hlsl:136); the specular
+ *                      clamp is uint(cap * roughness) (ind_spec_temporal_resample.hlsl:252)
+ *   diff_spatial_taps  golden-spiral taps MAX_ITERATION = 9
+ *                      (ind_diff_spatial_resample.hlsl:12), at most 16
+ *   spatial_radius_px  initial search radius INIT_RADIUS_FACTOR * min(w, h)
+ *                      = 0.05 * min(w, h) (ind_diff_spatial_resample.hlsl:13,
+ *                      ind_spec_spatial_resample.hlsl:13), at most 16384 */
+typedef struct vgi_resample_params {
+    uint32_t temporal_m_cap;
+    uint32_t diff_spatial_taps;
+    float spatial_radius_px;
+} vgi_resample_params;
+
 vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
-                                               const vgi_pc_ind_diff_temporal* pc, void* stream);
+                                               const vgi_pc_ind_diff_temporal* pc,
+                                               const vgi_resample_params* params, void* stream);
 
 /* restir/ind_diff_spatial_resample.hlsl:29-49; restir_renderer.rs:632-652 */
 typedef struct vgi_ind_diff_spatial_io {
@@ -488,18 +507,15 @@ typedef struct vgi_ind_diff_spatial_io {
     void* rw_hit_radiance_texture;
     void* rw_lighting_texture;
 } vgi_ind_diff_spatial_io;
+/* PushConstants {uint frame_rand} of ind_diff_spatial_resample.hlsl:45-49 and
+ * ind_spec_spatial_resample.hlsl:31-35 (4 B). */
 typedef struct vgi_pc_frame_rand {
     uint32_t frame_rand;
 } vgi_pc_frame_rand;
-/* num_taps: golden-spiral taps (MAX_ITERATION, :12; 0 = 9, at most 16);
- * radius_px: initial search radius (0 = INIT_RADIUS_FACTOR 0.05 * min(w, h)). */
-typedef struct vgi_pc_ind_diff_spatial {
-    uint32_t frame_rand;
-    uint32_t num_taps;
-    float radius_px;
-} vgi_pc_ind_diff_spatial;
+typedef vgi_pc_frame_rand vgi_pc_ind_diff_spatial;
 vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
-                                              const vgi_pc_ind_diff_spatial* pc, void* stream);
+                                              const vgi_pc_ind_diff_spatial* pc,
+                                              const vgi_resample_params* params, void* stream);
 
 /* restir/ind_spec_temporal_resample.hlsl:26-44; restir_renderer.rs:743-761
  * (rw_hit_pos/rw_hit_radiance hold the new sample on entry). */
@@ -516,13 +532,14 @@ typedef struct vgi_ind_spec_temporal_io {
     float* rw_hit_pos_texture;              /* float4 */
     void* rw_hit_radiance_texture;
 } vgi_ind_spec_temporal_io;
+/* PushConstants of ind_spec_temporal_resample.hlsl:39-44, field for field (8 B). */
 typedef struct vgi_pc_ind_spec_temporal {
     uint32_t frame_rand;
     uint32_t has_prev_frame;
-    uint32_t m_cap;     /* M clamp is uint(m_cap * roughness); 0 = M_MAX 16 (:252) */
 } vgi_pc_ind_spec_temporal;
 vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params* fp, const vgi_ind_spec_temporal_io* io,
-                                               const vgi_pc_ind_spec_temporal* pc, void* stream);
+                                               const vgi_pc_ind_spec_temporal* pc,
+                                               const vgi_resample_params* params, void* stream);
 
 /* restir/ind_spec_spatial_resample.hlsl:23-35; restir_renderer.rs:770-784 */
 typedef struct vgi_ind_spec_spatial_io {
@@ -535,13 +552,10 @@ typedef struct vgi_ind_spec_spatial_io {
     void* rw_lighting_texture;
     void* rw_ray_len_texture; /* scalar */
 } vgi_ind_spec_spatial_io;
-/* radius_px: search radius (0 = INIT_RADIUS_FACTOR 0.05 * min(w, h), :13). */
-typedef struct vgi_pc_ind_spec_spatial {
-    uint32_t frame_rand;
-    float radius_px;
-} vgi_pc_ind_spec_spatial;
+typedef vgi_pc_frame_rand vgi_pc_ind_spec_spatial;
 vgi_status vgi_pass_ind_spec_spatial_resample(const vgi_frame_params* fp, const vgi_ind_spec_spatial_io* io,
-                                              const vgi_pc_ind_spec_spatial* pc, void* stream);
+                                              const vgi_pc_ind_spec_spatial* pc,
+                                              const vgi_resample_params* params, void* stream);
 
 /* restir/hash_grid_decay.hlsl:5-6; restir_renderer.rs:322-329 */
 typedef struct vgi_hash_grid_decay_io {

@@ -99,6 +99,10 @@ void oracle_denoise_temporal(const float* fpf, int w, int h, const real* depth, 
                 st3(o_diff, i, V3s(0));
                 st3(o_spec, i, V3s(0));
                 st2(o_mom, i, V2(0, 0));
+                if (fast_history) {
+                    st3(o_diff_fast, i, V3s(0));
+                    st3(o_spec_fast, i, V3s(0));
+                }
                 continue;
             }
             real sux = ((real)px + RC(0.5)) / bsx, suy = ((real)py + RC(0.5)) / bsy;

@@ -41,8 +41,39 @@ def _mirrors():
         "vgi_raytraced_shadow_io": _abi.RaytracedShadowIO, "vgi_final_lighting_io": _abi.FinalLightingIO,
         "vgi_temporal_aa_io": _abi.TemporalAAIO, "vgi_pc_temporal_aa": _abi.PCTemporalAA,
         "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
+        "vgi_resample_params": _abi.ResampleParams,
     }
     return m
+
+
+# The byte size of every HLSL PushConstants block a pass entry point takes
+# (4-byte fields, PushConstantsBuilder shader.rs:1436-1449): a violet caller
+# passes exactly these bytes.
+HLSL_PUSH_CONSTANTS = {
+    "vgi_pc_frame_hash": 4,         # hash_grid_query_select.hlsl:16-20, hash_grid_raygen.hlsl:19-22
+    "vgi_pc_sample_gen": 8,         # ind_diff_sample_gen.hlsl:17-23, ind_spec_sample_gen.hlsl:21-26
+    "vgi_pc_ao_gen": 12,            # raytraced_ao_gen.hlsl:10-16
+    "vgi_pc_ind_diff_temporal": 12,  # ind_diff_temporal_resample.hlsl:31-36
+    "vgi_pc_ind_diff_spatial": 4,   # ind_diff_spatial_resample.hlsl:45-49
+    "vgi_pc_ind_spec_temporal": 8,  # ind_spec_temporal_resample.hlsl:39-44
+    "vgi_pc_ind_spec_spatial": 4,   # ind_spec_spatial_resample.hlsl:31-35
+    "vgi_pc_denoise_temporal": 16,  # svgf_temporal_filter.hlsl:41-47
+    "vgi_pc_denoise_atrous": 4,     # svgf_atrous_filter.hlsl:17-21
+}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_push_constants_are_the_hlsl_size(tmp_path):
+    lines = ["#include <stdio.h>", '#include "vgi.h"', "int main(void) {"]
+    for cname in HLSL_PUSH_CONSTANTS:
+        lines.append(f'    printf("%zu\\n", sizeof({cname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "pc.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "pc"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert dict(zip(HLSL_PUSH_CONSTANTS, got)) == HLSL_PUSH_CONSTANTS
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")

@@ -140,8 +140,11 @@ def test_denoise_passes_vs_oracle(vgi, oracle64, oracle32, w, h):
         g.keep.clear()
 
 
-def test_denoiser_sequence_vs_oracle(vgi):
-    """Denoiser (vgi_denoiser_add_passes) over an 8-frame pan vs the FP64 oracle sequencer."""
+@pytest.mark.parametrize("fast_history_clamp", [False, True], ids=["no_fast_clamp", "fast_clamp"])
+def test_denoiser_sequence_vs_oracle(vgi, fast_history_clamp):
+    """Denoiser (vgi_denoiser_add_passes) over an 8-frame pan vs the FP64
+    oracle sequencer, with and without the fast-history clamp
+    (relax_fast_history_clamp.hlsl:12-75, denoising.rs:192-207)."""
     import oracle as O
 
     from violet_amd import denoiser as D
@@ -151,7 +154,8 @@ def test_denoiser_sequence_vs_oracle(vgi):
     seq = _seq(w, h, 8, seed=3)
     den = D.Denoiser(w, h)
     den.atrous_iterations = 5
-    ref = O.OracleDenoiser(64, atrous_iterations=5)
+    den.fast_history_clamp = fast_history_clamp
+    ref = O.OracleDenoiser(64, atrous_iterations=5, fast_history_clamp=fast_history_clamp)
     prev = None
     prev_t = None
     for fi, f in enumerate(seq):
@@ -169,6 +173,40 @@ def test_denoiser_sequence_vs_oracle(vgi):
         assert_close(gs, r["specular"], f"sequence specular frame {fi}")
         assert np.mean(gl != r["history_len"]) < 1e-3
         prev, prev_t = f, t
+
+
+@pytest.mark.parametrize("storage", [0, 1], ids=["f32", "f16"])
+def test_fast_history_clamp_pass_vs_oracle(vgi, oracle64, storage):
+    """vgi_pass_denoise_fast_history_clamp on its own: the colour is clamped
+    into the fast history's 3x3 YCoCg mean +- 0.75 sigma box
+    (relax_fast_history_clamp.hlsl:12-75).  Inputs are a smooth fast history
+    and a noisy slow colour, so most pixels move.  FP32 and FP16 storage vs
+    the FP64 oracle on the storage-rounded inputs."""
+    from violet_amd import _lib
+
+    w, h = 173, 97
+    rng = np.random.default_rng(11)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = np.stack([np.sin(xx / 9.0) + 1.5, np.cos(yy / 7.0) + 1.5, np.sin((xx + yy) / 13.0) + 1.5], -1)
+    fast_d = base * (1.0 + 0.05 * rng.standard_normal((h, w, 3)))
+    fast_s = 0.5 * base[..., ::-1] * (1.0 + 0.05 * rng.standard_normal((h, w, 3)))
+    diff = base * rng.exponential(1.0, (h, w, 3))
+    spec = 0.5 * base[..., ::-1] * rng.exponential(1.0, (h, w, 3))
+    dt = np.float16 if storage else np.float32
+    fast_d, fast_s, diff, spec = (a.astype(dt) for a in (fast_d, fast_s, diff, spec))
+    g = Pass(storage)
+    od, osp = g.out((h, w, 3), np.float32), g.out((h, w, 3), np.float32)
+    od[0].copy_(g.torch.from_numpy(np.ascontiguousarray(diff).view(np.uint8).ravel()).cuda())
+    osp[0].copy_(g.torch.from_numpy(np.ascontiguousarray(spec).view(np.uint8).ravel()).cuda())
+    io = _lib.DenoiseFastClampIO(w, h, storage, g.d(fast_d), g.d(fast_s), od[0].data_ptr(), osp[0].data_ptr())
+    _lib.call("vgi_pass_denoise_fast_history_clamp", C.byref(io), g.gpu.stream())
+    gd, gs = g.get(od), g.get(osp)
+    rd, rs = oracle64.denoise_fast_clamp(fast_d, fast_s, diff, spec)
+    assert np.mean(np.abs(rd - diff.astype(np.float64)) > 1e-3) > 0.3, "the clamp moved too few pixels"
+    # f16 stores round each output once (half an ulp, 4.9e-4 relative)
+    tol = dict(rel_tol=1e-3, pix_tol=1e-2) if storage == 0 else dict(rel_tol=1e-3, pix_tol=1.5e-3)
+    assert_close(gd, rd, "fast clamp diffuse", **tol)
+    assert_close(gs, rs, "fast clamp specular", **tol)
 
 
 def test_denoise_invalid_arguments_do_not_launch(vgi):

@@ -43,20 +43,21 @@ DiffTemporalIO = _struct("DiffTemporalIO", WH + [
     ("new_hit_radiance_texture", vp), ("prev_reservoir_texture", vp), ("prev_hit_pos_normal_texture", vp),
     ("prev_hit_radiance_texture", vp), ("rw_reservoir_texture", vp), ("rw_hit_pos_normal_texture", vp),
     ("rw_hit_radiance_texture", vp)])
-PCDiffTemporal = _struct("PCDiffTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("has_new_sample", u32),
-                                            ("m_cap", u32)])
+PCDiffTemporal = _struct("PCDiffTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("has_new_sample", u32)])
+ResampleParams = _struct("ResampleParams", [("temporal_m_cap", u32), ("diff_spatial_taps", u32),
+                                            ("spatial_radius_px", f32)])
 DiffSpatialIO = _struct("DiffSpatialIO", WH + [
     ("gbuffer_color", vp), ("gbuffer_depth", vp), ("ao_texture", vp), ("temp_reservoir_texture", vp),
     ("temp_hit_pos_normal_texture", vp), ("temp_hit_radiance_texture", vp), ("rw_reservoir_texture", vp),
     ("rw_hit_pos_normal_texture", vp), ("rw_hit_radiance_texture", vp), ("rw_lighting_texture", vp)])
 PCFrameRand = _struct("PCFrameRand", [("frame_rand", u32)])
-PCDiffSpatial = _struct("PCDiffSpatial", [("frame_rand", u32), ("num_taps", u32), ("radius_px", f32)])
-PCSpecSpatial = _struct("PCSpecSpatial", [("frame_rand", u32), ("radius_px", f32)])
+PCDiffSpatial = PCFrameRand
+PCSpecSpatial = PCFrameRand
 SpecTemporalIO = _struct("SpecTemporalIO", WH + [
     ("prev_gbuffer_color", vp), ("prev_gbuffer_depth", vp), ("gbuffer_color", vp), ("gbuffer_depth", vp),
     ("prev_reservoir_texture", vp), ("prev_hit_pos_texture", vp), ("prev_hit_radiance_texture", vp),
     ("rw_reservoir_texture", vp), ("rw_hit_pos_texture", vp), ("rw_hit_radiance_texture", vp)])
-PCSpecTemporal = _struct("PCSpecTemporal", [("frame_rand", u32), ("has_prev_frame", u32), ("m_cap", u32)])
+PCSpecTemporal = _struct("PCSpecTemporal", [("frame_rand", u32), ("has_prev_frame", u32)])
 SpecSpatialIO = _struct("SpecSpatialIO", WH + [
     ("gbuffer_color", vp), ("gbuffer_depth", vp), ("reservoir_texture", vp), ("hit_pos_texture", vp),
     ("hit_radiance_texture", vp), ("rw_lighting_texture", vp), ("rw_ray_len_texture", vp)])
@@ -102,10 +103,10 @@ _SIGS = {
     "vgi_pass_ind_spec_sample_validate": [FPp, TRp, P(SpecSampleValidateIO), vp],
     "vgi_pass_raytraced_ao_gen": [FPp, P(AOGenIO), P(PCAOGen), vp],
     "vgi_pass_raytraced_ao_filter": [P(AOFilterIO), vp],
-    "vgi_pass_ind_diff_temporal_resample": [FPp, P(DiffTemporalIO), P(PCDiffTemporal), vp],
-    "vgi_pass_ind_diff_spatial_resample": [FPp, P(DiffSpatialIO), P(PCDiffSpatial), vp],
-    "vgi_pass_ind_spec_temporal_resample": [FPp, P(SpecTemporalIO), P(PCSpecTemporal), vp],
-    "vgi_pass_ind_spec_spatial_resample": [FPp, P(SpecSpatialIO), P(PCSpecSpatial), vp],
+    "vgi_pass_ind_diff_temporal_resample": [FPp, P(DiffTemporalIO), P(PCDiffTemporal), P(ResampleParams), vp],
+    "vgi_pass_ind_diff_spatial_resample": [FPp, P(DiffSpatialIO), P(PCDiffSpatial), P(ResampleParams), vp],
+    "vgi_pass_ind_spec_temporal_resample": [FPp, P(SpecTemporalIO), P(PCSpecTemporal), P(ResampleParams), vp],
+    "vgi_pass_ind_spec_spatial_resample": [FPp, P(SpecSpatialIO), P(PCSpecSpatial), P(ResampleParams), vp],
     "vgi_pass_hash_grid_decay": [P(HGDecayIO), vp],
     "vgi_pass_hash_grid_query_select": [P(HGSelectIO), P(PCFrameHash), vp],
     "vgi_pass_hash_grid_query_compact": [P(HGCompactIO), vp],

@@ -100,10 +100,16 @@ __device__ __forceinline__ void k_temporal_body(const FrameParams& fp, const vgi
 
     float depth = G(io.depth_texture)[i];
     if (depth == 0.0f) {
+        // (the reference leaves these texels unwritten; zeros keep the fast
+        // history clamp's 3x3 box at the sky's edge defined)
         G(io.rw_history_len_texture)[i] = 0;
         S::st3(io.rw_diff_texture, i, mk3s(0.0f));
         S::st3(io.rw_spec_texture, i, mk3s(0.0f));
         st_moments<S>(io.rw_moments_texture, io.rw_moments_comp_texture, i, mk2(0.0f, 0.0f));
+        if (pc.fast_history) {
+            S::st3(io.rw_diff_fast_history_texture, i, mk3s(0.0f));
+            S::st3(io.rw_spec_fast_history_texture, i, mk3s(0.0f));
+        }
         return;
     }
 

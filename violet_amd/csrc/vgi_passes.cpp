@@ -65,15 +65,15 @@ vgi_status DirectSink::launch(const PassRec& r, hipStream_t st) {
         case kPassAoGen: return vgi_pass_raytraced_ao_gen(&r.fp, &r.io.ao_gen, &r.pc.ao_gen, s);
         case kPassAoFilter: return vgi_pass_raytraced_ao_filter(&r.io.ao_filter, s);
         case kPassDiffTemporal:
-            return vgi_pass_ind_diff_temporal_resample(&r.fp, &r.io.diff_temporal, &r.pc.diff_temporal, s);
+            return vgi_pass_ind_diff_temporal_resample(&r.fp, &r.io.diff_temporal, &r.pc.diff_temporal, &r.resample, s);
         case kPassDiffSpatial:
-            return vgi_pass_ind_diff_spatial_resample(&r.fp, &r.io.diff_spatial, &r.pc.diff_spatial, s);
+            return vgi_pass_ind_diff_spatial_resample(&r.fp, &r.io.diff_spatial, &r.pc.diff_spatial, &r.resample, s);
         case kPassSpecGen: return vgi_pass_ind_spec_sample_gen(&r.fp, &r.tr, &r.io.spec_gen, &r.pc.sample_gen, s);
         case kPassSpecValidate: return vgi_pass_ind_spec_sample_validate(&r.fp, &r.tr, &r.io.spec_validate, s);
         case kPassSpecTemporal:
-            return vgi_pass_ind_spec_temporal_resample(&r.fp, &r.io.spec_temporal, &r.pc.spec_temporal, s);
+            return vgi_pass_ind_spec_temporal_resample(&r.fp, &r.io.spec_temporal, &r.pc.spec_temporal, &r.resample, s);
         case kPassSpecSpatial:
-            return vgi_pass_ind_spec_spatial_resample(&r.fp, &r.io.spec_spatial, &r.pc.spec_spatial, s);
+            return vgi_pass_ind_spec_spatial_resample(&r.fp, &r.io.spec_spatial, &r.pc.spec_spatial, &r.resample, s);
         case kPassDenTemporal: return vgi_pass_denoise_temporal(&r.fp, &r.io.den_temporal, &r.pc.den_temporal, s);
         case kPassDenDisocclusion: return vgi_pass_denoise_disocclusion_fix(&r.io.den_spatial, s);
         case kPassDenFastClamp: return vgi_pass_denoise_fast_history_clamp(&r.io.den_fast_clamp, s);

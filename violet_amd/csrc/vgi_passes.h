@@ -132,6 +132,7 @@ struct PassRec {
         vgi_pc_denoise_atrous den_atrous;
         vgi_pc_temporal_aa taa;
     } pc;
+    vgi_resample_params resample;  // the four resampling passes (zero = reference constants)
     void* clear_ptr;     // kPassClear
     size_t clear_bytes;  // kPassClear
 };

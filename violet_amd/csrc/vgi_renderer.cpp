@@ -146,6 +146,7 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
         vgi_renderer_default_config(&c);
     VGI_REQUIRE(c.storage == VGI_STORAGE_F32 || c.storage == VGI_STORAGE_F16, "vgi_renderer_create: bad storage");
     VGI_REQUIRE(c.diff_spatial_taps <= 16u, "vgi_renderer_create: diff_spatial_taps %u > 16", c.diff_spatial_taps);
+    VGI_REQUIRE(c.temporal_m_cap <= 255u, "vgi_renderer_create: temporal_m_cap %u > 255", c.temporal_m_cap);
     VGI_REQUIRE(c.spatial_radius_px >= 0.0f && c.spatial_radius_px <= 16384.0f, "vgi_renderer_create: bad spatial_radius_px");
     VGI_REQUIRE(!c.taa || c.compose, "vgi_renderer_create: taa needs compose");
     vgi_renderer* r = new (std::nothrow) vgi_renderer();
@@ -234,6 +235,10 @@ extern "C" vgi_status vgi_renderer_reset(vgi_renderer* r) {
         vgi_status _s = (x);              \
         if (_s != VGI_OK) return _s;      \
     } while (0)
+
+static vgi_resample_params resample_params(const vgi_renderer* r) {
+    return vgi_resample_params{r->cfg.temporal_m_cap, r->cfg.diff_spatial_taps, r->cfg.spatial_radius_px};
+}
 
 // One frame of restir_renderer.rs:258-829 issued to `sink` (host state -
 // history flags, ping-pong indices, FrameRand chain - advances as the passes
@@ -355,14 +360,16 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         // has_prev_frame = has_prev_diffuse_reservoir (restir_renderer.rs:605); the
         // previous depth exists whenever the previous reservoir does
         t.pc.diff_temporal = vgi_pc_ind_diff_temporal{frame_rand_next(frame_rand), (has_prev_diff_rsv && has_prev_depth) ? 1u : 0u,
-                                    has_new_sample, r->cfg.temporal_m_cap};
+                                    has_new_sample};
+        t.resample = resample_params(r);
         CALL(sink.submit(kSlotDiffTemporal, t));
         PassRec sp = make_rec(kPassDiffSpatial, fp);
         sp.io.diff_spatial = vgi_ind_diff_spatial_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(), r->ao_filtered.p,
                                     r->dtm_rsv.as<uint32_t>(), r->dtm_hit.as<uint32_t>(), r->dtm_rad.p,
                                     r->dsp_rsv[c].as<uint32_t>(), r->dsp_hit[c].as<uint32_t>(), r->dsp_rad[c].p,
                                     r->diff_light[c].p};
-        sp.pc.diff_spatial = vgi_pc_ind_diff_spatial{frame_rand_next(frame_rand), r->cfg.diff_spatial_taps, r->cfg.spatial_radius_px};
+        sp.pc.diff_spatial = vgi_pc_ind_diff_spatial{frame_rand_next(frame_rand)};
+        sp.resample = resample_params(r);
         CALL(sink.submit(kSlotDiffSpatial, sp));
     }
 
@@ -383,13 +390,15 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
                                     r->depth[c].as<float>(), hp ? r->spec_rsv[p].as<uint32_t>() : nullptr,
                                     hp ? r->spec_hitpos[p].as<float>() : nullptr, hp ? r->spec_rad[p].p : nullptr,
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p};
-        t.pc.spec_temporal = vgi_pc_ind_spec_temporal{frame_rand_next(frame_rand), hp ? 1u : 0u, r->cfg.temporal_m_cap};
+        t.pc.spec_temporal = vgi_pc_ind_spec_temporal{frame_rand_next(frame_rand), hp ? 1u : 0u};
+        t.resample = resample_params(r);
         CALL(sink.submit(kSlotSpecTemporal, t));
         PassRec sp = make_rec(kPassSpecSpatial, fp);
         sp.io.spec_spatial = vgi_ind_spec_spatial_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(),
                                     r->spec_rsv[c].as<uint32_t>(), r->spec_hitpos[c].as<float>(), r->spec_rad[c].p,
                                     r->spec_light.p, r->spec_raylen.p};
-        sp.pc.spec_spatial = vgi_pc_ind_spec_spatial{frame_rand_next(frame_rand), r->cfg.spatial_radius_px};
+        sp.pc.spec_spatial = vgi_pc_ind_spec_spatial{frame_rand_next(frame_rand)};
+        sp.resample = resample_params(r);
         CALL(sink.submit(kSlotSpecSpatial, sp));
     }
 

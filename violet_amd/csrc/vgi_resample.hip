@@ -151,8 +151,18 @@ __global__ void __launch_bounds__(64) k_ao_filter(vgi_ao_filter_io io) {
 // ---------------------------------------------------------------------------
 // Indirect diffuse temporal resampling
 // ---------------------------------------------------------------------------
+
+// Kernel arguments of the two temporal passes: the HLSL push constants plus
+// the history M clamp, resolved on the host (M_MAX 16 unless configured).
+struct DiffTemporalArgs {
+    uint32_t frame_rand, has_prev_frame, has_new_sample, m_cap;
+};
+struct SpecTemporalArgs {
+    uint32_t frame_rand, has_prev_frame, m_cap;
+};
+
 template <class S>
-__device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, const vgi_ind_diff_temporal_io& io, const vgi_pc_ind_diff_temporal& pc, uint32_t bx_, uint32_t by_) {
+__device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, const vgi_ind_diff_temporal_io& io, const DiffTemporalArgs& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
     tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
@@ -188,8 +198,7 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
             rad = mk3s(0.0f);
             ex = ey = 0u;
         }
-        const uint32_t m_cap = pc.m_cap ? pc.m_cap : 16u;  // M_MAX
-        M = M < m_cap ? M : m_cap;
+        M = M < pc.m_cap ? M : pc.m_cap;
     }
     if (pc.has_new_sample) {
         float target = luminance(rad);
@@ -219,7 +228,7 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
 }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_diff_temporal_io io, vgi_pc_ind_diff_temporal pc) {
+__global__ void __launch_bounds__(64) k_diff_temporal(FrameParams fp, vgi_ind_diff_temporal_io io, DiffTemporalArgs pc) {
     k_diff_temporal_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
 }
 
@@ -393,7 +402,7 @@ __device__ __forceinline__ GB load_gb(const uint32_t* g, size_t plane, size_t i)
 }
 
 template <class S>
-__device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, const vgi_ind_spec_temporal_io& io, const vgi_pc_ind_spec_temporal& pc, uint32_t bx_, uint32_t by_) {
+__device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, const vgi_ind_spec_temporal_io& io, const SpecTemporalArgs& pc, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
     tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
@@ -439,7 +448,7 @@ __device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, cons
     f3 new_pos = mk3(nhp[0], nhp[1], nhp[2]);
     f3 new_rad = S::ld3(io.rw_hit_radiance_texture, i);
     GB g = load_gb(io.gbuffer_color, plane, i);
-    uint32_t M_max = f2u((float)(pc.m_cap ? pc.m_cap : 16u) * g.roughness);
+    uint32_t M_max = f2u((float)pc.m_cap * g.roughness);
     M = M < M_max ? M : M_max;
     f3 f0 = get_specular_f0(g.color, g.metallic);
     SpecSampling cur = calc_things(xyz(fp.view_pos), position, g.normal, g.roughness, f0, new_pos);
@@ -486,7 +495,7 @@ __device__ __forceinline__ void k_spec_temporal_body(const FrameParams& fp, cons
 }
 
 template <class S>
-__global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_spec_temporal_io io, vgi_pc_ind_spec_temporal pc) {
+__global__ void __launch_bounds__(64) k_spec_temporal(FrameParams fp, vgi_ind_spec_temporal_io io, SpecTemporalArgs pc) {
     k_spec_temporal_body<S>(fp, io, pc, blockIdx.x, blockIdx.y);
 }
 
@@ -630,8 +639,18 @@ SpiralTable make_spiral_table(int n) {
 }
 
 // Initial spatial search radius: INIT_RADIUS_FACTOR * min(w, h) unless set.
-static float spatial_radius(float radius_px, uint32_t w, uint32_t h) {
-    return radius_px > 0.0f ? radius_px : (float)(w < h ? w : h) * 0.05f;
+static float spatial_radius(const vgi_resample_params* p, uint32_t w, uint32_t h) {
+    return p && p->spatial_radius_px > 0.0f ? p->spatial_radius_px : (float)(w < h ? w : h) * 0.05f;
+}
+static uint32_t temporal_m_cap(const vgi_resample_params* p) { return p && p->temporal_m_cap ? p->temporal_m_cap : 16u; }
+static int spiral_taps(const vgi_resample_params* p) { return p && p->diff_spatial_taps ? (int)p->diff_spatial_taps : 9; }
+static vgi_status check_resample_params(const vgi_resample_params* p, const char* who) {
+    if (!p) return VGI_OK;
+    VGI_REQUIRE(p->diff_spatial_taps <= (uint32_t)kMaxSpiralTaps, "%s: diff_spatial_taps %u > %d", who,
+                p->diff_spatial_taps, kMaxSpiralTaps);
+    VGI_REQUIRE(p->spatial_radius_px >= 0.0f && p->spatial_radius_px <= 16384.0f, "%s: bad spatial_radius_px", who);
+    VGI_REQUIRE(p->temporal_m_cap <= 255u, "%s: temporal_m_cap %u > 255", who, p->temporal_m_cap);
+    return VGI_OK;
 }
 
 #define VGI_LAUNCH_S(mode, kernel, grid, ...)                                             \
@@ -663,7 +682,7 @@ template <class S>
 struct JobDiffTemporal {
     FrameParams fp;
     vgi_ind_diff_temporal_io io;
-    vgi_pc_ind_diff_temporal pc;
+    DiffTemporalArgs pc;
     __device__ static void run(const JobDiffTemporal& j, uint32_t bx, uint32_t by) {
         k_diff_temporal_body<S>(j.fp, j.io, j.pc, bx, by);
     }
@@ -683,7 +702,7 @@ template <class S>
 struct JobSpecTemporal {
     FrameParams fp;
     vgi_ind_spec_temporal_io io;
-    vgi_pc_ind_spec_temporal pc;
+    SpecTemporalArgs pc;
     __device__ static void run(const JobSpecTemporal& j, uint32_t bx, uint32_t by) {
         k_spec_temporal_body<S>(j.fp, j.io, j.pc, bx, by);
     }
@@ -721,7 +740,9 @@ static vgi_status fill_diff_temporal(const PassRec* const* recs, int n, void* ho
     auto* t = batch_table<JobDiffTemporal>(host);
     for (int v = 0; v < n; ++v) {
         const PassRec& r = *recs[v];
-        t->jobs[v] = {to_fp(&r.fp), r.io.diff_temporal, r.pc.diff_temporal};
+        const vgi_pc_ind_diff_temporal& pc = r.pc.diff_temporal;
+        t->jobs[v] = {to_fp(&r.fp), r.io.diff_temporal,
+                      {pc.frame_rand, pc.has_prev_frame, pc.has_new_sample, temporal_m_cap(&r.resample)}};
         batch_add_pixel_tiles(t->hd, v, r.io.diff_temporal.width, r.io.diff_temporal.height);
     }
     return VGI_OK;
@@ -730,11 +751,10 @@ static vgi_status fill_diff_spatial(const PassRec* const* recs, int n, void* hos
     auto* t = batch_table<JobDiffSpatial>(host);
     for (int v = 0; v < n; ++v) {
         const PassRec& r = *recs[v];
-        const vgi_pc_ind_diff_spatial& pc = r.pc.diff_spatial;
-        VGI_REQUIRE(pc.num_taps <= (uint32_t)kMaxSpiralTaps, "batched diffuse spatial: num_taps %u", pc.num_taps);
-        t->jobs[v] = {to_fp(&r.fp), r.io.diff_spatial, pc.frame_rand,
-                      spatial_radius(pc.radius_px, r.io.diff_spatial.width, r.io.diff_spatial.height),
-                      make_spiral_table(pc.num_taps ? (int)pc.num_taps : 9)};
+        VGI_CHECK(check_resample_params(&r.resample, "batched diffuse spatial"));
+        t->jobs[v] = {to_fp(&r.fp), r.io.diff_spatial, r.pc.diff_spatial.frame_rand,
+                      spatial_radius(&r.resample, r.io.diff_spatial.width, r.io.diff_spatial.height),
+                      make_spiral_table(spiral_taps(&r.resample))};
         batch_add_pixel_tiles(t->hd, v, r.io.diff_spatial.width, r.io.diff_spatial.height);
     }
     return VGI_OK;
@@ -743,7 +763,9 @@ static vgi_status fill_spec_temporal(const PassRec* const* recs, int n, void* ho
     auto* t = batch_table<JobSpecTemporal>(host);
     for (int v = 0; v < n; ++v) {
         const PassRec& r = *recs[v];
-        t->jobs[v] = {to_fp(&r.fp), r.io.spec_temporal, r.pc.spec_temporal};
+        const vgi_pc_ind_spec_temporal& pc = r.pc.spec_temporal;
+        t->jobs[v] = {to_fp(&r.fp), r.io.spec_temporal,
+                      {pc.frame_rand, pc.has_prev_frame, temporal_m_cap(&r.resample)}};
         batch_add_pixel_tiles(t->hd, v, r.io.spec_temporal.width, r.io.spec_temporal.height);
     }
     return VGI_OK;
@@ -754,7 +776,7 @@ static vgi_status fill_spec_spatial(const PassRec* const* recs, int n, void* hos
     for (int v = 0; v < n; ++v) {
         const PassRec& r = *recs[v];
         t->jobs[v] = {to_fp(&r.fp), r.io.spec_spatial, r.pc.spec_spatial.frame_rand,
-                      spatial_radius(r.pc.spec_spatial.radius_px, r.io.spec_spatial.width, r.io.spec_spatial.height),
+                      spatial_radius(&r.resample, r.io.spec_spatial.width, r.io.spec_spatial.height),
                       tab.cos_tol};
         batch_add_pixel_tiles(t->hd, v, r.io.spec_spatial.width, r.io.spec_spatial.height);
     }
@@ -814,8 +836,10 @@ extern "C" vgi_status vgi_pass_raytraced_ao_filter(const vgi_ao_filter_io* io, v
 }
 
 extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
-                                                          const vgi_pc_ind_diff_temporal* pc, void* stream) {
+                                                          const vgi_pc_ind_diff_temporal* pc,
+                                                          const vgi_resample_params* params, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_temporal_resample: null argument");
+    VGI_CHECK(check_resample_params(params, "vgi_pass_ind_diff_temporal_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_diff_temporal_resample: bad size/storage");
     VGI_REQUIRE(io->gbuffer_depth && io->rw_reservoir_texture && io->rw_hit_pos_normal_texture && io->rw_hit_radiance_texture,
@@ -828,36 +852,37 @@ extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params
     hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_diff_temporal_resample", st, (double)io->width * io->height * (4 + 4 + 3 * (4 + 8 + R) - 4));
-    VGI_LAUNCH_S(io->storage, k_diff_temporal, tile_grid(io->width, io->height), to_fp(fp), *io, *pc);
+    const DiffTemporalArgs args{pc->frame_rand, pc->has_prev_frame, pc->has_new_sample, temporal_m_cap(params)};
+    VGI_LAUNCH_S(io->storage, k_diff_temporal, tile_grid(io->width, io->height), to_fp(fp), *io, args);
     return check_launch("k_diff_temporal");
 }
 
 extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
-                                                         const vgi_pc_ind_diff_spatial* pc, void* stream) {
+                                                         const vgi_pc_ind_diff_spatial* pc,
+                                                         const vgi_resample_params* params, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_spatial_resample: null argument");
+    VGI_CHECK(check_resample_params(params, "vgi_pass_ind_diff_spatial_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_diff_spatial_resample: bad size/storage");
     VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->ao_texture && io->temp_reservoir_texture &&
                     io->temp_hit_pos_normal_texture && io->temp_hit_radiance_texture && io->rw_reservoir_texture &&
                     io->rw_hit_pos_normal_texture && io->rw_hit_radiance_texture && io->rw_lighting_texture,
                 "vgi_pass_ind_diff_spatial_resample: missing binding");
-    VGI_REQUIRE(pc->num_taps <= (uint32_t)kMaxSpiralTaps, "vgi_pass_ind_diff_spatial_resample: num_taps %u > %d",
-                pc->num_taps, kMaxSpiralTaps);
-    VGI_REQUIRE(pc->radius_px >= 0.0f && pc->radius_px <= 16384.0f,
-                "vgi_pass_ind_diff_spatial_resample: bad radius_px");
-    const SpiralTable tab = make_spiral_table(pc->num_taps ? (int)pc->num_taps : 9);
+    const SpiralTable tab = make_spiral_table(spiral_taps(params));
     hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_diff_spatial_resample", st,
                    (double)io->width * io->height * (4 + 4 + pair_bytes(io->storage) / 2 + 2 * (4 + 8 + R) + R));
     VGI_LAUNCH_S(io->storage, k_diff_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand,
-                 spatial_radius(pc->radius_px, io->width, io->height), tab);
+                 spatial_radius(params, io->width, io->height), tab);
     return check_launch("k_diff_spatial");
 }
 
 extern "C" vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params* fp, const vgi_ind_spec_temporal_io* io,
-                                                          const vgi_pc_ind_spec_temporal* pc, void* stream) {
+                                                          const vgi_pc_ind_spec_temporal* pc,
+                                                          const vgi_resample_params* params, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_spec_temporal_resample: null argument");
+    VGI_CHECK(check_resample_params(params, "vgi_pass_ind_spec_temporal_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_spec_temporal_resample: bad size/storage");
     VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->rw_reservoir_texture && io->rw_hit_pos_texture &&
@@ -870,26 +895,27 @@ extern "C" vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_spec_temporal_resample", st,
                    (double)io->width * io->height * (4 + 4 + 8 + 4 + 8 + 8 + 16 + R + 16 + R + 8 + 16 + R));
-    VGI_LAUNCH_S(io->storage, k_spec_temporal, tile_grid(io->width, io->height), to_fp(fp), *io, *pc);
+    const SpecTemporalArgs args{pc->frame_rand, pc->has_prev_frame, temporal_m_cap(params)};
+    VGI_LAUNCH_S(io->storage, k_spec_temporal, tile_grid(io->width, io->height), to_fp(fp), *io, args);
     return check_launch("k_spec_temporal");
 }
 
 extern "C" vgi_status vgi_pass_ind_spec_spatial_resample(const vgi_frame_params* fp, const vgi_ind_spec_spatial_io* io,
-                                                         const vgi_pc_ind_spec_spatial* pc, void* stream) {
+                                                         const vgi_pc_ind_spec_spatial* pc,
+                                                         const vgi_resample_params* params, void* stream) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_spec_spatial_resample: null argument");
+    VGI_CHECK(check_resample_params(params, "vgi_pass_ind_spec_spatial_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_spec_spatial_resample: bad size/storage");
     VGI_REQUIRE(io->gbuffer_color && io->gbuffer_depth && io->reservoir_texture && io->hit_pos_texture &&
                     io->hit_radiance_texture && io->rw_lighting_texture && io->rw_ray_len_texture,
                 "vgi_pass_ind_spec_spatial_resample: missing binding");
-    VGI_REQUIRE(pc->radius_px >= 0.0f && pc->radius_px <= 16384.0f,
-                "vgi_pass_ind_spec_spatial_resample: bad radius_px");
     static const SpiralTable tab = make_spiral_table(9);
     hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
     ProfScope prof("ind_spec_spatial_resample", st,
                    (double)io->width * io->height * (4 + 8 + 8 + 16 + R + R + scalar_bytes(io->storage)));
     VGI_LAUNCH_S(io->storage, k_spec_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand,
-                 spatial_radius(pc->radius_px, io->width, io->height), tab.cos_tol);
+                 spatial_radius(params, io->width, io->height), tab.cos_tol);
     return check_launch("k_spec_spatial");
 }

@@ -28,6 +28,13 @@ vgi_status check_hip(hipError_t e, const char* what);
         if (!(cond)) return ::vgi::set_error(VGI_E_INVALID, __VA_ARGS__); \
     } while (0)
 
+// Propagates a non-OK status (the callee has already set the message).
+#define VGI_CHECK(expr)                          \
+    do {                                         \
+        vgi_status _s = (expr);                  \
+        if (_s != VGI_OK) return _s;             \
+    } while (0)
+
 // Launch-after check: reports a launch configuration error synchronously.
 vgi_status check_launch(const char* kernel);
 
