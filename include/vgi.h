@@ -259,6 +259,64 @@ vgi_status vgi_denoiser_add_passes(vgi_denoiser* d, const vgi_frame_params* fp, 
                                    vgi_denoiser_output* out, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Allocation-free denoiser frame.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * vgi_denoise_frame_static runs the pass chain of vgi_denoiser_add_passes
+ * (same kernels, same results) with the caller owning every buffer that
+ * outlives the call: the inputs, the outputs and the history.  The history
+ * is read from history_* (history_len NULL, or prev_depth/prev_gbuffer
+ * NULL, = no history, like vgi_denoiser_reset) and written to rw_history_*,
+ * which must not alias history_*: pass this frame's rw_history_* as the
+ * next frame's history_*.  The frame's intermediate images live in one
+ * fixed device array of VGI_DENOISE_STATIC_SCRATCH_BYTES per GPU, which the
+ * library holds until the process ends.  They take 64 B/px with FP32
+ * storage and 36 B/px with FP16 (32 without compensated moments), so at most
+ * about 2.09 Mpx FP32 (1920x1080 fits) or 3.7 Mpx FP16 (2560x1440 fits);
+ * larger images return VGI_E_INVALID.  cfg NULL = the defaults
+ * {1, 0, 3, 1}; fast_history_clamp must be 0.  The call makes no device
+ * allocation (with the profiler off, its default).  Calls on one GPU share
+ * the array: calls that overlap on one GPU (two streams or two host
+ * threads) race.
+ * ---------------------------------------------------------------------- */
+#define VGI_DENOISE_STATIC_SCRATCH_BYTES (128u << 20)
+
+typedef struct vgi_denoise_frame_io {
+    uint32_t width, height, storage;
+    /* denoising.rs:10-18 Input */
+    const float* depth;
+    const uint32_t* gbuffer;
+    const float* prev_depth;
+    const uint32_t* prev_gbuffer;
+    const void* diffuse;
+    const void* specular;
+    const void* specular_ray_len;
+    /* history read (denoising.rs:41-49 History): radiance, radiance, pair,
+     * FP16 moment residual (may be NULL), R8 count */
+    const void* history_diffuse;
+    const void* history_specular;
+    const void* history_moments;
+    const void* history_moments_comp;
+    const uint8_t* history_len;
+    /* history written; rw_history_moments_comp is required for FP16 storage
+     * with compensated moments and unused otherwise */
+    void* rw_history_diffuse;
+    void* rw_history_specular;
+    void* rw_history_moments;
+    void* rw_history_moments_comp;
+    uint8_t* rw_history_len;
+    /* denoising.rs:27-31 Output; rw_variance (DebugViews.variance, the
+     * spatial fallback's variance) may be NULL */
+    void* rw_diffuse;
+    void* rw_specular;
+    void* rw_variance;
+} vgi_denoise_frame_io;
+
+vgi_status vgi_denoise_frame_static(const vgi_frame_params* fp, const vgi_denoise_frame_io* io,
+                                    const vgi_denoiser_config* cfg, void* stream);
+
+/* ------------------------------------------------------------------------
  * Scene and software ray tracing.
  * Replaces the BLAS/TLAS build of crates/violet/src/render_scene.rs:
  * 1170-1210 (BLAS per geometry group, UINT16 indices, f32x3 positions) and

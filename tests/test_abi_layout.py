@@ -41,7 +41,7 @@ def _mirrors():
         "vgi_raytraced_shadow_io": _abi.RaytracedShadowIO, "vgi_final_lighting_io": _abi.FinalLightingIO,
         "vgi_temporal_aa_io": _abi.TemporalAAIO, "vgi_pc_temporal_aa": _abi.PCTemporalAA,
         "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
-        "vgi_resample_params": _abi.ResampleParams,
+        "vgi_resample_params": _abi.ResampleParams, "vgi_denoise_frame_io": _lib.DenoiseFrameIO,
     }
     return m
 

@@ -122,6 +122,24 @@ class DenoiserOutput(C.Structure):
     _fields_ = [("diffuse", vp), ("specular", vp), ("history_len", vp), ("variance", vp)]
 
 
+class DenoiseFrameIO(C.Structure):
+    """vgi_denoise_frame_io: one allocation-free denoiser frame."""
+
+    _fields_ = [
+        ("width", u32), ("height", u32), ("storage", u32),
+        ("depth", vp), ("gbuffer", vp), ("prev_depth", vp), ("prev_gbuffer", vp),
+        ("diffuse", vp), ("specular", vp), ("specular_ray_len", vp),
+        ("history_diffuse", vp), ("history_specular", vp), ("history_moments", vp), ("history_moments_comp", vp),
+        ("history_len", vp),
+        ("rw_history_diffuse", vp), ("rw_history_specular", vp), ("rw_history_moments", vp),
+        ("rw_history_moments_comp", vp), ("rw_history_len", vp),
+        ("rw_diffuse", vp), ("rw_specular", vp), ("rw_variance", vp),
+    ]
+
+
+DENOISE_STATIC_SCRATCH_BYTES = 128 << 20  # VGI_DENOISE_STATIC_SCRATCH_BYTES
+
+
 class ProfileEntry(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("count", C.c_uint64), ("bytes", C.c_double),
                 ("rays_closest", C.c_double), ("rays_any", C.c_double), ("nodes", C.c_double), ("tris", C.c_double)]
@@ -152,6 +170,8 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_denoiser_set_config": (C.c_int32, [vp, C.POINTER(DenoiserConfig)]),
     "vgi_denoiser_add_passes": (C.c_int32, [vp, C.POINTER(FrameParams), C.POINTER(DenoiserInput),
                                             C.POINTER(DenoiserOutput), vp]),
+    "vgi_denoise_frame_static": (C.c_int32, [C.POINTER(FrameParams), C.POINTER(DenoiseFrameIO),
+                                             C.POINTER(DenoiserConfig), vp]),
 }
 
 _lib: C.CDLL | None = None
