@@ -325,6 +325,15 @@ class OracleRestir:
         tr.query_counter = _np_ptr(counter)
         return tr
 
+    def trace_radiance(self, fp, tr, rays, has_prev):
+        """raytrace.inc.hlsl:65-208 for a batch of rays (n, 8) = {o, tmin, d,
+        tmax}: returns (n, 9) = hit position, shading normal, radiance."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(rays), 9), self.rt_dtype)
+        self.lib.oracle_trace_radiance(_np_ptr(_fp_arr(fp)), C.byref(tr), _np_ptr(rays), len(rays), int(has_prev),
+                                       _np_ptr(out))
+        return out
+
     def gbuffer(self, fp, w, h):
         depth = np.zeros((h, w), self.rt_dtype)
         gb = np.zeros((4, h, w), np.uint32)

@@ -489,6 +489,9 @@ void oracle_ao_gen(const float* fpf, int w, int h, const uint32_t* hit, const re
         }
 }
 
+/* HLSL clamp(a, b, c) = min(max(a, b), c): the filter's "median"
+ * (raytraced_ao_filter.hlsl:52-60) is this clamp chain, a median only for
+ * some orderings of its arguments. */
 static inline real med3c(real a, real b, real c) { return R_FMIN(R_FMAX(a, b), c); }
 
 void oracle_ao_filter(int w, int h, const real* depth, const real* ao, real* o_ao) {

@@ -131,6 +131,7 @@ __device__ __forceinline__ void k_ao_filter_body(const vgi_ao_filter_io& io, uin
             }
             row[x + 1] = S::ld2(io.ao_texture, j);
         }
+        // the reference's "median" is clamp(a, b, c) = min(max(a, b), c)
         col_med[y + 1] = clamp2v(row[0], row[1], row[2]);
         col_min[y + 1] = min2v(min2v(row[0], row[1]), row[2]);
         col_max[y + 1] = max2v(max2v(row[0], row[1]), row[2]);
