@@ -9,8 +9,11 @@ per-step latency percentiles, the dominant kernel's roofline from live
 per-pass HIP-event timers, and the CPU baseline = the oracle's FP32 build
 timed on a bounded sample of the same workload on this host's cores).
 
-Beside the plain 1280x720 frame loop (the `value`), the line carries the
-other forms on a shared 1M-triangle scene: "batched" (16 views of seeded
+The `value` is the plain single-view frame loop at 1920x1080 (the metric's
+resolution) on the seeded 1M-triangle scene, FP32 storage, violet's own
+ReSTIR constants; `--width 1280 --height 720 --triangles 200000 --resample
+frame_loop` gives the 1280x720 frame-loop configuration.  Beside it the line
+carries the other forms on the same scene: "batched" (16 views of seeded
 resolutions, plain vs one fused persistent launch per pass, graph-replayed,
 autotuned), "serving" (2000 Poisson requests at 600/s over 64 persistent
 views, continuously batched, p50/p99 latency) and "precision" (1920x1080
@@ -31,6 +34,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+SCENE_SEED = 13  # the seeded boxes + icospheres scene shared by every form
 
 # Resampling parameters: the 1280x720 frame-loop workload's (temporal M cap
 # 30, 5 spatial taps in a 30 px radius) or the reference's constants.
@@ -183,10 +187,12 @@ def orbit_view(frame: int, w: int, h: int, radius=30.0, height=9.0, deg_per_fram
 
 
 def frame_params_seq(w, h, n, jitter_cycle=0):
+    """Frame parameters of an n-frame cycle of the orbit; frame 0's previous
+    view is frame n-1 (the frame rendered before it when the loop wraps)."""
     from violet_amd import synth
 
-    views = [orbit_view(f, w, h, jitter_cycle=jitter_cycle) for f in range(n + 1)]
-    return [synth.make_frame_params(views[f], views[f - 1] if f else views[-1]) for f in range(n)]
+    views = [orbit_view(f, w, h, jitter_cycle=jitter_cycle) for f in range(n)]
+    return [synth.make_frame_params(views[f], views[f - 1]) for f in range(n)]
 
 
 def profile_passes(step, first: int, n: int) -> dict:
@@ -239,7 +245,7 @@ def run_frame(args):
 
     w, h = args.width, args.height
     log(f"[bench] building the {args.triangles}-triangle scene and its BVH")
-    mesh = scene.make_scene(args.triangles, seed=11)
+    mesh = scene.make_scene(args.triangles, seed=SCENE_SEED)
     sc = scene.Scene(mesh)
     st = sc.stats()
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
@@ -252,6 +258,14 @@ def run_frame(args):
 
     def step(k: int):
         r.add_passes(sc, fps[k % len(fps)], k)
+
+    # the same frames on a one-stream renderer: per-pass times without the
+    # overlap of the two pass chains, which pick the roofline pass
+    iso_cfg = renderer.RestirConfig(**{**cfg.__dict__, "concurrent_lanes": False})
+    r_iso = renderer.RestirRenderer(w, h, iso_cfg)
+
+    def step_iso(k: int):
+        r_iso.add_passes(sc, fps[k % len(fps)], k)
 
     for k in range(args.warmup):
         step(k)
@@ -268,6 +282,10 @@ def run_frame(args):
     base += min(args.steps, 120)
     # per-pass times (every pass timed with HIP events; outside the timed region)
     prof = profile_passes(step, base, min(args.steps, 60))
+    for k in range(args.warmup):
+        step_iso(base + k)
+    prof_iso = profile_passes(step_iso, base + args.warmup, min(args.steps, 60))
+    r_iso.close()
     base += min(args.steps, 60)
     # traversal statistics: the instrumented trace kernels over a few more
     # frames of the same orbit (outside the timed region)
@@ -282,33 +300,43 @@ def run_frame(args):
     counts = _lib.profiler_read()
     base += args.count_frames
     trace_passes = {k for k, v in counts.items() if v["rays_closest"] + v["rays_any"] > 0}
-    dname = dominant_pass(prof, trace_passes)
+    dname = dominant_pass(prof_iso, trace_passes)
+    # the dependent-fetch rate over a buffer of the BVH's size (the trace passes' ceiling)
+    import ctypes as C
+
+    rate, rate_incoherent = C.c_double(), C.c_double()
+    _lib.call("vgi_probe_dependent_fetch", int(st.device_bytes), 64, 64, C.byref(rate), None)
+    _lib.call("vgi_probe_dependent_fetch", int(st.device_bytes), 64, 1, C.byref(rate_incoherent), None)
     elapsed, timed = timed_region(step, base, args.steps, dname)
-    return dict(elapsed=elapsed, lat=lat, prof=prof, timed=timed, dname=dname, counts=counts,
-                trace_passes=trace_passes, pixels=w * h, mesh=mesh)
+    return dict(elapsed=elapsed, lat=lat, prof=prof, prof_iso=prof_iso, timed=timed, dname=dname, counts=counts,
+                trace_passes=trace_passes, pixels=w * h, mesh=mesh, sc=sc, fetch_steps_per_s=rate.value,
+                fetch_steps_per_s_incoherent=rate_incoherent.value,
+                bvh_bytes=int(st.device_bytes))
 
 
 def cpu_baseline_frame(args, mesh, budget_s: float = 12.0):
-    """FP32 twin of the oracle (OpenMP) rendering frames of the same workload
-    at a reduced resolution (the orbit cycled as in the timed GPU loop) for
-    about `budget_s` seconds."""
+    """FP32 twin of the oracle (OpenMP) rendering the same orbit frames at the
+    same size: frame 0 (no history) untimed, then frames with history for
+    about `budget_s` seconds (at least two)."""
     import oracle as O
 
     threads = O.load(32).oracle_num_threads()
     w, h = args.cpu_width, args.cpu_height
     fps = frame_params_seq(w, h, args.frames)
     R = O.OracleRenderer(32, mesh, w, h, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
+    R.add_passes(fps[0], 0)
     t0 = time.perf_counter()
     n = 0
     while True:
-        R.add_passes(fps[n % len(fps)], n)
         n += 1
-        if time.perf_counter() - t0 > budget_s:
+        R.add_passes(fps[n % len(fps)], n)
+        if time.perf_counter() - t0 > budget_s and n >= 2:
             break
     el = time.perf_counter() - t0
     return dict(value=n * w * h / el / 1e6, unit="Mpix/s", cores=threads, kind="port",
-                sample=f"{n} full frames (GI trace, ReSTIR, hash grid, denoise) of the {len(mesh.indices)}-triangle "
-                       f"orbit workload at {w}x{h}, FP32 twin of the oracle, {el:.1f} s")
+                sample=f"frames 1-{n} (GI trace, ReSTIR, hash grid, denoise; frame 0 untimed) of the "
+                       f"{len(mesh.indices)}-triangle orbit workload at {w}x{h}, FP32 twin of the oracle "
+                       f"(gcc -O3 -mavx2 -mfma, OpenMP), {el:.1f} s")
 
 
 # ---------------------------------------------------------------------------
@@ -318,7 +346,7 @@ def big_scene(args):
     from violet_amd import scene
 
     log(f"[bench] building the {args.big_triangles}-triangle scene of the batched/serving/precision forms")
-    mesh = scene.make_scene(args.big_triangles, seed=13)
+    mesh = scene.make_scene(args.big_triangles, seed=SCENE_SEED)
     sc = scene.Scene(mesh)
     st = sc.stats()
     log(f"[bench] BVH: {st.num_nodes} nodes, depth {st.depth}, {st.device_bytes / 1e6:.1f} MB")
@@ -491,12 +519,12 @@ def main():
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--iterations", type=int, default=None)
-    ap.add_argument("--triangles", type=int, default=200_000)
+    ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16"])
     ap.add_argument("--cpu-width", type=int, default=None)
     ap.add_argument("--cpu-height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--resample", default="frame_loop", choices=sorted(RESAMPLE))
+    ap.add_argument("--resample", default="reference", choices=sorted(RESAMPLE))
     ap.add_argument("--count-frames", type=int, default=8, help="frames run with traversal statistics")
     ap.add_argument("--no-lanes", action="store_true",
                     help="every pass on one stream (per-pass times without overlap between pass chains)")
@@ -522,14 +550,16 @@ def main():
                     help="persistent groups of the fused trace passes (0 = one per tile)")
     args = ap.parse_args()
     frame = args.workload == "frame"
-    # full frame: 1280x720, 60-frame orbit, 3 a-trous iterations (the reference
-    # default); denoise only: 1920x1080, 8-frame pan, 5 iterations
-    args.width = args.width or (1280 if frame else 1920)
-    args.height = args.height or (720 if frame else 1080)
+    # full frame: 1920x1080, 60-frame orbit, 3 a-trous iterations (the reference
+    # default); denoise only: 1920x1080, 8-frame pan, 5 iterations.  The CPU
+    # baseline renders the same frames at the same size (frame) or at 640x360
+    # (denoise only)
+    args.width = args.width or 1920
+    args.height = args.height or 1080
     args.frames = args.frames or (60 if frame else 8)
     args.iterations = args.iterations if args.iterations is not None else (3 if frame else 5)
-    args.cpu_width = args.cpu_width or (160 if frame else 640)
-    args.cpu_height = args.cpu_height or (90 if frame else 360)
+    args.cpu_width = args.cpu_width or (args.width if frame else 640)
+    args.cpu_height = args.cpu_height or (args.height if frame else 360)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
               "path is designed but not built", file=sys.stderr)
@@ -552,13 +582,18 @@ def main():
     bytes_per_launch = d["bytes"] / max(d["count"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     lat = np.array(r["lat"])
-    passes = {k: dict(avg_us=round(v["ms"] / max(v["count"], 1) * 1e3, 2), launches=v["count"],
-                      gbs=round(v["bytes"] / max(v["ms"] * 1e-3, 1e-12) / 1e9, 1),
-                      share=round(v["ms"] / max(sum(x["ms"] for x in prof.values()), 1e-12), 4))
-              for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    prof_iso = r.get("prof_iso", prof)
+
+    def pass_table(p):
+        return {k: dict(avg_us=round(v["ms"] / max(v["count"], 1) * 1e3, 2), launches=v["count"],
+                        gbs=round(v["bytes"] / max(v["ms"] * 1e-3, 1e-12) / 1e9, 1),
+                        share=round(v["ms"] / max(sum(x["ms"] for x in p.values()), 1e-12), 4))
+                for k, v in sorted(p.items(), key=lambda kv: -kv[1]["ms"])}
+
+    passes = pass_table(prof_iso)
     trace = {}
-    for k in sorted(trace_passes, key=lambda n: -prof.get(n, {"ms": 0})["ms"]):
-        c, t = counts[k], prof.get(k)
+    for k in sorted(trace_passes, key=lambda n: -prof_iso.get(n, {"ms": 0})["ms"]):
+        c, t = counts[k], prof_iso.get(k)
         if not t or not c["count"]:
             continue
         rays = (c["rays_closest"] + c["rays_any"]) / c["count"]
@@ -570,6 +605,29 @@ def main():
                         tris_per_ray=round(c["tris"] / max(c["rays_closest"] + c["rays_any"], 1), 2),
                         traversal_bytes_per_ray=round(tb / max(rays, 1), 1),
                         traversal_gbs=round(tb / (us * 1e-6) / 1e9, 1))
+    trace_roofline = None
+    if trace:
+        tk = next(iter(trace))
+        tr_ = trace[tk]
+        c = counts[tk]
+        rays = c["rays_closest"] + c["rays_any"]
+        steps_per_ray = (c["nodes"] + c["tris"]) / max(rays, 1)
+        achieved_steps = tr_["mrays_per_s"] * 1e6 * steps_per_ray
+        ceiling = r["fetch_steps_per_s"]
+        trace_roofline = {
+            "kernel": tk, "bound": "dependent-fetch latency",
+            "achieved": round(achieved_steps / 1e9, 3), "peak": round(ceiling / 1e9, 3), "unit": "Gfetch/s",
+            "frac": round(achieved_steps / ceiling, 4),
+            "fetches_per_ray": round(steps_per_ray, 2), "mrays_per_s": tr_["mrays_per_s"],
+            "ceiling_mrays_per_s": round(ceiling / steps_per_ray / 1e6, 1),
+            "incoherent_gfetch_per_s": round(r["fetch_steps_per_s_incoherent"] / 1e9, 3),
+            "model": ("each ray makes one dependent round trip per interior node (128 B) and per triangle "
+                      "tested (48 B); the peak is vgi_probe_dependent_fetch's rate of dependent 128-byte "
+                      "fetches with every lane of the chip busy and the 64 lanes of each wave on one chain "
+                      f"(perfectly coherent rays) through {r['bvh_bytes'] / 1e6:.0f} MB (the scene's BVH size), "
+                      "launched like the trace kernels; incoherent_gfetch_per_s is the same with every lane "
+                      "on its own chain"),
+        }
     if args.no_cpu_baseline:
         cpu = None
     else:
@@ -606,18 +664,22 @@ def main():
                      "traffic": measured_traffic(workload_name, dname),
                      "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": bytes_per_launch},
         "passes": passes,
-        "passes_note": ("per-pass HIP-event times from a separate run of the same frames (the timed region "
-                        "times the roofline pass only); with concurrent lanes the hash-grid update overlaps the "
-                        "G-buffer and specular overlaps diffuse resampling, so overlapped passes read longer "
-                        "and shares do not add up to the frame" if frame and not args.no_lanes else
+        "passes_note": ("per-pass HIP-event times from a separate run of the same frames on one stream (the "
+                        "frame itself runs the hash-grid update beside the G-buffer and specular beside diffuse "
+                        "on two streams); the roofline pass is the HBM-bound pixel pass with the largest share "
+                        "here, timed live in the timed region" if frame else
                         "per-pass HIP-event times from a separate run (the timed region times the roofline "
                         "pass only), one stream"),
         "trace": trace,
+        "trace_roofline": trace_roofline,
         "cpu_baseline": cpu,
     }
     forms = [f for f in args.forms.split(",") if f] if frame else []
     if forms:
-        mesh, sc = big_scene(args)
+        if args.big_triangles == args.triangles:
+            mesh, sc = r["mesh"], r["sc"]
+        else:
+            mesh, sc = big_scene(args)
         ntris = int(len(mesh.indices))
         runners = {"batched": run_batched_form, "serving": run_serving_form, "precision": run_precision_form}
         for f in forms:

@@ -91,6 +91,20 @@ int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap);
 #define VGI_BVH_TRI_BYTES 48
 vgi_status vgi_trace_stats_enable(uint32_t enable);
 
+/* Dependent-fetch probe (no reference counterpart; measurement only): the
+ * rate at which the GPU completes dependent 128-byte fetches when every
+ * resident lane chases a random chain through a buffer of `bytes` (say, the
+ * scene BVH's size), launched like the ray-tracing kernels (64-lane groups,
+ * 8 waves per SIMD on every CU).  A step loads one 128-byte record (eight
+ * 16-byte loads issued together) and follows the index in its first word, as
+ * a traversal fetches a node and then one of its children.  Groups of
+ * `lanes_per_chain` consecutive lanes (a power of two, 1..64) follow the
+ * same chain: 64 is a wave of perfectly coherent rays, 1 a wave of rays
+ * that share no node.  Writes the completed lane-steps per second.
+ * Allocates and frees its own buffers and waits for the stream. */
+vgi_status vgi_probe_dependent_fetch(size_t bytes, uint32_t steps, uint32_t lanes_per_chain, double* steps_per_s,
+                                     void* stream);
+
 /* FrameParams (shader/frame_bindings.hlsl:6-25; filled by
  * crates/violet-app/src/render_loop.rs:108-158).  384 bytes, matrices
  * column-major (glam / #pragma pack_matrix(column_major)). */

@@ -251,10 +251,14 @@ def test_corrupted_documents_never_crash(doc, tmp_path):
     assert outcomes["error"] > 100 and sum(outcomes.values()) == 200
 
 
-def test_exponential_node_reuse_is_capped(doc, tmp_path):
+@pytest.mark.parametrize("leaf_mesh", [True, False], ids=["mesh_leaf", "meshless_chain"])
+def test_exponential_node_reuse_is_rejected(doc, tmp_path, leaf_mesh):
     """A chain of nodes that each list the next one twice as children would
-    instance the leaf 2^40 times: the importer stops at its cap with an
-    error instead of exhausting host memory."""
+    walk 2^40 nodes (and, with a mesh at the end, instance it 2^40 times):
+    glTF nodes form strict trees, so the second visit of a node is an error,
+    returned at once, whether or not any node of the chain carries a mesh."""
+    import time
+
     from violet_amd import _lib
 
     d, _, _ = doc
@@ -263,10 +267,110 @@ def test_exponential_node_reuse_is_capped(doc, tmp_path):
     depth = 40
     for k in range(depth):
         j["nodes"].append({"children": [base + k + 1, base + k + 1]})
-    j["nodes"].append({"mesh": 0})
+    j["nodes"].append({"mesh": 0} if leaf_mesh else {})
     j["scenes"] = [{"nodes": [base]}]
     p = tmp_path / "bomb.gltf"
     p.write_text(json.dumps(j))
+    t0 = time.perf_counter()
     with pytest.raises(_lib.VgiError) as e:
         _imported(p)
-    assert "exceeds" in str(e.value)
+    assert "reached twice" in str(e.value)
+    assert time.perf_counter() - t0 < 5.0
+
+
+def test_node_shared_by_two_roots_is_rejected(doc, tmp_path):
+    from violet_amd import _lib
+
+    d, _, _ = doc
+    j = json.loads(d.save_gltf(tmp_path / "base.gltf").read_text())
+    roots = j["scenes"][j.get("scene", 0)]["nodes"]
+    j["scenes"][j.get("scene", 0)]["nodes"] = roots + [roots[0]]
+    p = tmp_path / "twice.gltf"
+    p.write_text(json.dumps(j))
+    with pytest.raises(_lib.VgiError) as e:
+        _imported(p)
+    assert "reached twice" in str(e.value)
+
+
+# ---------------------------------------------------------------------------
+# the reference's own self-contained assets (read in place, when present)
+# ---------------------------------------------------------------------------
+REF_ASSETS = {
+    # name: (triangles, vertices, textured materials, base colour, metallic, roughness)
+    "Box/Box.gltf": (12, 24, 0, (0.8, 0.0, 0.0), 0.0, 1.0),
+    "Duck/Duck.gltf": (4212, 2399, 1, (1.0, 1.0, 1.0), 0.0, 1.0),
+}
+
+
+def _read_gltf(path):
+    """Independent reader: the default scene's triangles in world space
+    (node matrix, or T*R*S, after the parent's, under the Y-up -> Z-up root)
+    and the baked per-vertex normals."""
+    import base64
+
+    j = json.loads(path.read_text())
+    bufs = [base64.b64decode(b["uri"].split(",", 1)[1]) for b in j["buffers"]]
+    comp = {5121: np.uint8, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
+    ncomp = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4}
+
+    def accessor(i):
+        a = j["accessors"][i]
+        v = j["bufferViews"][a["bufferView"]]
+        dt = np.dtype(comp[a["componentType"]])
+        k = ncomp[a["type"]]
+        stride = v.get("byteStride", dt.itemsize * k)
+        off = v.get("byteOffset", 0) + a.get("byteOffset", 0)
+        raw = np.frombuffer(bufs[v["buffer"]], np.uint8)
+        rows = [raw[off + r * stride: off + r * stride + dt.itemsize * k].view(dt) for r in range(a["count"])]
+        return np.array(rows).reshape(a["count"], k)
+
+    def local(n):
+        if "matrix" in n:
+            return np.array(n["matrix"], np.float64).reshape(4, 4).T
+        return _trs(n.get("translation", (0, 0, 0)), n.get("rotation", (0, 0, 0, 1)), n.get("scale", (1, 1, 1)))
+
+    tris, nrms = [], []
+
+    def walk(i, parent):
+        n = j["nodes"][i]
+        m = parent @ local(n)
+        if "mesh" in n:
+            for p in j["meshes"][n["mesh"]]["primitives"]:
+                P = accessor(p["attributes"]["POSITION"]).astype(np.float64)
+                N = accessor(p["attributes"]["NORMAL"]).astype(np.float64)
+                I = accessor(p["indices"]).reshape(-1, 3).astype(np.int64)
+                W = P @ m[:3, :3].T + m[:3, 3]
+                tris.append(W[I])
+                nrms.append(N @ np.linalg.inv(m[:3, :3]))
+        for c in n.get("children", []):
+            walk(c, m)
+
+    for r in j["scenes"][j.get("scene", 0)]["nodes"]:
+        walk(r, _root(1.0))
+    return np.concatenate(tris), np.concatenate(nrms)
+
+
+@pytest.mark.parametrize("name", sorted(REF_ASSETS))
+def test_reference_assets_import(name):
+    """violet's own self-contained glTF assets (COLLADA2GLTF exports with
+    embedded buffers) import with the counts, material factors and world
+    transforms an independent reader of the document gives."""
+    from pathlib import Path
+
+    path = Path("/root/reference/assets") / name
+    if not path.exists():
+        pytest.skip("the reference's assets are not present on this machine")
+    tris_n, verts_n, textured, base, metal, rough = REF_ASSETS[name]
+    mesh, st = _imported(path)
+    assert (st.triangles, st.vertices, st.instances, st.textured_materials) == (tris_n, verts_n, 1, textured)
+    assert mesh.indices.shape == (tris_n, 3) and mesh.positions.shape == (verts_n, 3)
+    np.testing.assert_allclose(mesh.albedo[0], base, rtol=1e-6)
+    assert (float(mesh.metallic[0]), float(mesh.roughness[0])) == (metal, rough)
+    want, want_n = _read_gltf(path)
+    got = mesh.positions[mesh.indices].astype(np.float64)
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, atol=2e-6 * scale)
+    np.testing.assert_allclose(mesh.normals, want_n, atol=1e-5 * np.abs(want_n).max())
+    # Z up: the Y-up documents' up axis is now +z
+    if name.startswith("Duck"):
+        assert want[..., 2].mean() > 0.3 * want[..., 2].max()

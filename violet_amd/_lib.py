@@ -157,6 +157,7 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_profiler_reset": (C.c_int32, []),
     "vgi_profiler_read": (C.c_int32, [C.POINTER(ProfileEntry), C.c_int32]),
     "vgi_trace_stats_enable": (C.c_int32, [u32]),
+    "vgi_probe_dependent_fetch": (C.c_int32, [C.c_size_t, u32, u32, C.POINTER(C.c_double), vp]),
     "vgi_profiler_only": (C.c_int32, [C.c_char_p]),
     "vgi_pass_denoise_temporal": (C.c_int32, [C.POINTER(FrameParams), C.POINTER(DenoiseTemporalIO),
                                               C.POINTER(PCDenoiseTemporal), vp]),

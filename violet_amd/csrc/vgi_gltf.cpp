@@ -370,6 +370,7 @@ namespace {
 
 constexpr uint32_t kMaxInstances = 1u << 20;
 constexpr uint32_t kMaxTriangles = 0x7FFFFFFFu / 48u;
+constexpr uint64_t kMaxVertices = 3ull * kMaxTriangles;  // baked vertices (indices are u32)
 
 struct Importer {
     JVal doc;
@@ -738,11 +739,20 @@ struct Importer {
         m.stats.triangles += (uint32_t)ntri;
     }
 
+    std::vector<uint8_t> visited;  // nodes walked so far in the default scene
+
     vgi_status walk(vgi_gltf_model& m, uint32_t node, const Mat4& parent, int depth) {
         const JVal* nodes = doc.get("nodes");
         if (!nodes || nodes->kind != JVal::kArr || node >= nodes->arr.size())
             return set_error(VGI_E_INVALID, "vgi_gltf_import: node %u does not exist", node);
         if (depth > 1024) return set_error(VGI_E_INVALID, "vgi_gltf_import: node hierarchy deeper than 1024 (a cycle?)");
+        // glTF node hierarchies are disjoint strict trees (each node has at
+        // most one parent): a node reached twice would be baked twice, and a
+        // chain of such nodes doubles the walk per level
+        if (visited.size() != nodes->arr.size()) visited.assign(nodes->arr.size(), 0);
+        if (visited[node]) return set_error(VGI_E_INVALID, "vgi_gltf_import: node %u is reached twice (nodes must form "
+                                                           "strict trees)", node);
+        visited[node] = 1;
         const JVal& n = nodes->arr[node];
         Mat4 local;
         vgi_status s = node_local(n, local);
@@ -751,14 +761,18 @@ struct Importer {
         uint32_t mesh = 0;
         if (as_u32(n.get("mesh"), mesh)) {
             if (mesh >= meshes.size()) return set_error(VGI_E_INVALID, "vgi_gltf_import: node %u: mesh %u", node, mesh);
-            // a hierarchy that reuses nodes can multiply instances without
-            // bound: cap what one scene may bake (the BVH's own limit is
-            // 2^31 / 48 triangles)
-            size_t add = 0;
-            for (const Prim& p : meshes[mesh]) add += p.idx.size() / 3;
-            if (m.stats.instances >= kMaxInstances || m.stats.triangles + add > kMaxTriangles)
-                return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: the scene exceeds %u instances or %u triangles",
-                                 kMaxInstances, kMaxTriangles);
+            // cap what one scene may bake (the BVH's own limit is 2^31 / 48
+            // triangles; vertex indices are u32)
+            size_t add = 0, addv = 0;
+            for (const Prim& p : meshes[mesh]) {
+                add += p.idx.size() / 3;
+                addv += p.has_normals ? p.pos.size() / 3 : p.idx.size();
+            }
+            if (m.stats.instances >= kMaxInstances || m.stats.triangles + add > kMaxTriangles ||
+                (uint64_t)m.stats.vertices + addv > kMaxVertices)
+                return set_error(VGI_E_UNSUPPORTED,
+                                 "vgi_gltf_import: the scene exceeds %u instances, %u triangles or %llu vertices",
+                                 kMaxInstances, kMaxTriangles, (unsigned long long)kMaxVertices);
             ++m.stats.instances;
             for (const Prim& p : meshes[mesh]) bake(m, p, x);
         }
