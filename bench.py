@@ -621,8 +621,8 @@ def main():
             "fetches_per_ray": round(steps_per_ray, 2), "mrays_per_s": tr_["mrays_per_s"],
             "ceiling_mrays_per_s": round(ceiling / steps_per_ray / 1e6, 1),
             "incoherent_gfetch_per_s": round(r["fetch_steps_per_s_incoherent"] / 1e9, 3),
-            "model": ("each ray makes one dependent round trip per interior node (128 B) and per triangle "
-                      "tested (48 B); the peak is vgi_probe_dependent_fetch's rate of dependent 128-byte "
+            "model": (f"each ray makes one dependent round trip per interior node ({_lib_const('BVH_NODE_BYTES')} B) "
+                      f"and per triangle tested ({_lib_const('BVH_TRI_BYTES')} B); the peak is vgi_probe_dependent_fetch's rate of dependent 128-byte "
                       "fetches with every lane of the chip busy and the 64 lanes of each wave on one chain "
                       f"(perfectly coherent rays) through {r['bvh_bytes'] / 1e6:.0f} MB (the scene's BVH size), "
                       "launched like the trace kernels; incoherent_gfetch_per_s is the same with every lane "

@@ -87,7 +87,7 @@ int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap);
  * waits for it, adding the counts to the pass's profiler entry (the
  * profiler must be enabled too).  Counts do not depend on scheduling.
  * Allocates a small counter buffer on the current device when turned on. */
-#define VGI_BVH_NODE_BYTES 128
+#define VGI_BVH_NODE_BYTES 64
 #define VGI_BVH_TRI_BYTES 48
 vgi_status vgi_trace_stats_enable(uint32_t enable);
 

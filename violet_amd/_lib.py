@@ -145,7 +145,7 @@ class ProfileEntry(C.Structure):
                 ("rays_closest", C.c_double), ("rays_any", C.c_double), ("nodes", C.c_double), ("tris", C.c_double)]
 
 
-BVH_NODE_BYTES = 128  # VGI_BVH_NODE_BYTES
+BVH_NODE_BYTES = 64  # VGI_BVH_NODE_BYTES
 BVH_TRI_BYTES = 48  # VGI_BVH_TRI_BYTES
 
 

@@ -9,7 +9,8 @@
  * depth stays bounded by kBvhMaxDepth), then collapsed into a 4-wide tree:
  * each 4-wide node takes its binary node's two children and repeatedly
  * opens the inner child with the largest surface area until it has four
- * children (or only leaves are left).
+ * children (or only leaves are left).  Each 4-wide node stores its
+ * children's padded boxes on an 8-bit grid spanning them (vgi_bvh.h).
  */
 #include <algorithm>
 #include <cmath>
@@ -59,6 +60,10 @@ struct Prim {
 };
 
 constexpr int kBins = 32;
+// Largest vertex coordinate magnitude a scene may have: keeps every node's
+// grid scale a float and the traversal's plane distances finite
+// (scale * |1/d| and q * scale * |1/d| stay below 2^120 for |1/d| <= 1e20).
+constexpr float kMaxCoord = 1099511627776.0f;  // 2^40
 constexpr int kMaxLeaf = 15;
 constexpr int kMedianDepth = 36;
 
@@ -223,20 +228,57 @@ struct Collapser {
         int32_t child[4];
         for (int k = 0; k < 4; ++k) child[k] = k < cnt && link[k] >= 0 ? emit(link[k], level + 1) : (k < cnt ? link[k] : ~0);
         BvhNode& nd = out[idx];
-        float* f[6] = {&nd.lox.x, &nd.hix.x, &nd.loy.x, &nd.hiy.x, &nd.loz.x, &nd.hiz.x};
-        for (int k = 0; k < 4; ++k) {
-            const bool used = k < cnt;
-            for (int a = 0; a < 3; ++a) {
-                f[2 * a][k] = used ? box[k].lo[a] : INFINITY;
-                f[2 * a + 1][k] = used ? box[k].hi[a] : INFINITY;
-            }
-            nd.link[k] = child[k];
-            nd.pad[k] = 0;
-        }
+        quantise(nd, box, child, cnt);
         return idx;
     }
+
+    // Smallest power of two >= x (x >= 0), at least 2^-60.  Positions are
+    // bounded by kMaxCoord, so x < 2^34 and the power of two is a float.
+    static double pow2_at_least(double x) {
+        int e = 0;
+        const double m = std::frexp(x, &e);  // x = m * 2^e, m in [0.5, 1)
+        if (!(x > 0.0)) e = -60;
+        else if (m == 0.5) e -= 1;
+        return std::ldexp(1.0, std::max(-60, e));
+    }
+
+    // The node's grid: per axis, the origin is the lowest used child plane
+    // and 255 steps of the (power-of-two) scale reach the highest.  A child's
+    // lo rounds down and its hi up, so the quantised box contains the padded
+    // one exactly (computed in double: org, lo and hi are floats and the
+    // scale a power of two, so (lo - org) / scale is exact).
+    static void quantise(BvhNode& nd, const Box* box, const int32_t* child, int cnt) {
+        bool used[4];
+        for (int k = 0; k < 4; ++k) used[k] = k < cnt && child[k] != ~0;
+        for (int a = 0; a < 3; ++a) {
+            float lo = INFINITY, hi = -INFINITY;
+            for (int k = 0; k < 4; ++k)
+                if (used[k]) {
+                    lo = std::min(lo, box[k].lo[a]);
+                    hi = std::max(hi, box[k].hi[a]);
+                }
+            if (!(lo <= hi)) lo = hi = 0.0f;  // no used child (cannot happen for a built tree)
+            const double s = pow2_at_least(((double)hi - (double)lo) / 255.0);
+            nd.org[a] = lo;
+            nd.scale[a] = (float)s;
+            uint32_t qlo = 0, qhi = 0;
+            for (int k = 0; k < 4; ++k) {
+                uint32_t l = 255u, h = 0u;  // unused: an empty interval
+                if (used[k]) {
+                    const double fl = std::floor(((double)box[k].lo[a] - (double)lo) / s);
+                    const double ce = std::ceil(((double)box[k].hi[a] - (double)lo) / s);
+                    l = (uint32_t)std::min(255.0, std::max(0.0, fl));
+                    h = (uint32_t)std::min(255.0, std::max(0.0, ce));
+                }
+                qlo |= l << (8 * k);
+                qhi |= h << (8 * k);
+            }
+            nd.q[2 * a] = qlo;
+            nd.q[2 * a + 1] = qhi;
+        }
+        for (int k = 0; k < 4; ++k) nd.link[k] = k < cnt ? child[k] : ~0;
+    }
 };
-static_assert(offsetof(float4_, y) == 4 && offsetof(float4_, w) == 12, "float4_ component layout");
 
 }  // namespace
 
@@ -266,6 +308,9 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     VGI_REQUIRE(desc->num_triangles < (1u << 27), "vgi_scene_create: too many triangles (%u)", desc->num_triangles);
     VGI_REQUIRE(desc->materials && desc->num_materials > 0, "vgi_scene_create: no materials");
     const uint32_t nt = desc->num_triangles, nv = desc->num_vertices;
+    for (size_t i = 0; i < 3 * (size_t)nv; ++i)
+        VGI_REQUIRE(std::fabs(desc->positions[i]) <= kMaxCoord,
+                    "vgi_scene_create: vertex coordinate %zu is not finite or beyond 2^40", i);
     for (uint32_t i = 0; i < 3 * nt; ++i)
         VGI_REQUIRE(desc->indices[i] < nv, "vgi_scene_create: index %u out of range at %u", desc->indices[i], i);
     if (desc->triangle_material)

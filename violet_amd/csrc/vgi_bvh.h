@@ -5,12 +5,14 @@
  * (restir/raytrace.inc.hlsl:31-56,67-82; render_scene.rs:1170-1712).
  *
  * Layout (built on the host by vgi_bvh.cpp, uploaded once per scene):
- *   nodes : 4-wide BVH, one 128-byte node (one L2 line) holds its four
- *           children's boxes as six float4 planes (lo.x of children 0..3,
- *           hi.x, lo.y, hi.y, lo.z, hi.z) and four child links, fetched with
- *           eight 16-byte loads.  A child link >= 0 is an inner node index;
- *           < 0 encodes a leaf ~((first << 4) | n).  Unused children have a
- *           box at +inf on every axis, which no slab interval reaches.
+ *   nodes : 4-wide BVH, one 64-byte node (half an L2 line, four 16-byte
+ *           loads) holds its four children's boxes quantised to 8 bits per
+ *           plane on a per-node grid, plane = org + q * scale with scale a
+ *           power of two per axis, and four child links.  A child link >= 0
+ *           is an inner node index; < 0 encodes a leaf ~((first << 4) | n).
+ *           A child's quantised box contains its padded box (lo rounded
+ *           down, hi rounded up).  An unused child has lo = 255, hi = 0 on
+ *           every axis (an empty interval) and the empty leaf link ~0.
  *   tris  : leaf-ordered triangles as {v0, e1 = v1-v0, e2 = v2-v0} (3 x
  *           float4, 48 bytes); v0.w carries the scene triangle id.
  * Child boxes are padded by a small relative margin so the slab test is
@@ -27,11 +29,12 @@
 namespace vgi {
 
 struct BvhNode {
-    float4_ lox, hix, loy, hiy, loz, hiz;  // component c = child c
+    float org[3];    // grid origin (at or below every used child's padded box)
+    float scale[3];  // grid step per axis, a power of two (255 steps span the node)
+    uint32_t q[6];   // planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z: byte k = child k
     int32_t link[4];
-    int32_t pad[4];
 };
-static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 
 struct BvhTri {
     float4_ v0;  // w = triangle id (bits)

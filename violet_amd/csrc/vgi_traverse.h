@@ -1,15 +1,16 @@
 /*
  * vgi_traverse.h — device-side BVH traversal (closest hit and any hit).
  *
- * One lane per ray over the 4-wide BVH.  The traversal stack lives in LDS,
- * interleaved across the 64 lanes of the wave (entry k of lane l at
- * [k*64 + l]) so that a push or pop by all lanes touches 64 consecutive
- * dwords (no bank conflicts).  At a node the hit children are ordered by
- * entry distance; the nearest is visited next and the others are pushed
- * farthest first.  Boxes are tested with the slab method, one fused
- * multiply-add per plane (conservative within the boxes' padding, see
- * slab_nf); NaNs from 0*inf drop out of the max/min, which only widens the
- * interval.
+ * One lane per ray over the 4-wide BVH of 64-byte quantised nodes.  The
+ * traversal stack lives in LDS, interleaved across the 64 lanes of the wave
+ * (entry k of lane l at [k*64 + l]) so that a push or pop by all lanes
+ * touches 64 consecutive dwords (no bank conflicts).  At a node the hit
+ * children are ordered by entry distance; the nearest is visited next and
+ * the others are pushed farthest first.  Boxes are tested with the slab
+ * method on the node's 8-bit grid: per axis the ray's distance to grid line
+ * q is q * (scale / d) + (org - o) / d, one byte-to-float conversion and one
+ * fused multiply-add per plane (conservative within the boxes' padding, see
+ * slab_q).
  */
 #pragma once
 #include "vgi_bvh.h"
@@ -63,20 +64,27 @@ __device__ __forceinline__ RayInv make_ray(f3 o, f3 d) {
     return r;
 }
 
-// Entry distance of a box given each axis' near and far plane for this ray
-// (near = lo where the direction component is >= 0, else hi), or +inf if
-// missed.  A plane's distance is one fused multiply-add, P * idir + oid with
-// oid = -o * idir: against the exact (P - o) * idir it errs by at most an
-// ulp of o * idir plus the final rounding, i.e. a world distance of about
-// ulp(|o|), far inside the 2^-16 relative padding of every child box, so
-// the test stays conservative.  A NaN from 0 * inf (origin on a plane of a
-// box the ray runs parallel to) drops out of the max/min, which only widens
-// the interval.
-__device__ __forceinline__ float slab_nf(const RayInv& r, f3 oid, float nx, float fx, float ny, float fy, float nz,
-                                         float fz, float tmin, float tmax) {
-    const float tnx = __builtin_fmaf(nx, r.idir.x, oid.x), tfx = __builtin_fmaf(fx, r.idir.x, oid.x);
-    const float tny = __builtin_fmaf(ny, r.idir.y, oid.y), tfy = __builtin_fmaf(fy, r.idir.y, oid.y);
-    const float tnz = __builtin_fmaf(nz, r.idir.z, oid.z), tfz = __builtin_fmaf(fz, r.idir.z, oid.z);
+// Byte k of a plane word as a float (one v_cvt_f32_ubyte<k>).
+template <int k>
+__device__ __forceinline__ float qbyte(uint32_t w) {
+    return (float)((w >> (8 * k)) & 0xFFu);
+}
+
+// Entry distance of child k of a quantised node, or +inf if missed.  Per
+// axis, qn / qf are the plane words of the near and far side for this ray
+// (near = lo where the direction component is >= 0, else hi) and the
+// distance to grid line q is q * a + b with a = scale * idir (exact: scale
+// is a power of two) and b = (org - o) * idir.  Against the exact distance
+// (org + q * scale - o) * idir this errs by about an ulp of |org - o| in
+// world units plus the final rounding, inside the 2^-16 relative padding of
+// every child box, so the test stays conservative.  An unused child (lo =
+// 255, hi = 0) has its near plane beyond its far one on every axis.
+template <int k>
+__device__ __forceinline__ float slab_q(f3 a, f3 b, uint32_t qnx, uint32_t qfx, uint32_t qny, uint32_t qfy,
+                                        uint32_t qnz, uint32_t qfz, float tmin, float tmax) {
+    const float tnx = __builtin_fmaf(qbyte<k>(qnx), a.x, b.x), tfx = __builtin_fmaf(qbyte<k>(qfx), a.x, b.x);
+    const float tny = __builtin_fmaf(qbyte<k>(qny), a.y, b.y), tfy = __builtin_fmaf(qbyte<k>(qfy), a.y, b.y);
+    const float tnz = __builtin_fmaf(qbyte<k>(qnz), a.z, b.z), tfz = __builtin_fmaf(qbyte<k>(qfz), a.z, b.z);
     const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
     const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tmax);
     return tn <= tf ? tn : INFINITY;
@@ -120,9 +128,9 @@ constexpr int32_t kTraversalDone = INT32_MIN;
 template <bool kAnyHit, bool kCount = false>
 __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tmin, float tmax, LdsStack stack) {
     const RayInv r = make_ray(o, d);
-    // byte offset of each axis' near plane in a node (float4 slot lo = 2 axis,
-    // hi = 2 axis + 1; the far plane is the other one, offset ^ 16)
-    const uint32_t nx = r.idir.x >= 0.0f ? 0u : 16u, ny = r.idir.y >= 0.0f ? 32u : 48u, nz = r.idir.z >= 0.0f ? 64u : 80u;
+    // which plane word of each axis is the near one (lo where the direction
+    // component is >= 0, else hi)
+    const bool px = r.idir.x >= 0.0f, py = r.idir.y >= 0.0f, pz = r.idir.z >= 0.0f;
     const BufRsrc nodes = scene_rsrc(s.nodes, s.num_nodes * (uint32_t)sizeof(BvhNode));
     const BufRsrc tris = scene_rsrc(s.tris, s.num_tris * (uint32_t)sizeof(BvhTri));
     Hit h;
@@ -160,20 +168,22 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             }
             if (kCount) ++n_nodes;
             const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(BvhNode);
-            const Vf4 nrx = ld_buf16(nodes, nb + nx), frx = ld_buf16(nodes, nb + (nx ^ 16u));
-            const Vf4 nry = ld_buf16(nodes, nb + ny), fry = ld_buf16(nodes, nb + (ny ^ 16u));
-            const Vf4 nrz = ld_buf16(nodes, nb + nz), frz = ld_buf16(nodes, nb + (nz ^ 16u));
-            const Vf4 lk = ld_buf16(nodes, nb + 96u);
-            // -o * idir per node: kept out of loop-invariant hoisting (three
-            // more live registers across the traversal cost more than three
-            // multiplies per node)
-            f3 oo = r.o;
-            asm volatile("" : "+v"(oo.x), "+v"(oo.y), "+v"(oo.z));
-            const f3 oid = mk3(-(oo.x * r.idir.x), -(oo.y * r.idir.y), -(oo.z * r.idir.z));
-            float t0 = slab_nf(r, oid, nrx.x, frx.x, nry.x, fry.x, nrz.x, frz.x, tmin, h.t);
-            float t1 = slab_nf(r, oid, nrx.y, frx.y, nry.y, fry.y, nrz.y, frz.y, tmin, h.t);
-            float t2 = slab_nf(r, oid, nrx.z, frx.z, nry.z, fry.z, nrz.z, frz.z, tmin, h.t);
-            float t3 = slab_nf(r, oid, nrx.w, frx.w, nry.w, fry.w, nrz.w, frz.w, tmin, h.t);
+            const Vf4 g0 = ld_buf16(nodes, nb);        // org.xyz, scale.x
+            const Vf4 g1 = ld_buf16(nodes, nb + 16u);  // scale.yz, q lo.x, q hi.x
+            const Vf4 g2 = ld_buf16(nodes, nb + 32u);  // q lo.y, hi.y, lo.z, hi.z
+            const Vf4 lk = ld_buf16(nodes, nb + 48u);
+            const f3 a = mk3(g0.w * r.idir.x, g1.x * r.idir.y, g1.y * r.idir.z);
+            const f3 b = mk3((g0.x - r.o.x) * r.idir.x, (g0.y - r.o.y) * r.idir.y, (g0.z - r.o.z) * r.idir.z);
+            const uint32_t qlx = __float_as_uint(g1.z), qhx = __float_as_uint(g1.w);
+            const uint32_t qly = __float_as_uint(g2.x), qhy = __float_as_uint(g2.y);
+            const uint32_t qlz = __float_as_uint(g2.z), qhz = __float_as_uint(g2.w);
+            const uint32_t qnx = px ? qlx : qhx, qfx = px ? qhx : qlx;
+            const uint32_t qny = py ? qly : qhy, qfy = py ? qhy : qly;
+            const uint32_t qnz = pz ? qlz : qhz, qfz = pz ? qhz : qlz;
+            float t0 = slab_q<0>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
+            float t1 = slab_q<1>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
+            float t2 = slab_q<2>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
+            float t3 = slab_q<3>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
             int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
             int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
             if (kAnyHit) {
