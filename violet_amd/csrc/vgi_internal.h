@@ -37,6 +37,26 @@ struct DevBuf {
     }
 };
 
+// One pixel as the diffuse spatial pass reads it, written by the diffuse
+// temporal pass in the renderer's own frames: depth bits, G-buffer layer 1,
+// the reservoir word, the two hit words and the hit radiance as f32 (the
+// value the spatial pass would read back from the storage mode's radiance
+// image).  32 bytes: a spatial tap is one aligned pair of 16-byte loads from
+// one cache line instead of loads from five images, and its geometry test
+// and payload arrive together.
+struct alignas(32) DiffTap {
+    uint32_t w[8];
+};
+
+// The diffuse temporal and spatial passes with the tap records (taps NULL:
+// exactly the C entry points vgi_pass_ind_diff_{temporal,spatial}_resample).
+vgi_status launch_diff_temporal(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
+                                const vgi_pc_ind_diff_temporal* pc, const vgi_resample_params* params, DiffTap* taps,
+                                const uint32_t* gbuffer_color, hipStream_t st);
+vgi_status launch_diff_spatial(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
+                               const vgi_pc_ind_diff_spatial* pc, const vgi_resample_params* params,
+                               const DiffTap* taps, hipStream_t st);
+
 inline size_t radiance_bytes(uint32_t storage) { return storage == VGI_STORAGE_F16 ? 6 : 12; }
 inline size_t pair_bytes(uint32_t storage) { return storage == VGI_STORAGE_F16 ? 4 : 8; }
 inline size_t scalar_bytes(uint32_t storage) { return storage == VGI_STORAGE_F16 ? 2 : 4; }

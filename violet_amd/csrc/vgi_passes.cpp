@@ -65,9 +65,10 @@ vgi_status DirectSink::launch(const PassRec& r, hipStream_t st) {
         case kPassAoGen: return vgi_pass_raytraced_ao_gen(&r.fp, &r.io.ao_gen, &r.pc.ao_gen, s);
         case kPassAoFilter: return vgi_pass_raytraced_ao_filter(&r.io.ao_filter, s);
         case kPassDiffTemporal:
-            return vgi_pass_ind_diff_temporal_resample(&r.fp, &r.io.diff_temporal, &r.pc.diff_temporal, &r.resample, s);
+            return launch_diff_temporal(&r.fp, &r.io.diff_temporal, &r.pc.diff_temporal, &r.resample, r.diff_taps,
+                                        r.diff_taps_gbuffer, st);
         case kPassDiffSpatial:
-            return vgi_pass_ind_diff_spatial_resample(&r.fp, &r.io.diff_spatial, &r.pc.diff_spatial, &r.resample, s);
+            return launch_diff_spatial(&r.fp, &r.io.diff_spatial, &r.pc.diff_spatial, &r.resample, r.diff_taps, st);
         case kPassSpecGen: return vgi_pass_ind_spec_sample_gen(&r.fp, &r.tr, &r.io.spec_gen, &r.pc.sample_gen, s);
         case kPassSpecValidate: return vgi_pass_ind_spec_sample_validate(&r.fp, &r.tr, &r.io.spec_validate, s);
         case kPassSpecTemporal:

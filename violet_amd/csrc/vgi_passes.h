@@ -133,6 +133,11 @@ struct PassRec {
         vgi_pc_temporal_aa taa;
     } pc;
     vgi_resample_params resample;  // the four resampling passes (zero = reference constants)
+    // diffuse temporal / spatial of the renderer's own frames: the tap
+    // records the temporal pass writes (with the G-buffer it reads layer 1
+    // from) and the spatial pass reads; NULL for the plain entry points
+    DiffTap* diff_taps;
+    const uint32_t* diff_taps_gbuffer;
     void* clear_ptr;     // kPassClear
     size_t clear_bytes;  // kPassClear
 };

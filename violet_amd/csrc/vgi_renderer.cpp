@@ -33,6 +33,7 @@ struct vgi_renderer {
     // indirect diffuse reservoirs: spatial output (= next frame's prev), 2 sets
     DevBuf dsp_rsv[2], dsp_hit[2], dsp_rad[2];
     DevBuf dtm_rsv, dtm_hit, dtm_rad;  // temporal output (transient)
+    DevBuf dtaps;                      // the temporal output as the spatial pass's tap records (transient)
     DevBuf dnew_hit, dnew_rad;         // new diffuse sample (transient)
     DevBuf ao[2], ao_filtered;
     DevBuf diff_light[2];  // raw indirect diffuse (double-buffered: can be next frame's screen cache)
@@ -177,6 +178,7 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     A(r->dtm_rsv, 4 * n);
     A(r->dtm_hit, 8 * n);
     A(r->dtm_rad, R * n);
+    A(r->dtaps, sizeof(DiffTap) * n);
     A(r->dnew_hit, 8 * n);
     A(r->dnew_rad, R * n);
     A(r->ao_filtered, Pb * n);
@@ -362,6 +364,8 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         t.pc.diff_temporal = vgi_pc_ind_diff_temporal{frame_rand_next(frame_rand), (has_prev_diff_rsv && has_prev_depth) ? 1u : 0u,
                                     has_new_sample};
         t.resample = resample_params(r);
+        t.diff_taps = r->dtaps.as<DiffTap>();
+        t.diff_taps_gbuffer = r->gbuffer[c].as<uint32_t>();
         CALL(sink.submit(kSlotDiffTemporal, t));
         PassRec sp = make_rec(kPassDiffSpatial, fp);
         sp.io.diff_spatial = vgi_ind_diff_spatial_io{w, h, S, r->gbuffer[c].as<uint32_t>(), r->depth[c].as<float>(), r->ao_filtered.p,
@@ -370,6 +374,7 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
                                     r->diff_light[c].p};
         sp.pc.diff_spatial = vgi_pc_ind_diff_spatial{frame_rand_next(frame_rand)};
         sp.resample = resample_params(r);
+        sp.diff_taps = r->dtaps.as<DiffTap>();
         CALL(sink.submit(kSlotDiffSpatial, sp));
     }
 
@@ -552,7 +557,7 @@ extern "C" size_t vgi_renderer_device_bytes(const vgi_renderer* rc) {
     vgi_renderer* r = const_cast<vgi_renderer*>(rc);
     std::vector<DevBuf*> v;
     all_buffers(r, v);
-    for (DevBuf* b : {&r->dtm_rsv, &r->dtm_hit, &r->dtm_rad, &r->dnew_hit, &r->dnew_rad, &r->ao_filtered, &r->spec_light,
+    for (DevBuf* b : {&r->dtm_rsv, &r->dtm_hit, &r->dtm_rad, &r->dtaps, &r->dnew_hit, &r->dnew_rad, &r->ao_filtered, &r->spec_light,
                       &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount, &r->shadow_mask,
                       &r->color})
         if (std::find(v.begin(), v.end(), b) == v.end()) v.push_back(b);

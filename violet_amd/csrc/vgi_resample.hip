@@ -153,10 +153,38 @@ __global__ void __launch_bounds__(64) k_ao_filter(vgi_ao_filter_io io) {
 // Indirect diffuse temporal resampling
 // ---------------------------------------------------------------------------
 
+// Tap records as two 16-byte words.
+typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ U4 mk_u4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    U4 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    r.w = w;
+    return r;
+}
+struct TapWords {
+    U4 a, b;  // {depth, layer 1, reservoir, hit.x}, {hit.y, radiance rgb}
+};
+__device__ __forceinline__ void st_tap(DiffTap* p, const TapWords& t) {
+    G((U4*)p)[0] = t.a;
+    G((U4*)p)[1] = t.b;
+}
+__device__ __forceinline__ TapWords ld_tap(const DiffTap* p) {
+    TapWords t;
+    t.a = G((const U4*)p)[0];
+    t.b = G((const U4*)p)[1];
+    return t;
+}
+
 // Kernel arguments of the two temporal passes: the HLSL push constants plus
 // the history M clamp, resolved on the host (M_MAX 16 unless configured).
+// taps / gbuffer_color: when set (the renderer's own frames), the diffuse
+// temporal pass also writes each pixel's tap record for the spatial pass.
 struct DiffTemporalArgs {
     uint32_t frame_rand, has_prev_frame, has_new_sample, m_cap;
+    DiffTap* taps;
+    const uint32_t* gbuffer_color;
 };
 struct SpecTemporalArgs {
     uint32_t frame_rand, has_prev_frame, m_cap;
@@ -222,10 +250,19 @@ __device__ __forceinline__ void k_diff_temporal_body(const FrameParams& fp, cons
     ReservoirSimple rs;
     rs.M = M;
     rs.W = W;
-    G(io.rw_reservoir_texture)[i] = reservoir_encode_32b(rs);
+    const uint32_t rsv = reservoir_encode_32b(rs);
+    G(io.rw_reservoir_texture)[i] = rsv;
     G(io.rw_hit_pos_normal_texture)[2 * i] = ex;
     G(io.rw_hit_pos_normal_texture)[2 * i + 1] = ey;
     S::st3(io.rw_hit_radiance_texture, i, rad);
+    if (pc.taps) {
+        // the radiance as the spatial pass would read it back from storage
+        const f3 rs3 = S::round3(rad);
+        TapWords t;
+        t.a = mk_u4(__float_as_uint(depth), G(pc.gbuffer_color)[(size_t)w * h + i], rsv, ex);
+        t.b = mk_u4(ey, __float_as_uint(rs3.x), __float_as_uint(rs3.y), __float_as_uint(rs3.z));
+        st_tap(pc.taps + i, t);
+    }
 }
 
 template <class S>
@@ -252,30 +289,44 @@ struct DiffWR {
     float target;
 };
 
-template <class S>
-__device__ __forceinline__ DiffWR diff_load(const vgi_ind_diff_spatial_io& io, size_t j, f3 pix_pos, f3 pix_n) {
+__device__ __forceinline__ DiffWR diff_make(uint32_t rsv, uint32_t ex, uint32_t ey, f3 rad, f3 pix_pos, f3 pix_n) {
     DiffWR r;
-    ReservoirSimple rs = reservoir_decode_32b(G(io.temp_reservoir_texture)[j]);
+    ReservoirSimple rs = reservoir_decode_32b(rsv);
     r.M = rs.M;
     r.W = rs.W;
-    HitPosNormal hit = hit_decode(G(io.temp_hit_pos_normal_texture)[2 * j], G(io.temp_hit_pos_normal_texture)[2 * j + 1]);
+    HitPosNormal hit = hit_decode(ex, ey);
     r.pos = hit.pos;
     r.nrm = hit.normal;
-    r.rad = S::ld3(io.temp_hit_radiance_texture, j);
+    r.rad = rad;
     float nol = saturate(dot3(pix_n, normalize3(hit.pos - pix_pos)));
     r.target = nol * luminance(r.rad);
     return r;
 }
 
 template <class S>
-__device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const vgi_ind_diff_spatial_io& io, uint32_t frame_rand, float radius0, const SpiralTable& tab, uint32_t bx_, uint32_t by_) {
+__device__ __forceinline__ DiffWR diff_load(const vgi_ind_diff_spatial_io& io, size_t j, f3 pix_pos, f3 pix_n) {
+    return diff_make(G(io.temp_reservoir_texture)[j], G(io.temp_hit_pos_normal_texture)[2 * j],
+                     G(io.temp_hit_pos_normal_texture)[2 * j + 1], S::ld3(io.temp_hit_radiance_texture, j), pix_pos,
+                     pix_n);
+}
+__device__ __forceinline__ DiffWR diff_make_tap(const TapWords& t, f3 pix_pos, f3 pix_n) {
+    return diff_make(t.a.z, t.a.w, t.b.x, mk3(__uint_as_float(t.b.y), __uint_as_float(t.b.z), __uint_as_float(t.b.w)),
+                     pix_pos, pix_n);
+}
+
+// kTaps: the pixels' depth, normal layer and temporal reservoirs come from
+// the tap records (same values as the separate images, one load pair each).
+template <class S, bool kTaps>
+__device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const vgi_ind_diff_spatial_io& io, const DiffTap* taps, uint32_t frame_rand, float radius0, const SpiralTable& tab, uint32_t bx_, uint32_t by_) {
     uint32_t px, py;
     tile_pixel_at(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const size_t plane = (size_t)w * h;
-    float depth = G(io.gbuffer_depth)[i];
+    TapWords ct;
+    if constexpr (kTaps) ct = ld_tap(taps + i);
+    float depth = kTaps ? __uint_as_float(ct.a.x) : G(io.gbuffer_depth)[i];
     if (depth == 0.0f) {
         G(io.rw_reservoir_texture)[i] = 0u;
         G(io.rw_hit_pos_normal_texture)[2 * i] = 0u;
@@ -285,9 +336,9 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
         return;
     }
     const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
-    f3 nrm = decode_gbuffer_normal(G(io.gbuffer_color)[plane + i]).normal;
+    f3 nrm = decode_gbuffer_normal(kTaps ? ct.a.y : G(io.gbuffer_color)[plane + i]).normal;
     float ao = S::ld2(io.ao_texture, i).x;
-    DiffWR res = diff_load<S>(io, i, position, nrm);
+    DiffWR res = kTaps ? diff_make_tap(ct, position, nrm) : diff_load<S>(io, i, position, nrm);
     float w_sum = (res.W * res.target) * (float)res.M;
     float radius = fmaxf(radius0 * ao, 1.5f);
     uint32_t rng = lcg_init_with_seed(px, py, frame_rand);
@@ -300,8 +351,10 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
         int ox = (int)(x * radius), oy = (int)(y * radius);
         int cx = clampi((int)px + ox, 0, (int)w - 1), cy = clampi((int)py + oy, 0, (int)h - 1);
         size_t j = (size_t)cy * w + cx;
-        f3 nn = decode_gbuffer_normal(G(io.gbuffer_color)[plane + j]).normal;
-        float depth_n = G(io.gbuffer_depth)[j];
+        TapWords tt;
+        if constexpr (kTaps) tt = ld_tap(taps + j);
+        f3 nn = decode_gbuffer_normal(kTaps ? tt.a.y : G(io.gbuffer_color)[plane + j]).normal;
+        float depth_n = kTaps ? __uint_as_float(tt.a.x) : G(io.gbuffer_depth)[j];
         bool geometrical_diff = dot3(nrm, nn) < tab.cos_tol;
         f3 position_n = depth_to_position(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, depth_n);
         float dist_to_plane = dot3(position_n - position, nrm);
@@ -310,7 +363,7 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
             radius = fmaxf(radius * 0.5f, 1.5f);
             continue;
         }
-        DiffWR nb = diff_load<S>(io, j, position, nrm);
+        DiffWR nb = kTaps ? diff_make_tap(tt, position, nrm) : diff_load<S>(io, j, position, nrm);
         f3 off_qq = position_n - nb.pos;
         f3 off_rq = position - nb.pos;
         float L2_qq = dot3(off_qq, off_qq);
@@ -351,7 +404,11 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
 
 template <class S>
 __global__ void __launch_bounds__(64) k_diff_spatial(FrameParams fp, vgi_ind_diff_spatial_io io, uint32_t frame_rand, float radius0, SpiralTable tab) {
-    k_diff_spatial_body<S>(fp, io, frame_rand, radius0, tab, blockIdx.x, blockIdx.y);
+    k_diff_spatial_body<S, false>(fp, io, nullptr, frame_rand, radius0, tab, blockIdx.x, blockIdx.y);
+}
+template <class S>
+__global__ void __launch_bounds__(64) k_diff_spatial_taps(FrameParams fp, vgi_ind_diff_spatial_io io, const DiffTap* taps, uint32_t frame_rand, float radius0, SpiralTable tab) {
+    k_diff_spatial_body<S, true>(fp, io, taps, frame_rand, radius0, tab, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -692,11 +749,15 @@ template <class S>
 struct JobDiffSpatial {
     FrameParams fp;
     vgi_ind_diff_spatial_io io;
+    const DiffTap* taps;
     uint32_t frame_rand;
     float radius0;
     SpiralTable tab;
     __device__ static void run(const JobDiffSpatial& j, uint32_t bx, uint32_t by) {
-        k_diff_spatial_body<S>(j.fp, j.io, j.frame_rand, j.radius0, j.tab, bx, by);
+        if (j.taps)
+            k_diff_spatial_body<S, true>(j.fp, j.io, j.taps, j.frame_rand, j.radius0, j.tab, bx, by);
+        else
+            k_diff_spatial_body<S, false>(j.fp, j.io, nullptr, j.frame_rand, j.radius0, j.tab, bx, by);
     }
 };
 template <class S>
@@ -743,7 +804,8 @@ static vgi_status fill_diff_temporal(const PassRec* const* recs, int n, void* ho
         const PassRec& r = *recs[v];
         const vgi_pc_ind_diff_temporal& pc = r.pc.diff_temporal;
         t->jobs[v] = {to_fp(&r.fp), r.io.diff_temporal,
-                      {pc.frame_rand, pc.has_prev_frame, pc.has_new_sample, temporal_m_cap(&r.resample)}};
+                      {pc.frame_rand, pc.has_prev_frame, pc.has_new_sample, temporal_m_cap(&r.resample), r.diff_taps,
+                       r.diff_taps_gbuffer}};
         batch_add_pixel_tiles(t->hd, v, r.io.diff_temporal.width, r.io.diff_temporal.height);
     }
     return VGI_OK;
@@ -753,7 +815,7 @@ static vgi_status fill_diff_spatial(const PassRec* const* recs, int n, void* hos
     for (int v = 0; v < n; ++v) {
         const PassRec& r = *recs[v];
         VGI_CHECK(check_resample_params(&r.resample, "batched diffuse spatial"));
-        t->jobs[v] = {to_fp(&r.fp), r.io.diff_spatial, r.pc.diff_spatial.frame_rand,
+        t->jobs[v] = {to_fp(&r.fp), r.io.diff_spatial, r.diff_taps, r.pc.diff_spatial.frame_rand,
                       spatial_radius(&r.resample, r.io.diff_spatial.width, r.io.diff_spatial.height),
                       make_spiral_table(spiral_taps(&r.resample))};
         batch_add_pixel_tiles(t->hd, v, r.io.diff_spatial.width, r.io.diff_spatial.height);
@@ -836,10 +898,11 @@ extern "C" vgi_status vgi_pass_raytraced_ao_filter(const vgi_ao_filter_io* io, v
     return check_launch("k_ao_filter");
 }
 
-extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
-                                                          const vgi_pc_ind_diff_temporal* pc,
-                                                          const vgi_resample_params* params, void* stream) {
+vgi_status vgi::launch_diff_temporal(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
+                                     const vgi_pc_ind_diff_temporal* pc, const vgi_resample_params* params,
+                                     DiffTap* taps, const uint32_t* gbuffer_color, hipStream_t st) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_temporal_resample: null argument");
+    VGI_REQUIRE(!taps || gbuffer_color, "internal: diffuse tap records without the G-buffer");
     VGI_CHECK(check_resample_params(params, "vgi_pass_ind_diff_temporal_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
                 "vgi_pass_ind_diff_temporal_resample: bad size/storage");
@@ -850,17 +913,24 @@ extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params
     VGI_REQUIRE(!pc->has_prev_frame || (io->prev_gbuffer_depth && io->prev_reservoir_texture &&
                                         io->prev_hit_pos_normal_texture && io->prev_hit_radiance_texture),
                 "vgi_pass_ind_diff_temporal_resample: missing previous reservoir");
-    hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
-    ProfScope prof("ind_diff_temporal_resample", st, (double)io->width * io->height * (4 + 4 + 3 * (4 + 8 + R) - 4));
-    const DiffTemporalArgs args{pc->frame_rand, pc->has_prev_frame, pc->has_new_sample, temporal_m_cap(params)};
+    ProfScope prof("ind_diff_temporal_resample", st,
+                   (double)io->width * io->height * (4 + 4 + 3 * (4 + 8 + R) - 4 + (taps ? 4 + sizeof(DiffTap) : 0)));
+    const DiffTemporalArgs args{pc->frame_rand, pc->has_prev_frame, pc->has_new_sample, temporal_m_cap(params), taps,
+                                gbuffer_color};
     VGI_LAUNCH_S(io->storage, k_diff_temporal, tile_grid(io->width, io->height), to_fp(fp), *io, args);
     return check_launch("k_diff_temporal");
 }
 
-extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
-                                                         const vgi_pc_ind_diff_spatial* pc,
-                                                         const vgi_resample_params* params, void* stream) {
+extern "C" vgi_status vgi_pass_ind_diff_temporal_resample(const vgi_frame_params* fp, const vgi_ind_diff_temporal_io* io,
+                                                          const vgi_pc_ind_diff_temporal* pc,
+                                                          const vgi_resample_params* params, void* stream) {
+    return launch_diff_temporal(fp, io, pc, params, nullptr, nullptr, (hipStream_t)stream);
+}
+
+vgi_status vgi::launch_diff_spatial(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
+                                    const vgi_pc_ind_diff_spatial* pc, const vgi_resample_params* params,
+                                    const DiffTap* taps, hipStream_t st) {
     VGI_REQUIRE(fp && io && pc, "vgi_pass_ind_diff_spatial_resample: null argument");
     VGI_CHECK(check_resample_params(params, "vgi_pass_ind_diff_spatial_resample"));
     VGI_REQUIRE(valid_dims(io->width, io->height) && storage_ok(io->storage),
@@ -870,13 +940,24 @@ extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params*
                     io->rw_hit_pos_normal_texture && io->rw_hit_radiance_texture && io->rw_lighting_texture,
                 "vgi_pass_ind_diff_spatial_resample: missing binding");
     const SpiralTable tab = make_spiral_table(spiral_taps(params));
-    hipStream_t st = (hipStream_t)stream;
     size_t R = radiance_bytes(io->storage);
+    // algorithmic bytes of the pass's fields (a tap record carries the same ones)
     ProfScope prof("ind_diff_spatial_resample", st,
                    (double)io->width * io->height * (4 + 4 + pair_bytes(io->storage) / 2 + 2 * (4 + 8 + R) + R));
+    if (taps) {
+        VGI_LAUNCH_S(io->storage, k_diff_spatial_taps, tile_grid(io->width, io->height), to_fp(fp), *io, taps,
+                     pc->frame_rand, spatial_radius(params, io->width, io->height), tab);
+        return check_launch("k_diff_spatial_taps");
+    }
     VGI_LAUNCH_S(io->storage, k_diff_spatial, tile_grid(io->width, io->height), to_fp(fp), *io, pc->frame_rand,
                  spatial_radius(params, io->width, io->height), tab);
     return check_launch("k_diff_spatial");
+}
+
+extern "C" vgi_status vgi_pass_ind_diff_spatial_resample(const vgi_frame_params* fp, const vgi_ind_diff_spatial_io* io,
+                                                         const vgi_pc_ind_diff_spatial* pc,
+                                                         const vgi_resample_params* params, void* stream) {
+    return launch_diff_spatial(fp, io, pc, params, nullptr, (hipStream_t)stream);
 }
 
 extern "C" vgi_status vgi_pass_ind_spec_temporal_resample(const vgi_frame_params* fp, const vgi_ind_spec_temporal_io* io,

@@ -46,6 +46,8 @@ struct StorageF32 {
     }
     VGI_HD static float ld1(const void* p, size_t i) { return gld((const float*)p + i); }
     VGI_HD static void st1(void* p, size_t i, float v) { gst((float*)p + i, v); }
+    // the value ld3 returns after st3 of v
+    VGI_HD static f3 round3(f3 v) { return v; }
 };
 
 struct StorageF16 {
@@ -73,6 +75,10 @@ struct StorageF16 {
     }
     VGI_HD static float ld1(const void* p, size_t i) { return f16tof32(gld((const uint16_t*)p + i)); }
     VGI_HD static void st1(void* p, size_t i, float v) { gst((uint16_t*)p + i, (uint16_t)f32tof16(v)); }
+    // the value ld3 returns after st3 of v
+    VGI_HD static f3 round3(f3 v) {
+        return mk3(f16tof32(f32tof16(v.x)), f16tof32(f32tof16(v.y)), f16tof32(f32tof16(v.z)));
+    }
 };
 
 // Bilinear fetch with clamp-to-edge addressing (sampler_linear_clamp), in
