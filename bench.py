@@ -353,16 +353,6 @@ def big_scene(args):
     return mesh, sc
 
 
-def batch_view_specs(n: int, seed: int):
-    """Seeded resolutions w ~ U{320..1920}, h ~ U{240..1080}, one orbit camera each."""
-    from violet_amd import serve
-
-    rng = np.random.default_rng(seed)
-    ws, hs = rng.integers(320, 1921, n), rng.integers(240, 1081, n)
-    ph = rng.uniform(0.0, 360.0, n)
-    return [serve.ViewSpec(int(w), int(h), float(p)) for w, h, p in zip(ws, hs, ph)]
-
-
 def run_batched_form(args, sc, ntris):
     """16 views over the shared scene, FP16 storage: the plain path (each
     view's frame issued alone, one after the other) against one fused
@@ -372,7 +362,7 @@ def run_batched_form(args, sc, ntris):
 
     from violet_amd import _lib, autotune, batch, renderer, serve
 
-    specs = batch_view_specs(args.batch_views, seed=21)
+    specs = serve.batch_views(args.batch_views, seed=21)
     px = sum(s.width * s.height for s in specs)
     cfg = renderer.RestirConfig(storage=_lib.STORAGE_F16, atrous_iterations=args.iterations, **RESAMPLE[args.resample])
     fps_cache = {}

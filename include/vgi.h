@@ -445,6 +445,13 @@ typedef struct vgi_trace_resources {
     const vgi_hash_grid_cell* hash_grid_storage_buffer;
     vgi_hash_grid_query* rw_hash_grid_query_buffer;
     uint32_t* rw_hash_grid_query_counter_buffer;
+    /* NULL: every query is appended, as in the reference.  Otherwise a
+     * device word T written by vgi_pass_hash_grid_query_select_ordered: a
+     * query is appended only if the hash of its 24 bytes is >= T, a
+     * deterministic sample of about (2^32 - T) / 2^32 of the queries, so
+     * the appended set stays below VGI_HASH_GRID_MAX_QUERIES and does not
+     * depend on the order of the appends (no reference counterpart). */
+    const uint32_t* hash_grid_query_threshold;
 } vgi_trace_resources;
 
 /* restir/ind_diff_sample_gen.hlsl:11-23; restir_renderer.rs:478-492 */
@@ -672,6 +679,76 @@ typedef struct vgi_hash_grid_raygen_io {
 vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, const vgi_trace_resources* rt,
                                      const vgi_hash_grid_raygen_io* io, const vgi_pc_frame_hash* pc, void* stream);
 
+/* Order-independent cache update.  No counterpart in the reference: there
+ * (and in the three passes above) the slot a new cell takes, the query a
+ * cell selects (a random key drawn from the query's index in the append
+ * order), the order of the compacted list and the cache values the cache
+ * rays read all depend on how the GPU schedules the lanes, so two runs of
+ * the same frames drift apart (about 1e-2 relative mean after a few frames).
+ * These three variants give the same result for every schedule, so a view
+ * renders the same alone and inside a batch, run after run:
+ *
+ *   select   the keys to insert into a bucket go in one at a time, in
+ *            ascending checksum order, through the reference's find-or-
+ *            insert (the reference's result for that schedule); a cell
+ *            selects the query with the largest key drawn from the query's
+ *            contents (uniform over the cell's queries, like the reference's
+ *            key drawn from the query's index), and the query's index goes
+ *            to the winner buffer (identical queries are interchangeable);
+ *   compact  the selected cells in cell order, indices from the winner
+ *            buffer, appended after the counter's value on entry;
+ *   raygen   every cache ray reads the cache as it was before the pass:
+ *            blends go to a staging buffer (4 floats per compacted query)
+ *            and are written into the cells after all rays have traced.
+ *
+ * At 1920x1080 the frame's trace passes append about ten times as many
+ * queries as the buffer holds; the reference keeps the first
+ * VGI_HASH_GRID_MAX_QUERIES to arrive (an order set by the schedule).  With
+ * the threshold word of vgi_trace_resources the appends are a deterministic
+ * sample of about VGI_HASH_GRID_QUERY_TARGET queries instead.
+ *
+ * Left to scheduling: which keys get in when more than 16 keys of one
+ * bucket are new in one frame, and which queries survive when the sampled
+ * appends still exceed VGI_HASH_GRID_MAX_QUERIES. */
+typedef struct vgi_hash_grid_ordered_select_io {
+    const vgi_hash_grid_query* hash_grid_query_buffer;
+    const uint32_t* hash_grid_query_counter_buffer;
+    vgi_hash_grid_cell* rw_hash_grid_storage_buffer;
+    uint32_t* rw_hash_grid_query_selection_buffer; /* 1 word per cell, zero on entry */
+    uint32_t* rw_hash_grid_query_winner_buffer;    /* 1 word per cell, zero on entry, 16-byte aligned: on
+                                                    * return the index of the query a selected cell chose */
+    /* NULL, or the threshold word of vgi_trace_resources: replaced by the
+     * threshold for the queries appended next, from the counter of the
+     * queries selected here (those appended under the word's old value) */
+    uint32_t* rw_hash_grid_query_threshold;
+} vgi_hash_grid_ordered_select_io;
+/* The query threshold word for a frame that expects to append about
+ * `expected_queries` queries (before any thinning); 0 (keep every query)
+ * when they fit.  Queries are sampled down to VGI_HASH_GRID_QUERY_TARGET. */
+#define VGI_HASH_GRID_QUERY_TARGET (VGI_HASH_GRID_MAX_QUERIES / 4u * 3u)
+uint32_t vgi_hash_grid_query_threshold_for(uint64_t expected_queries);
+vgi_status vgi_pass_hash_grid_query_select_ordered(const vgi_hash_grid_ordered_select_io* io,
+                                                   const vgi_pc_frame_hash* pc, void* stream);
+
+#define VGI_HASH_GRID_COMPACT_WORKSPACE_WORDS 257u
+typedef struct vgi_hash_grid_ordered_compact_io {
+    const uint32_t* hash_grid_query_selection_buffer; /* 16-byte aligned */
+    const uint32_t* hash_grid_query_winner_buffer;
+    uint32_t* rw_compact_query_index_buffer;
+    uint32_t* rw_compact_query_cell_addr_buffer;
+    uint32_t* rw_compact_query_counter_buffer;
+    uint32_t* rw_workspace_buffer; /* VGI_HASH_GRID_COMPACT_WORKSPACE_WORDS words */
+} vgi_hash_grid_ordered_compact_io;
+vgi_status vgi_pass_hash_grid_query_compact_ordered(const vgi_hash_grid_ordered_compact_io* io, void* stream);
+
+typedef struct vgi_hash_grid_ordered_raygen_io {
+    vgi_hash_grid_raygen_io raygen;    /* as for vgi_pass_hash_grid_raygen */
+    float* rw_radiance_staging_buffer; /* 4 floats x VGI_HASH_GRID_MAX_QUERIES, 16-byte aligned */
+} vgi_hash_grid_ordered_raygen_io;
+vgi_status vgi_pass_hash_grid_raygen_ordered(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                             const vgi_hash_grid_ordered_raygen_io* io, const vgi_pc_frame_hash* pc,
+                                             void* stream);
+
 /* util/clear_buffer.hlsl:3-17; util_passes.rs:9-41 */
 vgi_status vgi_pass_clear_buffer(uint32_t* buffer, uint32_t length, uint32_t value, void* stream);
 
@@ -779,6 +856,11 @@ typedef struct vgi_renderer_config {
      * Halton(2,3) of (frame_index mod jitter_cycle) + 1, render_loop.rs:89-106. */
     uint32_t taa;
     vgi_denoiser_config denoiser;    /* default {1, 0, 3} */
+    /* default 1: the cache update runs the order-independent passes
+     * (vgi_pass_hash_grid_*_ordered), so a view renders the same frames run
+     * after run and alone or in a batch; 0 = the reference's passes, whose
+     * cache contents depend on the GPU's scheduling */
+    uint32_t ordered_cache_update;
 } vgi_renderer_config;
 
 /* Outputs of one frame: device images owned by the renderer, valid until
@@ -812,7 +894,8 @@ vgi_status vgi_renderer_add_passes(vgi_renderer* r, const vgi_scene* scene, cons
  * "diff_new_hit_radiance", "ao", "ao_filtered", "indirect_diffuse_raw",
  * "spec_reservoir", "spec_hit_pos", "spec_hit_radiance",
  * "indirect_specular_raw", "spec_ray_len", "hash_grid_storage",
- * "hash_grid_decay", "hash_grid_queries", "hash_grid_query_counter". */
+ * "hash_grid_decay", "hash_grid_queries", "hash_grid_query_counter",
+ * "hash_grid_query_threshold" (ordered_cache_update = 1 only). */
 vgi_status vgi_renderer_get_image(const vgi_renderer* r, const char* name, void** ptr, size_t* bytes);
 /* Per-view state snapshot (checkpoint/resume, serving eviction): copies all
  * temporal state to/from a host blob of vgi_renderer_state_size() bytes. */

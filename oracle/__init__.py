@@ -221,7 +221,7 @@ class OracleDenoiser:
 # ---------------------------------------------------------------------------
 class _TraceRes(C.Structure):
     _fields_ = [("scene", P), ("sky", C.c_float * 3), ("prev_ind_diff", P), ("prev_depth", P), ("pw", C.c_int32),
-                ("ph", C.c_int32), ("hg_cells", P), ("queries", P), ("query_counter", P)]
+                ("ph", C.c_int32), ("hg_cells", P), ("queries", P), ("query_counter", P), ("query_threshold", P)]
 
 
 def _restir_sigs(lib):
@@ -246,6 +246,13 @@ def _restir_sigs(lib):
     lib.oracle_hg_select.argtypes = [P, U32, P, P, U32]
     lib.oracle_hg_compact.restype = U32
     lib.oracle_hg_compact.argtypes = [P, P, P]
+    lib.oracle_hg_select_ordered.argtypes = [P, U32, P, P, P, P, U32]
+    lib.oracle_hg_query_threshold_for.restype = U32
+    lib.oracle_hg_query_threshold_for.argtypes = [C.c_uint64]
+    lib.oracle_hg_next_query_threshold.restype = U32
+    lib.oracle_hg_next_query_threshold.argtypes = [U32, U32]
+    lib.oracle_hg_compact_ordered.restype = U32
+    lib.oracle_hg_compact_ordered.argtypes = [P, P, P, P]
     lib.oracle_hg_raygen.argtypes = [P, TR, P, P, U32, P, P, P, U32]
     lib.oracle_raytraced_shadow.argtypes = [P, P, I, I, P, P]
     lib.oracle_final_lighting.argtypes = [P, I, I, P, P, P, P, P, P]
@@ -311,7 +318,7 @@ class OracleRestir:
         with np.errstate(invalid="ignore"):
             return np.ascontiguousarray(a, dtype=self.rt_dtype)
 
-    def trace_res(self, hg_cells, queries, counter, prev_ind_diff=None, prev_depth=None):
+    def trace_res(self, hg_cells, queries, counter, prev_ind_diff=None, prev_depth=None, threshold=None):
         tr = _TraceRes()
         tr.scene = self.scene.h
         tr.sky[:] = list(self.sky)
@@ -323,6 +330,7 @@ class OracleRestir:
         tr.hg_cells = _np_ptr(hg_cells)
         tr.queries = _np_ptr(queries)
         tr.query_counter = _np_ptr(counter)
+        tr.query_threshold = None if threshold is None else _np_ptr(threshold)
         return tr
 
     def trace_radiance(self, fp, tr, rays, has_prev):
@@ -473,6 +481,31 @@ class OracleRestir:
         n = self.lib.oracle_hg_compact(_np_ptr(np.ascontiguousarray(sel, np.uint32)), _np_ptr(idx), _np_ptr(addr))
         return idx[:n], addr[:n]
 
+    def hg_select_ordered(self, queries, counter, cells, frame_hash, threshold=None):
+        """Order-independent select: (cells, selection words, winner indices);
+        `threshold` (a 1-word array) is updated in place when given."""
+        cells = cells.copy()
+        sel = np.zeros(65536 * 16, np.uint32)
+        win = np.zeros(65536 * 16, np.uint32)
+        self.lib.oracle_hg_select_ordered(_np_ptr(np.ascontiguousarray(queries, np.uint32)), int(counter),
+                                          _np_ptr(cells), _np_ptr(sel), _np_ptr(win),
+                                          None if threshold is None else _np_ptr(threshold), frame_hash)
+        return cells, sel, win
+
+    def hg_query_threshold_for(self, expected: int) -> int:
+        return int(self.lib.oracle_hg_query_threshold_for(int(expected)))
+
+    def hg_next_query_threshold(self, kept: int, thr: int) -> int:
+        return int(self.lib.oracle_hg_next_query_threshold(int(kept), int(thr)))
+
+    def hg_compact_ordered(self, sel, win):
+        idx = np.zeros(self.NQ, np.uint32)
+        addr = np.zeros(self.NQ, np.uint32)
+        n = self.lib.oracle_hg_compact_ordered(_np_ptr(np.ascontiguousarray(sel, np.uint32)),
+                                               _np_ptr(np.ascontiguousarray(win, np.uint32)), _np_ptr(idx),
+                                               _np_ptr(addr))
+        return idx[:n], addr[:n]
+
     def hg_raygen(self, fp, tr, cells, queries, cidx, caddr, decay, frame_hash):
         cells, decay = cells.copy(), decay.copy()
         cidx = np.ascontiguousarray(cidx, np.uint32)
@@ -500,7 +533,8 @@ class OracleRenderer:
 
     def __init__(self, bits: int, mesh, width: int, height: int, sky=(1.0, 1.0, 1.0), denoise=True,
                  ind_diff_validate=True, ind_spec_validate=True, hash_grid_cache_decay=True, ao_radius=2.0,
-                 atrous_iterations=3, temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0):
+                 atrous_iterations=3, temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0,
+                 ordered_cache_update=True):
         self.bits = bits
         self.scene = OracleScene(bits, mesh)
         self.R = OracleRestir(bits, self.scene, sky)
@@ -509,6 +543,7 @@ class OracleRenderer:
         self.denoise, self.ind_diff_validate, self.ind_spec_validate = denoise, ind_diff_validate, ind_spec_validate
         self.decay_on, self.ao_radius = hash_grid_cache_decay, ao_radius
         self.m_cap, self.taps, self.radius_px = temporal_m_cap, diff_spatial_taps, spatial_radius_px
+        self.ordered = ordered_cache_update
         self.cells = None
         self.prev = None  # dict of last frame's state
 
@@ -521,23 +556,34 @@ class OracleRenderer:
             self.decay = np.zeros(self.NC, np.uint8)
             self.q = np.zeros((131072, 6), np.uint32)
             self.cnt = np.zeros(1, np.uint32)
+            # ordered update: the query threshold word before the first count
+            self.thr = (np.array([R.hg_query_threshold_for(2 * w * h + 131072)], np.uint32) if self.ordered
+                        else None)
         if self.decay_on:
             self.decay, self.cells = R.hg_decay(self.decay, self.cells)
         p = self.prev
         has_prev = p is not None
         hg = False
         if has_prev:
-            self.cells, sel = R.hg_select(self.q, int(self.cnt[0]), self.cells, next(seeds))
-            cidx, caddr = R.hg_compact(sel)
+            if self.ordered:
+                self.cells, sel, win = R.hg_select_ordered(self.q, int(self.cnt[0]), self.cells, next(seeds),
+                                                           self.thr)
+                cidx, caddr = R.hg_compact_ordered(sel, win)
+                lookup = self.cells.copy()  # the cache rays read the cache from before the pass
+            else:
+                self.cells, sel = R.hg_select(self.q, int(self.cnt[0]), self.cells, next(seeds))
+                cidx, caddr = R.hg_compact(sel)
+                lookup = self.cells
             nq = np.zeros((131072, 6), np.uint32)
             ncnt = np.zeros(1, np.uint32)
-            tr = R.trace_res(self.cells, nq, ncnt, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"])
+            tr = R.trace_res(lookup, nq, ncnt, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"],
+                             threshold=self.thr)
             self.cells, self.decay = R.hg_raygen(fp, tr, self.cells, self.q, cidx, caddr, self.decay, next(seeds))
             self.q, self.cnt = nq, ncnt
             hg = True
         validation = has_prev and (frame_index & 3) == 0 and self.ind_diff_validate
         tr = R.trace_res(self.cells, self.q, self.cnt, prev_ind_diff=p["out_diffuse"] if has_prev else None,
-                         prev_depth=p["depth"] if has_prev else None)
+                         prev_depth=p["depth"] if has_prev else None, threshold=self.thr)
         if not validation:
             hit, rad = R.diff_sample_gen(fp, tr, gb, depth, next(seeds), has_prev)
             prev_rsv = (p["d_rsv"], p["d_hit"], p["d_rad"]) if has_prev else (None, None, None)
@@ -565,4 +611,4 @@ class OracleRenderer:
         self.prev = dict(depth=depth, gb=gb, d_rsv=d_rsv, d_hit=d_hit, d_rad=d_rad, s_rsv=s_rsv, s_hp=s_hp,
                          s_rad=s_rad, ao=ao, out_diffuse=out_d)
         return dict(diffuse=out_d, specular=out_s, depth=depth, gbuffer=gb, hash_grid_updated=hg,
-                    validation=validation, diff_light=d_light, spec_light=s_light)
+                    validation=validation, diff_light=d_light, spec_light=s_light, diff_rad=d_rad, spec_rad=s_rad)

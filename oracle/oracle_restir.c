@@ -64,7 +64,25 @@ typedef struct {
     const uint32_t* hg_cells; /* 4 words per cell: checksum, rgb (f32 bits) */
     uint32_t* queries;        /* 6 words per query */
     uint32_t* query_counter;
+    const uint32_t* query_threshold; /* NULL: append every query (the reference) */
 } otrace_res;
+
+/* Hash of a query's six words (jenkins chain); see oracle_hg_select_ordered. */
+static uint32_t o_query_hash(const uint32_t* q) {
+    uint32_t h = o_jenkins(q[0]);
+    for (int k = 1; k < 6; ++k) h = o_jenkins(h ^ q[k]);
+    return h;
+}
+
+/* Threshold word keeping about 3/4 * HG_MAX_QUERIES of `expected` query
+ * candidates (a candidate is kept if its hash is >= the word). */
+#define HG_QUERY_TARGET (128u * 1024u / 4u * 3u)
+uint32_t oracle_hg_query_threshold_for(uint64_t expected) {
+    if (expected <= HG_QUERY_TARGET) return 0u;
+    uint64_t keep = ((uint64_t)HG_QUERY_TARGET << 32) / expected;
+    if (keep == 0) keep = 1;
+    return (uint32_t)((1ull << 32) - keep);
+}
 
 typedef struct {
     v3 pos, normal, radiance;
@@ -246,17 +264,12 @@ static otrace_out o_trace_radiance(const otrace_res* rt, const ofp* fp, v3 o, v3
         o_hg_key(pos, o_xyz(fp->view_pos), &hh, &ck);
         v3 cr;
         if (o_hg_find(rt->hg_cells, hh, ck, &cr, &addr)) rad = add3(rad, mul3(rho, cr));
-        uint32_t qi;
+        uint32_t qw[6] = {f_bits((float)pos.x), f_bits((float)pos.y), f_bits((float)pos.z), o_oct_enc_u32(ng), hh, ck};
+        if (!rt->query_threshold || o_query_hash(qw) >= *rt->query_threshold) {
+            uint32_t qi;
 #pragma omp atomic capture
-        qi = (*rt->query_counter)++;
-        if (qi < HG_MAX_QUERIES) {
-            uint32_t* q = rt->queries + 6 * (size_t)qi;
-            q[0] = f_bits((float)pos.x);
-            q[1] = f_bits((float)pos.y);
-            q[2] = f_bits((float)pos.z);
-            q[3] = o_oct_enc_u32(ng);
-            q[4] = hh;
-            q[5] = ck;
+            qi = (*rt->query_counter)++;
+            if (qi < HG_MAX_QUERIES) memcpy(rt->queries + 6 * (size_t)qi, qw, sizeof(qw));
         }
     }
     r.pos = pos;
@@ -384,6 +397,8 @@ void oracle_spec_sample_gen(const float* fpf, const otrace_res* rt, int w, int h
             size_t i = (size_t)py * w + px;
             real dpt = depth[i];
             if (dpt == RC(0.0)) {
+                /* unwritten in the reference; zero here and in the library */
+                for (int k = 0; k < 4; ++k) o_hitpos[4 * i + k] = RC(0.0);
                 st3r(o_rad, i, V3s(0));
                 continue;
             }
@@ -1054,8 +1069,107 @@ uint32_t oracle_hg_compact(const uint32_t* selection, uint32_t* out_index, uint3
     return n;
 }
 
+/* Order-independent cache update (the library's *_ordered passes; no
+ * reference counterpart).  The keys that the reference's probe would not
+ * find in front of their bucket's first free slot go in one at a time, in
+ * ascending checksum order, through find-or-insert (the reference's result
+ * for the schedule that runs them in that order).  Each query draws a
+ * selection word from its 24 bytes (the reference's jenkins + xorshift draw
+ * seeded by a hash of the contents instead of the index, bit 31 set); a cell
+ * keeps the largest word and the index of a query that drew it. */
+static uint32_t o_query_select_word(const uint32_t* q, uint32_t frame_hash) {
+    uint32_t st = o_jenkins(o_query_hash(q) ^ frame_hash);
+    return o_xorshift(&st) | 0x80000000u;
+}
+
+/* First slot of the bucket holding `checksum`, or -1. */
+static int64_t o_key_cell(const uint32_t* cells, uint32_t hash, uint32_t checksum) {
+    for (uint32_t a = hash << 4; a < (hash << 4) + 16; ++a)
+        if (cells[4 * (size_t)a] == checksum) return a;
+    return -1;
+}
+
+static int o_cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* The `n` queries kept under the word `thr` sample about n * 2^32 / (2^32 -
+ * thr) candidates: the word that keeps the target of as many. */
+uint32_t oracle_hg_next_query_threshold(uint32_t n, uint32_t thr) {
+    uint64_t keep = (1ull << 32) - thr;
+    return oracle_hg_query_threshold_for(((uint64_t)n << 32) / keep);
+}
+
+void oracle_hg_select_ordered(const uint32_t* queries, uint32_t counter, uint32_t* cells, uint32_t* selection,
+                              uint32_t* winner, uint32_t* threshold, uint32_t frame_hash) {
+    uint32_t n = counter < HG_MAX_QUERIES ? counter : HG_MAX_QUERIES;
+    if (threshold) *threshold = oracle_hg_next_query_threshold(counter, *threshold);
+    /* candidate keys per bucket (at most 16, as the library's table) */
+    uint32_t* cand = (uint32_t*)calloc(65536u * 16u, 4);
+    for (uint32_t id = 0; id < n; ++id) {
+        uint32_t hash = queries[6 * (size_t)id + 4], ck = queries[6 * (size_t)id + 5], a = hash << 4;
+        for (; a < (hash << 4) + 16; ++a) {
+            uint32_t c = cells[4 * (size_t)a];
+            if (c == ck || c == 0) break;
+        }
+        if (a == (hash << 4) + 16 || cells[4 * (size_t)a] == ck) continue;
+        for (uint32_t t = hash << 4; t < (hash << 4) + 16; ++t) {
+            if (cand[t] == ck) break;
+            if (cand[t] == 0) {
+                cand[t] = ck;
+                break;
+            }
+        }
+    }
+    for (uint32_t b = 0; b < 65536u; ++b) {
+        uint32_t* key = cand + 16 * (size_t)b;
+        uint32_t m = 0;
+        while (m < 16 && key[m]) ++m;
+        qsort(key, m, 4, o_cmp_u32);
+        for (uint32_t i = 0; i < m; ++i)
+            for (uint32_t a = b * 16; a < b * 16 + 16; ++a) {
+                uint32_t* c = cells + 4 * (size_t)a;
+                if (c[0] == key[i]) break;
+                if (c[0] == 0) {
+                    c[0] = key[i];
+                    break;
+                }
+            }
+    }
+    free(cand);
+    for (uint32_t id = 0; id < n; ++id) {
+        const uint32_t* q = queries + 6 * (size_t)id;
+        int64_t a = o_key_cell(cells, q[4], q[5]);
+        if (a < 0) continue;
+        uint32_t w = o_query_select_word(q, frame_hash);
+        if (w > selection[a]) selection[a] = w;
+    }
+    for (uint32_t id = 0; id < n; ++id) {
+        const uint32_t* q = queries + 6 * (size_t)id;
+        int64_t a = o_key_cell(cells, q[4], q[5]);
+        if (a >= 0 && selection[a] == o_query_select_word(q, frame_hash)) winner[a] = id;
+    }
+}
+
+uint32_t oracle_hg_compact_ordered(const uint32_t* selection, const uint32_t* winner, uint32_t* out_index,
+                                   uint32_t* out_addr) {
+    uint32_t n = 0;
+    for (uint32_t c = 0; c < 65536u * 16u; ++c)
+        if (selection[c] >> 31) {
+            if (n < HG_MAX_QUERIES) {
+                out_index[n] = winner[c] & (HG_MAX_QUERIES - 1u);
+                out_addr[n] = c;
+            }
+            ++n;
+        }
+    return n;
+}
+
 /* One cosine ray per compacted query, in compacted order; the cache lookup
- * of the trace reads `cells` as updated so far (sequential schedule). */
+ * of the trace reads rt->hg_cells (`cells` as updated so far when they are
+ * the same buffer: one sequential schedule of the reference's pass; a copy
+ * of the cache from before the pass: the ordered pass). */
 void oracle_hg_raygen(const float* fpf, const otrace_res* rt, uint32_t* cells, const uint32_t* queries, uint32_t count,
                       const uint32_t* cidx, const uint32_t* caddr, uint8_t* decay, uint32_t frame_hash) {
     ofp fp;

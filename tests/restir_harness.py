@@ -53,6 +53,7 @@ SNAP = {
     "hash_grid_decay": (np.uint8, lambda w, h: (65536 * 16,)),
     "hash_grid_queries": (np.uint32, lambda w, h: (128 * 1024, 6)),
     "hash_grid_query_counter": (np.uint32, lambda w, h: (1,)),
+    "hash_grid_query_threshold": (np.uint32, lambda w, h: (1,)),
 }
 
 

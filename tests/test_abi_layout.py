@@ -42,6 +42,8 @@ def _mirrors():
         "vgi_temporal_aa_io": _abi.TemporalAAIO, "vgi_pc_temporal_aa": _abi.PCTemporalAA,
         "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
         "vgi_resample_params": _abi.ResampleParams, "vgi_denoise_frame_io": _lib.DenoiseFrameIO,
+        "vgi_hash_grid_ordered_select_io": _abi.HGOSelectIO, "vgi_hash_grid_ordered_compact_io": _abi.HGOCompactIO,
+        "vgi_hash_grid_ordered_raygen_io": _abi.HGORaygenIO,
     }
     return m
 

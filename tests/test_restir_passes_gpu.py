@@ -1,6 +1,8 @@
 """Per-pass replay through the C ABI of the ReSTIR passes whose results the
 frame tests only see folded into later passes: the hash-grid cache update
-(decay, query select, compaction, cache rays), the diffuse and specular
+(decay, query select, compaction, cache rays; the reference's passes and
+the order-independent ones, which must match the oracle bit for bit
+everywhere), the diffuse and specular
 sample validation, specular sample generation with a screen cache, and the
 specular temporal resampling with history (its balance-heuristic MIS merge).
 
@@ -46,11 +48,22 @@ NQ = 128 * 1024
 
 @pytest.fixture(scope="module")
 def run():
+    """The renderer with the reference's cache passes."""
     from violet_amd import renderer
 
     mesh = make_mesh()
-    fps, outs, snaps, r, sc = run_gpu_frames(mesh, W, H, FRAMES, config=renderer.RestirConfig())
+    fps, outs, snaps, r, sc = run_gpu_frames(mesh, W, H, FRAMES,
+                                             config=renderer.RestirConfig(ordered_cache_update=False))
     return dict(mesh=mesh, fps=fps, snaps=snaps, r=r, sc=sc)
+
+
+@pytest.fixture(scope="module")
+def run_ordered(run):
+    """The renderer with its default, order-independent cache passes."""
+    from violet_amd import renderer
+
+    fps, outs, snaps, r, sc = run_gpu_frames(run["mesh"], W, H, FRAMES, config=renderer.RestirConfig())
+    return dict(mesh=run["mesh"], fps=fps, snaps=snaps, r=r, sc=sc)
 
 
 @pytest.fixture(scope="module")
@@ -261,6 +274,77 @@ def test_hash_grid_cache_update_replayed(vgi, run, o32, k):
     s = snaps[k]
     assert np.array_equal(bucket_sets(cells3, np.zeros(NCELLS, np.uint32))[:, :2],
                           bucket_sets(s["hash_grid_storage"], np.zeros(NCELLS, np.uint32))[:, :2])
+
+
+@pytest.mark.parametrize("k", [2, 5, 8])
+def test_hash_grid_ordered_update_replayed(vgi, run_ordered, o32, k):
+    """Frame k's order-independent cache update (decay, ordered select,
+    ordered compaction, ordered cache rays) pass by pass: every output bit-
+    exact with the oracle's FP32 twin, and the replay's cache equals the
+    renderer's frame k cache bit for bit."""
+    from violet_amd import _abi
+
+    run = run_ordered
+    snaps, fp = run["snaps"], run["fps"][k]
+    assert snaps[k]["flags"]["hg"], f"frame {k} should update the cache"
+    p = snaps[k - 1]
+    seeds = seeds_for_frame(k, True, snaps[k]["flags"]["validation"])
+    dev = Dev()
+    cells0, decay0 = p["hash_grid_storage"], p["hash_grid_decay"]
+    queries, counter = p["hash_grid_queries"], p["hash_grid_query_counter"]
+    nq = int(counter[0])
+    assert nq > 500, f"too few queries to exercise the cache ({nq})"
+
+    gc, gd = dev.put(cells0), dev.put(decay0)
+    call("vgi_pass_hash_grid_decay", C.byref(_abi.HGDecayIO(gd, gc)))
+    cells1, decay1 = Dev.get(gc, np.uint32, (NCELLS, 4)), Dev.get(gd, np.uint8, (NCELLS,))
+
+    gq, gn, gs, gw = dev.put(queries), dev.put(counter), dev.zeros(NCELLS * 4), dev.zeros(NCELLS * 4)
+    thr0 = p["hash_grid_query_threshold"]
+    gt = dev.put(thr0)
+    call("vgi_pass_hash_grid_query_select_ordered", C.byref(_abi.HGOSelectIO(gq, gn, gc, gs, gw, gt)),
+         C.byref(_abi.PCFrameHash(seeds["hg_select"])))
+    cells2, sel, win = (Dev.get(gc, np.uint32, (NCELLS, 4)), Dev.get(gs, np.uint32, (NCELLS,)),
+                        Dev.get(gw, np.uint32, (NCELLS,)))
+    o_thr = thr0.copy()
+    o_cells2, o_sel, o_win = o32.hg_select_ordered(queries, nq, cells1, seeds["hg_select"], o_thr)
+    thr1 = Dev.get(gt, np.uint32, (1,))
+    assert thr1[0] == o_thr[0] and thr1[0] == snaps[k]["hash_grid_query_threshold"][0]
+    bits_equal(cells2, o_cells2, f"ordered select cells[{k}]")
+    bits_equal(sel, o_sel, f"ordered select words[{k}]")
+    chosen = np.nonzero(sel)[0]
+    assert len(chosen) > 100
+    assert np.array_equal(queries[win[chosen]], queries[o_win[chosen]]), f"selected queries differ [{k}]"
+    inserted = int((cells2[:, 0] != 0).sum() - (cells1[:, 0] != 0).sum())
+    assert inserted > 0
+
+    gi, ga, gcc = dev.zeros(NQ * 4), dev.zeros(NQ * 4), dev.zeros(4)
+    gws = dev.zeros(_abi.HG_COMPACT_WORKSPACE_WORDS * 4)
+    call("vgi_pass_hash_grid_query_compact_ordered", C.byref(_abi.HGOCompactIO(gs, gw, gi, ga, gcc, gws)))
+    n = int(Dev.get(gcc, np.uint32, (1,))[0])
+    idx, addr = Dev.get(gi, np.uint32, (NQ,))[:n], Dev.get(ga, np.uint32, (NQ,))[:n]
+    o_idx, o_addr = o32.hg_compact_ordered(o_sel, o_win)
+    assert np.array_equal(addr, o_addr), f"compaction order differs [{k}]"
+    assert np.array_equal(queries[idx], queries[o_idx])
+
+    tr, q_new, c_new = gpu_trace_res(dev, run, gc, p)
+    tr.hash_grid_query_threshold = gt
+    gst = dev.zeros(NQ * 16)
+    io = _abi.HGORaygenIO(_abi.HGRaygenIO(gc, gq, gcc, gi, ga, gd), gst)
+    call("vgi_pass_hash_grid_raygen_ordered", C.byref(fp), C.byref(tr), C.byref(io),
+         C.byref(_abi.PCFrameHash(seeds["hg_raygen"])))
+    cells3, decay3 = Dev.get(gc, np.uint32, (NCELLS, 4)), Dev.get(gd, np.uint8, (NCELLS,))
+    g_newq, _ = read_queries(q_new, c_new)
+    oq, oc = np.zeros((NQ, 6), np.uint32), np.zeros(1, np.uint32)
+    before = np.ascontiguousarray(cells2)  # the cache rays read the cache from before the pass
+    otr = o32.trace_res(before, oq, oc, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"], threshold=o_thr)
+    o_cells3, o_decay3 = o32.hg_raygen(fp, otr, cells2, queries, idx, addr, decay1, seeds["hg_raygen"])
+    bits_equal(cells3, o_cells3, f"ordered cache rays cells[{k}]")
+    bits_equal(decay3, o_decay3, f"ordered cache rays decay[{k}]")
+    assert multiset(g_newq) == multiset(oq[: int(oc[0])]), f"cache-ray queries differ [{k}]"
+    # the renderer's own frame k
+    bits_equal(cells3, snaps[k]["hash_grid_storage"], f"renderer cache[{k}]")
+    bits_equal(decay3, snaps[k]["hash_grid_decay"], f"renderer decay[{k}]")
 
 
 @pytest.mark.parametrize("k", [4, 8])

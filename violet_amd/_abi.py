@@ -22,7 +22,8 @@ WH = [("width", u32), ("height", u32), ("storage", u32)]
 TraceResources = _struct("TraceResources", [
     ("scene_tlas", vp), ("sky_radiance", f32 * 3), ("storage", u32),
     ("prev_indirect_diffuse_texture", vp), ("prev_depth_texture", vp), ("prev_width", u32), ("prev_height", u32),
-    ("hash_grid_storage_buffer", vp), ("rw_hash_grid_query_buffer", vp), ("rw_hash_grid_query_counter_buffer", vp)])
+    ("hash_grid_storage_buffer", vp), ("rw_hash_grid_query_buffer", vp), ("rw_hash_grid_query_counter_buffer", vp),
+    ("hash_grid_query_threshold", vp)])
 
 DiffSampleGenIO = _struct("DiffSampleGenIO", WH + [
     ("gbuffer_color", vp), ("gbuffer_depth", vp), ("rw_hit_pos_normal_texture", vp), ("rw_hit_radiance_texture", vp)])
@@ -72,6 +73,16 @@ HGCompactIO = _struct("HGCompactIO", [
 HGRaygenIO = _struct("HGRaygenIO", [
     ("rw_hash_grid_storage_buffer", vp), ("hash_grid_query_buffer", vp), ("compact_query_counter_buffer", vp),
     ("compact_query_index_buffer", vp), ("compact_query_cell_addr_buffer", vp), ("rw_hash_grid_decay_buffer", vp)])
+HGOSelectIO = _struct("HGOSelectIO", [
+    ("hash_grid_query_buffer", vp), ("hash_grid_query_counter_buffer", vp), ("rw_hash_grid_storage_buffer", vp),
+    ("rw_hash_grid_query_selection_buffer", vp), ("rw_hash_grid_query_winner_buffer", vp),
+    ("rw_hash_grid_query_threshold", vp)])
+HG_COMPACT_WORKSPACE_WORDS = 257
+HGOCompactIO = _struct("HGOCompactIO", [
+    ("hash_grid_query_selection_buffer", vp), ("hash_grid_query_winner_buffer", vp),
+    ("rw_compact_query_index_buffer", vp), ("rw_compact_query_cell_addr_buffer", vp),
+    ("rw_compact_query_counter_buffer", vp), ("rw_workspace_buffer", vp)])
+HGORaygenIO = _struct("HGORaygenIO", [("raygen", HGRaygenIO), ("rw_radiance_staging_buffer", vp)])
 GBufferIO = _struct("GBufferIO", [("width", u32), ("height", u32), ("rw_depth", vp), ("rw_gbuffer", vp)])
 RaytracedShadowIO = _struct("RaytracedShadowIO", [("width", u32), ("height", u32), ("gbuffer_depth", vp),
                                                   ("rw_shadow", vp)])
@@ -86,7 +97,7 @@ RendererConfig = _struct("RendererConfig", [
     ("storage", u32), ("denoise", u32), ("ind_diff_validate", u32), ("ind_spec_validate", u32),
     ("hash_grid_cache_decay", u32), ("ao_radius", f32), ("sky_radiance", f32 * 3),
     ("temporal_m_cap", u32), ("diff_spatial_taps", u32), ("spatial_radius_px", f32), ("concurrent_lanes", u32),
-    ("compose", u32), ("taa", u32), ("denoiser", _lib.DenoiserConfig)])
+    ("compose", u32), ("taa", u32), ("denoiser", _lib.DenoiserConfig), ("ordered_cache_update", u32)])
 RendererOutput = _struct("RendererOutput", [
     ("indirect_diffuse", vp), ("indirect_specular", vp), ("depth", vp), ("gbuffer", vp), ("denoised", u32),
     ("validation_frame", u32), ("hash_grid_updated", u32), ("color", vp)])
@@ -111,6 +122,9 @@ _SIGS = {
     "vgi_pass_hash_grid_query_select": [P(HGSelectIO), P(PCFrameHash), vp],
     "vgi_pass_hash_grid_query_compact": [P(HGCompactIO), vp],
     "vgi_pass_hash_grid_raygen": [FPp, TRp, P(HGRaygenIO), P(PCFrameHash), vp],
+    "vgi_pass_hash_grid_query_select_ordered": [P(HGOSelectIO), P(PCFrameHash), vp],
+    "vgi_pass_hash_grid_query_compact_ordered": [P(HGOCompactIO), vp],
+    "vgi_pass_hash_grid_raygen_ordered": [FPp, TRp, P(HGORaygenIO), P(PCFrameHash), vp],
     "vgi_pass_clear_buffer": [vp, u32, u32, vp],
     "vgi_pass_gbuffer_primary": [FPp, vp, P(GBufferIO), vp],
     "vgi_pass_raytraced_shadow": [FPp, vp, P(RaytracedShadowIO), vp],
@@ -133,3 +147,4 @@ for _n, _a in _SIGS.items():
     _lib.register(_n, C.c_int32, _a)
 _lib.register("vgi_renderer_state_size", C.c_size_t, [vp])
 _lib.register("vgi_renderer_device_bytes", C.c_size_t, [vp])
+_lib.register("vgi_hash_grid_query_threshold_for", u32, [C.c_uint64])

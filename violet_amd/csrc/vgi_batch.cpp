@@ -45,7 +45,9 @@ static const char* kind_name(int kind) {
         "batch_ind_spec_sample_gen", "batch_ind_spec_sample_validate", "batch_ind_spec_temporal_resample",
         "batch_ind_spec_spatial_resample", "batch_denoise_temporal", "batch_denoise_disocclusion_fix",
         "batch_denoise_fast_history_clamp", "batch_denoise_spatial_fallback", "batch_denoise_atrous",
-        "batch_raytraced_shadow", "batch_final_lighting", "batch_temporal_aa"};
+        "batch_raytraced_shadow", "batch_final_lighting", "batch_temporal_aa",
+        "batch_hash_grid_query_select_ordered", "batch_hash_grid_query_compact_ordered",
+        "batch_hash_grid_raygen_ordered"};
     return kind >= 0 && kind < kPassKindCount ? n[kind] : "batch_unknown";
 }
 }  // namespace vgi

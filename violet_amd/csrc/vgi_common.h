@@ -522,6 +522,18 @@ constexpr uint32_t kHashGridBucketShift = 4;
 constexpr uint32_t kHashGridNumCells = kHashGridNumBuckets * kHashGridBucketSize;
 constexpr uint32_t kHashGridMaxQueries = 128 * 1024;
 constexpr uint32_t kHashGridMaxQueriesShift = 17;
+// Appends the ordered cache update samples the frame's queries down to.
+constexpr uint32_t kHashGridQueryTarget = kHashGridMaxQueries / 4u * 3u;
+
+// Threshold word that keeps about kHashGridQueryTarget of `expected` query
+// candidates: a candidate is kept if its 32-bit hash is >= the word, which
+// happens with probability (2^32 - word) / 2^32.  0 keeps every candidate.
+VGI_HD uint32_t query_threshold_for(uint64_t expected) {
+    if (expected <= kHashGridQueryTarget) return 0u;
+    uint64_t keep = ((uint64_t)kHashGridQueryTarget << 32) / expected;  // < 2^32
+    if (keep == 0) keep = 1;
+    return (uint32_t)((1ull << 32) - keep);
+}
 
 // uint(max(0.5 * log2(d2), 0)) evaluated exactly: for d2 >= 1 with binary
 // exponent e, 0.5*log2(d2) lies in [e/2, (e+1)/2), so the truncation is e>>1.

@@ -23,15 +23,15 @@ vgi_status Lanes::enter(int slot, hipStream_t& st) {
         VGI_TRY_HIP(hipEventRecord(ev_fork, main));
         VGI_TRY_HIP(hipStreamWaitEvent(side, ev_fork, 0));
         phase = ph;
-        cross_from = nullptr;
+        cross_pending = false;
     }
     if (slot_lane(slot)) {
         st = side;
         side_work = true;
     }
-    if (slot_waits(slot) && cross_from && cross_from != st) {
+    if (slot_waits(slot) && cross_pending && cross_from != st) {
         VGI_TRY_HIP(hipStreamWaitEvent(st, ev_cross, 0));
-        cross_from = nullptr;
+        cross_pending = false;
     }
     return VGI_OK;
 }
@@ -39,7 +39,8 @@ vgi_status Lanes::enter(int slot, hipStream_t& st) {
 vgi_status Lanes::leave(int slot, hipStream_t st) {
     if (!side || !ev_cross || !slot_signals(slot)) return VGI_OK;
     VGI_TRY_HIP(hipEventRecord(ev_cross, st));
-    cross_from = st;
+    cross_from = st;  // may be the null stream: cross_pending says that ev_cross is set
+    cross_pending = true;
     return VGI_OK;
 }
 
@@ -54,12 +55,23 @@ vgi_status DirectSink::submit(int slot, const PassRec& r) {
 vgi_status DirectSink::launch(const PassRec& r, hipStream_t st) {
     void* s = (void*)st;
     switch (r.kind) {
-        case kPassClear: return fill_zero(r.clear_ptr, r.clear_bytes, st);
+        case kPassClear:
+            if (r.clear_value) {
+                VGI_REQUIRE(r.clear_bytes % 4 == 0 && ((uintptr_t)r.clear_ptr & 3u) == 0,
+                            "internal: word fill of an unaligned buffer");
+                return fill_u32((uint32_t*)r.clear_ptr, r.clear_bytes / 4, r.clear_value, st);
+            }
+            return fill_zero(r.clear_ptr, r.clear_bytes, st);
         case kPassGBuffer: return vgi_pass_gbuffer_primary(&r.fp, r.scene, &r.io.gbuffer, s);
         case kPassHgDecay: return vgi_pass_hash_grid_decay(&r.io.hg_decay, s);
         case kPassHgSelect: return vgi_pass_hash_grid_query_select(&r.io.hg_select, &r.pc.frame_hash, s);
         case kPassHgCompact: return vgi_pass_hash_grid_query_compact(&r.io.hg_compact, s);
         case kPassHgRaygen: return vgi_pass_hash_grid_raygen(&r.fp, &r.tr, &r.io.hg_raygen, &r.pc.frame_hash, s);
+        case kPassHgSelectOrdered:
+            return vgi_pass_hash_grid_query_select_ordered(&r.io.hg_oselect, &r.pc.frame_hash, s);
+        case kPassHgCompactOrdered: return vgi_pass_hash_grid_query_compact_ordered(&r.io.hg_ocompact, s);
+        case kPassHgRaygenOrdered:
+            return vgi_pass_hash_grid_raygen_ordered(&r.fp, &r.tr, &r.io.hg_oraygen, &r.pc.frame_hash, s);
         case kPassDiffGen: return vgi_pass_ind_diff_sample_gen(&r.fp, &r.tr, &r.io.diff_gen, &r.pc.sample_gen, s);
         case kPassDiffValidate: return vgi_pass_ind_diff_sample_validate(&r.fp, &r.tr, &r.io.diff_validate, s);
         case kPassAoGen: return vgi_pass_raytraced_ao_gen(&r.fp, &r.io.ao_gen, &r.pc.ao_gen, s);

@@ -40,6 +40,9 @@ enum PassKind : int {
     kPassShadow,          // raytraced_shadow.hlsl
     kPassFinalLighting,   // restir/final_lighting.hlsl
     kPassTemporalAA,      // temporal_aa.hlsl
+    kPassHgSelectOrdered,   // order-independent cache update (no reference counterpart)
+    kPassHgCompactOrdered,
+    kPassHgRaygenOrdered,
     kPassKindCount
 };
 
@@ -48,13 +51,18 @@ enum Slot : int {
     kSlotClearHgStorage,
     kSlotClearHgDecay,
     kSlotClearHgCounter,
+    kSlotFillHgThreshold,
     kSlotHgDecay,
     kSlotClearSelection,
     kSlotHgSelect,
+    kSlotClearWinner,
+    kSlotHgSelectOrdered,
     kSlotClearCompactCount,
     kSlotHgCompact,
+    kSlotHgCompactOrdered,
     kSlotClearNextCounter,
     kSlotHgRaygen,
+    kSlotHgRaygenOrdered,
     kSlotSpecValidate,  // issued ahead of the diffuse passes (see slot_lane)
     kSlotDiffGen,
     kSlotDiffValidate,
@@ -81,9 +89,10 @@ enum Slot : int {
 // The pass kind each slot carries.
 inline int slot_kind(int slot) {
     static const int k[kSlotCount] = {
-        kPassGBuffer,      kPassClear,        kPassClear,         kPassClear,       kPassHgDecay,
-        kPassClear,        kPassHgSelect,     kPassClear,         kPassHgCompact,   kPassClear,
-        kPassHgRaygen,     kPassSpecValidate, kPassDiffGen,       kPassDiffValidate, kPassAoGen,
+        kPassGBuffer,      kPassClear,        kPassClear,         kPassClear,       kPassClear,       kPassHgDecay,
+        kPassClear,        kPassHgSelect,     kPassClear,         kPassHgSelectOrdered, kPassClear, kPassHgCompact,
+        kPassHgCompactOrdered, kPassClear,    kPassHgRaygen,      kPassHgRaygenOrdered,
+        kPassSpecValidate, kPassDiffGen,      kPassDiffValidate,  kPassAoGen,
         kPassAoFilter,     kPassDiffTemporal, kPassDiffSpatial,   kPassSpecGen,     kPassSpecTemporal,
         kPassSpecSpatial,  kPassClear,        kPassClear,         kPassDenTemporal, kPassDenDisocclusion,
         kPassDenFastClamp, kPassDenFallback,  kPassDenAtrous,     kPassDenAtrous,   kPassDenAtrous,
@@ -102,6 +111,9 @@ struct PassRec {
         vgi_hash_grid_query_select_io hg_select;
         vgi_hash_grid_query_compact_io hg_compact;
         vgi_hash_grid_raygen_io hg_raygen;
+        vgi_hash_grid_ordered_select_io hg_oselect;
+        vgi_hash_grid_ordered_compact_io hg_ocompact;
+        vgi_hash_grid_ordered_raygen_io hg_oraygen;
         vgi_ind_diff_sample_gen_io diff_gen;
         vgi_ind_diff_sample_validate_io diff_validate;
         vgi_ao_gen_io ao_gen;
@@ -140,6 +152,7 @@ struct PassRec {
     const uint32_t* diff_taps_gbuffer;
     void* clear_ptr;     // kPassClear
     size_t clear_bytes;  // kPassClear
+    uint32_t clear_value;  // kPassClear: the 32-bit word every word of the buffer gets (clear_bytes % 4 == 0 unless 0)
 };
 
 inline PassRec make_rec(int kind, const vgi_frame_params* fp = nullptr) {
@@ -147,6 +160,13 @@ inline PassRec make_rec(int kind, const vgi_frame_params* fp = nullptr) {
     memset(&r, 0, sizeof(r));
     r.kind = kind;
     if (fp) r.fp = *fp;
+    return r;
+}
+inline PassRec make_fill(void* p, size_t bytes, uint32_t value) {
+    PassRec r = make_rec(kPassClear);
+    r.clear_ptr = p;
+    r.clear_bytes = bytes;
+    r.clear_value = value;
     return r;
 }
 inline PassRec make_clear(void* p, size_t bytes) {
@@ -177,7 +197,7 @@ inline int slot_phase(int slot) {
     return slot < kSlotSpecValidate ? 0 : slot < kSlotDenClearFastDiff ? 1 : slot < kSlotFinalLighting ? 2 : 3;
 }
 inline int slot_lane(int slot) {
-    return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygen) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial) ||
+    return (slot >= kSlotClearHgStorage && slot <= kSlotHgRaygenOrdered) || (slot >= kSlotSpecGen && slot <= kSlotSpecSpatial) ||
            slot == kSlotShadow;
 }
 inline bool slot_signals(int slot) { return slot == kSlotSpecValidate; }
@@ -196,7 +216,8 @@ struct Lanes {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cross = nullptr;
     int phase = -1;
     bool side_work = false;
-    hipStream_t cross_from = nullptr;  // stream ev_cross was recorded on this phase, or null
+    bool cross_pending = false;        // ev_cross was recorded in this phase
+    hipStream_t cross_from = nullptr;  // the stream it was recorded on
     vgi_status enter(int slot, hipStream_t& st);
     vgi_status leave(int slot, hipStream_t st);
     vgi_status finish();

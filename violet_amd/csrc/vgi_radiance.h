@@ -33,7 +33,20 @@ struct TraceRes {
     const HashCell* hg_storage;
     HashQuery* queries;
     uint32_t* query_counter;
+    const uint32_t* query_threshold;  // null: append every query
 };
+
+// Hash of a query's 24 bytes (a jenkins chain over its words): the thinning
+// of the appends and the ordered select's random key are functions of it,
+// so equal queries are treated alike wherever they were appended.
+__device__ __forceinline__ uint32_t query_content_hash(const HashQuery& q) {
+    uint32_t h = jenkins_hash(__float_as_uint(q.px));
+    h = jenkins_hash(h ^ __float_as_uint(q.py));
+    h = jenkins_hash(h ^ __float_as_uint(q.pz));
+    h = jenkins_hash(h ^ q.normal);
+    h = jenkins_hash(h ^ q.hash);
+    return jenkins_hash(h ^ q.checksum);
+}
 
 struct RadianceTraceResult {
     f3 position_ws;
@@ -159,16 +172,16 @@ __device__ __forceinline__ RadianceTraceResult radiance_at(const TraceRes& rt, c
         uint32_t cell_addr;
         if (hash_grid_find(rt.hg_storage, hsh, chk, cell_radiance, cell_addr))
             radiance = radiance + diffuse_rho * cell_radiance;
-        uint32_t qi = g_atomic_add(rt.query_counter, 1u);
-        if (qi < kHashGridMaxQueries) {
-            HashQuery q;
-            q.px = sh.position.x;
-            q.py = sh.position.y;
-            q.pz = sh.position.z;
-            q.normal = normal_encode_oct_u32(sh.normal_geo);
-            q.hash = hsh;
-            q.checksum = chk;
-            gst(rt.queries + qi, q);
+        HashQuery q;
+        q.px = sh.position.x;
+        q.py = sh.position.y;
+        q.pz = sh.position.z;
+        q.normal = normal_encode_oct_u32(sh.normal_geo);
+        q.hash = hsh;
+        q.checksum = chk;
+        if (!rt.query_threshold || query_content_hash(q) >= G(rt.query_threshold)[0]) {
+            uint32_t qi = g_atomic_add(rt.query_counter, 1u);
+            if (qi < kHashGridMaxQueries) gst(rt.queries + qi, q);
         }
     }
     ret.radiance = radiance;

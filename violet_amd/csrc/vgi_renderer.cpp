@@ -44,6 +44,8 @@ struct vgi_renderer {
     DevBuf taa[2];              // taa: resolved colour, double-buffered (= next frame's history)
     // hash grid
     DevBuf hg_storage, hg_decay, hg_query[2], hg_counter[2], hg_selection, hg_cidx, hg_caddr, hg_ccount;
+    DevBuf hg_winner, hg_workspace, hg_staging;  // ordered cache update (transient)
+    DevBuf hg_threshold;                         // ordered cache update: query threshold word (state)
     int q = 0;  // query buffer holding pending queries
     vgi_denoiser* den = nullptr;
     // history state
@@ -79,6 +81,7 @@ struct vgi_renderer {
                 {"hg_storage", &hg_storage},   {"hg_decay", &hg_decay},
                 {"hg_query0", &hg_query[0]},   {"hg_query1", &hg_query[1]},
                 {"hg_counter0", &hg_counter[0]}, {"hg_counter1", &hg_counter[1]},
+                {"hg_threshold", &hg_threshold},
                 {"taa0", &taa[0]},             {"taa1", &taa[1]}};
     }
 };
@@ -106,6 +109,7 @@ vgi_trace_resources trace_res(vgi_renderer* r, const vgi_scene* scene, bool with
     t.hash_grid_storage_buffer = r->hg_storage.as<vgi_hash_grid_cell>();
     t.rw_hash_grid_query_buffer = r->hg_query[r->q].as<vgi_hash_grid_query>();
     t.rw_hash_grid_query_counter_buffer = r->hg_counter[r->q].as<uint32_t>();
+    t.hash_grid_query_threshold = r->cfg.ordered_cache_update ? r->hg_threshold.as<uint32_t>() : nullptr;
     return t;
 }
 
@@ -126,6 +130,7 @@ extern "C" vgi_status vgi_renderer_default_config(vgi_renderer_config* c) {
     c->ind_spec_validate = 1;
     c->hash_grid_cache_decay = 1;
     c->concurrent_lanes = 1;
+    c->ordered_cache_update = 1;
     c->ao_radius = 2.0f;
     c->sky_radiance[0] = c->sky_radiance[1] = c->sky_radiance[2] = 1.0f;
     c->denoiser.disocclusion_fix = 1;
@@ -190,6 +195,12 @@ extern "C" vgi_status vgi_renderer_create(uint32_t width, uint32_t height, const
     A(r->hg_cidx, 4 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
     A(r->hg_caddr, 4 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
     A(r->hg_ccount, 4);
+    if (c.ordered_cache_update) {
+        A(r->hg_winner, 4 * (size_t)VGI_HASH_GRID_NUM_CELLS);
+        A(r->hg_workspace, 4 * (size_t)VGI_HASH_GRID_COMPACT_WORKSPACE_WORDS);
+        A(r->hg_staging, 16 * (size_t)VGI_HASH_GRID_MAX_QUERIES);
+    }
+    A(r->hg_threshold, 4);  // read only by the ordered update (kept in every mode: a fixed state layout)
     if (c.compose) {
         A(r->shadow_mask, n);
         A(r->color, R * n);
@@ -265,6 +276,12 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         CALL(sink.submit(kSlotClearHgStorage, make_clear(r->hg_storage.p, r->hg_storage.bytes)));
         CALL(sink.submit(kSlotClearHgDecay, make_clear(r->hg_decay.p, r->hg_decay.bytes)));
         CALL(sink.submit(kSlotClearHgCounter, make_clear(r->hg_counter[r->q].p, 4)));
+        // the ordered update's query threshold before the first count: at
+        // most one diffuse and one specular query per pixel, plus the cache
+        // rays' own
+        const uint64_t expected = 2ull * w * h + VGI_HASH_GRID_MAX_QUERIES;
+        CALL(sink.submit(kSlotFillHgThreshold,
+                         make_fill(r->hg_threshold.p, 4, vgi_hash_grid_query_threshold_for(expected))));
         r->hash_grid_ready = true;
     }
     if (r->cfg.hash_grid_cache_decay) {
@@ -278,29 +295,54 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
     // Cache update (:361-433)
     bool hg_updated = false;
     if (has_prev_depth && has_prev_ind_diff) {
+        const bool ordered = r->cfg.ordered_cache_update != 0;
         CALL(sink.submit(kSlotClearSelection, make_clear(r->hg_selection.p, r->hg_selection.bytes)));
-        PassRec sel = make_rec(kPassHgSelect);
-        sel.io.hg_select = vgi_hash_grid_query_select_io{
-            r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
-            r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_selection.as<uint32_t>()};
-        sel.pc.frame_hash = vgi_pc_frame_hash{frame_rand_next(frame_rand)};
-        CALL(sink.submit(kSlotHgSelect, sel));
+        const uint32_t select_hash = frame_rand_next(frame_rand);
+        if (ordered) {
+            CALL(sink.submit(kSlotClearWinner, make_clear(r->hg_winner.p, r->hg_winner.bytes)));
+            PassRec sel = make_rec(kPassHgSelectOrdered);
+            sel.io.hg_oselect = vgi_hash_grid_ordered_select_io{
+                r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
+                r->hg_storage.as<vgi_hash_grid_cell>(),      r->hg_selection.as<uint32_t>(),
+                r->hg_winner.as<uint32_t>(),                 r->hg_threshold.as<uint32_t>()};
+            sel.pc.frame_hash = vgi_pc_frame_hash{select_hash};
+            CALL(sink.submit(kSlotHgSelectOrdered, sel));
+        } else {
+            PassRec sel = make_rec(kPassHgSelect);
+            sel.io.hg_select = vgi_hash_grid_query_select_io{
+                r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
+                r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_selection.as<uint32_t>()};
+            sel.pc.frame_hash = vgi_pc_frame_hash{select_hash};
+            CALL(sink.submit(kSlotHgSelect, sel));
+        }
         CALL(sink.submit(kSlotClearCompactCount, make_clear(r->hg_ccount.p, 4)));
-        PassRec cmp = make_rec(kPassHgCompact);
-        cmp.io.hg_compact = vgi_hash_grid_query_compact_io{r->hg_selection.as<uint32_t>(), r->hg_cidx.as<uint32_t>(),
-                                                           r->hg_caddr.as<uint32_t>(), r->hg_ccount.as<uint32_t>()};
-        CALL(sink.submit(kSlotHgCompact, cmp));
+        if (ordered) {
+            PassRec cmp = make_rec(kPassHgCompactOrdered);
+            cmp.io.hg_ocompact = vgi_hash_grid_ordered_compact_io{
+                r->hg_selection.as<uint32_t>(), r->hg_winner.as<uint32_t>(), r->hg_cidx.as<uint32_t>(),
+                r->hg_caddr.as<uint32_t>(),     r->hg_ccount.as<uint32_t>(), r->hg_workspace.as<uint32_t>()};
+            CALL(sink.submit(kSlotHgCompactOrdered, cmp));
+        } else {
+            PassRec cmp = make_rec(kPassHgCompact);
+            cmp.io.hg_compact = vgi_hash_grid_query_compact_io{r->hg_selection.as<uint32_t>(), r->hg_cidx.as<uint32_t>(),
+                                                               r->hg_caddr.as<uint32_t>(), r->hg_ccount.as<uint32_t>()};
+            CALL(sink.submit(kSlotHgCompact, cmp));
+        }
         const int nq = 1 - r->q;
         CALL(sink.submit(kSlotClearNextCounter, make_clear(r->hg_counter[nq].p, 4)));
-        PassRec rg = make_rec(kPassHgRaygen, fp);
+        PassRec rg = make_rec(ordered ? kPassHgRaygenOrdered : kPassHgRaygen, fp);
         rg.tr = trace_res(r, scene, true);
         rg.tr.rw_hash_grid_query_buffer = r->hg_query[nq].as<vgi_hash_grid_query>();
         rg.tr.rw_hash_grid_query_counter_buffer = r->hg_counter[nq].as<uint32_t>();
-        rg.io.hg_raygen = vgi_hash_grid_raygen_io{
+        const vgi_hash_grid_raygen_io rio{
             r->hg_storage.as<vgi_hash_grid_cell>(), r->hg_query[r->q].as<vgi_hash_grid_query>(),
             r->hg_ccount.as<uint32_t>(), r->hg_cidx.as<uint32_t>(), r->hg_caddr.as<uint32_t>(), r->hg_decay.as<uint8_t>()};
+        if (ordered)
+            rg.io.hg_oraygen = vgi_hash_grid_ordered_raygen_io{rio, r->hg_staging.as<float>()};
+        else
+            rg.io.hg_raygen = rio;
         rg.pc.frame_hash = vgi_pc_frame_hash{frame_rand_next(frame_rand)};
-        CALL(sink.submit(kSlotHgRaygen, rg));
+        CALL(sink.submit(ordered ? kSlotHgRaygenOrdered : kSlotHgRaygen, rg));
         r->q = nq;
         hg_updated = true;
     }
@@ -513,6 +555,7 @@ extern "C" vgi_status vgi_renderer_get_image(const vgi_renderer* rc, const char*
         {"hash_grid_decay", &r->hg_decay},
         {"hash_grid_queries", &r->hg_query[r->q]},
         {"hash_grid_query_counter", &r->hg_counter[r->q]},
+        {"hash_grid_query_threshold", &r->hg_threshold},
         {"shadow_mask", &r->shadow_mask},
         {"color", &r->color},
         {"taa_color", &r->taa[last]},
@@ -558,8 +601,8 @@ extern "C" size_t vgi_renderer_device_bytes(const vgi_renderer* rc) {
     std::vector<DevBuf*> v;
     all_buffers(r, v);
     for (DevBuf* b : {&r->dtm_rsv, &r->dtm_hit, &r->dtm_rad, &r->dtaps, &r->dnew_hit, &r->dnew_rad, &r->ao_filtered, &r->spec_light,
-                      &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount, &r->shadow_mask,
-                      &r->color})
+                      &r->spec_raylen, &r->hg_selection, &r->hg_cidx, &r->hg_caddr, &r->hg_ccount, &r->hg_winner,
+                      &r->hg_workspace, &r->hg_staging, &r->shadow_mask, &r->color})
         if (std::find(v.begin(), v.end(), b) == v.end()) v.push_back(b);
     size_t n = 0;
     for (auto* b : v) n += b->bytes;

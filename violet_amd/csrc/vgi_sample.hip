@@ -182,6 +182,13 @@ struct SpecGenPass {
         const size_t plane = (size_t)w * h;
         float depth = G(io.gbuffer_depth)[i];
         if (depth == 0.0f) {
+            // the reference leaves the hit position of a sky pixel unwritten;
+            // the specular temporal pass reads it back (times M = 0) when a
+            // reprojection lands on such a pixel, so it must not be stale
+            G(io.rw_hit_pos_texture)[4 * i] = 0.0f;
+            G(io.rw_hit_pos_texture)[4 * i + 1] = 0.0f;
+            G(io.rw_hit_pos_texture)[4 * i + 2] = 0.0f;
+            G(io.rw_hit_pos_texture)[4 * i + 3] = 0.0f;
             S::st3(io.rw_hit_radiance_texture, i, mk3s(0.0f));
             return false;
         }
@@ -266,9 +273,12 @@ __global__ void __launch_bounds__(64, kTraceWaves) k_spec_sample_validate(FrameP
 }
 
 // One cache ray: hash_grid_raygen.hlsl:141-204 for compacted query `id`.
+// With `staging` the blended radiance goes to staging[id] (4 floats) and
+// k_hgo_apply writes it into the cell after every ray has traced, so each
+// ray's cache lookups read the cache as it was before the pass.
 template <class S, bool kCount>
 __device__ __forceinline__ void hash_grid_ray(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io,
-                                              uint32_t frame_hash, uint32_t id, LdsStack stack) {
+                                              float* staging, uint32_t frame_hash, uint32_t id, LdsStack stack) {
     uint32_t qi = G(io.compact_query_index_buffer)[id];
     uint32_t cell = G(io.compact_query_cell_addr_buffer)[id];
     uint32_t rng = jenkins_hash(id ^ frame_hash);
@@ -283,10 +293,34 @@ __device__ __forceinline__ void hash_grid_ray(const FrameParams& fp, const Trace
     f3 prev = mk3(storage[cell].r, storage[cell].g, storage[cell].b);
     const float bf = 1.0f / 16.0f;
     f3 blend = r.radiance * bf + prev * (1.0f - bf);
-    storage[cell].r = blend.x;
-    storage[cell].g = blend.y;
-    storage[cell].b = blend.z;
+    if (staging) {
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        F4 v = {blend.x, blend.y, blend.z, 0.0f};
+        G(reinterpret_cast<F4*>(staging))[id] = v;
+    } else {
+        storage[cell].r = blend.x;
+        storage[cell].g = blend.y;
+        storage[cell].b = blend.z;
+    }
     G(io.rw_hash_grid_decay_buffer)[cell] = 10;
+}
+
+// Writes the staged blends of an ordered cache-ray pass into their cells.
+__device__ __forceinline__ void k_hgo_apply_body(const vgi_hash_grid_raygen_io& io, const float* staging,
+                                                 uint32_t bx_) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const uint32_t id = bx_ * 64u + threadIdx.x;
+    if (id >= kHashGridMaxQueries || id >= G(io.compact_query_counter_buffer)[0]) return;
+    const uint32_t cell = G(io.compact_query_cell_addr_buffer)[id];
+    const F4 v = G(reinterpret_cast<const F4*>(staging))[id];
+    HashCell* c = reinterpret_cast<HashCell*>(io.rw_hash_grid_storage_buffer) + cell;
+    G(&c->r)[0] = v.x;
+    G(&c->g)[0] = v.y;
+    G(&c->b)[0] = v.z;
+}
+
+__global__ void __launch_bounds__(64) k_hgo_apply(vgi_hash_grid_raygen_io io, const float* staging) {
+    k_hgo_apply_body(io, staging, blockIdx.x);
 }
 
 // Cache rays start at scattered surface points in random directions, so a
@@ -298,18 +332,18 @@ constexpr uint32_t kRaygenRaysPerWave = 16;
 constexpr uint32_t kRaygenGroups = kHashGridMaxQueries / 64;  // groups per launch (per view in a batch)
 
 template <class S, bool kCount>
-__device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
+__device__ __forceinline__ void k_hash_grid_raygen_body(const FrameParams& fp, const TraceRes& rt, const vgi_hash_grid_raygen_io& io, float* staging, uint32_t frame_hash, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     const uint32_t lane = threadIdx.x & 63u;
     if (lane >= kRaygenRaysPerWave) return;
-    const uint32_t count = G(io.compact_query_counter_buffer)[0];
+    const uint32_t count = minu(G(io.compact_query_counter_buffer)[0], kHashGridMaxQueries);
     for (uint32_t id = bx_ * kRaygenRaysPerWave + lane; id < count; id += kRaygenGroups * kRaygenRaysPerWave)
-        hash_grid_ray<S, kCount>(fp, rt, io, frame_hash, id, stack);
+        hash_grid_ray<S, kCount>(fp, rt, io, staging, frame_hash, id, stack);
 }
 
 template <class S, bool kCount>
-__global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io, uint32_t frame_hash) {
-    k_hash_grid_raygen_body<S, kCount>(fp, rt, io, frame_hash, blockIdx.x, blockIdx.y);
+__global__ void __launch_bounds__(64) k_hash_grid_raygen(FrameParams fp, TraceRes rt, vgi_hash_grid_raygen_io io, float* staging, uint32_t frame_hash) {
+    k_hash_grid_raygen_body<S, kCount>(fp, rt, io, staging, frame_hash, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,6 +366,7 @@ static vgi_status make_trace_res(const vgi_trace_resources* r, bool need_prev, T
     t.hg_storage = reinterpret_cast<const HashCell*>(r->hash_grid_storage_buffer);
     t.queries = reinterpret_cast<HashQuery*>(r->rw_hash_grid_query_buffer);
     t.query_counter = r->rw_hash_grid_query_counter_buffer;
+    t.query_threshold = r->hash_grid_query_threshold;
     return VGI_OK;
 }
 
@@ -402,16 +437,25 @@ struct JobSpecValidate {
         k_spec_sample_validate_body<S, false>(j.fp, j.rt, j.io, bx, by);
     }
 };
-template <class S>
-struct JobHgRaygen {
+struct HgRaygenArgs {
     FrameParams fp;
     TraceRes rt;
     vgi_hash_grid_raygen_io io;
+    float* staging;  // null: blends go straight into the cells
     uint32_t frame_hash;
+};
+template <class S>
+struct JobHgRaygen : HgRaygenArgs {
     __device__ static void run(const JobHgRaygen& j, uint32_t bx, uint32_t by) {
-        k_hash_grid_raygen_body<S, false>(j.fp, j.rt, j.io, j.frame_hash, bx, by);
+        k_hash_grid_raygen_body<S, false>(j.fp, j.rt, j.io, j.staging, j.frame_hash, bx, by);
     }
 };
+template <class S>
+struct JobHgOApply : HgRaygenArgs {
+    __device__ static void run(const JobHgOApply& j, uint32_t bx, uint32_t) { k_hgo_apply_body(j.io, j.staging, bx); }
+};
+static_assert(sizeof(JobHgOApply<StorageF32>) == sizeof(JobHgRaygen<StorageF32>), "one table, two launches");
+static_assert(kRaygenGroups * 64 >= kHashGridMaxQueries, "the cache-ray tiles cover the compact list");
 template <class S>
 struct BatchMinWaves<JobDiffGen<S>> {
     static constexpr int value = kTraceWaves;
@@ -492,10 +536,33 @@ static vgi_status fill_hg_raygen(const PassRec* const* recs, int n, void* host) 
         if (s != VGI_OK) return s;
         j.fp = to_fp(&r.fp);
         j.io = r.io.hg_raygen;
+        j.staging = nullptr;
         j.frame_hash = r.pc.frame_hash.frame_hash;
         batch_add_tiles(t->hd, v, kRaygenGroups, 1);
     }
     return VGI_OK;
+}
+static vgi_status fill_hg_oraygen(const PassRec* const* recs, int n, void* host) {
+    auto* t = batch_table<JobHgRaygen>(host);
+    for (int v = 0; v < n; ++v) {
+        const PassRec& r = *recs[v];
+        auto& j = t->jobs[v];
+        vgi_status s = make_trace_res(&r.tr, true, j.rt);
+        if (s != VGI_OK) return s;
+        VGI_REQUIRE(r.io.hg_oraygen.rw_radiance_staging_buffer &&
+                        ((uintptr_t)r.io.hg_oraygen.rw_radiance_staging_buffer & 15u) == 0,
+                    "batched ordered cache rays: staging buffer missing or not 16-byte aligned");
+        j.fp = to_fp(&r.fp);
+        j.io = r.io.hg_oraygen.raygen;
+        j.staging = r.io.hg_oraygen.rw_radiance_staging_buffer;
+        j.frame_hash = r.pc.frame_hash.frame_hash;
+        batch_add_tiles(t->hd, v, kRaygenGroups, 1);
+    }
+    return VGI_OK;
+}
+static vgi_status launch_hg_oraygen(const void* dev, uint32_t storage, uint32_t grid, size_t lds, hipStream_t st) {
+    VGI_CHECK(batch_launch<JobHgRaygen>(dev, storage, grid, lds, st));
+    return batch_launch<JobHgOApply>(dev, storage, grid, 0, st);
 }
 
 const BatchOps* batch_ops_sample(int kind) {
@@ -509,12 +576,15 @@ const BatchOps* batch_ops_sample(int kind) {
                                         batch_launch<JobSpecValidate>, 64, true};
     static const BatchOps hg_raygen{sizeof(BatchTable<JobHgRaygen<StorageF32>>), fill_hg_raygen,
                                     batch_launch<JobHgRaygen>, 64, true};
+    static const BatchOps hg_oraygen{sizeof(BatchTable<JobHgRaygen<StorageF32>>), fill_hg_oraygen, launch_hg_oraygen,
+                                     64, true};
     switch (kind) {
         case kPassDiffGen: return &diff_gen;
         case kPassDiffValidate: return &diff_validate;
         case kPassSpecGen: return &spec_gen;
         case kPassSpecValidate: return &spec_validate;
         case kPassHgRaygen: return &hg_raygen;
+        case kPassHgRaygenOrdered: return &hg_oraygen;
         default: return nullptr;
     }
 }
@@ -609,8 +679,33 @@ extern "C" vgi_status vgi_pass_hash_grid_raygen(const vgi_frame_params* fp, cons
     hipStream_t st = (hipStream_t)stream;
     // ray per compacted query: query 24 B + index 4 + addr 4 + cell rmw 32 + decay 1
     ProfScope prof("hash_grid_raygen", st, 0.0);
-    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kRaygenGroups), to_fp(fp), t, *io,
+    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kRaygenGroups), to_fp(fp), t, *io, (float*)nullptr,
                          pc->frame_hash);
     trace_stats_collect("hash_grid_raygen", st);
     return check_launch("k_hash_grid_raygen");
+}
+
+extern "C" vgi_status vgi_pass_hash_grid_raygen_ordered(const vgi_frame_params* fp, const vgi_trace_resources* rt,
+                                                        const vgi_hash_grid_ordered_raygen_io* oio,
+                                                        const vgi_pc_frame_hash* pc, void* stream) {
+    VGI_REQUIRE(fp && oio && pc && rt, "vgi_pass_hash_grid_raygen_ordered: null argument");
+    const vgi_hash_grid_raygen_io* io = &oio->raygen;
+    VGI_REQUIRE(io->rw_hash_grid_storage_buffer && io->hash_grid_query_buffer && io->compact_query_counter_buffer &&
+                    io->compact_query_index_buffer && io->compact_query_cell_addr_buffer &&
+                    io->rw_hash_grid_decay_buffer && oio->rw_radiance_staging_buffer,
+                "vgi_pass_hash_grid_raygen_ordered: missing binding");
+    VGI_REQUIRE(((uintptr_t)oio->rw_radiance_staging_buffer & 15u) == 0,
+                "vgi_pass_hash_grid_raygen_ordered: staging buffer not 16-byte aligned");
+    TraceRes t;
+    vgi_status s = make_trace_res(rt, true, t);
+    if (s != VGI_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope prof("hash_grid_raygen_ordered", st, 0.0);
+    VGI_DISPATCH_STORAGE(rt->storage, k_hash_grid_raygen, dim3(kRaygenGroups), to_fp(fp), t, *io,
+                         oio->rw_radiance_staging_buffer, pc->frame_hash);
+    trace_stats_collect("hash_grid_raygen_ordered", st);
+    VGI_CHECK(check_launch("k_hash_grid_raygen"));
+    hipLaunchKernelGGL(k_hgo_apply, dim3(kHashGridMaxQueries / 64), dim3(64), 0, st, *io,
+                       (const float*)oio->rw_radiance_staging_buffer);
+    return check_launch("k_hgo_apply");
 }

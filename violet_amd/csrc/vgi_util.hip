@@ -48,12 +48,13 @@ struct JobClear {
     size_t nwords;
     uint8_t* tail;
     uint32_t tail_bytes;
+    uint32_t value;
     __device__ static void run(const JobClear& j, uint32_t bx, uint32_t) {
         const size_t base = (size_t)bx * kClearTileWords;
 #pragma unroll
         for (uint32_t k = 0; k < kClearTileWords / 64; ++k) {
             const size_t i = base + k * 64u + (threadIdx.x & 63u);
-            if (i < j.nwords) j.words[i] = 0u;
+            if (i < j.nwords) j.words[i] = j.value;
         }
         if (bx == 0 && threadIdx.x < j.tail_bytes) j.tail[threadIdx.x] = 0u;
     }
@@ -69,6 +70,8 @@ static vgi_status fill_clear(const PassRec* const* recs, int n, void* host) {
         j.nwords = r.clear_bytes / 4;
         j.tail = (uint8_t*)r.clear_ptr + 4 * j.nwords;
         j.tail_bytes = (uint32_t)(r.clear_bytes & 3u);
+        j.value = r.clear_value;
+        VGI_REQUIRE(r.clear_value == 0 || j.tail_bytes == 0, "batched fill: buffer not whole words");
         const size_t tiles = std::max<size_t>(1, (j.nwords + kClearTileWords - 1) / kClearTileWords);
         VGI_REQUIRE(tiles < (1u << 30), "batched clear: buffer too large");
         batch_add_tiles(t->hd, v, (uint32_t)tiles, 1);

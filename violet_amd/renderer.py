@@ -46,6 +46,9 @@ class RestirConfig:
     fast_history_clamp: bool = False
     atrous_iterations: int = 3
     compensated_moments: bool = True  # FP16 storage: moments as hi + lo f16 pairs
+    # order-independent cache update: the same frames run after run and alone
+    # or in a batch (False: the reference's schedule-dependent cache passes)
+    ordered_cache_update: bool = True
 
     def to_c(self) -> _abi.RendererConfig:
         c = _abi.RendererConfig()
@@ -64,6 +67,7 @@ class RestirConfig:
         c.taa = int(self.taa)
         c.denoiser = _lib.DenoiserConfig(int(self.disocclusion_fix), int(self.fast_history_clamp),
                                          int(self.atrous_iterations), int(self.compensated_moments))
+        c.ordered_cache_update = int(self.ordered_cache_update)
         return c
 
 

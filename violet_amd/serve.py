@@ -251,5 +251,14 @@ def serve_views(num_views: int, seed: int) -> list[ViewSpec]:
     return [ViewSpec(*SERVE_RESOLUTIONS[p], float(a)) for p, a in zip(pick, phase)]
 
 
-__all__ = ["Request", "ViewSpec", "Server", "ServeStats", "poisson_requests", "serve_views", "view_frame_params",
-           "SERVE_RESOLUTIONS", "SERVE_WEIGHTS"]
+def batch_views(num_views: int, seed: int) -> list[ViewSpec]:
+    """Views of the batched form: seeded resolutions w ~ U{320..1920},
+    h ~ U{240..1080}, one orbit camera each."""
+    rng = np.random.default_rng(seed)
+    ws, hs = rng.integers(320, 1921, num_views), rng.integers(240, 1081, num_views)
+    ph = rng.uniform(0.0, 360.0, num_views)
+    return [ViewSpec(int(w), int(h), float(p)) for w, h, p in zip(ws, hs, ph)]
+
+
+__all__ = ["Request", "ViewSpec", "Server", "ServeStats", "poisson_requests", "serve_views", "batch_views",
+           "view_frame_params", "SERVE_RESOLUTIONS", "SERVE_WEIGHTS"]
