@@ -725,7 +725,7 @@ typedef struct vgi_hash_grid_ordered_select_io {
 /* The query threshold word for a frame that expects to append about
  * `expected_queries` queries (before any thinning); 0 (keep every query)
  * when they fit.  Queries are sampled down to VGI_HASH_GRID_QUERY_TARGET. */
-#define VGI_HASH_GRID_QUERY_TARGET (VGI_HASH_GRID_MAX_QUERIES / 4u * 3u)
+#define VGI_HASH_GRID_QUERY_TARGET (VGI_HASH_GRID_MAX_QUERIES / 2u)
 uint32_t vgi_hash_grid_query_threshold_for(uint64_t expected_queries);
 vgi_status vgi_pass_hash_grid_query_select_ordered(const vgi_hash_grid_ordered_select_io* io,
                                                    const vgi_pc_frame_hash* pc, void* stream);

@@ -74,9 +74,9 @@ static uint32_t o_query_hash(const uint32_t* q) {
     return h;
 }
 
-/* Threshold word keeping about 3/4 * HG_MAX_QUERIES of `expected` query
+/* Threshold word keeping about HG_MAX_QUERIES / 2 of `expected` query
  * candidates (a candidate is kept if its hash is >= the word). */
-#define HG_QUERY_TARGET (128u * 1024u / 4u * 3u)
+#define HG_QUERY_TARGET (128u * 1024u / 2u)
 uint32_t oracle_hg_query_threshold_for(uint64_t expected) {
     if (expected <= HG_QUERY_TARGET) return 0u;
     uint64_t keep = ((uint64_t)HG_QUERY_TARGET << 32) / expected;

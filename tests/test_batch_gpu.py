@@ -11,7 +11,10 @@ frame:
 * the batched form at full size: 16 views with seeded resolutions from
   320x240 to 1920x1080 over the shared 1M-triangle scene, FP16 storage, one
   graph-replayed fused launch per pass: the denoised diffuse and specular
-  and the raw diffuse of every view, every frame, over 6 frames.
+  and the raw diffuse of every view, every frame, over 10 frames (two
+  diffuse validation frames), with every frame's sampled cache queries
+  within the query buffer (beyond it, which queries are kept depends on the
+  order of the appends).
 """
 from __future__ import annotations
 
@@ -89,7 +92,7 @@ def test_views_same_alone_and_in_the_mix(vgi, graph):
 
 
 CONFIG_VIEWS = 16
-CONFIG_FRAMES = 6
+CONFIG_FRAMES = 10
 CONFIG_TRIANGLES = 1_000_000
 
 
@@ -115,6 +118,8 @@ def test_batched_form_views_bit_identical_alone_and_in_the_mix(vgi):
         for f in range(CONFIG_FRAMES):
             o = r.add_passes(sc, serve.view_frame_params(spec, f), f)
             frames.append(outputs(r, spec, o))
+            n = int(r.read("hash_grid_query_counter", np.uint32, (1,))[0])
+            assert n < 131072, f"{spec.width}x{spec.height} frame {f}: {n} sampled queries overflow the buffer"
         torch.cuda.synchronize()
         alone.append(frames)
         r.close()

@@ -19,7 +19,7 @@ reference guarantees for every schedule.  Checked here on the oracle:
 * compaction lists the selected cells in cell order, once each;
 * the query threshold word: a candidate is kept when its hash is at least
   the word, and the word set from one frame's kept count keeps about
-  3/4 * 2^17 of the next frame's candidates (binomial bounds), whatever the
+  2^16 of the next frame's candidates (binomial bounds), whatever the
   previous word was; 0 (keep all) when the candidates fit.
 """
 from __future__ import annotations
@@ -185,7 +185,7 @@ def test_compaction_lists_selected_cells_in_cell_order(R32):
 
 @pytest.mark.parametrize("candidates", [50_000, 400_000, 2_000_000])
 def test_query_threshold_keeps_about_the_target(R32, candidates):
-    target = 131072 * 3 // 4
+    target = 131072 // 2
     rng = np.random.default_rng(candidates)
     thr0 = R32.hg_query_threshold_for(3 * candidates)  # a first guess 3x too high
     h = rng.integers(0, 2**32, candidates, dtype=np.uint64)

@@ -522,8 +522,10 @@ constexpr uint32_t kHashGridBucketShift = 4;
 constexpr uint32_t kHashGridNumCells = kHashGridNumBuckets * kHashGridBucketSize;
 constexpr uint32_t kHashGridMaxQueries = 128 * 1024;
 constexpr uint32_t kHashGridMaxQueriesShift = 17;
-// Appends the ordered cache update samples the frame's queries down to.
-constexpr uint32_t kHashGridQueryTarget = kHashGridMaxQueries / 4u * 3u;
+// Appends the ordered cache update samples the frame's queries down to:
+// half the buffer, since the candidates of one frame can exceed the last
+// frame's by 40% (the frame after a diffuse validation frame).
+constexpr uint32_t kHashGridQueryTarget = kHashGridMaxQueries / 2u;
 
 // Threshold word that keeps about kHashGridQueryTarget of `expected` query
 // candidates: a candidate is kept if its 32-bit hash is >= the word, which
