@@ -84,3 +84,70 @@ def test_fp16_storage_error_vs_fp64_oracle(vgi):
         for k, b in bounds.items():
             assert table[name][k] <= b, f"{name} {k}: {table[name][k]:.3e} > {b:.0e}"
     assert table["f16c"]["variance"] <= table["f16"]["variance"] * 1.05
+
+
+# Full frame loop: the renderer (order-independent cache update, so a run is
+# reproducible) against the oracle's frame loop over the same 16 frames of the
+# orbit, 192x112, 4000-triangle scene.  Relative mean error per output,
+# averaged over the frames:
+#   FP32 storage vs the FP32 twin     the same arithmetic, so <= 1e-5 on every
+#                                      frame (the denoiser's exp2 folding and
+#                                      fused multiply-adds differ in the last bits)
+#   FP16 storage vs the FP32 twin     the cost of binary16 storage alone
+#   FP32 / FP16 storage vs FP64       dominated by reservoir selections that
+#                                      flip between FP32 and FP64 rounding: a
+#                                      flipped selection is a different sample,
+#                                      and temporal reuse carries it on, so the
+#                                      error grows over the frames for FP32 as
+#                                      much as for FP16
+LOOP_FRAMES = 16
+LOOP_OUTPUTS = {  # renderer snapshot image -> oracle frame-loop output
+    "out_diffuse": "diffuse", "out_specular": "specular", "indirect_diffuse_raw": "diff_light",
+    "indirect_specular_raw": "spec_light", "diff_hit_radiance": "diff_rad", "spec_hit_radiance": "spec_rad",
+}
+LOOP_BOUNDS = {
+    "f32_vs_twin": dict.fromkeys(LOOP_OUTPUTS, 1e-5),
+    "f16_vs_twin": dict(out_diffuse=2e-3, out_specular=2e-3, indirect_diffuse_raw=5e-3,
+                        indirect_specular_raw=5e-3, diff_hit_radiance=5e-3, spec_hit_radiance=5e-3),
+    "f32_vs_fp64": dict(out_diffuse=1e-2, out_specular=1.2e-2, indirect_diffuse_raw=4e-2,
+                        indirect_specular_raw=4e-2, diff_hit_radiance=4e-2, spec_hit_radiance=4e-2),
+    "f16_vs_fp64": dict(out_diffuse=1e-2, out_specular=1.2e-2, indirect_diffuse_raw=4e-2,
+                        indirect_specular_raw=4e-2, diff_hit_radiance=4e-2, spec_hit_radiance=4e-2),
+}
+
+
+def test_frame_loop_error_per_output_vs_fp64_oracle(vgi):
+    import oracle as O
+    from restir_harness import make_mesh, run_gpu_frames
+    from violet_amd import _lib, renderer
+
+    w, h = 192, 112
+    mesh = make_mesh()
+    gpu = {}
+    for name, st in (("f32", _lib.STORAGE_F32), ("f16", _lib.STORAGE_F16)):
+        fps, _, snaps, r, _ = run_gpu_frames(mesh, w, h, LOOP_FRAMES, config=renderer.RestirConfig(storage=st))
+        gpu[name] = snaps
+        r.close()
+    ref = {}
+    for bits in (64, 32):
+        R = O.OracleRenderer(bits, mesh, w, h)
+        ref[bits] = [R.add_passes(fp, f) for f, fp in enumerate(fps)]
+    pairs = {"f32_vs_twin": ("f32", 32), "f16_vs_twin": ("f16", 32), "f32_vs_fp64": ("f32", 64),
+             "f16_vs_fp64": ("f16", 64)}
+    table = {}
+    for key, (g, bits) in pairs.items():
+        per = {}
+        for img, out in LOOP_OUTPUTS.items():
+            e = [rel_mean(gpu[g][f][img].astype(np.float64), np.asarray(ref[bits][f][out], np.float64))
+                 for f in range(LOOP_FRAMES)]
+            per[img] = (float(np.mean(e)), float(np.max(e)))
+        table[key] = per
+    for key, per in table.items():
+        print(f"\n{key}: " + ", ".join(f"{k} {m:.2e} (max {x:.2e})" for k, (m, x) in per.items()), flush=True)
+    for key, bounds in LOOP_BOUNDS.items():
+        for img, b in bounds.items():
+            m, x = table[key][img]
+            assert (x if key == "f32_vs_twin" else m) <= b, f"{key} {img}: mean {m:.3e} max {x:.3e} > {b:.0e}"
+    # binary16 storage adds little to the FP32-vs-FP64 divergence
+    for img in LOOP_OUTPUTS:
+        assert table["f16_vs_fp64"][img][0] <= 1.25 * table["f32_vs_fp64"][img][0] + 1e-3, img
