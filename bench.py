@@ -458,7 +458,8 @@ def run_serving_form(args, sc, ntris):
     log(f"[bench] serving form: {len(reqs)} requests at {args.serve_rate:g}/s over {len(specs)} views")
     bcfg = batch.BatchConfig(trace_groups=args.serve_trace_groups, use_graph=args.serve_graph)
     srv = serve.Server(sc, specs, hbm_budget_bytes=args.serve_budget_gb * 1e9, config=cfg,
-                       max_batch=args.serve_max_batch, streams=2, batch_config=bcfg)
+                       max_batch=args.serve_max_batch, streams=args.serve_streams, batch_config=bcfg,
+                       max_batch_pixels=int(args.serve_max_pixels))
     admitted = srv.admit_all()
     stats = srv.run(reqs, timeout_s=600.0)
     srv.close()
@@ -468,7 +469,8 @@ def run_serving_form(args, sc, ntris):
             "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "mean_ms": round(s["mean_ms"], 3),
             "mpix_per_s": round(s["mpix_per_s"], 2), "wall_s": round(s["wall_s"], 3), "batches": s["batches"],
             "mean_batch": round(s["mean_batch"], 2), "evictions": s["evictions"],
-            "graph": args.serve_graph, "trace_groups": args.serve_trace_groups,
+            "graph": args.serve_graph, "trace_groups": args.serve_trace_groups, "streams": args.serve_streams,
+            "max_batch_pixels": int(args.serve_max_pixels),
             "peak_hbm_gb": round(s["peak_hbm_gb"], 2), "admitted_up_front": admitted}
 
 
@@ -536,6 +538,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-max-batch", type=int, default=16)
     ap.add_argument("--serve-graph", type=int, default=0, help="1: replay each batch slot's frame as a HIP graph")
+    ap.add_argument("--serve-streams", type=int, default=2, help="batch slots (streams) the server keeps busy")
+    ap.add_argument("--serve-max-pixels", type=float, default=4e6,
+                    help="pixel cap of one batch (0 = none; the oldest queued request always goes)")
     ap.add_argument("--serve-trace-groups", type=int, default=0,
                     help="persistent groups of the fused trace passes (0 = one per tile)")
     args = ap.parse_args()

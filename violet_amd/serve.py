@@ -14,7 +14,8 @@ Scheduling (continuous batching): requests join a FIFO queue when they
 arrive.  Whenever one of `streams` batch slots is free, the scheduler takes
 up to `max_batch` queued requests in arrival order - at most one per view
 and none for a view that still has a frame in flight, since a view's frames
-are sequential - and issues them as one fused multi-view frame
+are sequential, and, with `max_batch_pixels`, no more pixels than that
+(the oldest request always goes) - and issues them as one fused multi-view frame
 (violet_amd.batch) on that slot's stream.  Requests that arrive while other
 batches run are admitted as soon as a slot frees, so the work of different
 batches overlaps on the device.  A request's latency runs from its arrival
@@ -84,7 +85,7 @@ def view_frame_params(spec: ViewSpec, frame: int, deg_per_frame: float = 6.0):
 class Server:
     def __init__(self, scene: Scene, views: list[ViewSpec], hbm_budget_bytes: float,
                  config: renderer.RestirConfig | None = None, max_batch: int = 16, streams: int = 2,
-                 batch_config: batch.BatchConfig | None = None, on_complete=None):
+                 batch_config: batch.BatchConfig | None = None, on_complete=None, max_batch_pixels: int = 0):
         import torch
 
         gpu.require_gpu()
@@ -94,6 +95,7 @@ class Server:
         self.budget = hbm_budget_bytes
         self.config = config or renderer.RestirConfig()
         self.max_batch = max_batch
+        self.max_batch_pixels = max_batch_pixels  # 0: no limit
         self.on_complete = on_complete
         bc = batch_config or batch.BatchConfig()
         bc.max_views = max(bc.max_views, max_batch)
@@ -160,13 +162,19 @@ class Server:
 
     def _issue(self, k: int, queue: collections.deque, now: float, stats: ServeStats) -> None:
         picked, seen, keep = [], set(), collections.deque()
+        pixels = 0
         while queue and len(picked) < self.max_batch:
             req = queue.popleft()
-            if req.view in self.inflight_views or req.view in seen or not self._ensure_resident(req.view, stats, seen):
+            spec = self.views[req.view]
+            px = spec.width * spec.height
+            over = self.max_batch_pixels and picked and pixels + px > self.max_batch_pixels
+            if over or req.view in self.inflight_views or req.view in seen or \
+                    not self._ensure_resident(req.view, stats, seen):
                 keep.append(req)
                 continue
             picked.append(req)
             seen.add(req.view)
+            pixels += px
         queue.extendleft(reversed(keep))
         if not picked:
             return
