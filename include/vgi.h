@@ -716,7 +716,8 @@ typedef struct vgi_hash_grid_ordered_select_io {
     vgi_hash_grid_cell* rw_hash_grid_storage_buffer;
     uint32_t* rw_hash_grid_query_selection_buffer; /* 1 word per cell, zero on entry */
     uint32_t* rw_hash_grid_query_winner_buffer;    /* 1 word per cell, zero on entry, 16-byte aligned: on
-                                                    * return the index of the query a selected cell chose */
+                                                    * return the index of the query a selected cell chose
+                                                    * (and zero elsewhere) */
     /* NULL, or the threshold word of vgi_trace_resources: replaced by the
      * threshold for the queries appended next, from the counter of the
      * queries selected here (those appended under the word's old value) */
@@ -732,8 +733,11 @@ vgi_status vgi_pass_hash_grid_query_select_ordered(const vgi_hash_grid_ordered_s
 
 #define VGI_HASH_GRID_COMPACT_WORKSPACE_WORDS 257u
 typedef struct vgi_hash_grid_ordered_compact_io {
-    const uint32_t* hash_grid_query_selection_buffer; /* 16-byte aligned */
-    const uint32_t* hash_grid_query_winner_buffer;
+    /* the ordered select's outputs, 16-byte aligned; the compaction sets
+     * every word it lists back to zero, so both buffers hold zeros again
+     * (as the next ordered select needs) without a clear */
+    uint32_t* rw_hash_grid_query_selection_buffer;
+    uint32_t* rw_hash_grid_query_winner_buffer;
     uint32_t* rw_compact_query_index_buffer;
     uint32_t* rw_compact_query_cell_addr_buffer;
     uint32_t* rw_compact_query_counter_buffer;

@@ -326,6 +326,8 @@ def test_hash_grid_ordered_update_replayed(vgi, run_ordered, o32, k):
     o_idx, o_addr = o32.hg_compact_ordered(o_sel, o_win)
     assert np.array_equal(addr, o_addr), f"compaction order differs [{k}]"
     assert np.array_equal(queries[idx], queries[o_idx])
+    # the compaction leaves both buffers zero for the next frame's select
+    assert not Dev.get(gs, np.uint32, (NCELLS,)).any() and not Dev.get(gw, np.uint32, (NCELLS,)).any()
 
     tr, q_new, c_new = gpu_trace_res(dev, run, gc, p)
     tr.hash_grid_query_threshold = gt

@@ -79,7 +79,7 @@ HGOSelectIO = _struct("HGOSelectIO", [
     ("rw_hash_grid_query_threshold", vp)])
 HG_COMPACT_WORKSPACE_WORDS = 257
 HGOCompactIO = _struct("HGOCompactIO", [
-    ("hash_grid_query_selection_buffer", vp), ("hash_grid_query_winner_buffer", vp),
+    ("rw_hash_grid_query_selection_buffer", vp), ("rw_hash_grid_query_winner_buffer", vp),
     ("rw_compact_query_index_buffer", vp), ("rw_compact_query_cell_addr_buffer", vp),
     ("rw_compact_query_counter_buffer", vp), ("rw_workspace_buffer", vp)])
 HGORaygenIO = _struct("HGORaygenIO", [("raygen", HGRaygenIO), ("rw_radiance_staging_buffer", vp)])

@@ -256,6 +256,14 @@ __device__ __forceinline__ void k_hgo_insert_body(HashCell* storage, const uint3
         key[4 * k + 3] = v.w;
     }
     if (key[0] == 0u) return;  // candidates fill the table from its first slot
+    {
+        // leave the table zero for the next frame (the claim launch writes
+        // the winner indices into the same buffer, the compaction clears them)
+        __attribute__((address_space(1))) U4* cz = (__attribute__((address_space(1))) U4*)(cand + base);
+        const U4 z = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) cz[k] = z;
+    }
     // ascending checksums, the empty slots (0) last
 #pragma unroll
     for (uint32_t i = 0; i < kHashGridBucketSize; ++i)
@@ -347,7 +355,7 @@ __device__ __forceinline__ void k_hgo_count_body(const uint32_t* selection, cons
     }
 }
 
-__device__ __forceinline__ void k_hgo_place_body(const uint32_t* selection, const uint32_t* winner,
+__device__ __forceinline__ void k_hgo_place_body(uint32_t* selection, uint32_t* winner,
                                                  uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter,
                                                  const uint32_t* ws, uint32_t bx_) {
     static_assert(kCompactWaves == 4 * 64, "four wave counts per lane");
@@ -382,6 +390,10 @@ __device__ __forceinline__ void k_hgo_place_body(const uint32_t* selection, cons
             G(out_addr)[o] = cell;
         }
         ++o;
+        // the selection and winner words go back to zero: the next frame's
+        // ordered select needs no clear of the two 4 MiB buffers
+        G(selection)[cell] = 0u;
+        G(winner)[cell] = 0u;
     }
 }
 
@@ -404,7 +416,7 @@ __global__ void __launch_bounds__(64) k_hgo_claim(const HashQuery* queries, cons
 __global__ void __launch_bounds__(64) k_hgo_count(const uint32_t* selection, const uint32_t* counter, uint32_t* ws) {
     k_hgo_count_body(selection, counter, ws, blockIdx.x);
 }
-__global__ void __launch_bounds__(64) k_hgo_place(const uint32_t* selection, const uint32_t* winner,
+__global__ void __launch_bounds__(64) k_hgo_place(uint32_t* selection, uint32_t* winner,
                                                   uint32_t* out_index, uint32_t* out_addr, uint32_t* out_counter,
                                                   const uint32_t* ws) {
     k_hgo_place_body(selection, winner, out_index, out_addr, out_counter, ws, blockIdx.x);
@@ -475,7 +487,7 @@ struct JobHgOClaim : HgOSelectArgs {
     }
 };
 struct HgOCompactArgs {
-    const uint32_t *selection, *winner;
+    uint32_t *selection, *winner;
     uint32_t *out_index, *out_addr, *out_counter, *ws;
 };
 template <class>
@@ -558,11 +570,11 @@ static vgi_status fill_hg_ocompact(const PassRec* const* recs, int n, void* host
     auto* t = batch_table<JobHgOCount>(host);
     for (int v = 0; v < n; ++v) {
         const vgi_hash_grid_ordered_compact_io& io = recs[v]->io.hg_ocompact;
-        VGI_REQUIRE(((uintptr_t)io.hash_grid_query_selection_buffer & 15u) == 0,
+        VGI_REQUIRE(((uintptr_t)io.rw_hash_grid_query_selection_buffer & 15u) == 0,
                     "batched ordered compaction: selection buffer not 16-byte aligned");
         HgOCompactArgs& a = t->jobs[v];
-        a.selection = io.hash_grid_query_selection_buffer;
-        a.winner = io.hash_grid_query_winner_buffer;
+        a.selection = io.rw_hash_grid_query_selection_buffer;
+        a.winner = io.rw_hash_grid_query_winner_buffer;
         a.out_index = io.rw_compact_query_index_buffer;
         a.out_addr = io.rw_compact_query_cell_addr_buffer;
         a.out_counter = io.rw_compact_query_counter_buffer;
@@ -670,20 +682,20 @@ extern "C" vgi_status vgi_pass_hash_grid_query_select_ordered(const vgi_hash_gri
 
 extern "C" vgi_status vgi_pass_hash_grid_query_compact_ordered(const vgi_hash_grid_ordered_compact_io* io,
                                                                void* stream) {
-    VGI_REQUIRE(io && io->hash_grid_query_selection_buffer && io->hash_grid_query_winner_buffer &&
+    VGI_REQUIRE(io && io->rw_hash_grid_query_selection_buffer && io->rw_hash_grid_query_winner_buffer &&
                     io->rw_compact_query_index_buffer && io->rw_compact_query_cell_addr_buffer &&
                     io->rw_compact_query_counter_buffer && io->rw_workspace_buffer,
                 "vgi_pass_hash_grid_query_compact_ordered: missing binding");
-    VGI_REQUIRE(((uintptr_t)io->hash_grid_query_selection_buffer & 15u) == 0,
+    VGI_REQUIRE(((uintptr_t)io->rw_hash_grid_query_selection_buffer & 15u) == 0,
                 "vgi_pass_hash_grid_query_compact_ordered: selection buffer not 16-byte aligned");
     static_assert(VGI_HASH_GRID_COMPACT_WORKSPACE_WORDS == kCompactWaves + 1, "workspace size");
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("hash_grid_query_compact_ordered", st, (double)kHashGridNumCells * 8.0);
-    hipLaunchKernelGGL(k_hgo_count, dim3(kCompactWaves), dim3(64), 0, st, io->hash_grid_query_selection_buffer,
+    hipLaunchKernelGGL(k_hgo_count, dim3(kCompactWaves), dim3(64), 0, st, io->rw_hash_grid_query_selection_buffer,
                        io->rw_compact_query_counter_buffer, io->rw_workspace_buffer);
     VGI_CHECK(check_launch("k_hgo_count"));
-    hipLaunchKernelGGL(k_hgo_place, dim3(kCompactWaves), dim3(64), 0, st, io->hash_grid_query_selection_buffer,
-                       io->hash_grid_query_winner_buffer, io->rw_compact_query_index_buffer,
+    hipLaunchKernelGGL(k_hgo_place, dim3(kCompactWaves), dim3(64), 0, st, io->rw_hash_grid_query_selection_buffer,
+                       io->rw_hash_grid_query_winner_buffer, io->rw_compact_query_index_buffer,
                        io->rw_compact_query_cell_addr_buffer, io->rw_compact_query_counter_buffer,
                        io->rw_workspace_buffer);
     return check_launch("k_hgo_place");

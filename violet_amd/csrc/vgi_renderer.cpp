@@ -53,6 +53,7 @@ struct vgi_renderer {
     bool has_prev_depth = false, has_prev_gbuffer = false, has_prev_indirect_diffuse = false;
     bool has_prev_diffuse_reservoir = false, has_prev_ind_spec = false, has_prev_ao = false;
     bool hash_grid_ready = false;
+    bool hg_scratch_zero = false;  // ordered update: selection and winner buffers hold zeros
     bool has_prev_color = false;  // TAA history (restir_renderer.rs:914-918)
     const void* prev_indirect_diffuse = nullptr;  // last frame's denoised (or raw) diffuse
     int prev_ind_diff_slot = -1;                  // 0/1 = diff_light[k]; 2/3 = denoiser pp[k-2]
@@ -237,6 +238,7 @@ extern "C" vgi_status vgi_renderer_reset(vgi_renderer* r) {
     r->has_prev_depth = r->has_prev_gbuffer = r->has_prev_indirect_diffuse = false;
     r->has_prev_diffuse_reservoir = r->has_prev_ind_spec = r->has_prev_ao = false;
     r->hash_grid_ready = false;
+    r->hg_scratch_zero = false;
     r->has_prev_color = false;
     r->prev_indirect_diffuse = nullptr;
     r->prev_ind_diff_slot = -1;
@@ -296,10 +298,15 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
     bool hg_updated = false;
     if (has_prev_depth && has_prev_ind_diff) {
         const bool ordered = r->cfg.ordered_cache_update != 0;
-        CALL(sink.submit(kSlotClearSelection, make_clear(r->hg_selection.p, r->hg_selection.bytes)));
+        // the ordered passes leave the selection and winner buffers zero
+        // again, so they are cleared once; the reference's select every frame
+        if (!ordered || !r->hg_scratch_zero)
+            CALL(sink.submit(kSlotClearSelection, make_clear(r->hg_selection.p, r->hg_selection.bytes)));
         const uint32_t select_hash = frame_rand_next(frame_rand);
         if (ordered) {
-            CALL(sink.submit(kSlotClearWinner, make_clear(r->hg_winner.p, r->hg_winner.bytes)));
+            if (!r->hg_scratch_zero)
+                CALL(sink.submit(kSlotClearWinner, make_clear(r->hg_winner.p, r->hg_winner.bytes)));
+            r->hg_scratch_zero = true;
             PassRec sel = make_rec(kPassHgSelectOrdered);
             sel.io.hg_oselect = vgi_hash_grid_ordered_select_io{
                 r->hg_query[r->q].as<vgi_hash_grid_query>(), r->hg_counter[r->q].as<uint32_t>(),
@@ -669,6 +676,7 @@ extern "C" vgi_status vgi_renderer_import_state(vgi_renderer* r, const void* blo
     r->has_prev_ind_spec = hd.flags[4];
     r->has_prev_ao = hd.flags[5];
     r->hash_grid_ready = hd.flags[6];
+    r->hg_scratch_zero = false;
     r->has_prev_color = hd.flags[7] && r->cfg.taa;
     r->prev_indirect_diffuse = slot_ptr(r, hd.slot);
     vgi_denoiser* d = r->den;
