@@ -57,6 +57,23 @@ const char* vgi_version(void);
  * violet's render graph reads such images through descriptor bindings). */
 vgi_status vgi_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 
+/* Frame fence with a deadline: violet waits for the oldest frame's command
+ * buffer fence for at most 5 s, warns above 33.3 ms and carries on after a
+ * time-out (crates/violet-app/src/render_loop.rs:445-482).  A fence marks
+ * everything issued on a stream so far; waiting never blocks past the
+ * deadline.  VGI_FENCE_TIMEOUT_MS is violet's deadline. */
+#define VGI_FENCE_TIMEOUT_MS 5000u
+typedef struct vgi_fence vgi_fence;
+vgi_status vgi_fence_create(vgi_fence** out);
+vgi_status vgi_fence_destroy(vgi_fence* fence);
+/* Marks the work issued on `stream` so far (replaces an earlier mark). */
+vgi_status vgi_fence_record(vgi_fence* fence, void* stream);
+/* VGI_OK once the marked work has finished (also for a fence never
+ * recorded); VGI_E_TIMEOUT if it has not within timeout_ms (0 = poll once):
+ * the work keeps running and the fence can be waited on again.  The time
+ * spent waiting goes to *waited_ms when it is not NULL. */
+vgi_status vgi_fence_wait(vgi_fence* fence, uint32_t timeout_ms, double* waited_ms);
+
 /* Per-pass GPU timers: the counterpart of violet's per-pass timestamp
  * queries (crates/violet/src/gpu_profiling.rs:253-355, written around every
  * pass by render_graph.rs:2585-2591,2664-2669).  When enabled, every pass

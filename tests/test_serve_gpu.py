@@ -149,3 +149,18 @@ def test_up_front_admission_stops_at_the_budget(vgi):
     srv.close()
     s = stats.summary()
     assert s["requests"] == len(reqs) and s["peak_hbm_gb"] * 1e9 <= srv.budget
+
+
+def test_batches_past_the_frame_deadline_are_counted_and_finish(vgi):
+    """A zero frame deadline marks every batch still running at its first
+    poll as timed out; as in violet, the server keeps waiting and every
+    request completes."""
+    from violet_amd import serve
+
+    sc, views, cfg = _setup()
+    srv = serve.Server(sc, views, hbm_budget_bytes=50e9, config=cfg, max_batch=4, streams=2, frame_timeout_ms=0)
+    reqs = serve.poisson_requests(16, 2000.0, len(views), seed=9)
+    stats = srv.run(reqs, timeout_s=60.0)
+    srv.close()
+    s = stats.summary()
+    assert s["requests"] == len(reqs) and s["timeouts"] > 0

@@ -159,6 +159,10 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_trace_stats_enable": (C.c_int32, [u32]),
     "vgi_probe_dependent_fetch": (C.c_int32, [C.c_size_t, u32, u32, C.POINTER(C.c_double), vp]),
     "vgi_profiler_only": (C.c_int32, [C.c_char_p]),
+    "vgi_fence_create": (C.c_int32, [C.POINTER(vp)]),
+    "vgi_fence_destroy": (C.c_int32, [vp]),
+    "vgi_fence_record": (C.c_int32, [vp, vp]),
+    "vgi_fence_wait": (C.c_int32, [vp, u32, C.POINTER(C.c_double)]),
     "vgi_pass_denoise_temporal": (C.c_int32, [C.POINTER(FrameParams), C.POINTER(DenoiseTemporalIO),
                                               C.POINTER(PCDenoiseTemporal), vp]),
     "vgi_pass_denoise_disocclusion_fix": (C.c_int32, [C.POINTER(DenoiseSpatialIO), vp]),

@@ -4,6 +4,9 @@
  * hipMemsetAsync so that a frame can be captured into a hipGraph.
  */
 #include <algorithm>
+#include <chrono>
+#include <new>
+#include <thread>
 
 #include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
@@ -87,6 +90,71 @@ const BatchOps* batch_ops_util(int kind) {
 }  // namespace vgi
 
 using namespace vgi;
+
+struct vgi_fence {
+    hipEvent_t ev = nullptr;
+    bool recorded = false;
+};
+
+extern "C" vgi_status vgi_fence_create(vgi_fence** out) {
+    VGI_REQUIRE(out, "vgi_fence_create: null out");
+    *out = nullptr;
+    vgi_fence* f = new (std::nothrow) vgi_fence();
+    if (!f) return set_error(VGI_E_NOMEM, "vgi_fence_create: out of host memory");
+    const hipError_t e = hipEventCreateWithFlags(&f->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete f;
+        return check_hip(e, "vgi_fence_create");
+    }
+    *out = f;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_fence_destroy(vgi_fence* f) {
+    if (!f) return VGI_OK;
+    if (f->ev) (void)hipEventDestroy(f->ev);
+    delete f;
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_fence_record(vgi_fence* f, void* stream) {
+    VGI_REQUIRE(f, "vgi_fence_record: null fence");
+    VGI_TRY_HIP(hipEventRecord(f->ev, (hipStream_t)stream));
+    f->recorded = true;
+    return VGI_OK;
+}
+
+// Polls the event: back-to-back for the first 100 us, then sleeping with a
+// doubling interval up to 1 ms, so short frames are seen at once and a stuck
+// one costs little host time.
+extern "C" vgi_status vgi_fence_wait(vgi_fence* f, uint32_t timeout_ms, double* waited_ms) {
+    VGI_REQUIRE(f, "vgi_fence_wait: null fence");
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t0 = clk::now();
+    const clk::time_point deadline = t0 + std::chrono::milliseconds(timeout_ms);
+    auto elapsed_ms = [&] { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
+    vgi_status st = VGI_OK;
+    std::chrono::microseconds nap(0);
+    while (f->recorded) {
+        const hipError_t e = hipEventQuery(f->ev);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) {
+            st = check_hip(e, "vgi_fence_wait");
+            break;
+        }
+        const clk::time_point now = clk::now();
+        if (now >= deadline) {
+            st = set_error(VGI_E_TIMEOUT, "vgi_fence_wait: the marked work did not finish within %u ms", timeout_ms);
+            break;
+        }
+        if (now - t0 > std::chrono::microseconds(100)) {
+            nap = nap.count() ? std::min(nap * 2, std::chrono::microseconds(1000)) : std::chrono::microseconds(20);
+            std::this_thread::sleep_for(std::min<clk::duration>(nap, deadline - now));
+        }
+    }
+    if (waited_ms) *waited_ms = elapsed_ms();
+    return st;
+}
 
 extern "C" vgi_status vgi_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
     VGI_REQUIRE(dst && src, "vgi_memcpy_d2d: null pointer");

@@ -68,3 +68,39 @@ def copy_from_ptr(src: int, nbytes: int) -> "torch.Tensor":
     out = empty(nbytes)
     _lib.call("vgi_memcpy_d2d", out.data_ptr(), src, nbytes, stream())
     return out
+
+
+class Fence:
+    """vgi_fence: marks the work issued on a stream and waits for it with a
+    deadline (violet's 5 s frame fence, render_loop.rs:445-482)."""
+
+    TIMEOUT_MS = 5000
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        _lib.call("vgi_fence_create", C.byref(self._h))
+
+    def record(self, stream_handle: int | None = None) -> None:
+        _lib.call("vgi_fence_record", self._h, stream() if stream_handle is None else stream_handle)
+
+    def wait(self, timeout_ms: int = TIMEOUT_MS) -> tuple[bool, float]:
+        """(finished, ms waited); finished is False after a time-out (the work
+        keeps running)."""
+        waited = C.c_double()
+        st = _lib.lib().vgi_fence_wait(self._h, int(timeout_ms), C.byref(waited))
+        if st == _lib.VGI_E_TIMEOUT:
+            return False, waited.value
+        if st != _lib.VGI_OK:
+            raise _lib.VgiError(st, _lib.lib().vgi_last_error().decode())
+        return True, waited.value
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().vgi_fence_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

@@ -31,6 +31,7 @@ def _counter_means(db_path: str, counter: str) -> dict[str, tuple[float, int]]:
 
 
 def _short(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("void ", "")
 
 

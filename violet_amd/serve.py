@@ -19,7 +19,10 @@ are sequential, and, with `max_batch_pixels`, no more pixels than that
 (violet_amd.batch) on that slot's stream.  Requests that arrive while other
 batches run are admitted as soon as a slot frees, so the work of different
 batches overlaps on the device.  A request's latency runs from its arrival
-to the host observing its batch's completion event.
+to the host observing its batch's completion fence (vgi_fence).  As violet's
+frame fence (crates/violet-app/src/render_loop.rs:445-482), a batch still
+running `frame_timeout_ms` after its issue is counted as timed out and
+waited for further, and one finishing after 33.3 ms as slow.
 """
 from __future__ import annotations
 
@@ -58,6 +61,8 @@ class ServeStats:
     batches: int = 0
     evictions: int = 0
     peak_bytes: int = 0
+    timeouts: int = 0       # batches still running frame_timeout_ms after their issue
+    slow_batches: int = 0   # batches that took longer than 33.3 ms (violet's warning threshold)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies_ms) if self.latencies_ms else np.zeros(1)
@@ -65,7 +70,8 @@ class ServeStats:
                     p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
                     mpix_per_s=self.pixels / max(self.wall_s, 1e-9) / 1e6, wall_s=self.wall_s,
                     batches=self.batches, mean_batch=len(self.latencies_ms) / max(self.batches, 1),
-                    evictions=self.evictions, peak_hbm_gb=self.peak_bytes / 1e9)
+                    evictions=self.evictions, peak_hbm_gb=self.peak_bytes / 1e9, timeouts=self.timeouts,
+                    slow_batches=self.slow_batches)
 
 
 def view_frame_params(spec: ViewSpec, frame: int, deg_per_frame: float = 6.0):
@@ -85,7 +91,8 @@ def view_frame_params(spec: ViewSpec, frame: int, deg_per_frame: float = 6.0):
 class Server:
     def __init__(self, scene: Scene, views: list[ViewSpec], hbm_budget_bytes: float,
                  config: renderer.RestirConfig | None = None, max_batch: int = 16, streams: int = 2,
-                 batch_config: batch.BatchConfig | None = None, on_complete=None, max_batch_pixels: int = 0):
+                 batch_config: batch.BatchConfig | None = None, on_complete=None, max_batch_pixels: int = 0,
+                 frame_timeout_ms: int = gpu.Fence.TIMEOUT_MS):
         import torch
 
         gpu.require_gpu()
@@ -96,11 +103,13 @@ class Server:
         self.config = config or renderer.RestirConfig()
         self.max_batch = max_batch
         self.max_batch_pixels = max_batch_pixels  # 0: no limit
+        self.frame_timeout_ms = frame_timeout_ms
         self.on_complete = on_complete
         bc = batch_config or batch.BatchConfig()
         bc.max_views = max(bc.max_views, max_batch)
         self.slots = [(batch.Batch(bc), torch.cuda.Stream()) for _ in range(streams)]
-        self.busy = [None] * streams           # per slot: (event, [(req, view)], issue time)
+        self.fences = [gpu.Fence() for _ in range(streams)]
+        self.busy = [None] * streams           # per slot: [issue time, [(req, outputs)], timed out]
         self.resident: dict[int, renderer.RestirRenderer] = {}
         self.frame_of: dict[int, int] = collections.defaultdict(int)
         self.last_use: dict[int, float] = {}
@@ -147,9 +156,17 @@ class Server:
     # -- loop -----------------------------------------------------------------
     def _poll(self, now: float, stats: ServeStats) -> None:
         for k, b in enumerate(self.busy):
-            if b is None or not b[0].query():
+            if b is None:
+                continue
+            done, _ = self.fences[k].wait(0)
+            if not done:
+                if not b[2] and (now - b[0]) * 1e3 > self.frame_timeout_ms:
+                    b[2] = True  # violet reports the time-out and waits on
+                    stats.timeouts += 1
                 continue
             t_done = time.perf_counter()
+            if (t_done - self.t0 - b[0]) * 1e3 > 33.3:
+                stats.slow_batches += 1
             for req, outs in b[1]:
                 stats.latencies_ms.append((t_done - self.t0 - req.arrival_s) * 1e3)
                 spec = self.views[req.view]
@@ -184,12 +201,11 @@ class Server:
         idx = [self.frame_of[r.view] for r in picked]
         with self.torch.cuda.stream(stream):
             outs = bt.add_passes(self.scene, rs, fps, idx, stream=stream.cuda_stream)
-            ev = self.torch.cuda.Event()
-            ev.record(stream)
+            self.fences[k].record(stream.cuda_stream)
         for r in picked:
             self.frame_of[r.view] += 1
             self.inflight_views.add(r.view)
-        self.busy[k] = (ev, list(zip(picked, outs)))
+        self.busy[k] = [now, list(zip(picked, outs)), False]
         stats.batches += 1
 
     def admit_all(self, stats: ServeStats | None = None) -> int:
@@ -238,6 +254,8 @@ class Server:
         self.resident.clear()
         for b, _ in self.slots:
             b.close()
+        for f in self.fences:
+            f.close()
 
 
 def poisson_requests(n: int, rate_hz: float, num_views: int, seed: int) -> list[Request]:
