@@ -120,3 +120,50 @@ def scene_from_mesh(mesh) -> Doc:
     d.j["scenes"] = [{"nodes": [0]}]
     d.j["scene"] = 0
     return d
+
+
+def corrupted_documents(d: Doc, out_dir: Path, seed: int = 99) -> list[Path]:
+    """200 seeded corruptions of a document: 60 truncations of the .gltf
+    text, 60 .glb forms with 1-7 random bytes replaced, and 80 .gltf forms
+    with one accessor / buffer view / mesh / node field deleted, set to -1,
+    to a string or to 2^40."""
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    gl = d.save_gltf(out_dir / "base.gltf")
+    glb = d.save_glb(out_dir / "base.glb").read_bytes()
+    text = gl.read_text()
+    j = json.loads(text)
+    paths = []
+    for k in range(60):
+        p = out_dir / f"t{k}.gltf"
+        p.write_text(text[: int(rng.integers(0, len(text)))])
+        paths.append(p)
+    for k in range(60):
+        b = bytearray(glb)
+        for _ in range(int(rng.integers(1, 8))):
+            b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        p = out_dir / f"f{k}.glb"
+        p.write_bytes(bytes(b))
+        paths.append(p)
+    keys = [("accessors", "count"), ("accessors", "componentType"), ("bufferViews", "byteOffset"),
+            ("bufferViews", "byteLength"), ("bufferViews", "buffer"), ("meshes", "primitives"), ("nodes", "mesh"),
+            ("nodes", "children"), ("nodes", "matrix"), ("accessors", "bufferView")]
+    for k in range(80):
+        jj = json.loads(json.dumps(j))
+        arr, key = keys[int(rng.integers(0, len(keys)))]
+        items = [x for x in jj[arr] if isinstance(x, dict)]
+        tgt = items[int(rng.integers(0, len(items)))]
+        choice = int(rng.integers(0, 4))
+        if choice == 0:
+            tgt.pop(key, None)
+        elif choice == 1:
+            tgt[key] = -1
+        elif choice == 2:
+            tgt[key] = "x"
+        else:
+            tgt[key] = 2 ** 40
+        p = out_dir / f"m{k}.gltf"
+        p.write_text(json.dumps(jj))
+        paths.append(p)
+    return paths

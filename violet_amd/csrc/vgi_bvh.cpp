@@ -300,9 +300,10 @@ static vgi_status upload(DevBuf& b, const std::vector<T>& v) {
     return VGI_OK;
 }
 
-extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** out) {
-    VGI_REQUIRE(desc && out, "vgi_scene_create: null argument");
-    *out = nullptr;
+namespace vgi {
+
+vgi_status build_scene_host(const vgi_scene_desc* desc, HostScene& hs) {
+    VGI_REQUIRE(desc, "vgi_scene_create: null argument");
     VGI_REQUIRE(desc->positions && desc->indices && desc->num_vertices > 0 && desc->num_triangles > 0,
                 "vgi_scene_create: empty scene");
     VGI_REQUIRE(desc->num_triangles < (1u << 27), "vgi_scene_create: too many triangles (%u)", desc->num_triangles);
@@ -350,8 +351,10 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     c4.out.reserve(b.nodes.size() / 2 + 1);
     root = c4.emit(root, 0);
 
-    std::vector<BvhTri> tris(nt);
-    std::vector<TriShade> shade(nt);
+    std::vector<BvhTri>& tris = hs.tris;
+    std::vector<TriShade>& shade = hs.shade;
+    tris.assign(nt, BvhTri{});
+    shade.assign(nt, TriShade{});
     for (uint32_t k = 0; k < nt; ++k) {
         uint32_t t = b.order[k];
         const float* p0 = desc->positions + 3 * desc->indices[3 * t + 0];
@@ -364,15 +367,17 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     for (uint32_t t = 0; t < nt; ++t)
         shade[t] = TriShade{desc->indices[3 * t], desc->indices[3 * t + 1], desc->indices[3 * t + 2],
                             desc->triangle_material ? desc->triangle_material[t] : 0u};
-    std::vector<float> pos(desc->positions, desc->positions + 3 * (size_t)nv);
-    std::vector<float> nrm;
+    std::vector<float>& pos = hs.positions;
+    pos.assign(desc->positions, desc->positions + 3 * (size_t)nv);
+    std::vector<float>& nrm = hs.normals;
     if (desc->normals) {
         nrm.assign(desc->normals, desc->normals + 3 * (size_t)nv);
     } else {
         // flat shading: per-vertex normal slot unused; use the geometric normal
         nrm.assign(3 * (size_t)nv, 0.0f);
     }
-    std::vector<Material> mats(desc->num_materials);
+    std::vector<Material>& mats = hs.materials;
+    mats.assign(desc->num_materials, Material{});
     for (uint32_t m = 0; m < desc->num_materials; ++m) {
         const vgi_material& s = desc->materials[m];
         Material& d = mats[m];
@@ -381,41 +386,48 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
         d.roughness = s.roughness;
         d.pad[0] = d.pad[1] = 0.0f;
     }
+    hs.nodes = std::move(c4.out);
+    hs.root = root;
+    hs.levels = (uint32_t)c4.max_level + 1;  // levels of 4-wide nodes
+    VGI_REQUIRE(3u * hs.levels <= (uint32_t)kStackMaxEntries,
+                "vgi_scene_create: BVH of %u levels exceeds the traversal stack", hs.levels);
+    // traversal reads both arrays through 32-bit buffer offsets
+    if ((uint64_t)hs.nodes.size() * sizeof(BvhNode) > 0x7FFFFFFFull || (uint64_t)nt * sizeof(BvhTri) > 0x7FFFFFFFull)
+        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: %u triangles exceed the 2 GiB node/triangle arrays", nt);
+    return VGI_OK;
+}
 
+}  // namespace vgi
+
+extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** out) {
+    VGI_REQUIRE(desc && out, "vgi_scene_create: null argument");
+    *out = nullptr;
+    HostScene hs;
+    VGI_CHECK(build_scene_host(desc, hs));
+    const uint32_t nt = desc->num_triangles, nv = desc->num_vertices;
     vgi_scene* sc = new (std::nothrow) vgi_scene();
     if (!sc) return set_error(VGI_E_NOMEM, "vgi_scene_create: out of host memory");
     vgi_status st;
-    if ((st = upload(sc->nodes, c4.out)) != VGI_OK || (st = upload(sc->tris, tris)) != VGI_OK ||
-        (st = upload(sc->shade, shade)) != VGI_OK || (st = upload(sc->positions, pos)) != VGI_OK ||
-        (st = upload(sc->normals, nrm)) != VGI_OK || (st = upload(sc->materials, mats)) != VGI_OK) {
+    if ((st = upload(sc->nodes, hs.nodes)) != VGI_OK || (st = upload(sc->tris, hs.tris)) != VGI_OK ||
+        (st = upload(sc->shade, hs.shade)) != VGI_OK || (st = upload(sc->positions, hs.positions)) != VGI_OK ||
+        (st = upload(sc->normals, hs.normals)) != VGI_OK || (st = upload(sc->materials, hs.materials)) != VGI_OK) {
         delete sc;
         return st;
     }
     sc->num_vertices = nv;
     sc->num_triangles = nt;
-    sc->depth = (uint32_t)c4.max_level + 1;  // levels of 4-wide nodes
+    sc->depth = hs.levels;
     sc->view.nodes = sc->nodes.as<BvhNode>();
     sc->view.tris = sc->tris.as<BvhTri>();
     sc->view.shade = sc->shade.as<TriShade>();
     sc->view.positions = sc->positions.as<float>();
     sc->view.normals = desc->normals ? sc->normals.as<float>() : nullptr;
     sc->view.materials = sc->materials.as<Material>();
-    sc->view.num_nodes = (uint32_t)c4.out.size();
+    sc->view.num_nodes = (uint32_t)hs.nodes.size();
     sc->view.num_tris = nt;
-    sc->view.root = root;
+    sc->view.root = hs.root;
     sc->view.stack_depth = 3u * sc->depth;  // at most 3 far children pushed per level
-    if (sc->view.stack_depth > kStackMaxEntries) {
-        delete sc;
-        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: BVH of %u levels exceeds the traversal stack",
-                         (unsigned)c4.max_level + 1);
-    }
     sc->view.counters = nullptr;
-    // traversal reads both arrays through 32-bit buffer offsets
-    if ((uint64_t)sc->view.num_nodes * sizeof(BvhNode) > 0x7FFFFFFFull ||
-        (uint64_t)nt * sizeof(BvhTri) > 0x7FFFFFFFull) {
-        delete sc;
-        return set_error(VGI_E_UNSUPPORTED, "vgi_scene_create: %u triangles exceed the 2 GiB node/triangle arrays", nt);
-    }
     *out = sc;
     return VGI_OK;
 }

@@ -24,6 +24,9 @@
  * shadow rays accept the first hit (RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH).
  */
 #pragma once
+#include <vector>
+
+#include "../../include/vgi.h"
 #include "vgi_common.h"
 
 namespace vgi {
@@ -88,6 +91,20 @@ constexpr uint32_t kStackMaxEntries = 3 * kBvhMaxDepth;  // 3 far children per 4
 // LDS stack per wave allows more of them than ~80 registers would.  (The
 // cache-ray pass runs too few waves to gain from it and keeps its registers.)
 constexpr int kTraceWaves = 8;
+
+// The host side of vgi_scene_create: validation, the binned-SAH build, the
+// 4-wide collapse and the device arrays, before any upload (so host tools
+// can build and check a scene without a GPU).
+struct HostScene {
+    std::vector<BvhNode> nodes;
+    std::vector<BvhTri> tris;
+    std::vector<TriShade> shade;
+    std::vector<float> positions, normals;
+    std::vector<Material> materials;
+    int32_t root = 0;
+    uint32_t levels = 0;  // levels of 4-wide nodes
+};
+vgi_status build_scene_host(const vgi_scene_desc* desc, HostScene& out);
 
 // Dynamic LDS bytes of a 64-lane traversal stack for this scene.
 inline size_t traversal_lds_bytes(const SceneView& s) {
