@@ -357,7 +357,8 @@ def run_batched_form(args, sc, ntris):
     """16 views over the shared scene, FP16 storage: the plain path (each
     view's frame issued alone, one after the other) against one fused
     persistent launch per pass over all views, graph-replayed, with the
-    group counts autotuned on these views."""
+    launch shapes (sample-pass tile, a-trous block) and persistent group
+    counts autotuned on these views; the default shapes are restored after."""
     import torch
 
     from violet_amd import _lib, autotune, batch, renderer, serve
@@ -401,7 +402,7 @@ def run_batched_form(args, sc, ntris):
     views = [renderer.RestirRenderer(s.width, s.height, cfg) for s in specs]
     n = len(views)
     log(f"[bench] batched form: {n} views, {px / 1e6:.2f} Mpx per frame; autotuning")
-    best, table, f0 = autotune.tune_batch(sc, views, fps_of, frame0=0)
+    best, table, f0, chosen = autotune.tune_batch(sc, views, fps_of, frame0=0)
     b = batch.Batch(best)
     for f in range(f0, f0 + 3):
         b.add_passes(sc, views, [fps_of(i, f) for i in range(n)], [f] * n)
@@ -434,6 +435,7 @@ def run_batched_form(args, sc, ntris):
     pb.close()
     for r in views:
         r.close()
+    _lib.set_tuning()  # default shapes for the other forms
     return {"workload": f"batched_{n}_views_f16", "views": n, "pixels_per_frame": px, "triangles": ntris,
             "resolutions": [[s.width, s.height] for s in specs], "frames": K,
             "plain_mpix_per_s": round(px / plain_s / 1e6, 2), "batched_mpix_per_s": round(px / batched_s / 1e6, 2),
@@ -441,9 +443,10 @@ def run_batched_form(args, sc, ntris):
             "batched_ms_per_frame": round(batched_s * 1e3, 3),
             "batched_p50_ms": round(float(np.percentile(lat, 50)), 3),
             "batched_p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "graph": True, "autotuned": {k: getattr(best, k) for k in ("pixel_groups", "trace_groups", "atrous_groups")},
-            "autotune_table_ms": [[c["trace_groups"], c["pixel_groups"], c["atrous_groups"], round(ms, 3)]
-                                  for c, ms in table],
+            "graph": True, "autotuned": chosen,
+            "autotune_columns": list(autotune.SHAPES + autotune.GROUPS) + ["median_ms", "spread_ms"],
+            "autotune_table": [[c[k] for k in autotune.SHAPES + autotune.GROUPS] + [round(ms, 3), round(sp, 3)]
+                               for c, ms, sp in table],
             "plain_pass_ms_per_frame": plain_passes, "batched_pass_ms_per_frame": batched_passes}
 
 

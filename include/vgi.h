@@ -98,6 +98,24 @@ vgi_status vgi_profiler_only(const char* name);
 vgi_status vgi_profiler_reset(void);
 int32_t vgi_profiler_read(vgi_profile_entry* out, int32_t cap);
 
+/* Launch shapes (no reference counterpart: violet's shaders fix them with
+ * [numthreads], e.g. svgf_atrous_filter.hlsl's 8x8).  Results do not depend
+ * on them; violet_amd/autotune.py times the candidates on the views to be
+ * rendered and sets the fastest.  The setting is process-wide and applies to
+ * launches issued after the call (set it before rendering, not while another
+ * thread issues frames).
+ *   atrous_block      a-trous lattice block edge staged in LDS per workgroup:
+ *                     8 (64 threads), 16 (256, the default) or 32 (1024)
+ *   trace_tile_width  pixel tile of one 64-lane wave in the four sample
+ *                     passes: 8 (8x8 in 2x2 quads, the default), 16 (16x4)
+ *                     or 32 (32x2) */
+typedef struct vgi_tuning {
+    uint32_t atrous_block;
+    uint32_t trace_tile_width;
+} vgi_tuning;
+vgi_status vgi_tuning_set(const vgi_tuning* tuning); /* NULL = the defaults */
+vgi_status vgi_tuning_get(vgi_tuning* tuning);
+
 /* Traversal statistics (no reference counterpart; measurement only).  When
  * enabled, every ray-tracing pass launches an instrumented build of its
  * kernel that counts rays, interior nodes visited and triangles tested, and

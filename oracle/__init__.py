@@ -557,7 +557,7 @@ class OracleRenderer:
             self.q = np.zeros((131072, 6), np.uint32)
             self.cnt = np.zeros(1, np.uint32)
             # ordered update: the query threshold word before the first count
-            self.thr = (np.array([R.hg_query_threshold_for(2 * w * h + 131072)], np.uint32) if self.ordered
+            self.thr = (np.array([R.hg_query_threshold_for(2 * w * h)], np.uint32) if self.ordered
                         else None)
         if self.decay_on:
             self.decay, self.cells = R.hg_decay(self.decay, self.cells)

@@ -43,7 +43,7 @@ def _mirrors():
         "vgi_renderer_output": _abi.RendererOutput, "vgi_batch_config": _abi.BatchConfig,
         "vgi_resample_params": _abi.ResampleParams, "vgi_denoise_frame_io": _lib.DenoiseFrameIO,
         "vgi_hash_grid_ordered_select_io": _abi.HGOSelectIO, "vgi_hash_grid_ordered_compact_io": _abi.HGOCompactIO,
-        "vgi_hash_grid_ordered_raygen_io": _abi.HGORaygenIO,
+        "vgi_hash_grid_ordered_raygen_io": _abi.HGORaygenIO, "vgi_tuning": _lib.Tuning,
     }
     return m
 
@@ -106,3 +106,22 @@ def test_ctypes_mirrors_match_c_layout(tmp_path):
     declared = set(re.findall(r"typedef struct (vgi_\w+) \{", (ROOT / "include" / "vgi.h").read_text()))
     unmirrored = declared - set(mirrors) - {"vgi_hash_grid_cell", "vgi_hash_grid_query"}
     assert not unmirrored, f"structs without a ctypes mirror: {sorted(unmirrored)}"
+
+
+def test_tuning_rejects_unknown_shapes():
+    """vgi_tuning_set takes 8, 16 or 32 for either shape (0 = the default)
+    and rejects anything else without touching the current setting."""
+    import ctypes as C
+
+    from violet_amd import _lib
+
+    lib = _lib.lib()
+    _lib.set_tuning()
+    for ab, tw in ((12, 8), (16, 64), (4, 4)):
+        assert lib.vgi_tuning_set(C.byref(_lib.Tuning(ab, tw))) == _lib.VGI_E_INVALID
+        assert _lib.get_tuning() == dict(atrous_block=16, trace_tile_width=8)
+    _lib.set_tuning(0, 0)
+    assert _lib.get_tuning() == dict(atrous_block=16, trace_tile_width=8)
+    _lib.set_tuning(32, 16)
+    assert _lib.get_tuning() == dict(atrous_block=32, trace_tile_width=16)
+    _lib.set_tuning()

@@ -79,7 +79,7 @@ def test_frame0_bit_exact_pipeline(run, o32):
     cells = np.zeros((65536 * 16, 4), np.uint32)
     q = np.zeros((128 * 1024, 6), np.uint32)
     cnt = np.zeros(1, np.uint32)
-    tr = o32.trace_res(cells, q, cnt)
+    tr = o32.trace_res(cells, q, cnt, threshold=s["hash_grid_query_threshold"])
     hit, rad = o32.diff_sample_gen(fp, tr, gb, depth, seeds["diff_gen"], False)
     bits_equal(s["diff_new_hit_pos_normal"], hit, "diff new hit")
     bits_equal(s["diff_new_hit_radiance"], rad, "diff new radiance")
@@ -174,7 +174,8 @@ def test_trace_passes_replayed(run, o32, k):
     seeds = seeds_for_frame(k, fl["hg"], fl["validation"])
     q = np.zeros((128 * 1024, 6), np.uint32)
     cnt = np.zeros(1, np.uint32)
-    tr = o32.trace_res(s["hash_grid_storage"], q, cnt, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"])
+    tr = o32.trace_res(s["hash_grid_storage"], q, cnt, prev_ind_diff=p["out_diffuse"], prev_depth=p["depth"],
+                       threshold=s["hash_grid_query_threshold"])
     if not fl["validation"]:
         hit, rad = o32.diff_sample_gen(fp, tr, s["gbuffer"], s["depth"], seeds["diff_gen"], True)
         bits_equal(s["diff_new_hit_pos_normal"], hit, f"diff new hit[{k}]")

@@ -149,6 +149,12 @@ BVH_NODE_BYTES = 64  # VGI_BVH_NODE_BYTES
 BVH_TRI_BYTES = 48  # VGI_BVH_TRI_BYTES
 
 
+class Tuning(C.Structure):
+    """vgi_tuning: process-wide launch shapes (results do not depend on them)."""
+
+    _fields_ = [("atrous_block", u32), ("trace_tile_width", u32)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES: dict[str, tuple] = {
     "vgi_last_error": (C.c_char_p, []),
@@ -159,6 +165,8 @@ _SIGNATURES: dict[str, tuple] = {
     "vgi_trace_stats_enable": (C.c_int32, [u32]),
     "vgi_probe_dependent_fetch": (C.c_int32, [C.c_size_t, u32, u32, C.POINTER(C.c_double), vp]),
     "vgi_profiler_only": (C.c_int32, [C.c_char_p]),
+    "vgi_tuning_set": (C.c_int32, [C.POINTER(Tuning)]),
+    "vgi_tuning_get": (C.c_int32, [C.POINTER(Tuning)]),
     "vgi_fence_create": (C.c_int32, [C.POINTER(vp)]),
     "vgi_fence_destroy": (C.c_int32, [vp]),
     "vgi_fence_record": (C.c_int32, [vp, vp]),
@@ -207,6 +215,17 @@ def call(name: str, *args) -> None:
     st = getattr(lib(), name)(*args)
     if st != VGI_OK:
         raise VgiError(st, lib().vgi_last_error().decode())
+
+
+def set_tuning(atrous_block: int = 16, trace_tile_width: int = 8) -> None:
+    """Launch shapes for the passes issued from now on (vgi_tuning_set)."""
+    call("vgi_tuning_set", C.byref(Tuning(atrous_block, trace_tile_width)))
+
+
+def get_tuning() -> dict:
+    t = Tuning()
+    call("vgi_tuning_get", C.byref(t))
+    return dict(atrous_block=t.atrous_block, trace_tile_width=t.trace_tile_width)
 
 
 def exported_symbols() -> list[str]:

@@ -1,19 +1,30 @@
-"""Autotuner for batched frames: picks the launch shape of the fused passes
-(ray-tracing passes, pixel and cache passes, a-trous passes) on the actual
-views and scene, by timing whole batched frames.
+"""Autotuner for batched frames: picks the launch shapes of the fused passes
+on the actual views and scene, by timing whole graph-replayed batched frames.
 
-Each class either launches one workgroup per tile (multiplier 0: the
-hardware dispatcher balances uneven tiles) or a persistent grid of
-`multiplier x compute units` groups striding over the tiles.  Coordinate
-descent over the three classes, trace passes first since they dominate a
-frame; every candidate runs graph-replayed frames and the fastest mean frame
-time wins.  Tuning advances the views by the frames it renders.
+Dimensions, in order (the trace passes dominate a frame):
+  trace_tile_width  pixel tile of a 64-lane wave in the four sample passes:
+                    8x8 (2x2 quads), 16x4 or 32x2 (vgi_tuning)
+  atrous_block      a-trous lattice block staged in LDS per workgroup:
+                    16x16 (256 threads), 8x8 (64) or 32x32 (1024) (vgi_tuning)
+  trace_groups,     fused launches: one workgroup per tile (0) or a
+  pixel_groups,     persistent grid of `multiplier x compute units` groups
+  atrous_groups     striding over the tiles (vgi_batch_config)
+
+Coordinate descent: for each dimension every option is timed `repeats`
+times (each a median over `frames` frames after `warmup` untimed ones,
+options interleaved round-robin so slow drifts hit them alike); an option
+replaces the incumbent only if its median beats the incumbent's by more
+than the incumbent's own spread across repeats (and at least 1%).  Shapes
+do not change results, so tuning only advances the views by the frames it
+renders.  The chosen launch shapes stay set (vgi_tuning_set) on return.
 """
 from __future__ import annotations
 
 import time
 
-from . import batch
+import numpy as np
+
+from . import _lib, batch
 
 
 def _time_frames(b, scene, views, fps_of, frame0: int, warmup: int, frames: int) -> float:
@@ -24,35 +35,59 @@ def _time_frames(b, scene, views, fps_of, frame0: int, warmup: int, frames: int)
         f = frame0 + k
         b.add_passes(scene, views, [fps_of(i, f) for i in range(n)], [f] * n)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    ts = []
     for k in range(frames):
         f = frame0 + warmup + k
+        t0 = time.perf_counter()
         b.add_passes(scene, views, [fps_of(i, f) for i in range(n)], [f] * n)
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / frames
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
 
 
-def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: int = 4,
-               trace=(0, 16, 32), pixel=(0, 8), atrous=(0, 4), use_graph: bool = True):
+DIMENSIONS = (("trace_tile_width", (8, 16, 32)), ("atrous_block", (16, 8, 32)), ("trace_groups", (0, 16, 32)),
+              ("pixel_groups", (0, 8)), ("atrous_groups", (0, 4)))
+SHAPES = ("trace_tile_width", "atrous_block")
+GROUPS = ("trace_groups", "pixel_groups", "atrous_groups")
+
+
+def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: int = 6, repeats: int = 3,
+               dimensions=DIMENSIONS, use_graph: bool = True):
     """fps_of(view_index, frame) -> FrameParams.  Returns (best BatchConfig,
-    list of (config dict, ms per frame)) and the next free frame index."""
+    table of (candidate dict, median ms, spread ms), the next free frame
+    index); the best launch shapes are left set."""
     import torch
 
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    best = dict(trace_groups=0, pixel_groups=0, atrous_groups=0)
+    best = dict(trace_tile_width=8, atrous_block=16, trace_groups=0, pixel_groups=0, atrous_groups=0)
     table = []
     f = frame0
-    for key, opts in (("trace_groups", trace), ("pixel_groups", pixel), ("atrous_groups", atrous)):
-        scores = {}
-        for m in opts:
-            cand = dict(best)
-            cand[key] = m * cus
-            cfg = batch.BatchConfig(max_views=max(len(views), 1), use_graph=use_graph, **cand)
-            b = batch.Batch(cfg)
-            ms = _time_frames(b, scene, views, fps_of, f, warmup, frames) * 1e3
-            f += warmup + frames
+
+    def config(c):
+        return batch.BatchConfig(max_views=max(len(views), 1), use_graph=use_graph,
+                                 **{k: c[k] * (cus if k in GROUPS else 1) for k in GROUPS})
+
+    for key, opts in dimensions:
+        cands = [dict(best, **{key: m}) for m in opts]
+        runs = {i: [] for i in range(len(cands))}
+        bs = {}
+        for _ in range(repeats):
+            for i, c in enumerate(cands):
+                _lib.set_tuning(c["atrous_block"], c["trace_tile_width"])
+                if i not in bs:
+                    bs[i] = batch.Batch(config(c))
+                runs[i].append(_time_frames(bs[i], scene, views, fps_of, f, warmup, frames) * 1e3)
+                f += warmup + frames
+        for b in bs.values():
             b.close()
-            scores[m] = ms
-            table.append((cand, ms))
-        best[key] = min(scores, key=scores.get) * cus
-    return batch.BatchConfig(max_views=max(len(views), 1), use_graph=use_graph, **best), table, f
+        med = {i: float(np.median(v)) for i, v in runs.items()}
+        spread = {i: float(np.max(v) - np.min(v)) for i, v in runs.items()}
+        inc = next(i for i, c in enumerate(cands) if c[key] == best[key])
+        win = inc
+        for i in med:
+            if med[i] < med[win] - max(spread[inc], 0.01 * med[inc]):
+                win = i
+        best = dict(cands[win])
+        table += [(cands[i], med[i], spread[i]) for i in med]
+    _lib.set_tuning(best["atrous_block"], best["trace_tile_width"])
+    return config(best), table, f, best

@@ -69,6 +69,7 @@ struct vgi_batch {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cross = nullptr;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t key_storage[2] = {0, 0};
+    uint32_t key_shape[2][2] = {{0, 0}, {0, 0}};  // launch shapes (vgi_tuning) the graph was captured with
     size_t key_lds[2] = {0, 0};
     // One-group-per-tile slots launch as many groups as the largest tile
     // count seen so far for the slot (a graph is re-captured when it grows).
@@ -255,8 +256,10 @@ extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene,
         if (r != VGI_OK) return r;
         VGI_TRY_HIP(hipEventRecord(b->done[a], st));
     } else {
+        const uint32_t shape[2] = {tuning_atrous_block(), tuning_trace_tile_width()};
         if (!b->exec[a] || b->key_storage[a] != storage || b->key_lds[a] != lds ||
-            memcmp(b->key_grid[a], b->grid_bound, sizeof(b->grid_bound)) != 0) {
+            memcmp(b->key_grid[a], b->grid_bound, sizeof(b->grid_bound)) != 0 ||
+            memcmp(b->key_shape[a], shape, sizeof(shape)) != 0) {
             if (b->exec[a]) {
                 VGI_TRY_HIP(hipGraphExecDestroy(b->exec[a]));
                 b->exec[a] = nullptr;
@@ -276,6 +279,7 @@ extern "C" vgi_status vgi_batch_add_passes(vgi_batch* b, const vgi_scene* scene,
             b->key_storage[a] = storage;
             b->key_lds[a] = lds;
             memcpy(b->key_grid[a], b->grid_bound, sizeof(b->grid_bound));
+            memcpy(b->key_shape[a], shape, sizeof(shape));
         }
         VGI_TRY_HIP(hipEventRecord(b->ev_in, st));
         VGI_TRY_HIP(hipStreamWaitEvent(b->cap, b->ev_in, 0));

@@ -572,20 +572,29 @@ __global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io)
 // Arithmetic is FP32 with hardware transcendentals (exp2, rsqrt, rcp): the
 // pass's contract is the FP64-oracle tolerance, not bit equality.
 // ---------------------------------------------------------------------------
-constexpr int kAtrousB = 16;                   // block edge in lattice points
-constexpr int kAtrousF = kAtrousB + 4;         // footprint edge (2 each side)
-constexpr int kAtrousN = kAtrousF * kAtrousF;  // footprint entries
-// LDS layout: each wave filters an 8x8 quarter of the block and footprint
-// rows are kAtrousP = 24 dwords apart, so the 8 rows of a wave's reads start
-// at 24 r mod 64 = 0, 24, 48, 8, 32, 56, 16, 40: the 64 lanes of one LDS
-// read hit 64 distinct banks (and any 32 consecutive lanes 32 distinct ones).
-constexpr int kAtrousP = 24;                   // footprint row pitch
-constexpr int kAtrousRow = kAtrousF * kAtrousP;  // dwords per field
-
+// Lattice block shapes: a workgroup of B x B threads filters a B x B block
+// of lattice points (one residue class of the step) from its (B + 4)^2
+// footprint staged in LDS.  B is a launch shape (vgi_tuning): 8, 16 (the
+// default) or 32.  Each wave filters an 8x8 part of the block and footprint
+// rows are P dwords apart, so the 8 rows of a wave's reads start at P r mod
+// 64: for P = 24, 0, 24, 48, 8, 32, 56, 16, 40; for P = 40, 0, 40, 16, 56,
+// 32, 8, 48, 24.  The 64 lanes of one LDS read hit 64 distinct banks (and any
+// 32 consecutive lanes 32 distinct ones).
+template <int B>
+struct AtrousShape {
+    static constexpr int F = B + 4;                // footprint edge (2 each side)
+    static constexpr int N = F * F;                // footprint entries
+    static constexpr int P = B == 32 ? 40 : 24;    // footprint row pitch
+    static constexpr int Row = F * P;              // dwords per field
+    static constexpr int T = B * B;                // threads per workgroup
+    static_assert(P >= F, "pitch holds a footprint row");
+};
 enum AtrousField { kFDepth, kFNx, kFNy, kFNz, kFLd, kFLs, kFDr, kFDg, kFDb, kFSr, kFSg, kFSb, kFVx, kFVy, kFCount };
 
-template <class S>
+template <class S, int B>
 __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, int step, uint32_t nbx, uint32_t nby, uint32_t bx_, uint32_t by_) {
+    typedef AtrousShape<B> Sh;
+    constexpr int kAtrousF = Sh::F, kAtrousN = Sh::N, kAtrousP = Sh::P, kAtrousRow = Sh::Row;
     __shared__ float lds[kFCount][kAtrousRow];
     const int w = (int)io.width, h = (int)io.height;
     const size_t plane = (size_t)w * h;
@@ -596,8 +605,8 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     const int t = (int)threadIdx.x;
 
 #pragma unroll 1
-    for (int e = t; e < kAtrousN; e += 256) {
-        int gx = bx * kAtrousB + e % kAtrousF - 2, gy = by * kAtrousB + e / kAtrousF - 2;
+    for (int e = t; e < kAtrousN; e += Sh::T) {
+        int gx = bx * B + e % kAtrousF - 2, gy = by * B + e / kAtrousF - 2;
         int cx = rx + step * gx, cy = ry + step * gy;
         float v[kFCount];
 #pragma unroll
@@ -622,8 +631,9 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     __syncthreads();
 
     const int lane = t & 63, wv = t >> 6;
-    const int li = (wv & 1) * 8 + (lane & 7), lj = (wv >> 1) * 8 + (lane >> 3);
-    const int px = rx + step * (bx * kAtrousB + li), py = ry + step * (by * kAtrousB + lj);
+    constexpr int kWavesPerRow = B / 8;
+    const int li = (wv % kWavesPerRow) * 8 + (lane & 7), lj = (wv / kWavesPerRow) * 8 + (lane >> 3);
+    const int px = rx + step * (bx * B + li), py = ry + step * (by * B + lj);
     if (px >= w || py >= h) return;
     const size_t i = (size_t)py * w + px;
     const int c = (lj + 2) * kAtrousP + (li + 2);
@@ -701,22 +711,19 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     S::st2(io.rw_variance_texture, io_px, mk2(vsum.x * (id * id), vsum.y * (is * is)));
 }
 
-// Register budget of the a-trous kernels: 6 waves per SIMD, the LDS limit
-// of 26.9 KB workgroups.
-template <class S>
+// Register budget of the a-trous kernels in waves per SIMD: the LDS limit of
+// the block shape (16: 26.9 KB workgroups of 4 waves, 6 per SIMD; 8: 16.1 KB
+// single-wave workgroups; 32: 80.6 KB workgroups of 16 waves, one per CU half).
+template <int B>
 struct AtrousWaves {
-    static constexpr int value = 6;
-};
-template <>
-struct AtrousWaves<StorageF16> {
-    static constexpr int value = 6;
+    static constexpr int value = B == 8 ? 8 : B == 16 ? 6 : 4;
 };
 
-template <class S>
-__global__ void __launch_bounds__(256, AtrousWaves<S>::value) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
+template <class S, int B>
+__global__ void __launch_bounds__(B * B, AtrousWaves<B>::value) k_atrous(vgi_denoise_atrous_io io, int step, uint32_t nbx, uint32_t nby) {
     const uint32_t gx = gridDim.x;
     const uint32_t l = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
-    k_atrous_body<S>(io, step, nbx, nby, l % gx, l / gx);
+    k_atrous_body<S, B>(io, step, nbx, nby, l % gx, l / gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -777,24 +784,35 @@ vgi_status launch_denoise_fast_clamp(const vgi_denoise_fast_clamp_io& io, hipStr
 struct AtrousGrid {
     uint32_t s, nbx, nby;
 };
-static AtrousGrid atrous_grid(uint32_t w, uint32_t h, uint32_t step) {
+static AtrousGrid atrous_grid(uint32_t w, uint32_t h, uint32_t step, uint32_t block) {
     AtrousGrid g;
     g.s = step < std::max(w, h) ? step : std::max(w, h);
     const uint32_t lw = (w + g.s - 1) / g.s, lh = (h + g.s - 1) / g.s;  // lattice extent of residue 0
-    g.nbx = (lw + kAtrousB - 1) / kAtrousB;
-    g.nby = (lh + kAtrousB - 1) / kAtrousB;
+    g.nbx = (lw + block - 1) / block;
+    g.nby = (lh + block - 1) / block;
     return g;
+}
+
+template <int B>
+static void launch_atrous_shape(const vgi_denoise_atrous_io& io, const AtrousGrid& ag, hipStream_t st) {
+    const dim3 g(ag.nbx * ag.s, ag.nby * ag.s, 1);
+    if (io.storage == VGI_STORAGE_F32)
+        hipLaunchKernelGGL((k_atrous<StorageF32, B>), g, dim3(B * B), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
+    else
+        hipLaunchKernelGGL((k_atrous<StorageF16, B>), g, dim3(B * B), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
 }
 
 vgi_status launch_denoise_atrous(const vgi_denoise_atrous_io& io, int step, hipStream_t st) {
     // reads depth 4, L1+L2 8, diff, spec, variance; writes diff, spec, variance
     ProfScope prof("denoise_atrous", st, px_bytes(io.storage, 4, 2, 0, 12) * io.width * io.height);
-    const AtrousGrid ag = atrous_grid(io.width, io.height, (uint32_t)step);
-    dim3 g(ag.nbx * ag.s, ag.nby * ag.s, 1);
-    if (io.storage == VGI_STORAGE_F32)
-        hipLaunchKernelGGL(k_atrous<StorageF32>, g, dim3(256), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
+    const uint32_t block = tuning_atrous_block();
+    const AtrousGrid ag = atrous_grid(io.width, io.height, (uint32_t)step, block);
+    if (block == 8)
+        launch_atrous_shape<8>(io, ag, st);
+    else if (block == 32)
+        launch_atrous_shape<32>(io, ag, st);
     else
-        hipLaunchKernelGGL(k_atrous<StorageF16>, g, dim3(256), 0, st, io, (int)ag.s, ag.nbx, ag.nby);
+        launch_atrous_shape<16>(io, ag, st);
     return check_launch("k_atrous");
 }
 
@@ -833,18 +851,31 @@ struct JobDenFastClamp {
         k_fast_clamp_body<S>(j.io, bx, by);
     }
 };
-template <class S>
-struct JobDenAtrous {
+struct AtrousArgs {
     vgi_denoise_atrous_io io;
     int step;
     uint32_t nbx, nby;
-    __device__ static void run(const JobDenAtrous& j, uint32_t bx, uint32_t by) {
-        k_atrous_body<S>(j.io, j.step, j.nbx, j.nby, bx, by);
-    }
+};
+template <int B>
+struct AtrousJobs {
+    template <class S>
+    struct J : AtrousArgs {
+        __device__ static void run(const J& j, uint32_t bx, uint32_t by) {
+            k_atrous_body<S, B>(j.io, j.step, j.nbx, j.nby, bx, by);
+        }
+    };
 };
 template <class S>
-struct BatchMinWaves<JobDenAtrous<S>> {
-    static constexpr int value = AtrousWaves<S>::value;
+struct BatchMinWaves<AtrousJobs<8>::J<S>> {
+    static constexpr int value = AtrousWaves<8>::value;
+};
+template <class S>
+struct BatchMinWaves<AtrousJobs<16>::J<S>> {
+    static constexpr int value = AtrousWaves<16>::value;
+};
+template <class S>
+struct BatchMinWaves<AtrousJobs<32>::J<S>> {
+    static constexpr int value = AtrousWaves<32>::value;
 };
 
 static vgi_status fill_den_temporal(const PassRec* const* recs, int n, void* host) {
@@ -886,16 +917,29 @@ static vgi_status fill_den_fast_clamp(const PassRec* const* recs, int n, void* h
     return VGI_OK;
 }
 static vgi_status fill_den_atrous(const PassRec* const* recs, int n, void* host) {
-    auto* t = batch_table<JobDenAtrous>(host);
+    auto* t = batch_table<AtrousJobs<16>::J>(host);
+    const uint32_t block = tuning_atrous_block();
     for (int v = 0; v < n; ++v) {
         const vgi_denoise_atrous_io& io = recs[v]->io.den_atrous;
         const int step = recs[v]->pc.den_atrous.step_size;
         VGI_REQUIRE(step >= 1, "batched a-trous: bad step %d", step);
-        const AtrousGrid g = atrous_grid(io.width, io.height, (uint32_t)step);
-        t->jobs[v] = {io, (int)g.s, g.nbx, g.nby};
+        const AtrousGrid g = atrous_grid(io.width, io.height, (uint32_t)step, block);
+        AtrousArgs& a = t->jobs[v];
+        a.io = io;
+        a.step = (int)g.s;
+        a.nbx = g.nbx;
+        a.nby = g.nby;
         batch_add_tiles(t->hd, v, g.nbx * g.s, g.nby * g.s);
     }
     return VGI_OK;
+}
+// The table was filled for the current block shape; the launch uses it too.
+static vgi_status launch_den_atrous(const void* dev, uint32_t storage, uint32_t grid, size_t lds, hipStream_t st) {
+    switch (tuning_atrous_block()) {
+        case 8: return batch_launch<AtrousJobs<8>::J, 64, true, true>(dev, storage, grid, lds, st);
+        case 32: return batch_launch<AtrousJobs<32>::J, 1024, true, true>(dev, storage, grid, lds, st);
+        default: return batch_launch<AtrousJobs<16>::J, 256, true, true>(dev, storage, grid, lds, st);
+    }
 }
 
 const BatchOps* batch_ops_denoise(int kind) {
@@ -907,8 +951,10 @@ const BatchOps* batch_ops_denoise(int kind) {
                                    batch_launch<JobDenFallback, 256, true>, 256, false};
     static const BatchOps fast_clamp{sizeof(BatchTable<JobDenFastClamp<StorageF32>>), fill_den_fast_clamp,
                                      batch_launch<JobDenFastClamp>, 64, false};
-    static const BatchOps atrous{sizeof(BatchTable<JobDenAtrous<StorageF32>>), fill_den_atrous,
-                                 batch_launch<JobDenAtrous, 256, true, true>, 256, false};
+    // (threads 256 marks the a-trous class of persistent group counts; the
+    // launch's workgroup size follows the block shape)
+    static const BatchOps atrous{sizeof(BatchTable<AtrousJobs<16>::J<StorageF32>>), fill_den_atrous,
+                                 launch_den_atrous, 256, false};
     switch (kind) {
         case kPassDenTemporal: return &temporal;
         case kPassDenDisocclusion: return &disocclusion;

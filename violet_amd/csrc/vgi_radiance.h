@@ -34,6 +34,7 @@ struct TraceRes {
     HashQuery* queries;
     uint32_t* query_counter;
     const uint32_t* query_threshold;  // null: append every query
+    uint32_t tile_w;                  // sample passes: a wave's pixel tile is tile_w x (64 / tile_w)
 };
 
 // Hash of a query's 24 bytes (a jenkins chain over its words): the thinning

@@ -279,9 +279,9 @@ vgi_status vgi_renderer_plan(vgi_renderer* r, const vgi_scene* scene, const vgi_
         CALL(sink.submit(kSlotClearHgDecay, make_clear(r->hg_decay.p, r->hg_decay.bytes)));
         CALL(sink.submit(kSlotClearHgCounter, make_clear(r->hg_counter[r->q].p, 4)));
         // the ordered update's query threshold before the first count: at
-        // most one diffuse and one specular query per pixel, plus the cache
-        // rays' own
-        const uint64_t expected = 2ull * w * h + VGI_HASH_GRID_MAX_QUERIES;
+        // most one diffuse and one specular query per pixel (the first frame
+        // runs no cache rays)
+        const uint64_t expected = 2ull * w * h;
         CALL(sink.submit(kSlotFillHgThreshold,
                          make_fill(r->hg_threshold.p, 4, vgi_hash_grid_query_threshold_for(expected))));
         r->hash_grid_ready = true;

@@ -1,4 +1,5 @@
 // vgi_runtime.cpp — error reporting for the C ABI (thread-local last error).
+#include <atomic>
 #include "vgi_runtime.h"
 
 #include <string.h>
@@ -165,6 +166,29 @@ extern "C" vgi_status vgi_trace_stats_enable(uint32_t enable) {
 extern "C" vgi_status vgi_profiler_enable(uint32_t enable) {
     std::lock_guard<std::mutex> lk(vgi::g_prof_mu);
     vgi::g_prof_on = enable != 0;
+    return VGI_OK;
+}
+
+namespace vgi {
+static std::atomic<uint32_t> g_atrous_block{16}, g_trace_tile{8};
+uint32_t tuning_atrous_block() { return g_atrous_block.load(std::memory_order_relaxed); }
+uint32_t tuning_trace_tile_width() { return g_trace_tile.load(std::memory_order_relaxed); }
+}  // namespace vgi
+
+extern "C" vgi_status vgi_tuning_set(const vgi_tuning* t) {
+    const uint32_t ab = t && t->atrous_block ? t->atrous_block : 16u;
+    const uint32_t tw = t && t->trace_tile_width ? t->trace_tile_width : 8u;
+    VGI_REQUIRE(ab == 8 || ab == 16 || ab == 32, "vgi_tuning_set: atrous_block %u is not 8, 16 or 32", ab);
+    VGI_REQUIRE(tw == 8 || tw == 16 || tw == 32, "vgi_tuning_set: trace_tile_width %u is not 8, 16 or 32", tw);
+    vgi::g_atrous_block.store(ab, std::memory_order_relaxed);
+    vgi::g_trace_tile.store(tw, std::memory_order_relaxed);
+    return VGI_OK;
+}
+
+extern "C" vgi_status vgi_tuning_get(vgi_tuning* t) {
+    VGI_REQUIRE(t, "vgi_tuning_get: null argument");
+    t->atrous_block = vgi::tuning_atrous_block();
+    t->trace_tile_width = vgi::tuning_trace_tile_width();
     return VGI_OK;
 }
 

@@ -67,6 +67,15 @@ inline bool valid_dims(uint32_t w, uint32_t h) { return w > 0 && h > 0 && w <= k
 
 inline dim3 tile_grid(uint32_t w, uint32_t h) { return dim3((w + 7) / 8, (h + 7) / 8, 1); }
 
+// Current launch shapes (vgi_tuning_set).
+uint32_t tuning_atrous_block();
+uint32_t tuning_trace_tile_width();
+// Tile grid of the sample passes for a tile `tw` pixels wide and 64 / tw high.
+inline dim3 trace_tile_grid(uint32_t w, uint32_t h, uint32_t tw) {
+    const uint32_t th = 64u / tw;
+    return dim3((w + tw - 1) / tw, (h + th - 1) / th, 1);
+}
+
 }  // namespace vgi
 
 #if defined(__HIPCC__)

@@ -64,13 +64,21 @@ struct DiffGenPass {
     }
 };
 
-// One lane per pixel of tile (bx_, by_).
+// One lane per pixel of tile (bx_, by_): 8x8 pixels in 2x2 quads, or rows of
+// rt.tile_w pixels (16 x 4, 32 x 2); the shape is a launch parameter
+// (vgi_tuning) and does not change any pixel's result.
 template <class P, class S, bool kCount>
 __device__ __forceinline__ void sample_pixel(const P& p, const TraceRes& rt, const FrameParams& fp, bool has_prev,
                                              uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
+    if (rt.tile_w == 8u) {
+        tile_pixel_at(bx_, by_, px, py);
+    } else {
+        const uint32_t lane = threadIdx.x & 63u, sh = rt.tile_w == 16u ? 4u : 5u;
+        px = (bx_ << sh) + (lane & (rt.tile_w - 1u));
+        py = by_ * (64u >> sh) + (lane >> sh);
+    }
     f3 o, d;
     if (!p.gen_px(px, py, o, d)) return;
     RadianceTraceResult r = trace_radiance<S, kCount>(rt, fp, o, d, has_prev, stack);
@@ -367,6 +375,7 @@ static vgi_status make_trace_res(const vgi_trace_resources* r, bool need_prev, T
     t.queries = reinterpret_cast<HashQuery*>(r->rw_hash_grid_query_buffer);
     t.query_counter = r->rw_hash_grid_query_counter_buffer;
     t.query_threshold = r->hash_grid_query_threshold;
+    t.tile_w = tuning_trace_tile_width();
     return VGI_OK;
 }
 
@@ -483,7 +492,10 @@ static vgi_status fill_diff_gen(const PassRec* const* recs, int n, void* host) {
         j.fp = to_fp(&r.fp);
         j.io = r.io.diff_gen;
         j.pc = r.pc.sample_gen;
-        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+        {
+            const dim3 tg = trace_tile_grid(j.io.width, j.io.height, j.rt.tile_w);
+            batch_add_tiles(t->hd, v, tg.x, tg.y);
+        }
     }
     return VGI_OK;
 }
@@ -496,7 +508,10 @@ static vgi_status fill_diff_validate(const PassRec* const* recs, int n, void* ho
         if (s != VGI_OK) return s;
         j.fp = to_fp(&r.fp);
         j.io = r.io.diff_validate;
-        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+        {
+            const dim3 tg = trace_tile_grid(j.io.width, j.io.height, j.rt.tile_w);
+            batch_add_tiles(t->hd, v, tg.x, tg.y);
+        }
     }
     return VGI_OK;
 }
@@ -510,7 +525,10 @@ static vgi_status fill_spec_gen(const PassRec* const* recs, int n, void* host) {
         j.fp = to_fp(&r.fp);
         j.io = r.io.spec_gen;
         j.pc = r.pc.sample_gen;
-        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+        {
+            const dim3 tg = trace_tile_grid(j.io.width, j.io.height, j.rt.tile_w);
+            batch_add_tiles(t->hd, v, tg.x, tg.y);
+        }
     }
     return VGI_OK;
 }
@@ -523,7 +541,10 @@ static vgi_status fill_spec_validate(const PassRec* const* recs, int n, void* ho
         if (s != VGI_OK) return s;
         j.fp = to_fp(&r.fp);
         j.io = r.io.spec_validate;
-        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+        {
+            const dim3 tg = trace_tile_grid(j.io.width, j.io.height, j.rt.tile_w);
+            batch_add_tiles(t->hd, v, tg.x, tg.y);
+        }
     }
     return VGI_OK;
 }
@@ -606,7 +627,7 @@ extern "C" vgi_status vgi_pass_ind_diff_sample_gen(const vgi_frame_params* fp, c
     if (s != VGI_OK) return s;
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_diff_sample_gen", st, (double)io->width * io->height * (8 + 4 + 8 + radiance_bytes(io->storage)));
-    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_gen, trace_tile_grid(io->width, io->height, t.tile_w), to_fp(fp), t, *io, *pc);
     trace_stats_collect("ind_diff_sample_gen", st);
     return check_launch("k_diff_sample_gen");
 }
@@ -625,7 +646,7 @@ extern "C" vgi_status vgi_pass_ind_diff_sample_validate(const vgi_frame_params* 
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_diff_sample_validate", st,
                    (double)io->width * io->height * (4 + 2 * (4 + 8 + radiance_bytes(io->storage))));
-    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    VGI_DISPATCH_STORAGE(io->storage, k_diff_sample_validate, trace_tile_grid(io->width, io->height, t.tile_w), to_fp(fp), t, *io);
     trace_stats_collect("ind_diff_sample_validate", st);
     return check_launch("k_diff_sample_validate");
 }
@@ -643,7 +664,7 @@ extern "C" vgi_status vgi_pass_ind_spec_sample_gen(const vgi_frame_params* fp, c
     if (s != VGI_OK) return s;
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_spec_sample_gen", st, (double)io->width * io->height * (4 + 4 + 16 + radiance_bytes(io->storage)));
-    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_gen, tile_grid(io->width, io->height), to_fp(fp), t, *io, *pc);
+    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_gen, trace_tile_grid(io->width, io->height, t.tile_w), to_fp(fp), t, *io, *pc);
     trace_stats_collect("ind_spec_sample_gen", st);
     return check_launch("k_spec_sample_gen");
 }
@@ -661,7 +682,7 @@ extern "C" vgi_status vgi_pass_ind_spec_sample_validate(const vgi_frame_params* 
     hipStream_t st = (hipStream_t)stream;
     ProfScope prof("ind_spec_sample_validate", st,
                    (double)io->width * io->height * (4 + 2 * (8 + 16 + radiance_bytes(io->storage))));
-    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_validate, tile_grid(io->width, io->height), to_fp(fp), t, *io);
+    VGI_DISPATCH_STORAGE(io->storage, k_spec_sample_validate, trace_tile_grid(io->width, io->height, t.tile_w), to_fp(fp), t, *io);
     trace_stats_collect("ind_spec_sample_validate", st);
     return check_launch("k_spec_sample_validate");
 }
