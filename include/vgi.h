@@ -375,13 +375,41 @@ vgi_status vgi_denoise_frame_static(const vgi_frame_params* fp, const vgi_denois
  * ---------------------------------------------------------------------- */
 typedef struct vgi_scene vgi_scene;
 
-/* MaterialParams factors (shader/scene_bindings.hlsl:24-51); textures are
- * not sampled (untextured scenes: every texture fetch returns 1). */
+/* Material textures (crates/violet/src/model.rs:800-945 with
+ * --disable-tex-compress, render_scene.rs:344-346,768-833,920-935): RGBA8
+ * texels with a full mip chain.  Level l is max(1, width >> l) x
+ * max(1, height >> l) texels, rows packed, levels back to back (violet's
+ * importer makes 1 + log2(next_pow2(max(width, height))) levels by halving).
+ * srgb = 1 marks R8G8B8A8_SRGB (base colour: texels are decoded to linear
+ * before filtering); 0 is UNORM.  A metallic-roughness texture carries
+ * metallic in R and roughness in G (violet's importer swizzles glTF's B/G,
+ * model.rs:811-814).  Sampling is violet's sampler_linear_wrap
+ * (render_loop.rs:199-212): bilinear, linear between mips, REPEAT. */
+typedef struct vgi_texture {
+    uint32_t width, height;
+    uint32_t levels; /* 1..16 */
+    uint32_t srgb;
+    const uint8_t* texels; /* host memory */
+} vgi_texture;
+
+/* Texture indices of a material, as violet's bindless table numbers them:
+ * 0 = a white 1x1 texture, 1 = a flat normal map (128, 128, 255, 255), and
+ * k >= 2 = vgi_scene_desc.textures[k - 2]. */
+#define VGI_TEXTURE_WHITE 0u
+#define VGI_TEXTURE_NORMAL_UP 1u
+#define VGI_TEXTURE_FIRST 2u
+
+/* MaterialParams (shader/scene_bindings.hlsl:24-51): the factors multiply
+ * the sampled textures (geometry_rchit.inc.hlsl:79-86).  The texture words
+ * pack two 16-bit indices as violet's do: base colour | metallic-roughness
+ * << 16, and normal map in the low 16 bits (the high half, violet's
+ * transmission map, is unused).  An all-zero word means white textures. */
 typedef struct vgi_material {
     float base_color[4];
     float metallic;
     float roughness; /* perceptual roughness */
-    float pad[2];
+    uint32_t color_metalrough_texture;
+    uint32_t normal_texture;
 } vgi_material;
 
 typedef struct vgi_scene_desc {
@@ -393,6 +421,15 @@ typedef struct vgi_scene_desc {
     const uint32_t* triangle_material; /* per triangle or NULL (= material 0) */
     const vgi_material* materials;
     uint32_t num_materials;
+    /* Texturing (all optional).  texcoords: TEXCOORD_0, 2 floats per vertex
+     * (NULL = (0, 0)).  tangents: 4 floats per vertex, xyz in world space
+     * and w the bitangent sign (MikkTSpace convention); NULL disables the
+     * tangent-space terms of normal mapping (violet's zero tangents).
+     * textures: indexed from VGI_TEXTURE_FIRST by the materials. */
+    const float* texcoords;
+    const float* tangents;
+    const vgi_texture* textures;
+    uint32_t num_textures;
 } vgi_scene_desc;
 
 typedef struct vgi_scene_stats {

@@ -227,6 +227,9 @@ class _TraceRes(C.Structure):
 def _restir_sigs(lib):
     lib.oracle_scene_create.restype = P
     lib.oracle_scene_create.argtypes = [P, P, P, U32, P, P]
+    lib.oracle_scene_set_textures.argtypes = [P, P, P, P, P, U32]
+    R = C.c_float if lib._bits == 32 else C.c_double
+    lib.oracle_tex_sample.argtypes = [P, U32, R, R, R, P, P]
     lib.oracle_scene_destroy.argtypes = [P]
     TR = C.POINTER(_TraceRes)
     lib.oracle_trace_radiance.argtypes = [P, TR, P, U32, I, P]
@@ -281,9 +284,45 @@ class OracleScene:
         mats[:, 3] = 1.0
         mats[:, 4] = mesh.metallic
         mats[:, 5] = mesh.roughness
+        # texture indices (base colour, metallic-roughness, normal map) packed
+        # as MaterialParams packs them (scene_bindings.hlsl:24-51); absent:
+        # white, white, flat normal
+        tex = getattr(mesh, "material_textures", None)
+        tex = np.tile(np.array([0, 0, 1], np.uint32), (len(mats), 1)) if tex is None else np.asarray(tex, np.uint32)
+        mats.view(np.uint32)[:, 6] = tex[:, 0] | (tex[:, 1] << 16)
+        mats.view(np.uint32)[:, 7] = tex[:, 2]
         self.mats = mats
         self.h = self.lib.oracle_scene_create(_np_ptr(self.P), _np_ptr(self.N), _np_ptr(self.I), len(self.I),
                                               _np_ptr(self.TM), _np_ptr(self.mats))
+        # texturing: texel words of every level back to back, 20 words of
+        # (width, height, levels, srgb, level offsets) per texture
+        self.UV = None if getattr(mesh, 'texcoords', None) is None else np.ascontiguousarray(mesh.texcoords, np.float32)
+        self.TG = None if getattr(mesh, 'tangents', None) is None else np.ascontiguousarray(mesh.tangents, np.float32)
+        meta = np.zeros((max(1, len(mesh.textures)), 20), np.uint32)
+        words, off = [], 0
+        textures = getattr(mesh, "textures", [])
+        for k, t in enumerate(textures):
+            meta[k, :4] = (t.width, t.height, len(t.levels), int(bool(t.srgb)))
+            for l, lv in enumerate(t.levels):
+                meta[k, 4 + l] = off
+                w = np.ascontiguousarray(lv, np.uint8).reshape(-1).view(np.uint32)
+                words.append(w)
+                off += len(w)
+        self.texels = np.concatenate(words) if words else np.zeros(1, np.uint32)
+        self.meta = meta
+        if self.UV is not None or self.TG is not None or textures:
+            self.lib.oracle_scene_set_textures(self.h, _np_ptr(self.UV) if self.UV is not None else None,
+                                               _np_ptr(self.TG) if self.TG is not None else None,
+                                               _np_ptr(self.texels), _np_ptr(self.meta), len(textures))
+
+    def tex_sample(self, t: int, u: float, v: float, lod: float = 0.0, grad=None) -> np.ndarray:
+        """Texture t (0-based) sampled at (u, v): explicit LOD, or implicit
+        from grad = (ds/dx, dt/dx, ds/dy, dt/dy)."""
+        dt = np.float32 if self.lib._bits == 32 else np.float64
+        out = np.zeros(4, dt)
+        g = None if grad is None else np.ascontiguousarray(grad, dt)
+        self.lib.oracle_tex_sample(self.h, t, u, v, lod, _np_ptr(g), _np_ptr(out))
+        return out
 
     def __del__(self):
         try:

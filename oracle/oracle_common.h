@@ -10,8 +10,8 @@
  *                    -ffp-contract=off, so that integer decisions taken from
  *                    float values (reservoir selections, neighbour choices,
  *                    hash keys) can be compared bit for bit with the GPU.
- * Only the single-precision sine/cosine (vgi_mathf.h, a Cephes-method
- * routine the GPU also uses) is shared with the product; it is pinned
+ * Only the single-precision sine/cosine and log2 (vgi_mathf.h, Cephes-method
+ * routines the GPU also uses) are shared with the product; they are pinned
  * against libm double in tests/test_oracle_kat.py.
  */
 #ifndef ORACLE_COMMON_H
@@ -39,6 +39,8 @@ typedef ORACLE_REAL real;
 #define R_FMIN fminf
 #define R_FMAX fmaxf
 #define R_RINT rintf
+#define R_CEIL ceilf
+#define R_LOG2 vgi_log2f
 static inline void r_sincos(real x, real* s, real* c) { vgi_sincosf(x, s, c); }
 #else
 #define R_SQRT sqrt
@@ -49,6 +51,8 @@ static inline void r_sincos(real x, real* s, real* c) { vgi_sincosf(x, s, c); }
 #define R_FMIN fmin
 #define R_FMAX fmax
 #define R_RINT rint
+#define R_CEIL ceil
+#define R_LOG2 log2
 static inline void r_sincos(real x, real* s, real* c) {
     *s = sin(x);
     *c = cos(x);

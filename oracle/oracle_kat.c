@@ -29,6 +29,7 @@ void oracle_gb_normal(uint32_t l1, real* n) { v3 v = o_gb_normal(l1); n[0] = v.x
 uint32_t oracle_rsv_enc(uint32_t M, float W) { return o_rsv_enc(M, W); }
 void oracle_rsv_dec(uint32_t u, uint32_t* M, float* W) { o_rsv_dec(u, M, W); }
 void oracle_sincos(real x, real* s, real* c) { r_sincos(x, s, c); }
+real oracle_log2(real x) { return R_LOG2(x); }
 int oracle_real_bytes(void) { return (int)sizeof(real); }
 #ifdef _OPENMP
 #include <omp.h>

@@ -27,14 +27,26 @@
  */
 #include "oracle_trace.inc"
 
+static inline v3 vtx(const float* A, uint32_t i) { return V3(A[3 * i], A[3 * i + 1], A[3 * i + 2]); }
+
+typedef struct {
+    uint32_t w, h, levels, srgb;
+    const uint32_t* level[16]; /* RGBA8 words, R in the low byte */
+} otex;
+
 typedef struct {
     obvh* bvh;
     const float* P;
     const float* N; /* may be NULL */
     const uint32_t* I;
     const uint32_t* TM;
-    const float* mats; /* 8 floats per material */
+    const float* mats; /* 8 words per material: base colour rgba, metallic, roughness, texture words */
     uint32_t nt;
+    const float* UV; /* 2 per vertex or NULL */
+    const float* TG; /* 4 per vertex or NULL */
+    otex* tex;
+    uint32_t ntex;
+    real lut[512]; /* byte -> UNORM [0,256), sRGB-decoded [256,512) */
 } oscene;
 
 oscene* oracle_scene_create(const float* P, const float* N, const uint32_t* I, uint32_t nt, const uint32_t* TM,
@@ -47,12 +59,208 @@ oscene* oracle_scene_create(const float* P, const float* N, const uint32_t* I, u
     s->TM = TM;
     s->mats = mats;
     s->nt = nt;
+    for (int i = 0; i < 256; ++i) {
+        double c = i / 255.0;
+        s->lut[i] = (real)c;
+        s->lut[256 + i] = (real)(c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4));
+    }
     return s;
+}
+/* Texturing inputs: texcoords (2 per vertex) and tangents (4 per vertex),
+ * either NULL, and ntex textures whose texels are `texels` at the word
+ * offsets of `meta` (per texture: width, height, levels, srgb, then one
+ * offset per level; 20 words). */
+void oracle_scene_set_textures(oscene* s, const float* UV, const float* TG, const uint32_t* texels,
+                               const uint32_t* meta, uint32_t ntex) {
+    s->UV = UV;
+    s->TG = TG;
+    free(s->tex);
+    s->tex = ntex ? (otex*)calloc(ntex, sizeof(otex)) : NULL;
+    s->ntex = ntex;
+    for (uint32_t t = 0; t < ntex; ++t) {
+        const uint32_t* m = meta + 20 * t;
+        s->tex[t].w = m[0], s->tex[t].h = m[1], s->tex[t].levels = m[2], s->tex[t].srgb = m[3];
+        for (uint32_t l = 0; l < m[2] && l < 16; ++l) s->tex[t].level[l] = texels + m[4 + l];
+    }
 }
 void oracle_scene_destroy(oscene* s) {
     if (!s) return;
     obvh_destroy(s->bvh);
+    free(s->tex);
     free(s);
+}
+
+/* ---- material textures: violet's sampler_linear_wrap (render_loop.rs:199-212)
+ * through the Vulkan texel filtering equations (bilinear within a level,
+ * linear between levels, REPEAT, anisotropy up to 16 in the specification's
+ * reference form), sRGB decoded per texel before filtering ---- */
+typedef struct { real r, g, b, a; } o4;
+static inline o4 o4lerp(o4 x, o4 y, real t) {
+    o4 o = {x.r + (y.r - x.r) * t, x.g + (y.g - x.g) * t, x.b + (y.b - x.b) * t, x.a + (y.a - x.a) * t};
+    return o;
+}
+static inline uint32_t o_wrap(real f, uint32_t n) {
+    int i = R_FABS(f) < RC(16777216.0) ? (int)f : 0;
+    int r = i % (int)n;
+    return (uint32_t)(r < 0 ? r + (int)n : r);
+}
+static inline o4 o_texel(const oscene* s, const otex* t, uint32_t l, uint32_t x, uint32_t y) {
+    uint32_t wl = t->w >> l ? t->w >> l : 1u;
+    uint32_t w = t->level[l][(size_t)y * wl + x];
+    const real* lut = s->lut + (t->srgb ? 256 : 0);
+    o4 o = {lut[w & 255u], lut[(w >> 8) & 255u], lut[(w >> 16) & 255u], s->lut[w >> 24]};
+    return o;
+}
+static o4 o_tex_bilinear(const oscene* s, const otex* t, uint32_t l, real u, real v) {
+    uint32_t wl = t->w >> l ? t->w >> l : 1u, hl = t->h >> l ? t->h >> l : 1u;
+    real fu = u * (real)wl - RC(0.5), fv = v * (real)hl - RC(0.5);
+    real x0f = R_FLOOR(fu), y0f = R_FLOOR(fv);
+    real a = fu - x0f, b = fv - y0f;
+    uint32_t x0 = o_wrap(x0f, wl), y0 = o_wrap(y0f, hl);
+    uint32_t x1 = x0 + 1 == wl ? 0 : x0 + 1, y1 = y0 + 1 == hl ? 0 : y0 + 1;
+    return o4lerp(o4lerp(o_texel(s, t, l, x0, y0), o_texel(s, t, l, x1, y0), a),
+                  o4lerp(o_texel(s, t, l, x0, y1), o_texel(s, t, l, x1, y1), a), b);
+}
+static o4 o_tex_trilinear(const oscene* s, const otex* t, real lod, real u, real v) {
+    uint32_t last = t->levels - 1;
+    real d = lod > RC(0.0) ? lod : RC(0.0);
+    if (d >= (real)last) return o_tex_bilinear(s, t, last, u, v);
+    real dl = R_FLOOR(d), fr = d - dl;
+    o4 lo = o_tex_bilinear(s, t, (uint32_t)dl, u, v);
+    if (fr == RC(0.0)) return lo;
+    return o4lerp(lo, o_tex_bilinear(s, t, (uint32_t)dl + 1, u, v), fr);
+}
+static o4 o_tex_grad(const oscene* s, const otex* t, real u, real v, real dsdx, real dtdx, real dsdy, real dtdy) {
+    real W = (real)t->w, H = (real)t->h;
+    real ax = dsdx * W, bx = dtdx * H, ay = dsdy * W, by = dtdy * H;
+    real rx = R_SQRT(ax * ax + bx * bx), ry = R_SQRT(ay * ay + by * by);
+    real rmax = R_FMAX(rx, ry), rmin = R_FMIN(rx, ry), lod = RC(0.0);
+    uint32_t n = 1;
+    if (!(rmax > RC(0.0))) {
+        lod = RC(0.0);
+    } else if (!(rmax <= RC(3.0e38))) {
+        lod = RC(1.0e30);
+    } else {
+        real eta = rmin > RC(0.0) ? R_FMIN(rmax / rmin, RC(16.0)) : RC(16.0);
+        n = (uint32_t)R_CEIL(eta);
+        lod = R_LOG2(rmax / eta);
+    }
+    if (n <= 1) return o_tex_trilinear(s, t, lod, u, v);
+    int ax_major = rx > ry;
+    real du = ax_major ? dsdx : dsdy, dv = ax_major ? dtdx : dtdy;
+    o4 sum = {0, 0, 0, 0};
+    for (uint32_t i = 1; i <= n; ++i) {
+        real o = (real)i / (real)(n + 1) - RC(0.5);
+        o4 x = o_tex_trilinear(s, t, lod, u + du * o, v + dv * o);
+        sum.r = sum.r + x.r, sum.g = sum.g + x.g, sum.b = sum.b + x.b, sum.a = sum.a + x.a;
+    }
+    o4 r = {sum.r / (real)n, sum.g / (real)n, sum.b / (real)n, sum.a / (real)n};
+    return r;
+}
+
+/* Test access to the sampler: texture `t` (0-based in the scene's list) at
+ * (u, v), either at an explicit level of detail (grad == NULL) or with the
+ * four derivatives grad = {ds/dx, dt/dx, ds/dy, dt/dy}; out = rgba. */
+void oracle_tex_sample(const oscene* s, uint32_t t, real u, real v, real lod, const real* grad, real* out) {
+    o4 r = grad ? o_tex_grad(s, &s->tex[t], u, v, grad[0], grad[1], grad[2], grad[3])
+                : o_tex_trilinear(s, &s->tex[t], lod, u, v);
+    out[0] = r.r, out[1] = r.g, out[2] = r.b, out[3] = r.a;
+}
+
+/* A primary ray's quad partners (mesh_gbuffer.hlsl's implicit derivatives). */
+typedef struct {
+    v3 o, dir_x, dir_y;
+    int x_odd, y_odd;
+} ofoot;
+
+typedef struct {
+    int missed;
+    v3 pos, n, ng, base;
+    real metal, rough;
+} osurf;
+
+static inline v2 o_uv(const float* UV, uint32_t i) { return V2(UV[2 * i], UV[2 * i + 1]); }
+static inline v2 o_bary2(v2 a, v2 b, v2 c, real w0, real u, real v) {
+    return V2((a.x * w0 + b.x * u) + c.x * v, (a.y * w0 + b.y * u) + c.y * v);
+}
+
+/* geometry_rchit.inc.hlsl:54-150 (indirect rays: textures at LOD 2,
+ * geometry_ind.rchit.hlsl:3) or mesh_gbuffer.hlsl:62-97 (foot != NULL:
+ * implicit derivatives) on a pre-transformed scene.  Texture indices 0 and
+ * 1 are violet's white and flat-normal textures (render_scene.rs:344-346). */
+static osurf o_surface(const oscene* s, ohit h, v3 d, const ofoot* foot) {
+    osurf r;
+    r.missed = h.id == 0xFFFFFFFFu;
+    if (r.missed) return r;
+    uint32_t i0 = s->I[3 * h.id], i1 = s->I[3 * h.id + 1], i2 = s->I[3 * h.id + 2];
+    v3 p0 = vtx(s->P, i0), p1 = vtx(s->P, i1), p2 = vtx(s->P, i2);
+    real w0 = (RC(1.0) - h.u) - h.v;
+    r.pos = add3(add3(scale3(p0, w0), scale3(p1, h.u)), scale3(p2, h.v));
+    v3 ng = norm3(cross3(sub3(p1, p0), sub3(p2, p0)));
+    v3 nl = ng;
+    if (s->N) nl = add3(add3(scale3(vtx(s->N, i0), w0), scale3(vtx(s->N, i1), h.u)), scale3(vtx(s->N, i2), h.v));
+    const float* m = s->mats + 8 * (s->TM ? s->TM[h.id] : 0);
+    uint32_t cm = f_bits(m[6]), nw = f_bits(m[7]);
+    uint32_t bci = cm & 0xFFFFu, mri = cm >> 16, nmi = nw & 0xFFFFu;
+    const real up = (real)(128.0 / 255.0);
+    o4 white = {1, 1, 1, 1}, flat = {up, up, 1, 1};
+    o4 bc = bci == 1 ? flat : white, mr = mri == 1 ? flat : white, nm = nmi == 1 ? flat : white;
+    if (bci >= 2 || mri >= 2 || nmi >= 2) {
+        v2 t0 = V2(0, 0), t1 = V2(0, 0), t2 = V2(0, 0);
+        if (s->UV) t0 = o_uv(s->UV, i0), t1 = o_uv(s->UV, i1), t2 = o_uv(s->UV, i2);
+        v2 uv = o_bary2(t0, t1, t2, w0, h.u, h.v);
+        real g[4] = {0, 0, 0, 0};
+        if (foot) {
+            v3 e1 = sub3(p1, p0), e2 = sub3(p2, p0), tv = sub3(foot->o, p0), qv = cross3(tv, e1);
+            v2 q[2];
+            for (int k = 0; k < 2; ++k) {
+                v3 dd = k ? foot->dir_y : foot->dir_x;
+                v3 pv = cross3(dd, e2);
+                real inv = RC(1.0) / dot3(e1, pv);
+                real bu = dot3(tv, pv) * inv, bv = dot3(dd, qv) * inv;
+                q[k] = o_bary2(t0, t1, t2, (RC(1.0) - bu) - bv, bu, bv);
+            }
+            g[0] = foot->x_odd ? uv.x - q[0].x : q[0].x - uv.x;
+            g[1] = foot->x_odd ? uv.y - q[0].y : q[0].y - uv.y;
+            g[2] = foot->y_odd ? uv.x - q[1].x : q[1].x - uv.x;
+            g[3] = foot->y_odd ? uv.y - q[1].y : q[1].y - uv.y;
+        }
+        uint32_t idx[3] = {bci, mri, nmi};
+        o4* dst[3] = {&bc, &mr, &nm};
+        for (int k = 0; k < 3; ++k) {
+            if (idx[k] < 2) continue;
+            const otex* t = &s->tex[idx[k] - 2];
+            *dst[k] = foot ? o_tex_grad(s, t, uv.x, uv.y, g[0], g[1], g[2], g[3])
+                           : o_tex_trilinear(s, t, RC(2.0), uv.x, uv.y);
+        }
+    }
+    v3 n;
+    if (!s->TG && nmi < 2) {
+        n = s->N ? norm3(nl) : ng;
+    } else {
+        v3 ts = V3(nm.r * RC(2.0) - RC(1.0), nm.g * RC(2.0) - RC(1.0), nm.b * RC(2.0) - RC(1.0));
+        v3 tg = V3s(0), bt = V3s(0);
+        if (s->TG) {
+            const float* a = s->TG + 4 * i0;
+            const float* b = s->TG + 4 * i1;
+            const float* c = s->TG + 4 * i2;
+            tg = add3(add3(scale3(V3(a[0], a[1], a[2]), w0), scale3(V3(b[0], b[1], b[2]), h.u)),
+                      scale3(V3(c[0], c[1], c[2]), h.v));
+            real tw = ((real)a[3] * w0 + (real)b[3] * h.u) + (real)c[3] * h.v;
+            bt = scale3(cross3(nl, tg), tw);
+        }
+        n = norm3(add3(add3(scale3(tg, ts.x), scale3(bt, ts.y)), scale3(nl, ts.z)));
+    }
+    if (dot3(ng, d) > RC(0.0)) {
+        ng = neg3(ng);
+        n = neg3(n);
+    }
+    r.n = n;
+    r.ng = ng;
+    r.base = V3(bc.r * m[0], bc.g * m[1], bc.b * m[2]);
+    r.metal = mr.r * m[4];
+    r.rough = mr.g * m[5];
+    return r;
 }
 
 typedef struct {
@@ -89,8 +297,6 @@ typedef struct {
 } otrace_out;
 
 #define HG_MAX_QUERIES (128u * 1024u)
-
-static inline v3 vtx(const float* A, uint32_t i) { return V3(A[3 * i], A[3 * i + 1], A[3 * i + 2]); }
 
 /* hash_grid.inc.hlsl:32-105 VertexDescriptor keys */
 static inline void o_hg_key(v3 pos, v3 cam, uint32_t* hash, uint32_t* checksum) {
@@ -217,20 +423,9 @@ static otrace_out o_trace_radiance(const otrace_res* rt, const ofp* fp, v3 o, v3
         r.normal = neg3(d);
         return r;
     }
-    uint32_t i0 = s->I[3 * h.id], i1 = s->I[3 * h.id + 1], i2 = s->I[3 * h.id + 2];
-    v3 p0 = vtx(s->P, i0), p1 = vtx(s->P, i1), p2 = vtx(s->P, i2);
-    real w0 = (RC(1.0) - h.u) - h.v;
-    v3 pos = add3(add3(scale3(p0, w0), scale3(p1, h.u)), scale3(p2, h.v));
-    v3 ng = norm3(cross3(sub3(p1, p0), sub3(p2, p0)));
-    v3 n = ng;
-    if (s->N) n = norm3(add3(add3(scale3(vtx(s->N, i0), w0), scale3(vtx(s->N, i1), h.u)), scale3(vtx(s->N, i2), h.v)));
-    if (dot3(ng, d) > RC(0.0)) {
-        ng = neg3(ng);
-        n = neg3(n);
-    }
-    const float* m = s->mats + 8 * (s->TM ? s->TM[h.id] : 0);
-    v3 base = V3(m[0], m[1], m[2]);
-    real metal = m[4], rough = m[5];
+    osurf sf = o_surface(s, h, d, NULL);
+    v3 pos = sf.pos, n = sf.n, ng = sf.ng, base = sf.base;
+    real metal = sf.metal, rough = sf.rough;
     v3 rho = scale3(base, RC(1.0) - metal);
     v3 sun_dir = o_xyz(fp->sun_dir), sun = o_xyz(fp->sun_inten);
     v3 rad = V3s(0);
@@ -936,6 +1131,13 @@ void oracle_spec_spatial(const float* fpf, int w, int h, const uint32_t* gb, con
  * raster pass gbuffer_pass.rs:45-137 / mesh_gbuffer.hlsl:62-126) ---- */
 static inline uint32_t enc_unorm(real x, real s) { return r2u(x * s); }
 
+static v3 o_primary_dir(const ofp* fp, v3 o, int px, int py, int w, int h) {
+    real ndx = ((real)px + RC(0.5)) / (real)w * RC(2.0) - RC(1.0);
+    real ndy = ((real)py + RC(0.5)) / (real)h * RC(2.0) - RC(1.0);
+    v4 e = o_mul44(fp->inv_view_proj, V4(ndx, ndy, RC(1.0), RC(1.0)));
+    return norm3(sub3(V3(e.x / e.w, e.y / e.w, e.z / e.w), o));
+}
+
 void oracle_gbuffer(const float* fpf, const oscene* s, int w, int h, real* o_depth, uint32_t* o_gb) {
     ofp fp;
     o_load_fp(fpf, &fp);
@@ -946,35 +1148,30 @@ void oracle_gbuffer(const float* fpf, const oscene* s, int w, int h, real* o_dep
     for (int py = 0; py < h; ++py)
         for (int px = 0; px < w; ++px) {
             size_t i = (size_t)py * w + px;
-            real ndx = ((real)px + RC(0.5)) / (real)w * RC(2.0) - RC(1.0);
-            real ndy = ((real)py + RC(0.5)) / (real)h * RC(2.0) - RC(1.0);
-            v4 e = o_mul44(fp.inv_view_proj, V4(ndx, ndy, RC(1.0), RC(1.0)));
             v3 o = o_xyz(fp.view_pos);
-            v3 d = norm3(sub3(V3(e.x / e.w, e.y / e.w, e.z / e.w), o));
+            v3 d = o_primary_dir(&fp, o, px, py, w, h);
             ohit hh = obvh_trace(s->bvh, o, d, RC(0.05), RC(100000.0), 0);
             o_depth[i] = 0;
             o_gb[i] = o_gb[plane + i] = o_gb[2 * plane + i] = o_gb[3 * plane + i] = 0;
             if (hh.id == 0xFFFFFFFFu) continue;
-            uint32_t i0 = s->I[3 * hh.id], i1 = s->I[3 * hh.id + 1], i2 = s->I[3 * hh.id + 2];
-            v3 p0 = vtx(s->P, i0), p1 = vtx(s->P, i1), p2 = vtx(s->P, i2);
-            real w0 = (RC(1.0) - hh.u) - hh.v;
-            v3 pos = add3(add3(scale3(p0, w0), scale3(p1, hh.u)), scale3(p2, hh.v));
-            v3 ng = norm3(cross3(sub3(p1, p0), sub3(p2, p0)));
-            v3 n = ng;
-            if (s->N) n = norm3(add3(add3(scale3(vtx(s->N, i0), w0), scale3(vtx(s->N, i1), hh.u)), scale3(vtx(s->N, i2), hh.v)));
-            if (dot3(ng, d) > RC(0.0)) n = neg3(n);
+            ofoot ft;
+            ft.o = o;
+            ft.dir_x = o_primary_dir(&fp, o, px ^ 1, py, w, h);
+            ft.dir_y = o_primary_dir(&fp, o, px, py ^ 1, w, h);
+            ft.x_odd = px & 1, ft.y_odd = py & 1;
+            osurf sf = o_surface(s, hh, d, &ft);
+            v3 pos = sf.pos, n = sf.n;
             v4 c = o_mul44(fp.view_proj, V4(pos.x, pos.y, pos.z, RC(1.0)));
             real dep = c.z / c.w;
             if (!(dep > RC(0.0))) continue;
-            const float* m = s->mats + 8 * (s->TM ? s->TM[hh.id] : 0);
             o_depth[i] = dep;
             geo[i] = 1;
             nrm[3 * i] = n.x, nrm[3 * i + 1] = n.y, nrm[3 * i + 2] = n.z;
-            o_gb[i] = enc_unorm(m[0], RC(255.0)) | (enc_unorm(m[1], RC(255.0)) << 8) | (enc_unorm(m[2], RC(255.0)) << 16) |
-                      (enc_unorm(m[4], RC(255.0)) << 24);
+            o_gb[i] = enc_unorm(sf.base.x, RC(255.0)) | (enc_unorm(sf.base.y, RC(255.0)) << 8) |
+                      (enc_unorm(sf.base.z, RC(255.0)) << 16) | (enc_unorm(sf.metal, RC(255.0)) << 24);
             v2 oe = o_oct_enc(n);
             o_gb[plane + i] = enc_unorm(oe.x, RC(4095.0)) | (enc_unorm(oe.y, RC(4095.0)) << 12) |
-                              (enc_unorm(m[5], RC(255.0)) << 24);
+                              (enc_unorm(sf.rough, RC(255.0)) << 24);
             o_gb[3 * plane + i] = 1;
         }
     /* fwidth from fine differences inside aligned 2x2 quads */

@@ -23,10 +23,11 @@ def orbit_view(frame: int, w: int, h: int, radius=11.0, height=3.0, deg_per_fram
     return synth.View(synth.Camera(pos, yaw, pitch), w, h, jitter=j)
 
 
-def make_mesh(tris=4000, seed=3, extent=8.0):
+def make_mesh(tris=4000, seed=3, extent=8.0, textured=False):
     from violet_amd import scene
 
-    return scene.make_scene(tris, seed=seed, extent=extent)
+    m = scene.make_scene(tris, seed=seed, extent=extent)
+    return scene.texture_scene(m) if textured else m
 
 
 SNAP = {

@@ -23,7 +23,7 @@ def _mirrors():
         "vgi_denoise_spatial_io": _lib.DenoiseSpatialIO, "vgi_denoise_fast_clamp_io": _lib.DenoiseFastClampIO,
         "vgi_denoise_atrous_io": _lib.DenoiseAtrousIO, "vgi_pc_denoise_atrous": _lib.PCDenoiseAtrous,
         "vgi_denoiser_config": _lib.DenoiserConfig, "vgi_denoiser_input": _lib.DenoiserInput,
-        "vgi_denoiser_output": _lib.DenoiserOutput, "vgi_material": scene.Material,
+        "vgi_denoiser_output": _lib.DenoiserOutput, "vgi_material": scene.Material, "vgi_texture": scene.TextureDesc,
         "vgi_scene_desc": scene.SceneDesc, "vgi_scene_stats": scene.SceneStats,
         "vgi_gltf_config": scene.GltfConfig, "vgi_gltf_stats": scene.GltfStats,
         "vgi_trace_resources": _abi.TraceResources, "vgi_ind_diff_sample_gen_io": _abi.DiffSampleGenIO,

@@ -162,6 +162,25 @@ def test_sincos_accuracy_vs_libm(oracle32, oracle64):
     assert worst <= 2.0
 
 
+def test_log2_accuracy_vs_libm(oracle32):
+    """The single-precision log2 shared with the GPU (texture LOD) against
+    libm double: within 2 ulp of the result, exact at powers of two,
+    including subnormal arguments."""
+    f = oracle32.lib.oracle_log2
+    f.restype, f.argtypes = C.c_float, [C.c_float]
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([np.exp2(rng.uniform(-149, 127, 20000)), rng.uniform(0.5, 2.0, 20000),
+                         [1.0, 2.0, 0.5, 1024.0, 2.0 ** -140, 3.0e38, 0.70710677, 1.4142135]]).astype(np.float32)
+    worst = 0.0
+    for x in xs[xs > 0]:
+        got, ref = f(float(x)), math.log2(float(x))
+        ulp = float(np.spacing(np.float32(abs(ref)))) if ref != 0 else 1e-45
+        worst = max(worst, abs(got - ref) / ulp)
+    assert worst <= 2.0
+    for e in range(-149, 128):
+        assert f(2.0 ** e) == float(e)
+
+
 def test_golden_file(oracle64):
     data = json.loads(GOLDEN.read_text())
     for x, h in data["jenkins"]:

@@ -12,8 +12,11 @@ sequencer) against the oracle.
 
 Every test runs with the reference's resampling constants (M cap 16, 9
 spiral taps, radius 0.05*min(w,h)), with the 1280x720 frame-loop workload's
-(M cap 30, 5 taps, 30 px), and with the latter on a single stream (no
-concurrent lanes: the renderer's default overlaps independent pass chains).
+(M cap 30, 5 taps, 30 px), with the latter on a single stream (no
+concurrent lanes: the renderer's default overlaps independent pass chains),
+and with the reference constants on a textured scene (material textures
+sampled at LOD 2 by the indirect rays and with quad derivatives and
+anisotropy by the G-buffer).
 """
 from __future__ import annotations
 
@@ -33,6 +36,9 @@ PARAMS = {
     "frame_loop": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0),
     "frame_loop_one_stream": dict(temporal_m_cap=30, diff_spatial_taps=5, spatial_radius_px=30.0,
                                   concurrent_lanes=False),
+    # reference constants on the textured scene (texture coordinates, tangents,
+    # base-colour / metallic-roughness / normal maps with mips)
+    "textured": dict(temporal_m_cap=0, diff_spatial_taps=0, spatial_radius_px=0.0),
 }
 
 
@@ -40,7 +46,7 @@ PARAMS = {
 def run(request):
     from violet_amd import renderer
 
-    mesh = make_mesh()
+    mesh = make_mesh(textured=request.param == "textured")
     prm = PARAMS[request.param]
     fps, outs, snaps, r, sc = run_gpu_frames(mesh, W, H, 6, config=renderer.RestirConfig(**prm))
     return dict(mesh=mesh, fps=fps, outs=outs, snaps=snaps, prm=prm)

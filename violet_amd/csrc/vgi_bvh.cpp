@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -283,7 +284,7 @@ struct Collapser {
 }  // namespace
 
 struct vgi_scene {
-    DevBuf nodes, tris, shade, positions, normals, materials;
+    DevBuf nodes, tris, shade, positions, normals, materials, texcoords, tangents, textures, texels, texel_lut;
     SceneView view{};
     uint32_t num_vertices = 0, num_triangles = 0, depth = 0;
 };
@@ -301,6 +302,56 @@ static vgi_status upload(DevBuf& b, const std::vector<T>& v) {
 }
 
 namespace vgi {
+
+// Texel table, texture descriptors and the byte -> float lookup of the
+// sampling code (render_scene.rs:344-346,768-833 for the two built-in
+// textures, which the device code treats as constants).
+static vgi_status build_textures(const vgi_scene_desc* desc, HostScene& hs) {
+    VGI_REQUIRE(desc->num_textures == 0 || desc->textures, "vgi_scene_create: null texture array");
+    VGI_REQUIRE(desc->num_textures <= 0xFFFFu - VGI_TEXTURE_FIRST, "vgi_scene_create: %u textures exceed the 16-bit indices",
+                desc->num_textures);
+    const uint32_t limit = VGI_TEXTURE_FIRST + desc->num_textures;
+    for (uint32_t m = 0; m < desc->num_materials; ++m) {
+        const vgi_material& mt = desc->materials[m];
+        const uint32_t bc = mt.color_metalrough_texture & 0xFFFFu, mr = mt.color_metalrough_texture >> 16,
+                       nm = mt.normal_texture & 0xFFFFu;
+        VGI_REQUIRE(bc < limit && mr < limit && nm < limit,
+                    "vgi_scene_create: material %u names texture %u but the scene has %u", m,
+                    bc >= limit ? bc : mr >= limit ? mr : nm, desc->num_textures);
+    }
+    hs.textures.assign(desc->num_textures, TexDesc{});
+    uint64_t total = 0;
+    for (uint32_t t = 0; t < desc->num_textures; ++t) {
+        const vgi_texture& x = desc->textures[t];
+        VGI_REQUIRE(x.texels && x.width >= 1 && x.height >= 1 && x.width <= 32768u && x.height <= 32768u,
+                    "vgi_scene_create: texture %u: bad size %ux%u or no texels", t, x.width, x.height);
+        VGI_REQUIRE(x.levels >= 1 && x.levels <= kMaxTextureLevels, "vgi_scene_create: texture %u: %u mip levels", t,
+                    x.levels);
+        TexDesc& d = hs.textures[t];
+        d.width = x.width;
+        d.height = x.height;
+        d.levels = x.levels;
+        d.srgb = x.srgb ? 1u : 0u;
+        for (uint32_t l = 0; l < x.levels; ++l) {
+            d.level_offset[l] = (uint32_t)total;
+            total += (uint64_t)std::max(1u, x.width >> l) * std::max(1u, x.height >> l);
+            VGI_REQUIRE(total < 0xFFFFFFFFull, "vgi_scene_create: textures exceed 2^32 texels");
+        }
+    }
+    hs.texels.resize((size_t)total);
+    for (uint32_t t = 0; t < desc->num_textures; ++t) {
+        const TexDesc& d = hs.textures[t];
+        const uint64_t end = t + 1 < desc->num_textures ? hs.textures[t + 1].level_offset[0] : total;
+        memcpy(hs.texels.data() + d.level_offset[0], desc->textures[t].texels, (size_t)(end - d.level_offset[0]) * 4u);
+    }
+    hs.texel_lut.resize(kTexelLutEntries);
+    for (uint32_t i = 0; i < 256; ++i) {
+        const double c = i / 255.0;
+        hs.texel_lut[i] = (float)c;
+        hs.texel_lut[256 + i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
+    return VGI_OK;
+}
 
 vgi_status build_scene_host(const vgi_scene_desc* desc, HostScene& hs) {
     VGI_REQUIRE(desc, "vgi_scene_create: null argument");
@@ -377,15 +428,11 @@ vgi_status build_scene_host(const vgi_scene_desc* desc, HostScene& hs) {
         nrm.assign(3 * (size_t)nv, 0.0f);
     }
     std::vector<Material>& mats = hs.materials;
-    mats.assign(desc->num_materials, Material{});
-    for (uint32_t m = 0; m < desc->num_materials; ++m) {
-        const vgi_material& s = desc->materials[m];
-        Material& d = mats[m];
-        for (int k = 0; k < 4; ++k) d.base_color[k] = s.base_color[k];
-        d.metallic = s.metallic;
-        d.roughness = s.roughness;
-        d.pad[0] = d.pad[1] = 0.0f;
-    }
+    mats.resize(desc->num_materials);
+    memcpy(mats.data(), desc->materials, sizeof(Material) * desc->num_materials);
+    if (desc->texcoords) hs.texcoords.assign(desc->texcoords, desc->texcoords + 2 * (size_t)nv);
+    if (desc->tangents) hs.tangents.assign(desc->tangents, desc->tangents + 4 * (size_t)nv);
+    VGI_CHECK(build_textures(desc, hs));
     hs.nodes = std::move(c4.out);
     hs.root = root;
     hs.levels = (uint32_t)c4.max_level + 1;  // levels of 4-wide nodes
@@ -410,7 +457,10 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     vgi_status st;
     if ((st = upload(sc->nodes, hs.nodes)) != VGI_OK || (st = upload(sc->tris, hs.tris)) != VGI_OK ||
         (st = upload(sc->shade, hs.shade)) != VGI_OK || (st = upload(sc->positions, hs.positions)) != VGI_OK ||
-        (st = upload(sc->normals, hs.normals)) != VGI_OK || (st = upload(sc->materials, hs.materials)) != VGI_OK) {
+        (st = upload(sc->normals, hs.normals)) != VGI_OK || (st = upload(sc->materials, hs.materials)) != VGI_OK ||
+        (st = upload(sc->texcoords, hs.texcoords)) != VGI_OK || (st = upload(sc->tangents, hs.tangents)) != VGI_OK ||
+        (st = upload(sc->textures, hs.textures)) != VGI_OK || (st = upload(sc->texels, hs.texels)) != VGI_OK ||
+        (st = upload(sc->texel_lut, hs.texel_lut)) != VGI_OK) {
         delete sc;
         return st;
     }
@@ -423,6 +473,11 @@ extern "C" vgi_status vgi_scene_create(const vgi_scene_desc* desc, vgi_scene** o
     sc->view.positions = sc->positions.as<float>();
     sc->view.normals = desc->normals ? sc->normals.as<float>() : nullptr;
     sc->view.materials = sc->materials.as<Material>();
+    sc->view.texcoords = desc->texcoords ? sc->texcoords.as<float>() : nullptr;
+    sc->view.tangents = desc->tangents ? sc->tangents.as<float>() : nullptr;
+    sc->view.textures = sc->textures.as<TexDesc>();
+    sc->view.texels = sc->texels.as<uint32_t>();
+    sc->view.texel_lut = sc->texel_lut.as<float>();
     sc->view.num_nodes = (uint32_t)hs.nodes.size();
     sc->view.num_tris = nt;
     sc->view.root = hs.root;
@@ -444,6 +499,7 @@ extern "C" vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out) {
     out->num_nodes = s->view.num_nodes;
     out->depth = s->depth;
     out->device_bytes = (uint64_t)(s->nodes.bytes + s->tris.bytes + s->shade.bytes + s->positions.bytes +
-                                   s->normals.bytes + s->materials.bytes);
+                                   s->normals.bytes + s->materials.bytes + s->texcoords.bytes + s->tangents.bytes +
+                                   s->textures.bytes + s->texels.bytes + s->texel_lut.bytes);
     return VGI_OK;
 }

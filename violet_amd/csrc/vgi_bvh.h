@@ -52,13 +52,27 @@ struct TriShade {
     uint32_t material;
 };
 
-// MaterialParams factors (scene_bindings.hlsl:24-51), untextured.
+// MaterialParams (scene_bindings.hlsl:24-51): the layout of vgi_material.
 struct Material {
     float base_color[4];
     float metallic;
     float roughness;
-    float pad[2];
+    uint32_t color_metalrough_texture;  // base colour | metallic-roughness << 16
+    uint32_t normal_texture;            // low 16 bits
 };
+static_assert(sizeof(Material) == sizeof(vgi_material), "Material layout");
+
+// One material texture on the device: RGBA8 texels (one u32 each, R in the
+// low byte) of every mip level, level l at texels + level_offset[l].
+constexpr uint32_t kMaxTextureLevels = 16;
+struct TexDesc {
+    uint32_t width, height, levels, srgb;
+    uint32_t level_offset[kMaxTextureLevels];
+};
+static_assert(sizeof(TexDesc) == 80, "TexDesc layout");
+// Texel byte -> float: [0, 256) UNORM (c / 255), [256, 512) the sRGB
+// transfer function's inverse, both evaluated in double on the host.
+constexpr uint32_t kTexelLutEntries = 512;
 
 struct SceneView {
     const BvhNode* nodes;
@@ -67,13 +81,17 @@ struct SceneView {
     const float* positions;  // 3 floats per vertex
     const float* normals;    // 3 floats per vertex
     const Material* materials;
+    const float* texcoords;  // 2 floats per vertex, or null
+    const float* tangents;   // 4 floats per vertex, or null
+    const TexDesc* textures; // the scene's textures (material index - 2)
+    const uint32_t* texels;
+    const float* texel_lut;  // kTexelLutEntries floats
     uint32_t num_nodes;
     uint32_t num_tris;
     int32_t root;           // link of the root (node 0)
     uint32_t stack_depth;   // most traversal-stack entries a ray can hold (3 per node level)
     unsigned long long* counters;  // traversal statistics slots (instrumented kernels only)
 };
-
 
 constexpr float kRayTMin = 0.0005f;  // raytrace.inc.hlsl:39,70
 constexpr float kRayTMax = 1000.0f;  // raytrace.inc.hlsl:40,71
@@ -99,8 +117,11 @@ struct HostScene {
     std::vector<BvhNode> nodes;
     std::vector<BvhTri> tris;
     std::vector<TriShade> shade;
-    std::vector<float> positions, normals;
+    std::vector<float> positions, normals, texcoords, tangents;
     std::vector<Material> materials;
+    std::vector<TexDesc> textures;
+    std::vector<uint32_t> texels;
+    std::vector<float> texel_lut;
     int32_t root = 0;
     uint32_t levels = 0;  // levels of 4-wide nodes
 };

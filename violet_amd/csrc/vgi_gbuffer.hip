@@ -9,7 +9,9 @@
  * quad, taken with lane shuffles: the quad's four pixels are lanes 4q..4q+3
  * (x partner = lane ^ 1, y partner = lane ^ 2), exactly the lanes a raster
  * pixel shader's derivatives come from.  A partner without geometry (or
- * outside the image) contributes 0.
+ * outside the image) contributes 0.  Material textures are sampled with the
+ * texture-coordinate derivatives of the same quad, evaluated on the hit
+ * triangle's plane (vgi_radiance.h shade_hit_t).
  */
 #include "vgi_batch_kernel.h"
 #include "vgi_internal.h"
@@ -17,6 +19,16 @@
 
 namespace vgi {
 const SceneView& scene_view(const vgi_scene* s);
+
+// Direction of the ray through pixel (px, py)'s centre.
+__device__ __forceinline__ f3 primary_dir(const FrameParams& fp, f3 o, uint32_t px, uint32_t py, uint32_t w,
+                                          uint32_t h) {
+    float ndx = ((float)px + 0.5f) / (float)w * 2.0f - 1.0f;
+    float ndy = ((float)py + 0.5f) / (float)h * 2.0f - 1.0f;
+    f4 e = mul44(fp.inv_view_proj, mk4(ndx, ndy, 1.0f, 1.0f));
+    f3 end = mk3(e.x / e.w, e.y / e.w, e.z / e.w);
+    return normalize3(end - o);
+}
 
 template <bool kCount>
 __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const SceneView& s, const vgi_gbuffer_io& io, uint32_t bx_, uint32_t by_) {
@@ -30,14 +42,16 @@ __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const Scen
     f3 n = mk3s(0.0f);
     SurfaceHit sh;
     if (inside) {
-        float ndx = ((float)px + 0.5f) / (float)w * 2.0f - 1.0f;
-        float ndy = ((float)py + 0.5f) / (float)h * 2.0f - 1.0f;
-        f4 e = mul44(fp.inv_view_proj, mk4(ndx, ndy, 1.0f, 1.0f));
-        f3 end = mk3(e.x / e.w, e.y / e.w, e.z / e.w);
-        f3 o = xyz(fp.view_pos);
-        f3 d = normalize3(end - o);
+        const f3 o = xyz(fp.view_pos);
+        const f3 d = primary_dir(fp, o, px, py, w, h);
         Hit hit = trace_closest<kCount>(s, o, d, 0.05f, 100000.0f, stack);
-        sh = shade_hit(s, hit, d);
+        PrimaryFootprint pf;
+        pf.o = o;
+        pf.dir_x = primary_dir(fp, o, px ^ 1u, py, w, h);
+        pf.dir_y = primary_dir(fp, o, px, py ^ 1u, w, h);
+        pf.x_odd = (px & 1u) != 0;
+        pf.y_odd = (py & 1u) != 0;
+        sh = shade_hit_t<true>(s, hit, d, &pf);
         if (!sh.missed) {
             f4 c = mul44(fp.view_proj, mk4(sh.position.x, sh.position.y, sh.position.z, 1.0f));
             depth = c.z / c.w;

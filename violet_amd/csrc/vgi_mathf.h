@@ -17,6 +17,12 @@
  * both host (SSE) and device (gfx950 VALU, compiled with
  * -ffp-contract=off).  Accuracy: < 2 ulp for |x| < 8192 (tested against
  * libm double).  This file is plain C99 so the C oracle can include it.
+ *
+ * vgi_log2f serves the texture level of detail (log2 of the texel
+ * footprint), whose fraction weights two mip levels: Cephes' log2f method
+ * (split x = m * 2^e with m in [sqrt(1/2), sqrt(2)), a degree-9 minimax
+ * polynomial for log(1 + (m - 1)), and log2(e) applied in two parts), with
+ * the same separate multiplies and adds.
  */
 #ifndef VGI_MATHF_H
 #define VGI_MATHF_H
@@ -97,6 +103,51 @@ VGI_FN float vgi_cosf(float x)
     float s, c;
     vgi_sincosf(x, &s, &c);
     return c;
+}
+
+/* log2 of a positive finite float (0, negative and non-finite arguments are
+ * the caller's to exclude: the result is then unspecified). */
+VGI_FN float vgi_log2f(float x)
+{
+    const float SQRTHF = 0.707106781186547524f;
+    const float LOG2EA = 0.44269504088896340736f; /* log2(e) - 1 */
+    unsigned int b;
+    __builtin_memcpy(&b, &x, 4);
+    int e = (int)((b >> 23) & 0xffu);
+    if (e == 0) { /* subnormal: scale into the normal range first */
+        x = x * 16777216.0f;
+        __builtin_memcpy(&b, &x, 4);
+        e = (int)((b >> 23) & 0xffu) - 24;
+    }
+    e -= 126;
+    b = (b & 0x807fffffu) | 0x3f000000u; /* mantissa in [0.5, 1) */
+    float m;
+    __builtin_memcpy(&m, &b, 4);
+    if (m < SQRTHF) {
+        e -= 1;
+        m = m + m - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = y * m + -1.1514610310e-1f;
+    y = y * m + 1.1676998740e-1f;
+    y = y * m + -1.2420140846e-1f;
+    y = y * m + 1.4249322787e-1f;
+    y = y * m + -1.6668057665e-1f;
+    y = y * m + 2.0000714765e-1f;
+    y = y * m + -2.4999993993e-1f;
+    y = y * m + 3.3333331174e-1f;
+    y = y * m;
+    y = y * z;
+    y = y + -0.5f * z;
+    float r = y * LOG2EA;
+    r = r + m * LOG2EA;
+    r = r + y;
+    r = r + m;
+    r = r + (float)e;
+    return r;
 }
 
 #endif /* VGI_MATHF_H */

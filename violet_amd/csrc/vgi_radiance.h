@@ -6,6 +6,7 @@
  */
 #pragma once
 #include "vgi_storage.h"
+#include "vgi_texture.h"
 #include "vgi_traverse.h"
 
 namespace vgi {
@@ -80,10 +81,41 @@ struct SurfaceHit {
     float roughness;  // perceptual
 };
 
-// geometry_rchit.inc.hlsl:376-472 for an untextured, pre-transformed scene:
-// barycentric interpolation of position and normal (normal map = flat),
-// geometric normal from the edges, two-sided flip.
-__device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h, f3 ray_dir) {
+// The G-buffer's screen-space footprint of a primary ray: the camera
+// position and the rays through the pixel's 2x2-quad partners, (px ^ 1, py)
+// and (px, py ^ 1).  A raster pixel shader's fine derivatives difference an
+// attribute between the two pixels of a quad row (column), both evaluated on
+// the pixel's own triangle plane; the partner's barycentrics come from the
+// same ray/plane intersection as the hit's, without the edge tests.
+struct PrimaryFootprint {
+    f3 o;
+    f3 dir_x, dir_y;
+    bool x_odd, y_odd;
+};
+
+constexpr float kNormalUpXY = (float)(128.0 / 255.0);  // render_scene.rs:808 texel, UNORM
+
+__device__ __forceinline__ float2_ tri_uv(const float* uv, uint32_t i) {
+    return float2_{gld(uv + 2 * i), gld(uv + 2 * i + 1)};
+}
+
+// Barycentric interpolation of a 2-vector, weights (w0, u, v).
+__device__ __forceinline__ float2_ bary2(float2_ a, float2_ b, float2_ c, float w0, float u, float v) {
+    return float2_{(a.x * w0 + b.x * u) + c.x * v, (a.y * w0 + b.y * u) + c.y * v};
+}
+
+// geometry_rchit.inc.hlsl:54-150 (and mesh_gbuffer.hlsl:62-97 for the
+// G-buffer, kPrimary) on a pre-transformed scene: barycentric interpolation
+// of position, normal, texture coordinates and tangent; material textures
+// sampled at LOD 2 (indirect) or with the quad derivatives (primary);
+// factors applied; normal mapping with the MikkTSpace bitangent; geometric
+// normal from the edges; two-sided flip.  Texture indices 0 and 1 are
+// violet's built-in white and flat-normal textures, taken as constants.  A
+// scene without tangents and a material without a normal map keep the
+// interpolated normal (the tangent terms vanish and the map's z is 1).
+template <bool kPrimary>
+__device__ __forceinline__ SurfaceHit shade_hit_t(const SceneView& s, const Hit& h, f3 ray_dir,
+                                                  const PrimaryFootprint* pf) {
     SurfaceHit r;
     r.missed = h.tri == 0xFFFFFFFFu;
     if (r.missed) return r;
@@ -95,15 +127,72 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
     float w0 = (1.0f - h.u) - h.v;
     r.position = (p0 * w0 + p1 * h.u) + p2 * h.v;
     f3 ng = normalize3(cross3(p1 - p0, p2 - p0));
-    f3 n;
+    f3 nl = ng;  // interpolated (unnormalised) vertex normal
     if (s.normals) {
         const float* N = s.normals;
         f3 n0 = mk3(gld(N + 3 * sh.i0), gld(N + 3 * sh.i0 + 1), gld(N + 3 * sh.i0 + 2));
         f3 n1 = mk3(gld(N + 3 * sh.i1), gld(N + 3 * sh.i1 + 1), gld(N + 3 * sh.i1 + 2));
         f3 n2 = mk3(gld(N + 3 * sh.i2), gld(N + 3 * sh.i2 + 1), gld(N + 3 * sh.i2 + 2));
-        n = normalize3((n0 * w0 + n1 * h.u) + n2 * h.v);
+        nl = (n0 * w0 + n1 * h.u) + n2 * h.v;
+    }
+    const Material m = gld(s.materials + sh.material);
+    const uint32_t bc_i = m.color_metalrough_texture & 0xFFFFu, mr_i = m.color_metalrough_texture >> 16,
+                   nm_i = m.normal_texture & 0xFFFFu;
+    Tex4 bc{1.0f, 1.0f, 1.0f, 1.0f}, mr{1.0f, 1.0f, 1.0f, 1.0f}, nm{1.0f, 1.0f, 1.0f, 1.0f};
+    if (bc_i == VGI_TEXTURE_NORMAL_UP) bc = Tex4{kNormalUpXY, kNormalUpXY, 1.0f, 1.0f};
+    if (mr_i == VGI_TEXTURE_NORMAL_UP) mr = Tex4{kNormalUpXY, kNormalUpXY, 1.0f, 1.0f};
+    if (nm_i == VGI_TEXTURE_NORMAL_UP) nm = Tex4{kNormalUpXY, kNormalUpXY, 1.0f, 1.0f};
+    if (bc_i >= VGI_TEXTURE_FIRST || mr_i >= VGI_TEXTURE_FIRST || nm_i >= VGI_TEXTURE_FIRST) {
+        float2_ t0{0.0f, 0.0f}, t1{0.0f, 0.0f}, t2{0.0f, 0.0f};
+        if (s.texcoords) {
+            t0 = tri_uv(s.texcoords, sh.i0);
+            t1 = tri_uv(s.texcoords, sh.i1);
+            t2 = tri_uv(s.texcoords, sh.i2);
+        }
+        const float2_ uv = bary2(t0, t1, t2, w0, h.u, h.v);
+        TexGrad g{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (kPrimary) {
+            const f3 e1 = p1 - p0, e2 = p2 - p0, tvec = pf->o - p0, qvec = cross3(tvec, e1);
+            auto uv_on_plane = [&](f3 d) {
+                const f3 pvec = cross3(d, e2);
+                const float inv_det = 1.0f / dot3(e1, pvec);
+                const float bu = dot3(tvec, pvec) * inv_det;
+                const float bv = dot3(d, qvec) * inv_det;
+                return bary2(t0, t1, t2, (1.0f - bu) - bv, bu, bv);
+            };
+            const float2_ ux = uv_on_plane(pf->dir_x), uy = uv_on_plane(pf->dir_y);
+            g.dsdx = pf->x_odd ? uv.x - ux.x : ux.x - uv.x;
+            g.dtdx = pf->x_odd ? uv.y - ux.y : ux.y - uv.y;
+            g.dsdy = pf->y_odd ? uv.x - uy.x : uy.x - uv.x;
+            g.dtdy = pf->y_odd ? uv.y - uy.y : uy.y - uv.y;
+        }
+        auto sample = [&](uint32_t idx) {
+            const TexDesc& t = s.textures[idx - VGI_TEXTURE_FIRST];
+            if constexpr (kPrimary)
+                return tex_sample_grad(s, t, uv.x, uv.y, g);
+            else
+                return tex_trilinear(s, t, kIndirectTexLod, uv.x, uv.y);
+        };
+        if (bc_i >= VGI_TEXTURE_FIRST) bc = sample(bc_i);
+        if (mr_i >= VGI_TEXTURE_FIRST) mr = sample(mr_i);
+        if (nm_i >= VGI_TEXTURE_FIRST) nm = sample(nm_i);
+    }
+    f3 n;
+    if (!s.tangents && nm_i < VGI_TEXTURE_FIRST) {
+        n = s.normals ? normalize3(nl) : ng;
     } else {
-        n = ng;
+        const f3 ts = mk3(nm.r * 2.0f - 1.0f, nm.g * 2.0f - 1.0f, nm.b * 2.0f - 1.0f);
+        f3 tg = mk3s(0.0f), bt = mk3s(0.0f);
+        if (s.tangents) {
+            const float* T = s.tangents;
+            const f4 a = mk4(gld(T + 4 * sh.i0), gld(T + 4 * sh.i0 + 1), gld(T + 4 * sh.i0 + 2), gld(T + 4 * sh.i0 + 3));
+            const f4 b = mk4(gld(T + 4 * sh.i1), gld(T + 4 * sh.i1 + 1), gld(T + 4 * sh.i1 + 2), gld(T + 4 * sh.i1 + 3));
+            const f4 c = mk4(gld(T + 4 * sh.i2), gld(T + 4 * sh.i2 + 1), gld(T + 4 * sh.i2 + 2), gld(T + 4 * sh.i2 + 3));
+            tg = (mk3(a.x, a.y, a.z) * w0 + mk3(b.x, b.y, b.z) * h.u) + mk3(c.x, c.y, c.z) * h.v;
+            const float tw = (a.w * w0 + b.w * h.u) + c.w * h.v;
+            bt = tw * cross3(nl, tg);
+        }
+        n = normalize3((ts.x * tg + ts.y * bt) + ts.z * nl);
     }
     if (dot3(ng, ray_dir) > 0.0f) {
         ng = -ng;
@@ -111,11 +200,15 @@ __device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h
     }
     r.normal = n;
     r.normal_geo = ng;
-    const Material m = gld(s.materials + sh.material);
-    r.base_color = mk3(m.base_color[0], m.base_color[1], m.base_color[2]);
-    r.metallic = m.metallic;
-    r.roughness = m.roughness;
+    r.base_color = mk3(bc.r * m.base_color[0], bc.g * m.base_color[1], bc.b * m.base_color[2]);
+    r.metallic = mr.r * m.metallic;
+    r.roughness = mr.g * m.roughness;
     return r;
+}
+
+// Indirect rays (geometry_ind.rchit.hlsl): textures at LOD 2.
+__device__ __forceinline__ SurfaceHit shade_hit(const SceneView& s, const Hit& h, f3 ray_dir) {
+    return shade_hit_t<false>(s, h, ray_dir, nullptr);
 }
 
 // raytrace.inc.hlsl:65-208 after its two TraceRay calls: the radiance of

@@ -11,7 +11,7 @@ plane flat ones (each face has its own vertices).
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from functools import lru_cache
 
 import numpy as np
@@ -22,13 +22,74 @@ from .synth import uniform
 
 class Material(C.Structure):
     _fields_ = [("base_color", C.c_float * 4), ("metallic", C.c_float), ("roughness", C.c_float),
-                ("pad", C.c_float * 2)]
+                ("color_metalrough_texture", C.c_uint32), ("normal_texture", C.c_uint32)]
+
+
+class TextureDesc(C.Structure):
+    """vgi_texture: RGBA8 mip chain in host memory."""
+
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("levels", C.c_uint32), ("srgb", C.c_uint32),
+                ("texels", C.c_void_p)]
 
 
 class SceneDesc(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("num_vertices", C.c_uint32),
                 ("indices", C.c_void_p), ("num_triangles", C.c_uint32), ("triangle_material", C.c_void_p),
-                ("materials", C.POINTER(Material)), ("num_materials", C.c_uint32)]
+                ("materials", C.POINTER(Material)), ("num_materials", C.c_uint32),
+                ("texcoords", C.c_void_p), ("tangents", C.c_void_p), ("textures", C.POINTER(TextureDesc)),
+                ("num_textures", C.c_uint32)]
+
+
+TEXTURE_WHITE = 0  # VGI_TEXTURE_WHITE
+TEXTURE_NORMAL_UP = 1  # VGI_TEXTURE_NORMAL_UP
+TEXTURE_FIRST = 2  # VGI_TEXTURE_FIRST
+
+
+@dataclass
+class Texture:
+    """An RGBA8 texture with its mip chain: levels[l] is (max(1, h >> l),
+    max(1, w >> l), 4) uint8.  srgb marks base-colour textures."""
+
+    levels: list
+    srgb: bool = False
+
+    @property
+    def width(self) -> int:
+        return self.levels[0].shape[1]
+
+    @property
+    def height(self) -> int:
+        return self.levels[0].shape[0]
+
+    def packed(self) -> np.ndarray:
+        return np.concatenate([np.ascontiguousarray(l, np.uint8).reshape(-1) for l in self.levels])
+
+
+def mip_count(width: int, height: int) -> int:
+    """violet's level count: 1 + log2(next power of two of max(w, h))
+    (model.rs:853-857)."""
+    m = max(width, height)
+    return 1 + (m - 1).bit_length()
+
+
+def box_mips(level0: np.ndarray, levels: int | None = None) -> list:
+    """A mip chain by 2x2 box averaging (rounded), halving with a floor of 1
+    as violet's importer does; odd sizes drop the last row/column.  A simple
+    chain for synthetic scenes (violet's own importer filters with
+    Lanczos3)."""
+    h, w = level0.shape[:2]
+    n = mip_count(w, h) if levels is None else levels
+    out = [np.ascontiguousarray(level0, np.uint8)]
+    for _ in range(1, n):
+        p = out[-1].astype(np.uint32)
+        ph, pw = p.shape[:2]
+        nh, nw = max(1, ph // 2), max(1, pw // 2)
+        acc = np.zeros((nh, nw, p.shape[2]), np.uint32)
+        for r in ((p[0:2 * nh:2], p[1:2 * nh:2]) if ph > 1 else (p, p)):
+            for c in ((r[:, 0:2 * nw:2], r[:, 1:2 * nw:2]) if pw > 1 else (r, r)):
+                acc += c
+        out.append(((acc + 2) // 4).astype(np.uint8))
+    return out
 
 
 class SceneStats(C.Structure):
@@ -67,6 +128,21 @@ class Mesh:
     albedo: np.ndarray  # (M,3)
     metallic: np.ndarray  # (M,)
     roughness: np.ndarray  # (M,)
+    texcoords: np.ndarray | None = None  # (V,2) f32
+    tangents: np.ndarray | None = None  # (V,4) f32, w = bitangent sign
+    textures: list = field(default_factory=list)  # Texture, numbered from TEXTURE_FIRST
+    # (M,3) u32 texture indices per material: base colour, metallic-roughness,
+    # normal map (None: white, white, flat normal)
+    material_textures: np.ndarray | None = None
+
+    def texture_words(self) -> np.ndarray:
+        """(M,2) u32: vgi_material's color_metalrough_texture, normal_texture."""
+        nm = len(self.albedo)
+        t = self.material_textures
+        if t is None:
+            t = np.tile(np.array([TEXTURE_WHITE, TEXTURE_WHITE, TEXTURE_NORMAL_UP], np.uint32), (nm, 1))
+        t = np.asarray(t, np.uint32)
+        return np.stack([t[:, 0] | (t[:, 1] << 16), t[:, 2]], -1).astype(np.uint32)
 
     @property
     def num_triangles(self) -> int:
@@ -168,21 +244,123 @@ def make_scene(target_triangles: int = 200_000, seed: int = 11, extent: float = 
                 np.zeros(obj), rough)
 
 
+def vertex_tangents(P: np.ndarray, N: np.ndarray, UV: np.ndarray, I: np.ndarray) -> np.ndarray:
+    """(V,4) f32 tangents for normal mapping: each triangle's dP/du summed
+    per vertex, made orthogonal to the vertex normal and normalised, with w
+    the sign of the bitangent against the summed dP/dv (E. Lengyel, 2001).
+    A vertex without a usable direction gets any unit vector orthogonal to
+    its normal.  (violet generates missing tangents with MikkTSpace,
+    model.rs:507-531, which also splits vertices on tangent seams.)"""
+    P = np.asarray(P, np.float64)
+    N = np.asarray(N, np.float64)
+    UV = np.asarray(UV, np.float64)
+    I = np.asarray(I, np.int64)
+    e1, e2 = P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]]
+    d1, d2 = UV[I[:, 1]] - UV[I[:, 0]], UV[I[:, 2]] - UV[I[:, 0]]
+    r = d1[:, 0] * d2[:, 1] - d2[:, 0] * d1[:, 1]
+    inv = np.where(np.abs(r) > 1e-20, 1.0 / np.where(r == 0, 1.0, r), 0.0)[:, None]
+    sdir = (e1 * d2[:, 1:2] - e2 * d1[:, 1:2]) * inv
+    tdir = (e2 * d1[:, 0:1] - e1 * d2[:, 0:1]) * inv
+    T = np.zeros_like(P)
+    B = np.zeros_like(P)
+    for k in range(3):
+        np.add.at(T, I[:, k], sdir)
+        np.add.at(B, I[:, k], tdir)
+    n = N / np.maximum(np.linalg.norm(N, axis=1, keepdims=True), 1e-30)
+    t = T - n * np.sum(n * T, axis=1, keepdims=True)
+    ln = np.linalg.norm(t, axis=1, keepdims=True)
+    alt = np.cross(n, np.where(np.abs(n[:, :1]) < 0.9, [[1.0, 0.0, 0.0]], [[0.0, 1.0, 0.0]]))
+    t = np.where(ln > 1e-12, t / np.maximum(ln, 1e-30), alt / np.maximum(np.linalg.norm(alt, axis=1, keepdims=True),
+                                                                         1e-30))
+    w = np.where(np.sum(np.cross(n, t) * B, axis=1) < 0.0, -1.0, 1.0)
+    return np.concatenate([t, w[:, None]], 1).astype(np.float32)
+
+
+def _smooth_image(seed: int, w: int, h: int, channels: int = 4, cells: int = 6) -> np.ndarray:
+    """Seeded RGBA noise: a coarse random lattice bilinearly upsampled plus
+    fine per-texel noise, values in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    g = rng.random((cells + 1, cells + 1, channels))
+    ys, xs = np.linspace(0, cells, h, endpoint=False), np.linspace(0, cells, w, endpoint=False)
+    y0, x0 = ys.astype(int), xs.astype(int)
+    fy, fx = (ys - y0)[:, None, None], (xs - x0)[None, :, None]
+    a = g[y0][:, x0] * (1 - fx) + g[y0][:, x0 + 1] * fx
+    b = g[y0 + 1][:, x0] * (1 - fx) + g[y0 + 1][:, x0 + 1] * fx
+    img = a * (1 - fy) + b * fy
+    return np.clip(0.8 * img + 0.2 * rng.random((h, w, channels)), 0.0, 1.0)
+
+
+def texture_scene(mesh: Mesh, seed: int = 5, uv_scale: float = 0.5) -> Mesh:
+    """The same geometry with texture coordinates, tangents and a set of
+    seeded textures: planar coordinates on the plane facing each vertex
+    normal's dominant axis (times uv_scale, so the ground repeats), tangents
+    from vertex_tangents, and materials cycling through untextured, base
+    colour, base colour + metallic-roughness, all three maps, a normal map
+    alone, and the built-in textures in unusual slots."""
+    P = np.asarray(mesh.positions, np.float64)
+    N = np.asarray(mesh.normals, np.float64)
+    ax = np.argmax(np.abs(N), axis=1)
+    uv = np.where((ax == 2)[:, None], P[:, [0, 1]], np.where((ax == 0)[:, None], P[:, [1, 2]], P[:, [0, 2]]))
+    uv = (uv * uv_scale).astype(np.float32)
+    tg = vertex_tangents(P, N, uv, mesh.indices)
+
+    def rgba8(x):
+        return np.round(np.clip(x, 0.0, 1.0) * 255.0).astype(np.uint8)
+
+    def normal_map(w, h):
+        rng = np.random.default_rng(seed + 7)
+        t = np.concatenate([(_smooth_image(seed + 3, w, h, 2) - 0.5) * 0.8, np.ones((h, w, 1))], 2)
+        t /= np.linalg.norm(t, axis=2, keepdims=True)
+        img = np.concatenate([t * 0.5 + 0.5, np.ones((h, w, 1))], 2)
+        img[..., :2] += (rng.random((h, w, 2)) - 0.5) * 0.02
+        return rgba8(img)
+
+    metal_rough = _smooth_image(seed + 2, 32, 128)
+    metal_rough[..., 0] = (metal_rough[..., 0] > 0.6)  # metallic mostly 0 or 1
+    textures = [
+        Texture(box_mips(rgba8(_smooth_image(seed, 64, 64))), srgb=True),  # 2
+        Texture(box_mips(rgba8(_smooth_image(seed + 1, 37, 53))), srgb=True),  # 3: odd sizes
+        Texture(box_mips(rgba8(metal_rough))),  # 4
+        Texture(box_mips(normal_map(64, 64))),  # 5
+        Texture([rgba8(np.array([[[0.9, 0.5, 0.2, 1.0]]]))], srgb=True),  # 6: 1x1
+        Texture(box_mips(rgba8(_smooth_image(seed + 4, 256, 16))), srgb=True),  # 7: wide
+    ]
+    nm = len(mesh.albedo)
+    mt = np.zeros((nm, 3), np.uint32)
+    for m in range(nm):
+        bc = (2, 3, 6, 7)[(m // 6) % 4]
+        mt[m] = [(0, 0, 1), (bc, 0, 1), (bc, 4, 1), (bc, 4, 5), (0, 0, 5), (1, bc, 0)][(m + 3) % 6]
+    # a metallic-roughness texture drives metallic through a factor of 1
+    metallic = np.where(mt[:, 1] >= TEXTURE_FIRST, 1.0, np.asarray(mesh.metallic, np.float64))
+    return Mesh(mesh.positions, mesh.normals, mesh.indices, mesh.tri_material, mesh.albedo, metallic,
+                mesh.roughness, texcoords=uv, tangents=tg, textures=textures, material_textures=mt)
+
+
 class Scene:
     """Device scene (BVH + shading data), wraps vgi_scene_create."""
 
     def __init__(self, mesh: Mesh):
         mats = (Material * len(mesh.albedo))()
+        words = mesh.texture_words()
         for m, (bc, met, rough) in enumerate(mesh.materials()):
             mats[m].base_color[:] = list(bc)
             mats[m].metallic = met
             mats[m].roughness = rough
+            mats[m].color_metalrough_texture = int(words[m, 0])
+            mats[m].normal_texture = int(words[m, 1])
         pos = np.ascontiguousarray(mesh.positions, np.float32)
         nrm = np.ascontiguousarray(mesh.normals, np.float32)
         idx = np.ascontiguousarray(mesh.indices, np.uint32)
         tm = np.ascontiguousarray(mesh.tri_material, np.uint32)
+        uv = None if mesh.texcoords is None else np.ascontiguousarray(mesh.texcoords, np.float32)
+        tg = None if mesh.tangents is None else np.ascontiguousarray(mesh.tangents, np.float32)
+        packed = [t.packed() for t in mesh.textures]
+        texs = (TextureDesc * max(1, len(packed)))()
+        for k, (t, p) in enumerate(zip(mesh.textures, packed)):
+            texs[k] = TextureDesc(t.width, t.height, len(t.levels), int(bool(t.srgb)), p.ctypes.data)
         d = SceneDesc(pos.ctypes.data, nrm.ctypes.data, len(pos), idx.ctypes.data, len(idx), tm.ctypes.data, mats,
-                      len(mats))
+                      len(mats), None if uv is None else uv.ctypes.data, None if tg is None else tg.ctypes.data,
+                      texs, len(packed))
         h = C.c_void_p()
         _lib.call("vgi_scene_create", C.byref(d), C.byref(h))
         self._h = h
