@@ -454,9 +454,13 @@ vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out);
  * through the inverse transpose without normalising, winding reversed under
  * a mirroring transform; a primitive without NORMAL gets per-corner vertices
  * with the geometric normal.  Materials keep the PBR factors (glTF defaults
- * 1); textures are not sampled (stats.textured_materials counts materials
- * that reference one).  Sources: .gltf with relative-file or base64 data-URI
- * buffers, or .glb. */
+ * 1) and their base-colour, metallic-roughness and normal textures: PNG
+ * images are decoded to RGBA8 with violet's Lanczos3 mip chain; other
+ * formats are counted in stats.images_unsupported and leave violet's
+ * default textures in their slots.  TEXCOORD_0 and TANGENT are kept
+ * (tangents are generated from the texture coordinates when absent).
+ * Sources: .gltf with relative-file or base64 data-URI buffers and images,
+ * or .glb. */
 typedef struct vgi_gltf_model vgi_gltf_model;
 typedef struct vgi_gltf_config {
     float scale; /* LoadConfig::scale (model.rs:20-33), default 1 */
@@ -464,6 +468,9 @@ typedef struct vgi_gltf_config {
 typedef struct vgi_gltf_stats {
     uint32_t meshes, primitives, primitives_skipped, instances, materials, textured_materials;
     uint32_t triangles, vertices, truncated_indices;
+    uint32_t images;             /* images the materials use */
+    uint32_t images_unsupported; /* of which not PNG (their slots keep the defaults) */
+    uint32_t textures;           /* textures made (decoded images) */
 } vgi_gltf_stats;
 vgi_status vgi_gltf_default_config(vgi_gltf_config* cfg);
 vgi_status vgi_gltf_import(const char* path, const vgi_gltf_config* cfg, vgi_gltf_model** out);

@@ -72,6 +72,20 @@ class Doc:
         self.j["materials"].append(m)
         return len(self.j["materials"]) - 1
 
+    def image(self, data: bytes, mime: str = "image/png", in_buffer: bool = False) -> int:
+        """An image as a base64 data URI, or in a bufferView of the blob."""
+        if in_buffer:
+            im = {"bufferView": self.view(data), "mimeType": mime}
+        else:
+            im = {"uri": f"data:{mime};base64," + base64.b64encode(data).decode()}
+        self.j.setdefault("images", []).append(im)
+        return len(self.j["images"]) - 1
+
+    def texture(self, image: int) -> int:
+        self.j.setdefault("textures", []).append({"source": image, "sampler": 0})
+        self.j.setdefault("samplers", [{"magFilter": 9729, "minFilter": 9987, "wrapS": 10497, "wrapT": 10497}])
+        return len(self.j["textures"]) - 1
+
     def _json(self, uri):
         j = json.loads(json.dumps(self.j))
         j["buffers"] = [{"byteLength": len(self.blob)}]
@@ -103,19 +117,50 @@ class Doc:
 
 def scene_from_mesh(mesh) -> Doc:
     """A Z-up world-space mesh as a glTF document (Y up: (x, y, z) -> (x, z, -y)),
-    one primitive per material, one node."""
+    one primitive per material, one node.  A textured mesh keeps its texture
+    coordinates and tangents, and its textures' base levels become PNG
+    images (a metallic-roughness texture with R and G moved back to glTF's B
+    and G)."""
+    from png_fixture import encode_png
+
     d = Doc()
     pos = np.stack([mesh.positions[:, 0], mesh.positions[:, 2], -mesh.positions[:, 1]], 1).astype(np.float32)
     nrm = np.stack([mesh.normals[:, 0], mesh.normals[:, 2], -mesh.normals[:, 1]], 1).astype(np.float32)
     pa, na = d.interleaved(pos, nrm)
+    attrs = {"POSITION": pa, "NORMAL": na}
+    if getattr(mesh, "texcoords", None) is not None:
+        attrs["TEXCOORD_0"] = d.accessor(np.ascontiguousarray(mesh.texcoords, np.float32))
+    if getattr(mesh, "tangents", None) is not None:
+        t = mesh.tangents
+        attrs["TANGENT"] = d.accessor(np.stack([t[:, 0], t[:, 2], -t[:, 1], t[:, 3]], 1).astype(np.float32))
+    mt = getattr(mesh, "material_textures", None)
+    tex_index = {}
+
+    def tex(k, metal_rough=False):
+        if (k, metal_rough) not in tex_index:
+            img = np.array(mesh.textures[k - 2].levels[0])
+            if metal_rough:
+                img = img[..., [2, 1, 0, 3]]
+            tex_index[(k, metal_rough)] = d.texture(d.image(encode_png(img, 6, 8)))
+        return tex_index[(k, metal_rough)]
+
     prims = []
     for m, (alb, met, rough) in enumerate(zip(mesh.albedo, mesh.metallic, mesh.roughness)):
-        d.material({"pbrMetallicRoughness": {"baseColorFactor": [float(alb[0]), float(alb[1]), float(alb[2]), 1.0],
-                                             "metallicFactor": float(met), "roughnessFactor": float(rough)}})
+        pbr = {"baseColorFactor": [float(alb[0]), float(alb[1]), float(alb[2]), 1.0], "metallicFactor": float(met),
+               "roughnessFactor": float(rough)}
+        mat = {"pbrMetallicRoughness": pbr}
+        if mt is not None:
+            if mt[m, 0] >= 2:
+                pbr["baseColorTexture"] = {"index": tex(int(mt[m, 0]))}
+            if mt[m, 1] >= 2:
+                pbr["metallicRoughnessTexture"] = {"index": tex(int(mt[m, 1]), metal_rough=True)}
+            if mt[m, 2] >= 2:
+                mat["normalTexture"] = {"index": tex(int(mt[m, 2]))}
+        d.material(mat)
         tri = mesh.indices[mesh.tri_material == m]
         if len(tri):
-            prims.append({"attributes": {"POSITION": pa, "NORMAL": na},
-                          "indices": d.accessor(tri.reshape(-1).astype(np.uint32)), "material": m})
+            prims.append({"attributes": attrs, "indices": d.accessor(tri.reshape(-1).astype(np.uint32)),
+                          "material": m})
     d.node(mesh=d.mesh(prims))
     d.j["scenes"] = [{"nodes": [0]}]
     d.j["scene"] = 0

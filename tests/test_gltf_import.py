@@ -24,6 +24,7 @@ import pytest
 from scipy.spatial.transform import Rotation
 
 from gltf_fixture import Doc, corrupted_documents
+from png_fixture import encode_png
 
 ANG = np.float32(1.57079637)
 
@@ -49,8 +50,7 @@ def _build():
     m_def = d.material({})
     m_tex = d.material({"pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.6, 0.9, 1.0],
                                                  "baseColorTexture": {"index": 0}, "metallicFactor": 0.25}})
-    d.j["textures"] = [{"source": 0}]
-    d.j["images"] = [{"uri": "unused.png"}]
+    d.texture(d.image(encode_png(np.full((3, 5, 4), 200, np.uint8), 6, 8)))
     # mesh 0: a quad with normals, interleaved POSITION/NORMAL, u16 indices
     qp = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0]], np.float32)
     qn = rng.normal(size=(4, 3)).astype(np.float32)

@@ -2,11 +2,13 @@
 sanitizers).
 
 * AddressSanitizer + UndefinedBehaviorSanitizer (gcc, -fno-sanitize-recover):
-  the glTF importer and the scene builder (the host half of vgi_scene_create:
-  validation, binned SAH, 4-wide collapse, node quantisation) run by
+  the glTF importer (with its PNG decoder and mip filter) and the scene
+  builder (the host half of vgi_scene_create: validation, binned SAH, 4-wide
+  collapse, node quantisation, texture tables) run by
   `sanitize/host_fuzz.cpp` on the corrupted-document corpus of the import
-  fuzz test, the reference's own glTF assets when present, and seeded
-  triangle soups (clustered, degenerate, huge, non-finite, bad indices).
+  fuzz test (untextured and textured documents), the reference's own glTF
+  assets when present, and seeded triangle soups (clustered, degenerate,
+  huge, non-finite, bad indices).
   Every built scene is checked: each triangle in exactly one leaf.
 * ThreadSanitizer on the oracle's OpenMP twin (clang with LLVM's OpenMP
   runtime and its Archer tool, which tells ThreadSanitizer about OpenMP's
@@ -48,11 +50,14 @@ def test_host_code_under_address_and_undefined_behaviour_sanitizers(tmp_path):
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
            "-fno-sanitize-recover=all", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", f"-I{ROOT / 'include'}",
            f"-I{csrc}", str(HERE / "sanitize" / "host_fuzz.cpp"), str(csrc / "vgi_gltf.cpp"), str(csrc / "vgi_bvh.cpp"),
+           str(csrc / "vgi_image.cpp"),
            str(csrc / "vgi_runtime.cpp"), f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}", "-o", str(exe)]
     rc, out = _run(cmd)
     assert rc == 0, out[-4000:]
     docs = corrupted_documents(_build()[0], tmp_path / "a", seed=99)
     docs += corrupted_documents(scene_from_mesh(make_mesh(300)), tmp_path / "b", seed=7)
+    # textured: PNG images (decoding, mips) and texture slots in the damaged files
+    docs += corrupted_documents(scene_from_mesh(make_mesh(300, textured=True)), tmp_path / "c", seed=5)
     ref = Path("/root/reference/assets")
     docs += [p for p in (ref / "Box" / "Box.gltf", ref / "Duck" / "Duck.gltf") if p.exists()]
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")

@@ -14,7 +14,20 @@
  *   - the root transform turns glTF's Y-up into violet's Z-up: rotation of
  *     +90 degrees about X times the configured uniform scale (:590-594);
  *   - materials keep the PBR factors (base colour, metallic, roughness;
- *     :700-709).
+ *     :700-709) and their base-colour, metallic-roughness and normal
+ *     textures (:640-702; texture coordinate set 0 only);
+ *   - TEXCOORD_0 and TANGENT are read; a primitive with texture coordinates
+ *     and normals but no TANGENT gets generated tangents (violet runs
+ *     MikkTSpace, :507-531; here per-vertex sums of the triangles' dP/du
+ *     made orthogonal to the normal, E. Lengyel 2001, without MikkTSpace's
+ *     vertex splitting);
+ *   - every image a material uses is decoded once (PNG; vgi_image.h),
+ *     numbered in image order, marked with the uses it has: an image used
+ *     only as a metallic-roughness map has its B and G moved to R and G
+ *     (:811-846), one used only as a base colour is sRGB (:936-941); its mip
+ *     chain is violet's Lanczos3 halving (:853-870).  An image in another
+ *     format (JPEG) is counted in stats.images_unsupported and its slots
+ *     keep violet's defaults (white, flat normal).
  * Instances are baked to world space, the form vgi_scene_create takes: a
  * position goes through the instance's affine transform, a normal through
  * the inverse transpose of its 3x3 part without normalisation (the hit
@@ -24,8 +37,10 @@
  * determinant) reverses the winding so that the baked geometric normal
  * keeps the direction the hit shader derives (:108-112).  A kept primitive
  * without NORMAL gets one vertex per triangle corner carrying the
- * geometric normal.  Textures are not imported: the materials count those
- * that reference any (the hit shading samples none).
+ * geometric normal.  Texture coordinates are copied and tangents go
+ * through the normal transform with their sign w multiplied by the sign of
+ * the determinant (the bitangent the hit shader forms in local space and
+ * then transforms).
  *
  * Sources: a .gltf JSON file with its buffers as relative files or base64
  * data URIs, or a .glb container (JSON chunk + BIN chunk).
@@ -40,6 +55,7 @@
 #include <utility>
 #include <vector>
 
+#include "vgi_image.h"
 #include "vgi_internal.h"
 
 using namespace vgi;
@@ -360,9 +376,12 @@ Mat4 trs(const float t[3], const float q[4], const float s[3]) {
 // the model
 // ---------------------------------------------------------------------------
 struct vgi_gltf_model {
-    std::vector<float> positions, normals;
+    std::vector<float> positions, normals, texcoords, tangents;
+    bool has_texcoords = false, has_tangents = false;
     std::vector<uint32_t> indices, tri_material;
     std::vector<vgi_material> materials;
+    std::vector<std::vector<uint8_t>> texels;  // per texture: every mip level back to back
+    std::vector<vgi_texture> textures;
     vgi_gltf_stats stats{};
 };
 
@@ -389,9 +408,15 @@ struct Importer {
     struct Prim {
         uint32_t material = 0;
         std::vector<uint32_t> idx;  // u16 values
-        std::vector<float> pos, nrm;
+        std::vector<float> pos, nrm, uv, tan;
         bool has_normals = false;
     };
+    // material texture references, as image indices (-1: none)
+    struct MatImages {
+        int64_t base_color = -1, metal_rough = -1, normal = -1;
+    };
+    std::vector<MatImages> mat_images;
+    enum : uint32_t { kUseBaseColor = 1, kUseMetalRough = 2, kUseNormal = 4 };
     std::vector<std::vector<Prim>> meshes;
 
     static uint32_t comp_size(uint32_t t) {
@@ -453,6 +478,34 @@ struct Importer {
         return load_buffers();
     }
 
+    // The bytes a uri names: a base64 data URI or a file relative to the
+    // document (percent-decoded; absolute paths and schemes are refused).
+    vgi_status uri_bytes(const std::string& u, std::vector<uint8_t>& data, const char* what, uint32_t k) {
+        if (u.compare(0, 5, "data:") == 0) {
+            size_t comma = u.find(',');
+            if (comma == std::string::npos || u.rfind(";base64", comma) == std::string::npos)
+                return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: %s %u: only base64 data URIs", what, k);
+            if (!base64_decode(u.data() + comma + 1, u.size() - comma - 1, data))
+                return set_error(VGI_E_INVALID, "vgi_gltf_import: %s %u: bad base64", what, k);
+            return VGI_OK;
+        }
+        if (u.find("://") != std::string::npos || (!u.empty() && u[0] == '/'))
+            return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: %s %u: only relative file URIs", what, k);
+        std::string path;
+        for (size_t i = 0; i < u.size(); ++i) {  // percent-decoding of the relative path
+            if (u[i] == '%' && i + 2 < u.size()) {
+                char hx[3] = {u[i + 1], u[i + 2], 0};
+                path += (char)strtol(hx, nullptr, 16);
+                i += 2;
+            } else {
+                path += u[i];
+            }
+        }
+        if (!read_file(dir + path, data))
+            return set_error(VGI_E_INVALID, "vgi_gltf_import: cannot read %s '%s'", what, (dir + path).c_str());
+        return VGI_OK;
+    }
+
     vgi_status load_buffers() {
         const JVal* bs = doc.get("buffers");
         if (!bs) return VGI_OK;
@@ -468,29 +521,8 @@ struct Importer {
                 data.assign(glb_bin, glb_bin + glb_bin_len);
             } else {
                 if (uri->kind != JVal::kStr) return set_error(VGI_E_INVALID, "vgi_gltf_import: buffer %zu: uri", k);
-                const std::string& u = uri->str;
-                if (u.compare(0, 5, "data:") == 0) {
-                    size_t comma = u.find(',');
-                    if (comma == std::string::npos || u.rfind(";base64", comma) == std::string::npos)
-                        return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: buffer %zu: only base64 data URIs", k);
-                    if (!base64_decode(u.data() + comma + 1, u.size() - comma - 1, data))
-                        return set_error(VGI_E_INVALID, "vgi_gltf_import: buffer %zu: bad base64", k);
-                } else {
-                    if (u.find("://") != std::string::npos || (!u.empty() && u[0] == '/'))
-                        return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: buffer %zu: only relative file URIs", k);
-                    std::string path;
-                    for (size_t i = 0; i < u.size(); ++i) {  // percent-decoding of the relative path
-                        if (u[i] == '%' && i + 2 < u.size()) {
-                            char hx[3] = {u[i + 1], u[i + 2], 0};
-                            path += (char)strtol(hx, nullptr, 16);
-                            i += 2;
-                        } else {
-                            path += u[i];
-                        }
-                    }
-                    if (!read_file(dir + path, data))
-                        return set_error(VGI_E_INVALID, "vgi_gltf_import: cannot read buffer '%s'", (dir + path).c_str());
-                }
+                vgi_status s = uri_bytes(uri->str, data, "buffer", (uint32_t)k);
+                if (s != VGI_OK) return s;
             }
             if (data.size() < len) return set_error(VGI_E_INVALID, "vgi_gltf_import: buffer %zu shorter than byteLength", k);
             data.resize(len);
@@ -621,6 +653,27 @@ struct Importer {
                         for (uint32_t c = 0; c < 3; ++c) p.nrm[3 * k + c] = read_f(na, k, c);
                     p.has_normals = true;
                 }
+                uint32_t tacc = 0, gacc = 0;
+                Accessor ta, ga;
+                if (as_u32(attrs->get("TEXCOORD_0"), tacc)) {
+                    if ((s = accessor(tacc, ta)) != VGI_OK) return s;
+                    if (ta.ncomp != 2 || ta.count != pa.count ||
+                        !(ta.comp_type == 5126 || (ta.normalized && (ta.comp_type == 5121 || ta.comp_type == 5123))))
+                        return set_error(VGI_E_INVALID, "vgi_gltf_import: mesh %zu primitive %zu: TEXCOORD_0", mi, pi);
+                    p.uv.resize(2 * ta.count);
+                    for (size_t k = 0; k < ta.count; ++k)
+                        for (uint32_t c = 0; c < 2; ++c) p.uv[2 * k + c] = read_f(ta, k, c);
+                }
+                if (as_u32(attrs->get("TANGENT"), gacc)) {
+                    if ((s = accessor(gacc, ga)) != VGI_OK) return s;
+                    if (ga.ncomp != 4 || ga.count != pa.count || ga.comp_type != 5126)
+                        return set_error(VGI_E_INVALID, "vgi_gltf_import: mesh %zu primitive %zu: TANGENT", mi, pi);
+                    p.tan.resize(4 * ga.count);
+                    for (size_t k = 0; k < ga.count; ++k)
+                        for (uint32_t c = 0; c < 4; ++c) p.tan[4 * k + c] = read_f(ga, k, c);
+                } else if (!p.uv.empty() && p.has_normals) {
+                    generate_tangents(p);
+                }
                 ++st.primitives;
                 group.push_back(std::move(p));
             }
@@ -640,6 +693,8 @@ struct Importer {
             o.metallic = 1.0f;  // glTF defaults of the PBR factors
             o.roughness = 1.0f;
             bool textured = mt.get("normalTexture") || mt.get("occlusionTexture") || mt.get("emissiveTexture");
+            MatImages mi;
+            vgi_status s = texture_image(mt.get("normalTexture"), mi.normal);
             if (const JVal* pbr = mt.get("pbrMetallicRoughness")) {
                 const JVal* bc = pbr->get("baseColorFactor");
                 if (bc && bc->kind == JVal::kArr && bc->arr.size() == 4)
@@ -650,12 +705,160 @@ struct Importer {
                 const JVal* rf = pbr->get("roughnessFactor");
                 if (rf && rf->is_num()) o.roughness = (float)rf->num;
                 textured = textured || pbr->get("baseColorTexture") || pbr->get("metallicRoughnessTexture");
+                if (s == VGI_OK) s = texture_image(pbr->get("baseColorTexture"), mi.base_color);
+                if (s == VGI_OK) s = texture_image(pbr->get("metallicRoughnessTexture"), mi.metal_rough);
             }
+            if (s != VGI_OK) return s;
             if (textured) ++m.stats.textured_materials;
+            o.color_metalrough_texture = 0;
+            o.normal_texture = VGI_TEXTURE_NORMAL_UP;
             m.materials.push_back(o);
+            mat_images.push_back(mi);
         }
         m.stats.materials = (uint32_t)m.materials.size();
         return VGI_OK;
+    }
+
+    // textureInfo -> the image index of its texture's source (-1 if absent).
+    vgi_status texture_image(const JVal* info, int64_t& image) {
+        image = -1;
+        if (!info) return VGI_OK;
+        uint32_t t = 0, src = 0;
+        const JVal* texs = doc.get("textures");
+        if (!as_u32(info->get("index"), t) || !texs || texs->kind != JVal::kArr || t >= texs->arr.size())
+            return set_error(VGI_E_INVALID, "vgi_gltf_import: material texture index");
+        const JVal* imgs = doc.get("images");
+        if (!as_u32(texs->arr[t].get("source"), src) || !imgs || imgs->kind != JVal::kArr || src >= imgs->arr.size())
+            return set_error(VGI_E_INVALID, "vgi_gltf_import: texture %u has no image source", t);
+        image = src;
+        return VGI_OK;
+    }
+
+    // The encoded bytes of image k (a data URI, a relative file or a bufferView).
+    vgi_status image_bytes(uint32_t k, std::vector<uint8_t>& out) {
+        const JVal& im = doc.get("images")->arr[k];
+        uint32_t bv = 0;
+        if (const JVal* uri = im.get("uri")) {
+            if (uri->kind != JVal::kStr) return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: uri", k);
+            return uri_bytes(uri->str, out, "image", k);
+        }
+        if (!as_u32(im.get("bufferView"), bv)) return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u has no source", k);
+        const JVal* views = doc.get("bufferViews");
+        uint32_t buf = 0, off = 0, len = 0;
+        if (!views || views->kind != JVal::kArr || bv >= views->arr.size() || !as_u32(views->arr[bv].get("buffer"), buf) ||
+            !as_u32(views->arr[bv].get("byteLength"), len) || buf >= buffers.size() ||
+            (views->arr[bv].get("byteOffset") && !as_u32(views->arr[bv].get("byteOffset"), off)) ||
+            (uint64_t)off + len > buffers[buf].size())
+            return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: bufferView %u", k, bv);
+        out.assign(buffers[buf].begin() + off, buffers[buf].begin() + off + len);
+        return VGI_OK;
+    }
+
+    // Decodes every image the materials use and sets their texture indices
+    // (model.rs:743-790: used images renumbered in image order).
+    vgi_status load_images(vgi_gltf_model& m) {
+        const JVal* imgs = doc.get("images");
+        const size_t nimg = imgs && imgs->kind == JVal::kArr ? imgs->arr.size() : 0;
+        std::vector<uint32_t> use(nimg, 0);
+        for (const MatImages& mi : mat_images) {
+            if (mi.base_color >= 0) use[mi.base_color] |= kUseBaseColor;
+            if (mi.metal_rough >= 0) use[mi.metal_rough] |= kUseMetalRough;
+            if (mi.normal >= 0) use[mi.normal] |= kUseNormal;
+        }
+        std::vector<int64_t> tex_of(nimg, -1);  // texture index, -1: unused or undecodable
+        for (uint32_t k = 0; k < nimg; ++k) {
+            if (!use[k]) continue;
+            ++m.stats.images;
+            std::vector<uint8_t> bytes;
+            vgi_status s = image_bytes(k, bytes);
+            if (s != VGI_OK) return s;
+            if (!is_png(bytes.data(), bytes.size())) {
+                ++m.stats.images_unsupported;
+                continue;
+            }
+            Rgba8Image img;
+            std::string err;
+            if (!decode_png(bytes.data(), bytes.size(), img, err))
+                return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: %s", k, err.c_str());
+            if (img.width > 32768u || img.height > 32768u)
+                return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: image %u is %ux%u (at most 32768 a side)", k,
+                                 img.width, img.height);
+            if (use[k] == kUseMetalRough)  // R: metallic (glTF B), G: roughness (glTF G)
+                for (size_t i = 0; i < img.texels.size(); i += 4) std::swap(img.texels[i], img.texels[i + 2]);
+            const std::vector<Rgba8Image> mips = mip_chain(img);
+            std::vector<uint8_t> all;
+            for (const Rgba8Image& l : mips) all.insert(all.end(), l.texels.begin(), l.texels.end());
+            tex_of[k] = (int64_t)m.textures.size();
+            m.texels.push_back(std::move(all));
+            vgi_texture t{};
+            t.width = img.width;
+            t.height = img.height;
+            t.levels = (uint32_t)mips.size();
+            t.srgb = use[k] == kUseBaseColor ? 1u : 0u;
+            m.textures.push_back(t);
+        }
+        for (size_t t = 0; t < m.textures.size(); ++t) m.textures[t].texels = m.texels[t].data();
+        auto slot = [&](int64_t img, uint32_t dflt) {
+            return img >= 0 && tex_of[img] >= 0 ? VGI_TEXTURE_FIRST + (uint32_t)tex_of[img] : dflt;
+        };
+        for (size_t i = 0; i < mat_images.size(); ++i) {
+            const MatImages& mi = mat_images[i];
+            m.materials[i].color_metalrough_texture =
+                slot(mi.base_color, VGI_TEXTURE_WHITE) | (slot(mi.metal_rough, VGI_TEXTURE_WHITE) << 16);
+            m.materials[i].normal_texture = slot(mi.normal, VGI_TEXTURE_NORMAL_UP);
+        }
+        m.stats.textures = (uint32_t)m.textures.size();
+        return VGI_OK;
+    }
+
+    // Tangents from the texture coordinates (E. Lengyel, "Computing Tangent
+    // Space Basis Vectors for an Arbitrary Mesh", 2001): per vertex, the sum
+    // of its triangles' dP/du and dP/dv; the tangent is dP/du made
+    // orthogonal to the normal and normalised, w the sign of the bitangent
+    // against dP/dv.  A vertex without a usable direction gets any unit
+    // vector orthogonal to its normal.
+    static void generate_tangents(Prim& p) {
+        const size_t nv = p.pos.size() / 3;
+        std::vector<double> T(3 * nv, 0.0), B(3 * nv, 0.0);
+        for (size_t t = 0; t + 2 < p.idx.size(); t += 3) {
+            const uint32_t i[3] = {p.idx[t], p.idx[t + 1], p.idx[t + 2]};
+            double e1[3], e2[3];
+            for (int c = 0; c < 3; ++c) {
+                e1[c] = (double)p.pos[3 * i[1] + c] - p.pos[3 * i[0] + c];
+                e2[c] = (double)p.pos[3 * i[2] + c] - p.pos[3 * i[0] + c];
+            }
+            const double d1u = (double)p.uv[2 * i[1]] - p.uv[2 * i[0]], d1v = (double)p.uv[2 * i[1] + 1] - p.uv[2 * i[0] + 1];
+            const double d2u = (double)p.uv[2 * i[2]] - p.uv[2 * i[0]], d2v = (double)p.uv[2 * i[2] + 1] - p.uv[2 * i[0] + 1];
+            const double r = d1u * d2v - d2u * d1v;
+            if (!(std::fabs(r) > 1e-20)) continue;
+            const double inv = 1.0 / r;
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) {
+                    T[3 * i[k] + c] += (e1[c] * d2v - e2[c] * d1v) * inv;
+                    B[3 * i[k] + c] += (e2[c] * d1u - e1[c] * d2u) * inv;
+                }
+        }
+        p.tan.assign(4 * nv, 0.0f);
+        for (size_t v = 0; v < nv; ++v) {
+            double n[3] = {p.nrm[3 * v], p.nrm[3 * v + 1], p.nrm[3 * v + 2]};
+            const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (nl > 0)
+                for (double& c : n) c /= nl;
+            const double dn = n[0] * T[3 * v] + n[1] * T[3 * v + 1] + n[2] * T[3 * v + 2];
+            double t[3] = {T[3 * v] - n[0] * dn, T[3 * v + 1] - n[1] * dn, T[3 * v + 2] - n[2] * dn};
+            double tl = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            if (!(tl > 1e-12)) {  // any unit vector orthogonal to n
+                const double a[3] = {std::fabs(n[0]) < 0.9 ? 1.0 : 0.0, std::fabs(n[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+                t[0] = n[1] * a[2] - n[2] * a[1];
+                t[1] = n[2] * a[0] - n[0] * a[2];
+                t[2] = n[0] * a[1] - n[1] * a[0];
+                tl = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            }
+            const double cx = n[1] * t[2] - n[2] * t[1], cy = n[2] * t[0] - n[0] * t[2], cz = n[0] * t[1] - n[1] * t[0];
+            const double hand = cx * B[3 * v] + cy * B[3 * v + 1] + cz * B[3 * v + 2];
+            for (int c = 0; c < 3; ++c) p.tan[4 * v + c] = (float)(t[c] / tl);
+            p.tan[4 * v + 3] = hand < 0.0 ? -1.0f : 1.0f;
+        }
     }
 
     vgi_status node_local(const JVal& n, Mat4& out) {
@@ -697,6 +900,18 @@ struct Importer {
         };
         const bool mirror = det < 0.0f;
         const size_t ntri = p.idx.size() / 3;
+        auto put_uv_tan = [&](uint32_t v) {  // texture coordinate and tangent of source vertex v
+            const float uv[2] = {p.uv.empty() ? 0.0f : p.uv[2 * v], p.uv.empty() ? 0.0f : p.uv[2 * v + 1]};
+            m.texcoords.insert(m.texcoords.end(), uv, uv + 2);
+            float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (!p.tan.empty()) {
+                xn(&p.tan[4 * v], t);
+                t[3] = mirror ? -p.tan[4 * v + 3] : p.tan[4 * v + 3];
+            }
+            m.tangents.insert(m.tangents.end(), t, t + 4);
+        };
+        m.has_texcoords = m.has_texcoords || !p.uv.empty();
+        m.has_tangents = m.has_tangents || !p.tan.empty();
         if (p.has_normals) {
             const uint32_t base = (uint32_t)(m.positions.size() / 3);
             const size_t nv = p.pos.size() / 3;
@@ -706,6 +921,7 @@ struct Importer {
                 xn(&p.nrm[3 * k], n);
                 m.positions.insert(m.positions.end(), o, o + 3);
                 m.normals.insert(m.normals.end(), n, n + 3);
+                put_uv_tan((uint32_t)k);
             }
             for (size_t t = 0; t < ntri; ++t) {
                 uint32_t i0 = p.idx[3 * t], i1 = p.idx[3 * t + 1], i2 = p.idx[3 * t + 2];
@@ -731,6 +947,7 @@ struct Importer {
                     m.positions.insert(m.positions.end(), w[k], w[k] + 3);
                     m.normals.insert(m.normals.end(), gn, gn + 3);
                     m.indices.push_back(base + (uint32_t)k);
+                    put_uv_tan(c[k]);
                 }
                 m.tri_material.push_back(p.material);
             }
@@ -790,6 +1007,7 @@ struct Importer {
     vgi_status run(const char* path, float scale, vgi_gltf_model& m) {
         vgi_status s = load(path);
         if (s == VGI_OK) s = load_materials(m);
+        if (s == VGI_OK) s = load_images(m);
         if (s == VGI_OK) s = load_meshes(m.stats);
         if (s != VGI_OK) return s;
         uint32_t scene = 0;
@@ -856,6 +1074,10 @@ extern "C" vgi_status vgi_gltf_model_desc(const vgi_gltf_model* m, vgi_scene_des
     desc->triangle_material = m->tri_material.data();
     desc->materials = m->materials.data();
     desc->num_materials = (uint32_t)m->materials.size();
+    desc->texcoords = m->has_texcoords ? m->texcoords.data() : nullptr;
+    desc->tangents = m->has_tangents ? m->tangents.data() : nullptr;
+    desc->textures = m->textures.data();
+    desc->num_textures = (uint32_t)m->textures.size();
     return VGI_OK;
 }
 

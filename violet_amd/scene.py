@@ -109,7 +109,8 @@ class GltfConfig(C.Structure):
 
 class GltfStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("meshes", "primitives", "primitives_skipped", "instances", "materials",
-                                           "textured_materials", "triangles", "vertices", "truncated_indices")]
+                                           "textured_materials", "triangles", "vertices", "truncated_indices",
+                                           "images", "images_unsupported", "textures")]
 
 
 _lib.register("vgi_gltf_default_config", C.c_int32, [C.POINTER(GltfConfig)])
@@ -418,8 +419,24 @@ def load_gltf(path, scale: float = 1.0) -> tuple[Mesh, GltfStats]:
         idx = arr(d.indices, 3 * nt, np.uint32, (nt, 3))
         tm = arr(d.triangle_material, nt, np.uint32, (nt,))
         mats = [d.materials[k] for k in range(nm)]
+        uv = arr(d.texcoords, 2 * nv, np.float32, (nv, 2)) if d.texcoords else None
+        tg = arr(d.tangents, 4 * nv, np.float32, (nv, 4)) if d.tangents else None
+        textures = []
+        for k in range(d.num_textures):
+            t = d.textures[k]
+            levels, off = [], 0
+            total = sum(max(1, t.width >> l) * max(1, t.height >> l) for l in range(t.levels)) * 4
+            raw = arr(t.texels, total, np.uint8, (total,))
+            for l in range(t.levels):
+                lw, lh = max(1, t.width >> l), max(1, t.height >> l)
+                levels.append(raw[off:off + lw * lh * 4].reshape(lh, lw, 4))
+                off += lw * lh * 4
+            textures.append(Texture(levels, srgb=bool(t.srgb)))
+        mt = np.array([[m.color_metalrough_texture & 0xFFFF, m.color_metalrough_texture >> 16,
+                        m.normal_texture & 0xFFFF] for m in mats], np.uint32).reshape(nm, 3)
         mesh = Mesh(pos, nrm, idx, tm, np.array([list(m.base_color)[:3] for m in mats], np.float32).reshape(nm, 3),
-                    np.array([m.metallic for m in mats], np.float32), np.array([m.roughness for m in mats], np.float32))
+                    np.array([m.metallic for m in mats], np.float32), np.array([m.roughness for m in mats], np.float32),
+                    texcoords=uv, tangents=tg, textures=textures, material_textures=mt)
         return mesh, st
     finally:
         _lib.lib().vgi_gltf_destroy(h)
