@@ -455,9 +455,10 @@ vgi_status vgi_scene_info(const vgi_scene* s, vgi_scene_stats* out);
  * a mirroring transform; a primitive without NORMAL gets per-corner vertices
  * with the geometric normal.  Materials keep the PBR factors (glTF defaults
  * 1) and their base-colour, metallic-roughness and normal textures: PNG
- * images are decoded to RGBA8 with violet's Lanczos3 mip chain; other
- * formats are counted in stats.images_unsupported and leave violet's
- * default textures in their slots.  TEXCOORD_0 and TANGENT are kept
+ * and baseline JPEG images are decoded to RGBA8 with violet's Lanczos3 mip
+ * chain; other forms (progressive JPEG, ...) are counted in
+ * stats.images_unsupported and leave violet's default textures in their
+ * slots.  TEXCOORD_0 and TANGENT are kept
  * (tangents are generated from the texture coordinates when absent).
  * Sources: .gltf with relative-file or base64 data-URI buffers and images,
  * or .glb. */
@@ -469,7 +470,7 @@ typedef struct vgi_gltf_stats {
     uint32_t meshes, primitives, primitives_skipped, instances, materials, textured_materials;
     uint32_t triangles, vertices, truncated_indices;
     uint32_t images;             /* images the materials use */
-    uint32_t images_unsupported; /* of which not PNG (their slots keep the defaults) */
+    uint32_t images_unsupported; /* of which not decodable here (their slots keep the defaults) */
     uint32_t textures;           /* textures made (decoded images) */
 } vgi_gltf_stats;
 vgi_status vgi_gltf_default_config(vgi_gltf_config* cfg);

@@ -115,12 +115,12 @@ class Doc:
         return path
 
 
-def scene_from_mesh(mesh) -> Doc:
+def scene_from_mesh(mesh, jpeg: bool = False) -> Doc:
     """A Z-up world-space mesh as a glTF document (Y up: (x, y, z) -> (x, z, -y)),
     one primitive per material, one node.  A textured mesh keeps its texture
     coordinates and tangents, and its textures' base levels become PNG
     images (a metallic-roughness texture with R and G moved back to glTF's B
-    and G)."""
+    and G); jpeg=True stores base-colour images as baseline JPEG."""
     from png_fixture import encode_png
 
     d = Doc()
@@ -141,7 +141,16 @@ def scene_from_mesh(mesh) -> Doc:
             img = np.array(mesh.textures[k - 2].levels[0])
             if metal_rough:
                 img = img[..., [2, 1, 0, 3]]
-            tex_index[(k, metal_rough)] = d.texture(d.image(encode_png(img, 6, 8)))
+            if jpeg and mesh.textures[k - 2].srgb and not metal_rough:
+                import io
+
+                from PIL import Image
+
+                b = io.BytesIO()
+                Image.fromarray(np.ascontiguousarray(img[..., :3])).save(b, "JPEG", quality=90, restart_marker_blocks=4)
+                tex_index[(k, metal_rough)] = d.texture(d.image(b.getvalue(), mime="image/jpeg"))
+            else:
+                tex_index[(k, metal_rough)] = d.texture(d.image(encode_png(img, 6, 8)))
         return tex_index[(k, metal_rough)]
 
     prims = []

@@ -1,6 +1,6 @@
 """A glTF scene through the whole path on the GPU: the seeded test scene
-(untextured, and textured: texture coordinates, tangents and PNG base
-colour / metallic-roughness / normal images) written as .glb (Y up),
+(untextured, and textured: texture coordinates, tangents and PNG or JPEG
+base colour with PNG metallic-roughness / normal images) written as .glb (Y up),
 imported by vgi_gltf_import (Z up, world space, decoded images with
 Lanczos3 mips) and rendered with composition; the frame's depth, G-buffer,
 sun shadow mask and first-frame GI samples equal the FP32 twin's on the
@@ -18,16 +18,16 @@ pytestmark = pytest.mark.gpu
 W, H = 96, 64
 
 
-@pytest.mark.parametrize("textured", [False, True])
+@pytest.mark.parametrize("textured", ["none", "png", "jpeg"])
 def test_glb_scene_renders_like_the_twin(tmp_path, textured):
     import oracle
     from violet_amd import renderer, scene
 
-    src = make_mesh(textured=textured)
-    path = scene_from_mesh(src).save_glb(tmp_path / "scene.glb")
+    src = make_mesh(textured=textured != "none")
+    path = scene_from_mesh(src, jpeg=textured == "jpeg").save_glb(tmp_path / "scene.glb")
     mesh, st = scene.load_gltf(path)
     assert st.triangles == src.num_triangles and st.instances == 1
-    if textured:
+    if textured != "none":
         assert st.textures >= len(src.textures) and st.images_unsupported == 0 and mesh.tangents is not None
     # Y-up -> Z-up by the f32 rotation of pi/2: the positions come back to rounding
     np.testing.assert_allclose(mesh.positions[mesh.indices], src.positions[src.indices], atol=1e-5)

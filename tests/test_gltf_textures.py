@@ -13,8 +13,14 @@
 * Material slots (model.rs:636-790, render_scene.rs:920-935): used images
   numbered in image order, an image used only as a metallic-roughness map
   swizzled (B, G -> R, G), only base colour marked sRGB, an image used for
-  two purposes neither; default slots white / white / flat normal; a JPEG
-  counted as unsupported with its slot left at the default.
+  two purposes neither; default slots white / white / flat normal; a
+  progressive JPEG counted as unsupported with its slot left at the default.
+* Baseline JPEG decoding against PIL (libjpeg): 4:4:4, 4:2:2 and 4:2:0
+  chroma, greyscale, restart intervals, odd sizes and two qualities agree
+  within 3 codes (mean below 0.1: decoders differ by their inverse-DCT and
+  colour-conversion rounding; the chroma upsampling is the same triangle
+  filter), as do violet's Sponza textures when the reference tree is
+  present; damaged files are refused or decoded, never read out of bounds.
 * Texture coordinates and generated tangents: a plane with u along +x
   gets tangents +x (w = +1), and w = -1 when v runs the other way.
 * violet's own Duck asset (when the reference tree is present): the
@@ -32,6 +38,46 @@ from gltf_fixture import Doc
 from png_fixture import encode_png, expected_rgba8
 
 REF_ASSETS = Path("/root/reference/assets")
+
+
+def _jpeg(rgb, quality=90, subsampling=2, progressive=False, restart=0):
+    import io
+
+    from PIL import Image
+
+    b = io.BytesIO()
+    mode = "L" if rgb.ndim == 2 else "RGB"
+    kw = dict(quality=quality, progressive=progressive)
+    if mode == "RGB":
+        kw["subsampling"] = subsampling
+    if restart:
+        kw["restart_marker_blocks"] = restart
+    Image.fromarray(np.ascontiguousarray(rgb, np.uint8), mode).save(b, "JPEG", **kw)
+    return b.getvalue()
+
+
+def _pil_rgba(data: bytes) -> np.ndarray:
+    import io
+
+    from PIL import Image
+
+    return np.asarray(Image.open(io.BytesIO(data)).convert("RGBA"))
+
+
+def _images_doc(images: list[tuple[bytes, str]]):
+    """One triangle per image, each with a base-colour texture of it."""
+    d = Doc()
+    pa = d.accessor(np.array([[0, 0, 0], [1, 0, 0], [1, 0, -1]], np.float32))
+    ia = d.accessor(np.array([0, 1, 2], np.uint16))
+    prims = []
+    for data, mime in images:
+        t = d.texture(d.image(data, mime=mime))
+        prims.append({"attributes": {"POSITION": pa}, "indices": ia,
+                      "material": d.material({"pbrMetallicRoughness": {"baseColorTexture": {"index": t}}})})
+    d.node(mesh=d.mesh(prims))
+    d.j["scenes"] = [{"nodes": [0]}]
+    d.j["scene"] = 0
+    return d
 
 
 def _quad_doc(materials: list[dict], uv=None):
@@ -183,9 +229,9 @@ def test_material_slots_swizzle_srgb_and_defaults(tmp_path):
          "normalTexture": {"index": 3}},
         {"pbrMetallicRoughness": {"baseColorTexture": {"index": 4}}, "normalTexture": {"index": 4}},  # two uses
         {"pbrMetallicRoughness": {"baseColorFactor": [0.5, 0.5, 0.5, 1.0]}},  # untextured
-        {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}},  # a JPEG
+        {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}},  # a progressive JPEG
     ])
-    d.texture(d.image(b"\xff\xd8\xff\xe0" + bytes(64), mime="image/jpeg"))  # texture 0 -> image 0 (JPEG)
+    d.texture(d.image(_jpeg(imgs[0][..., :3], progressive=True), mime="image/jpeg"))  # texture 0 -> image 0
     for k in range(4):  # textures 1..4 -> images 1..4
         d.texture(d.image(encode_png(imgs[k], 6, 8)))
     d.image(encode_png(imgs[4], 6, 8))  # image 5: not used by any material
@@ -247,3 +293,63 @@ def test_duck_asset_texture():
     assert np.abs(tl / nl - 1).max() < 1e-3
     cos = np.sum(mesh.tangents[:, :3] * mesh.normals, axis=1) / (tl * nl)
     assert np.abs(cos).max() < 1e-3
+
+
+def _smooth_rgb(rng, h, w):
+    g = rng.random((h // 8 + 2, w // 8 + 2, 3))
+    img = np.kron(g, np.ones((8, 8, 1)))[:h, :w]
+    return (np.clip(img * 0.85 + 0.15 * rng.random((h, w, 3)), 0, 1) * 255).astype(np.uint8)
+
+
+def test_baseline_jpeg_matches_libjpeg(tmp_path):
+    rng = np.random.default_rng(11)
+    cases = []
+    for (h, w) in ((33, 47), (17, 9), (64, 64), (120, 200)):
+        for sub in (0, 1, 2):
+            for q, rst in ((50, 3), (95, 0)):
+                cases.append(_jpeg(_smooth_rgb(rng, h, w), quality=q, subsampling=sub, restart=rst))
+        cases.append(_jpeg(_smooth_rgb(rng, h, w)[..., 1], quality=80))
+    mesh, st = _import(_images_doc([(c, "image/jpeg") for c in cases]), tmp_path, form="glb")
+    assert (st.images, st.textures, st.images_unsupported) == (len(cases), len(cases), 0)
+    for k, c in enumerate(cases):
+        got, want = mesh.textures[k].levels[0].astype(int), _pil_rgba(c).astype(int)
+        assert got.shape == want.shape
+        dd = np.abs(got - want)
+        assert dd.max() <= 3 and dd.mean() < 0.1, (k, dd.max(), dd.mean())
+
+
+def test_damaged_jpeg_is_refused_or_decoded(tmp_path):
+    from violet_amd import _lib
+
+    rng = np.random.default_rng(4)
+    good = _jpeg(_smooth_rgb(rng, 40, 56), quality=85, restart=2)
+    with pytest.raises(_lib.VgiError):
+        _import(_images_doc([(good[:len(good) // 3], "image/jpeg")]), tmp_path)
+    for k in range(60):
+        b = bytearray(good)
+        for _ in range(int(rng.integers(1, 8))):
+            b[int(rng.integers(2, len(b)))] = int(rng.integers(0, 256))
+        try:
+            _import(_images_doc([(bytes(b), "image/jpeg")]), tmp_path)
+        except _lib.VgiError:
+            pass
+
+
+def test_sponza_jpeg_textures():
+    import glob
+
+    from violet_amd import _lib  # noqa: F401
+
+    files = sorted(glob.glob(str(REF_ASSETS / "Sponza" / "*.jpg")))[:4]
+    if not files:
+        pytest.skip("reference assets absent")
+    import tempfile
+
+    tmp = Path(tempfile.mkdtemp())
+    data = [Path(f).read_bytes() for f in files]
+    mesh, st = _import(_images_doc([(b, "image/jpeg") for b in data]), tmp, form="glb")
+    assert st.textures == len(files)
+    for k, b in enumerate(data):
+        got, want = mesh.textures[k].levels[0].astype(int), _pil_rgba(b).astype(int)
+        dd = np.abs(got - want)
+        assert dd.max() <= 3 and dd.mean() < 0.1, (files[k], dd.max(), dd.mean())

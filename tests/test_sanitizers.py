@@ -2,11 +2,11 @@
 sanitizers).
 
 * AddressSanitizer + UndefinedBehaviorSanitizer (gcc, -fno-sanitize-recover):
-  the glTF importer (with its PNG decoder and mip filter) and the scene
+  the glTF importer (with its PNG and JPEG decoders and mip filter) and the scene
   builder (the host half of vgi_scene_create: validation, binned SAH, 4-wide
   collapse, node quantisation, texture tables) run by
   `sanitize/host_fuzz.cpp` on the corrupted-document corpus of the import
-  fuzz test (untextured and textured documents), the reference's own glTF
+  fuzz test (untextured, PNG- and JPEG-textured documents), the reference's own glTF
   assets when present, and seeded triangle soups (clustered, degenerate,
   huge, non-finite, bad indices).
   Every built scene is checked: each triangle in exactly one leaf.
@@ -58,6 +58,7 @@ def test_host_code_under_address_and_undefined_behaviour_sanitizers(tmp_path):
     docs += corrupted_documents(scene_from_mesh(make_mesh(300)), tmp_path / "b", seed=7)
     # textured: PNG images (decoding, mips) and texture slots in the damaged files
     docs += corrupted_documents(scene_from_mesh(make_mesh(300, textured=True)), tmp_path / "c", seed=5)
+    docs += corrupted_documents(scene_from_mesh(make_mesh(300, textured=True), jpeg=True), tmp_path / "d", seed=6)
     ref = Path("/root/reference/assets")
     docs += [p for p in (ref / "Box" / "Box.gltf", ref / "Duck" / "Duck.gltf") if p.exists()]
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")

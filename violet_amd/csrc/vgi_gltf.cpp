@@ -21,13 +21,14 @@
  *     MikkTSpace, :507-531; here per-vertex sums of the triangles' dP/du
  *     made orthogonal to the normal, E. Lengyel 2001, without MikkTSpace's
  *     vertex splitting);
- *   - every image a material uses is decoded once (PNG; vgi_image.h),
+ *   - every image a material uses is decoded once (PNG, baseline JPEG;
+ *     vgi_image.h),
  *     numbered in image order, marked with the uses it has: an image used
  *     only as a metallic-roughness map has its B and G moved to R and G
  *     (:811-846), one used only as a base colour is sRGB (:936-941); its mip
  *     chain is violet's Lanczos3 halving (:853-870).  An image in another
- *     format (JPEG) is counted in stats.images_unsupported and its slots
- *     keep violet's defaults (white, flat normal).
+ *     format (progressive JPEG, ...) is counted in stats.images_unsupported
+ *     and its slots keep violet's defaults (white, flat normal).
  * Instances are baked to world space, the form vgi_scene_create takes: a
  * position goes through the instance's affine transform, a normal through
  * the inverse transpose of its 3x3 part without normalisation (the hit
@@ -772,14 +773,22 @@ struct Importer {
             std::vector<uint8_t> bytes;
             vgi_status s = image_bytes(k, bytes);
             if (s != VGI_OK) return s;
-            if (!is_png(bytes.data(), bytes.size())) {
+            Rgba8Image img;
+            std::string err;
+            if (is_png(bytes.data(), bytes.size())) {
+                if (!decode_png(bytes.data(), bytes.size(), img, err))
+                    return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: %s", k, err.c_str());
+            } else if (is_jpeg(bytes.data(), bytes.size())) {
+                const int r = decode_jpeg(bytes.data(), bytes.size(), img, err);
+                if (r < 0) return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: %s", k, err.c_str());
+                if (r == 0) {  // a JPEG form this decoder does not read (progressive, ...)
+                    ++m.stats.images_unsupported;
+                    continue;
+                }
+            } else {
                 ++m.stats.images_unsupported;
                 continue;
             }
-            Rgba8Image img;
-            std::string err;
-            if (!decode_png(bytes.data(), bytes.size(), img, err))
-                return set_error(VGI_E_INVALID, "vgi_gltf_import: image %u: %s", k, err.c_str());
             if (img.width > 32768u || img.height > 32768u)
                 return set_error(VGI_E_UNSUPPORTED, "vgi_gltf_import: image %u is %ux%u (at most 32768 a side)", k,
                                  img.width, img.height);

@@ -398,6 +398,364 @@ bool decode_png(const uint8_t* d, size_t n, Rgba8Image& out, std::string& err, u
 }
 
 // ---------------------------------------------------------------------------
+// Baseline JPEG (ITU-T T.81, sequential DCT with Huffman coding)
+// ---------------------------------------------------------------------------
+namespace {
+
+// zig-zag scan position -> natural (row-major) coefficient index (T.81 A.3.6)
+constexpr uint8_t kZigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
+                                 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6,  7,  14, 21, 28,
+                                 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+                                 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+struct JHuff {
+    bool present = false;
+    uint8_t vals[256];
+    int32_t mincode[17], maxcode[18], valptr[17];  // T.81 F.2.2.3 decoding tables
+};
+
+struct JComp {
+    uint32_t id = 0, h = 1, v = 1, tq = 0, td = 0, ta = 0;
+    uint32_t bw = 0, bh = 0;  // blocks across / down in the padded plane
+    std::vector<uint8_t> plane;  // bw*8 x bh*8 samples
+    int32_t pred = 0;
+};
+
+struct JBits {
+    const uint8_t* p;
+    size_t n, i;
+    uint32_t buf = 0;
+    int cnt = 0;
+    bool marker = false;  // a marker was reached: zeros are fed from here on
+    void fill() {
+        while (cnt <= 24) {
+            uint32_t b = 0;
+            if (!marker && i < n) {
+                b = p[i];
+                if (b == 0xFF) {
+                    const uint8_t nx = i + 1 < n ? p[i + 1] : 0xD9;
+                    if (nx == 0x00) {
+                        i += 2;
+                    } else {
+                        marker = true;
+                        b = 0;
+                    }
+                } else {
+                    ++i;
+                }
+            }
+            buf |= b << (24 - cnt);
+            cnt += 8;
+        }
+    }
+    uint32_t bits(int k) {  // k <= 16, most significant bit first
+        if (k == 0) return 0;
+        fill();
+        const uint32_t v = buf >> (32 - k);
+        buf <<= k;
+        cnt -= k;
+        return v;
+    }
+    void reset_at(size_t pos) {
+        i = pos;
+        buf = 0;
+        cnt = 0;
+        marker = false;
+    }
+};
+
+bool jhuff_build(JHuff& t, const uint8_t* counts, const uint8_t* vals, int nvals) {
+    int total = 0;
+    for (int l = 0; l < 16; ++l) total += counts[l];
+    if (total > 256 || total != nvals) return false;
+    memcpy(t.vals, vals, (size_t)total);
+    int32_t code = 0, k = 0;
+    for (int l = 1; l <= 16; ++l) {
+        t.valptr[l] = k;
+        t.mincode[l] = code;
+        code += counts[l - 1];
+        k += counts[l - 1];
+        t.maxcode[l] = counts[l - 1] ? code - 1 : -1;
+        if (code > (1 << l)) return false;  // over-subscribed
+        code <<= 1;
+    }
+    t.maxcode[17] = 0x7FFFFFFF;
+    t.present = true;
+    return true;
+}
+
+int jhuff_decode(JBits& b, const JHuff& t) {
+    int32_t code = (int32_t)b.bits(1);
+    for (int l = 1; l <= 16; ++l) {
+        if (code <= t.maxcode[l] && t.maxcode[l] >= 0) return t.vals[t.valptr[l] + code - t.mincode[l]];
+        code = (code << 1) | (int32_t)b.bits(1);
+    }
+    return -1;
+}
+
+inline int32_t jextend(uint32_t v, int s) { return s && v < (1u << (s - 1)) ? (int32_t)v - (1 << s) + 1 : (int32_t)v; }
+
+// Separable inverse DCT (T.81 A.3.3) in double, level shift, round, clamp.
+void jidct(const int32_t* coef, uint8_t* out, size_t stride) {
+    static double c[8][8];
+    static bool init = false;
+    if (!init) {
+        for (int x = 0; x < 8; ++x)
+            for (int u = 0; u < 8; ++u)
+                c[x][u] = (u == 0 ? std::sqrt(0.5) : 1.0) * std::cos((2 * x + 1) * u * 3.14159265358979323846 / 16.0);
+        init = true;
+    }
+    double tmp[64];
+    for (int y = 0; y < 8; ++y)  // rows of the result, columns of frequency v
+        for (int u = 0; u < 8; ++u) {
+            double acc = 0.0;
+            for (int v = 0; v < 8; ++v) acc += c[y][v] * coef[v * 8 + u];
+            tmp[y * 8 + u] = acc;
+        }
+    for (int y = 0; y < 8; ++y)
+        for (int x = 0; x < 8; ++x) {
+            double acc = 0.0;
+            for (int u = 0; u < 8; ++u) acc += c[x][u] * tmp[y * 8 + u];
+            const double v = std::floor(acc / 4.0 + 128.5);
+            out[y * stride + x] = (uint8_t)(v < 0.0 ? 0.0 : v > 255.0 ? 255.0 : v);
+        }
+}
+
+inline uint16_t jbe16(const uint8_t* p) { return (uint16_t)((p[0] << 8) | p[1]); }
+
+// Triangle-filter upsampling of one component to the image size for 2:1
+// horizontal and/or vertical factors; other ratios replicate samples.
+std::vector<uint8_t> upsample(const JComp& c, uint32_t hmax, uint32_t vmax, uint32_t w, uint32_t h) {
+    const uint32_t sx = hmax / c.h, sy = vmax / c.v;
+    const uint32_t pw = c.bw * 8, ph = c.bh * 8;
+    const uint32_t cw = (w * c.h + hmax - 1) / hmax, ch = (h * c.v + vmax - 1) / vmax;  // the component's own size
+    std::vector<uint8_t> out((size_t)w * h);
+    auto at = [&](int64_t x, int64_t y) -> int {  // edge-replicated sample of the component's own extent
+        x = std::min<int64_t>(std::max<int64_t>(x, 0), (int64_t)cw - 1);
+        y = std::min<int64_t>(std::max<int64_t>(y, 0), (int64_t)ch - 1);
+        (void)pw;
+        (void)ph;
+        return c.plane[(size_t)y * pw + (size_t)x];
+    };
+    if (sx == 1 && sy == 1) {
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x) out[(size_t)y * w + x] = (uint8_t)at(x, y);
+    } else if (sx == 2 && sy == 1) {
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x) {
+                const int64_t i = x >> 1;
+                const int near = at(i, y);
+                int v;
+                if (x & 1)
+                    v = i + 1 >= (int64_t)cw ? near : (near * 3 + at(i + 1, y) + 2) >> 2;
+                else
+                    v = i == 0 ? near : (near * 3 + at(i - 1, y) + 1) >> 2;
+                out[(size_t)y * w + x] = (uint8_t)v;
+            }
+    } else if (sx == 2 && sy == 2) {
+        for (uint32_t y = 0; y < h; ++y) {
+            const int64_t j = y >> 1, jf = (y & 1) ? j + 1 : j - 1;  // near and far input rows
+            auto colsum = [&](int64_t i) { return at(i, j) * 3 + at(i, jf); };
+            for (uint32_t x = 0; x < w; ++x) {
+                const int64_t i = x >> 1;
+                int v;
+                if (x & 1)
+                    v = i + 1 >= (int64_t)cw ? (colsum(i) * 4 + 7) >> 4 : (colsum(i) * 3 + colsum(i + 1) + 7) >> 4;
+                else
+                    v = i == 0 ? (colsum(i) * 4 + 8) >> 4 : (colsum(i) * 3 + colsum(i - 1) + 8) >> 4;
+                out[(size_t)y * w + x] = (uint8_t)v;
+            }
+        }
+    } else {
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x) out[(size_t)y * w + x] = (uint8_t)at(x / sx, y / sy);
+    }
+    return out;
+}
+
+}  // namespace
+
+bool is_jpeg(const uint8_t* d, size_t n) { return n >= 3 && d[0] == 0xFF && d[1] == 0xD8 && d[2] == 0xFF; }
+
+int decode_jpeg(const uint8_t* d, size_t n, Rgba8Image& out, std::string& err, uint64_t max_texels) {
+    if (!is_jpeg(d, n)) return err = "not a JPEG file", -1;
+    uint16_t qt[4][64];
+    bool have_q[4] = {false, false, false, false};
+    JHuff hdc[4], hac[4];
+    std::vector<JComp> comps;
+    uint32_t W = 0, H = 0, hmax = 1, vmax = 1, restart = 0, mcux = 0, mcuy = 0;
+    bool have_frame = false, have_scan = false;
+    size_t pos = 2;
+    while (pos + 4 <= n) {
+        if (d[pos] != 0xFF) return err = "marker expected", -1;
+        const uint8_t m = d[pos + 1];
+        if (m == 0xFF) {  // fill byte
+            ++pos;
+            continue;
+        }
+        if (m == 0xD9) break;  // EOI
+        if (m == 0x01 || (m >= 0xD0 && m <= 0xD7)) {
+            pos += 2;
+            continue;
+        }
+        const uint32_t len = jbe16(d + pos + 2);
+        if (len < 2 || pos + 2 + len > n) return err = "truncated segment", -1;
+        const uint8_t* seg = d + pos + 4;
+        const uint32_t sl = len - 2;
+        pos += 2 + len;
+        if (m == 0xC2 || m == 0xC3 || (m >= 0xC5 && m <= 0xC7) || (m >= 0xC9 && m <= 0xCB) || (m >= 0xCD && m <= 0xCF))
+            return err = "not a baseline JPEG (progressive, lossless or arithmetic-coded)", 0;
+        if (m == 0xDB) {  // DQT
+            for (uint32_t o = 0; o < sl;) {
+                const uint32_t pq = seg[o] >> 4, tq = seg[o] & 15;
+                if (tq > 3 || pq > 1 || o + 1 + 64 * (pq + 1) > sl) return err = "bad DQT", -1;
+                for (int k = 0; k < 64; ++k)
+                    qt[tq][k] = pq ? jbe16(seg + o + 1 + 2 * k) : seg[o + 1 + k];
+                have_q[tq] = true;
+                o += 1 + 64 * (pq + 1);
+            }
+        } else if (m == 0xC4) {  // DHT
+            for (uint32_t o = 0; o < sl;) {
+                if (o + 17 > sl) return err = "bad DHT", -1;
+                const uint32_t tc = seg[o] >> 4, th = seg[o] & 15;
+                int total = 0;
+                for (int l = 0; l < 16; ++l) total += seg[o + 1 + l];
+                if (tc > 1 || th > 3 || o + 17 + (uint32_t)total > sl) return err = "bad DHT", -1;
+                if (!jhuff_build(tc ? hac[th] : hdc[th], seg + o + 1, seg + o + 17, total)) return err = "bad Huffman table", -1;
+                o += 17 + (uint32_t)total;
+            }
+        } else if (m == 0xDD) {  // DRI
+            if (sl < 2) return err = "bad DRI", -1;
+            restart = jbe16(seg);
+        } else if (m == 0xC0 || m == 0xC1) {  // SOF0 / SOF1
+            if (have_frame || sl < 6) return err = "bad SOF", -1;
+            if (seg[0] != 8) return err = "not 8-bit samples", 0;
+            H = jbe16(seg + 1);
+            W = jbe16(seg + 3);
+            const uint32_t nc = seg[5];
+            if (nc != 1 && nc != 3) return err = "unsupported component count", 0;
+            if (W == 0 || H == 0 || (uint64_t)W * H > max_texels) return err = "image size out of range", -1;
+            if (sl < 6 + 3 * nc) return err = "bad SOF", -1;
+            comps.resize(nc);
+            for (uint32_t k = 0; k < nc; ++k) {
+                comps[k].id = seg[6 + 3 * k];
+                comps[k].h = seg[7 + 3 * k] >> 4;
+                comps[k].v = seg[7 + 3 * k] & 15;
+                comps[k].tq = seg[8 + 3 * k];
+                if (comps[k].h < 1 || comps[k].h > 4 || comps[k].v < 1 || comps[k].v > 4 || comps[k].tq > 3)
+                    return err = "bad component parameters", -1;
+                hmax = std::max(hmax, comps[k].h);
+                vmax = std::max(vmax, comps[k].v);
+            }
+            mcux = (W + 8 * hmax - 1) / (8 * hmax);
+            mcuy = (H + 8 * vmax - 1) / (8 * vmax);
+            for (JComp& c : comps) {
+                c.bw = mcux * c.h;
+                c.bh = mcuy * c.v;
+                c.plane.assign((size_t)c.bw * 8 * c.bh * 8, 0);
+            }
+            have_frame = true;
+        } else if (m == 0xDA) {  // SOS: the entropy-coded data follows
+            if (!have_frame || sl < 1) return err = "scan before frame", -1;
+            const uint32_t ns = seg[0];
+            if (ns < 1 || ns > comps.size() || sl < 1 + 2 * ns + 3) return err = "bad SOS", -1;
+            std::vector<JComp*> sc;
+            for (uint32_t k = 0; k < ns; ++k) {
+                JComp* c = nullptr;
+                for (JComp& x : comps)
+                    if (x.id == seg[1 + 2 * k]) c = &x;
+                if (!c) return err = "scan names an unknown component", -1;
+                c->td = seg[2 + 2 * k] >> 4;
+                c->ta = seg[2 + 2 * k] & 15;
+                if (c->td > 3 || c->ta > 3 || !hdc[c->td].present || !hac[c->ta].present || !have_q[c->tq])
+                    return err = "scan without its tables", -1;
+                c->pred = 0;
+                sc.push_back(c);
+            }
+            if (ns > 1) {
+                uint32_t blocks = 0;
+                for (JComp* c : sc) blocks += c->h * c->v;
+                if (blocks > 10) return err = "too many blocks per MCU", -1;
+            }
+            JBits b{d, n, pos};
+            int32_t coef[64];
+            // units of the scan: MCUs (interleaved) or single blocks
+            const uint32_t ucx = ns > 1 ? mcux : (uint32_t)(((uint64_t)W * sc[0]->h + 8 * hmax - 1) / (8 * hmax));
+            const uint32_t ucy = ns > 1 ? mcuy : (uint32_t)(((uint64_t)H * sc[0]->v + 8 * vmax - 1) / (8 * vmax));
+            const uint64_t units = (uint64_t)ucx * ucy;
+            auto block = [&](JComp& c, uint32_t bx, uint32_t by) -> bool {
+                memset(coef, 0, sizeof(coef));
+                const int t = jhuff_decode(b, hdc[c.td]);
+                if (t < 0 || t > 11) return false;
+                c.pred += jextend(b.bits(t), t);
+                coef[0] = c.pred * (int32_t)qt[c.tq][0];
+                for (int k = 1; k < 64;) {
+                    const int rs = jhuff_decode(b, hac[c.ta]);
+                    if (rs < 0) return false;
+                    const int r = rs >> 4, sz = rs & 15;
+                    if (sz == 0) {
+                        if (r != 15) break;
+                        k += 16;
+                        continue;
+                    }
+                    k += r;
+                    if (k > 63 || sz > 10) return false;
+                    coef[kZigzag[k]] = jextend(b.bits(sz), sz) * (int32_t)qt[c.tq][k];
+                    ++k;
+                }
+                if (bx < c.bw && by < c.bh) jidct(coef, c.plane.data() + ((size_t)by * 8) * c.bw * 8 + (size_t)bx * 8, (size_t)c.bw * 8);
+                return true;
+            };
+            for (uint64_t u = 0; u < units; ++u) {
+                if (restart && u > 0 && u % restart == 0) {  // expect RSTn, realign, reset predictors
+                    size_t q = b.i;
+                    while (q + 1 < n && !(d[q] == 0xFF && d[q + 1] >= 0xD0 && d[q + 1] <= 0xD7)) ++q;
+                    if (q + 1 >= n) return err = "missing restart marker", -1;
+                    b.reset_at(q + 2);
+                    for (JComp* c : sc) c->pred = 0;
+                }
+                const uint32_t ux2 = (uint32_t)(u % ucx), uy2 = (uint32_t)(u / ucx);
+                if (ns > 1) {
+                    for (JComp* c : sc)
+                        for (uint32_t yy = 0; yy < c->v; ++yy)
+                            for (uint32_t xx = 0; xx < c->h; ++xx)
+                                if (!block(*c, ux2 * c->h + xx, uy2 * c->v + yy)) return err = "bad entropy-coded data", -1;
+                } else if (!block(*sc[0], ux2, uy2)) {
+                    return err = "bad entropy-coded data", -1;
+                }
+            }
+            // continue after the scan's data: the next marker that is not RSTn
+            size_t q = b.i;
+            while (q + 1 < n && !(d[q] == 0xFF && d[q + 1] != 0x00 && !(d[q + 1] >= 0xD0 && d[q + 1] <= 0xD7))) ++q;
+            pos = q;
+            have_scan = true;
+        } else if (m == 0xC8 || (m >= 0xF0 && m <= 0xFD) || m == 0xDC || m == 0xDE || m == 0xDF) {
+            return err = "unsupported JPEG marker", 0;
+        }
+        // APPn, COM and other segments are skipped
+    }
+    if (!have_frame || !have_scan) return err = "no frame or scan", -1;
+    out.width = W;
+    out.height = H;
+    out.texels.assign((size_t)W * H * 4, 255);
+    if (comps.size() == 1) {
+        const std::vector<uint8_t> y = upsample(comps[0], hmax, vmax, W, H);
+        for (size_t i = 0; i < (size_t)W * H; ++i) out.texels[4 * i] = out.texels[4 * i + 1] = out.texels[4 * i + 2] = y[i];
+        return 1;
+    }
+    const std::vector<uint8_t> Y = upsample(comps[0], hmax, vmax, W, H), Cb = upsample(comps[1], hmax, vmax, W, H),
+                               Cr = upsample(comps[2], hmax, vmax, W, H);
+    auto clamp8 = [](double v) { return (uint8_t)(v < 0.0 ? 0.0 : v > 255.0 ? 255.0 : v); };
+    for (size_t i = 0; i < (size_t)W * H; ++i) {  // JFIF (ITU-R BT.601 full range)
+        const double y = Y[i], cb = Cb[i] - 128.0, cr = Cr[i] - 128.0;
+        out.texels[4 * i] = clamp8(std::floor(y + 1.402 * cr + 0.5));
+        out.texels[4 * i + 1] = clamp8(std::floor(y - 0.344136 * cb - 0.714136 * cr + 0.5));
+        out.texels[4 * i + 2] = clamp8(std::floor(y + 1.772 * cb + 0.5));
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
 // image 0.24 imageops::sample::resize with the Lanczos3 filter
 // ---------------------------------------------------------------------------
 namespace {

@@ -11,6 +11,12 @@
  *     type and bit depth, palettes, tRNS and Adam7 interlacing, expanded to
  *     RGBA8 (16-bit samples round to 8 bits, grey replicates to RGB;
  *     violet itself stops on images other than 8-bit RGB or RGBA);
+ *   - baseline JPEG (ITU-T T.81 sequential Huffman, JFIF YCbCr): a float
+ *     8x8 inverse DCT, the triangle ("fancy") chroma upsampling for 2:1
+ *     factors that libjpeg and the jpeg-decoder crate use, and the JFIF
+ *     YCbCr -> RGB equations, each rounded to nearest (decoders differ
+ *     from one another by a unit or two here; violet decodes with the
+ *     jpeg-decoder crate);
  *   - Lanczos3 halving as the image crate 0.24's separable resampler
  *     computes it (vertical pass into f32, then horizontal pass with
  *     round-to-nearest and clamp to [0, 255]; the filter support widened by
@@ -35,6 +41,16 @@ bool decode_png(const uint8_t* data, size_t size, Rgba8Image& out, std::string& 
 
 // True if the bytes start with the PNG signature.
 bool is_png(const uint8_t* data, size_t size);
+
+// Decodes a baseline (sequential, Huffman) JPEG: greyscale or YCbCr, any
+// sampling factors up to 4, restart intervals.  Returns 1 on success, 0 for
+// a valid JPEG this decoder does not handle (progressive, arithmetic-coded,
+// 12-bit, CMYK), -1 for malformed or truncated input (reason in `err`).
+int decode_jpeg(const uint8_t* data, size_t size, Rgba8Image& out, std::string& err,
+                uint64_t max_texels = 1ull << 28);
+
+// True if the bytes start with a JPEG SOI marker.
+bool is_jpeg(const uint8_t* data, size_t size);
 
 // image::imageops::resize(src, nw, nh, FilterType::Lanczos3).
 Rgba8Image resize_lanczos3(const Rgba8Image& src, uint32_t nw, uint32_t nh);
