@@ -35,7 +35,8 @@ inline void batch_add_tiles(BatchHeader& hd, uint32_t v, uint32_t tiles_x, uint3
     hd.njobs = v + 1;
 }
 inline void batch_add_pixel_tiles(BatchHeader& hd, uint32_t v, uint32_t w, uint32_t h) {
-    batch_add_tiles(hd, v, (w + 7) / 8, (h + 7) / 8);
+    const dim3 g = tile_grid(w, h);
+    batch_add_tiles(hd, v, g.x, g.y);
 }
 
 // What a pass kind contributes to batching: the size of its table, the host

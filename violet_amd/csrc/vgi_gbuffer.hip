@@ -34,7 +34,7 @@ template <bool kCount>
 __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const SceneView& s, const vgi_gbuffer_io& io, uint32_t bx_, uint32_t by_) {
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
-    tile_pixel_at(bx_, by_, px, py);
+    quad_tile_pixel_at<8>(bx_, by_, px, py);
     const uint32_t w = io.width, h = io.height;
     const bool inside = px < w && py < h;
     bool geo = false;
@@ -116,7 +116,7 @@ static vgi_status fill_gbuffer(const PassRec* const* recs, int n, void* host) {
         j.s = scene_view(r.scene);
         j.s.counters = nullptr;
         j.io = r.io.gbuffer;
-        batch_add_pixel_tiles(t->hd, v, j.io.width, j.io.height);
+        batch_add_tiles(t->hd, v, (j.io.width + 7) / 8, (j.io.height + 7) / 8);
     }
     return VGI_OK;
 }
@@ -141,7 +141,7 @@ extern "C" vgi_status vgi_pass_gbuffer_primary(const vgi_frame_params* fp, const
     ProfScope prof("gbuffer_primary", st, (double)io->width * io->height * 20.0);
     SceneView sv = scene_view(scene);
     sv.counters = trace_counters();
-    const dim3 g = tile_grid(io->width, io->height);
+    const dim3 g = tile8_grid(io->width, io->height);
     if (sv.counters)
         hipLaunchKernelGGL(k_gbuffer<true>, g, dim3(64), traversal_lds_bytes(sv), st, f, sv, *io);
     else

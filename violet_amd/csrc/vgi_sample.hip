@@ -73,7 +73,7 @@ __device__ __forceinline__ void sample_pixel(const P& p, const TraceRes& rt, con
     VGI_TRAVERSAL_STACK(stack);
     uint32_t px, py;
     if (rt.tile_w == 8u) {
-        tile_pixel_at(bx_, by_, px, py);
+        quad_tile_pixel_at<8>(bx_, by_, px, py);
     } else {
         const uint32_t lane = threadIdx.x & 63u, sh = rt.tile_w == 16u ? 4u : 5u;
         px = (bx_ << sh) + (lane & (rt.tile_w - 1u));
