@@ -75,7 +75,9 @@ def test_requests_same_alone_and_in_the_mix(vgi):
         r.close()
 
 
-def test_serving_form_requests_bit_identical_alone_and_in_the_mix(vgi):
+def test_serving_form_requests_bit_identical_alone_and_in_the_mix(vgi, tmp_path):
+    import json
+
     import torch
 
     from violet_amd import _lib, batch, gpu, renderer, scene, serve
@@ -93,12 +95,20 @@ def test_serving_form_requests_bit_identical_alone_and_in_the_mix(vgi):
         got[(req.view, frame)] = (gpu.copy_from_ptr(out.indirect_diffuse, n),
                                   gpu.copy_from_ptr(out.indirect_specular, n))
 
+    mpath = tmp_path / "metrics.jsonl"
     srv = serve.Server(sc, views, hbm_budget_bytes=200e9, config=cfg, max_batch=16, streams=2,
-                       batch_config=batch.BatchConfig(), on_complete=keep)
+                       batch_config=batch.BatchConfig(), on_complete=keep, metrics=mpath)
     srv.admit_all()
     stats = srv.run(reqs, timeout_s=120.0)
     srv.close()
     s = stats.summary()
+    # the metrics stream: one line per batch, every request once, then the summary
+    lines = [json.loads(x) for x in mpath.read_text().splitlines()]
+    batches = [x for x in lines if x["event"] == "batch"]
+    assert len(batches) == s["batches"] and lines[-1]["event"] == "summary"
+    assert sorted(l for x in batches for l in x["latency_ms"]) == pytest.approx(sorted(stats.latencies_ms), abs=1e-3)
+    assert sum(len(x["views"]) for x in batches) == len(reqs) == lines[-1]["requests"]
+    assert lines[-1]["p99_ms"] == pytest.approx(s["p99_ms"])
     print(f"serving form: {s['requests']} requests, mean batch {s['mean_batch']:.2f}, p99 {s['p99_ms']:.1f} ms",
           flush=True)
     assert s["requests"] == len(reqs) and s["evictions"] == 0 and s["mean_batch"] > 1.0

@@ -71,7 +71,7 @@ def test_results_do_not_depend_on_launch_shapes(vgi, batched):
         _lib.set_tuning()
 
 
-def test_autotuner_descends_and_sets_the_shapes(vgi):
+def test_autotuner_descends_and_sets_the_shapes(vgi, tmp_path):
     from violet_amd import _lib, autotune, renderer, scene, synth
 
     sc = scene.Scene(make_mesh())
@@ -88,6 +88,17 @@ def test_autotuner_descends_and_sets_the_shapes(vgi):
         assert f == 6 * 2 * (1 + 2)  # every candidate's frames advanced the views
         assert _lib.get_tuning() == dict(atrous_block=best["atrous_block"], trace_tile_width=best["trace_tile_width"])
         assert cfg.trace_groups in (0, 16 * __import__("torch").cuda.get_device_properties(0).multi_processor_count)
+        # the cache file: a first tuning stores the result, a second applies it without timing
+        cache = autotune.TuningCache(tmp_path / "tune.json")
+        _lib.set_tuning()
+        cfg1, table1, f1, best1 = autotune.tune_batch(sc, views, fps_of, frame0=f, warmup=1, frames=2, repeats=1,
+                                                      dimensions=dims, cache=cache)
+        assert len(table1) == 6 and cache.get(autotune.workload_key(
+            views, sc.stats().num_triangles, __import__("torch").cuda.get_device_name(0))) == best1
+        _lib.set_tuning()
+        cfg2, table2, f2, best2 = autotune.tune_batch(sc, views, fps_of, frame0=f1, dimensions=dims, cache=cache)
+        assert table2 == [] and f2 == f1 and best2 == best1 and cfg2 == cfg1
+        assert _lib.get_tuning() == dict(atrous_block=best1["atrous_block"], trace_tile_width=best1["trace_tile_width"])
     finally:
         _lib.set_tuning()
         for r in views:

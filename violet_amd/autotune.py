@@ -17,10 +17,19 @@ replaces the incumbent only if its median beats the incumbent's by more
 than the incumbent's own spread across repeats (and at least 1%).  Shapes
 do not change results, so tuning only advances the views by the frames it
 renders.  The chosen launch shapes stay set (vgi_tuning_set) on return.
+
+Results persist in a JSON cache file (TuningCache; tune_batch(cache=...)):
+keyed by the device, the library version and the workload (the views'
+sizes, storage and resampling parameters, the scene's triangle count), a
+hit applies the stored shapes without timing anything.
 """
 from __future__ import annotations
 
+import hashlib
+import json
+import os
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -51,11 +60,61 @@ SHAPES = ("trace_tile_width", "atrous_block")
 GROUPS = ("trace_groups", "pixel_groups", "atrous_groups")
 
 
+class TuningCache:
+    """Tuned launch shapes on disk: a JSON object {key: {"shapes": {...},
+    "table": [...], "time": unix seconds}}.  Writes go through a temporary
+    file and a rename, so a reader never sees a partial file."""
+
+    def __init__(self, path):
+        self.path = Path(path)
+
+    def load(self) -> dict:
+        try:
+            d = json.loads(self.path.read_text())
+            return d if isinstance(d, dict) else {}
+        except (OSError, ValueError):
+            return {}
+
+    def get(self, key: str) -> dict | None:
+        e = self.load().get(key)
+        if not isinstance(e, dict) or not isinstance(e.get("shapes"), dict):
+            return None
+        sh = e["shapes"]
+        ok = all(sh.get(k) in opts for k, opts in DIMENSIONS)
+        return dict(sh) if ok else None
+
+    def put(self, key: str, shapes: dict, table=None) -> None:
+        d = self.load()
+        d[key] = {"shapes": {k: int(shapes[k]) for k, _ in DIMENSIONS}, "time": time.time(),
+                  "table": [[{k: int(v) for k, v in c.items()}, round(float(m), 4), round(float(sp), 4)]
+                            for c, m, sp in (table or [])]}
+        self.path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = self.path.with_name(self.path.name + f".{os.getpid()}.tmp")
+        tmp.write_text(json.dumps(d, indent=1, sort_keys=True))
+        os.replace(tmp, self.path)
+
+
+def workload_key(views, ntris: int, device: str, extra: str = "") -> str:
+    """Cache key: device, library version and the workload's shape (each
+    view's size, storage and resampling configuration; the triangle
+    count)."""
+    def cfg(v):
+        c = getattr(v, "config", None)
+        return json.dumps(vars(c) if hasattr(c, "__dict__") else c, sort_keys=True, default=str)
+
+    desc = {"device": device, "library": _lib.lib().vgi_version().decode(), "triangles": int(ntris),
+            "views": [[int(v.width), int(v.height), cfg(v)] for v in views], "extra": extra}
+    return hashlib.sha256(json.dumps(desc, sort_keys=True).encode()).hexdigest()[:32]
+
+
 def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: int = 6, repeats: int = 3,
-               dimensions=DIMENSIONS, use_graph: bool = True):
+               dimensions=DIMENSIONS, use_graph: bool = True, cache: TuningCache | None = None,
+               cache_key: str | None = None):
     """fps_of(view_index, frame) -> FrameParams.  Returns (best BatchConfig,
     table of (candidate dict, median ms, spread ms), the next free frame
-    index); the best launch shapes are left set."""
+    index, the best shapes); the best launch shapes are left set.  With a
+    cache, a stored entry for the workload is applied without timing (the
+    table is then empty), and a tuned result is stored."""
     import torch
 
     cus = torch.cuda.get_device_properties(0).multi_processor_count
@@ -66,6 +125,15 @@ def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: i
     def config(c):
         return batch.BatchConfig(max_views=max(len(views), 1), use_graph=use_graph,
                                  **{k: c[k] * (cus if k in GROUPS else 1) for k in GROUPS})
+
+    if cache is not None:
+        if cache_key is None:
+            ntris = scene.stats().num_triangles if hasattr(scene, "stats") else 0
+            cache_key = workload_key(views, ntris, torch.cuda.get_device_name(0))
+        hit = cache.get(cache_key)
+        if hit is not None:
+            _lib.set_tuning(hit["atrous_block"], hit["trace_tile_width"])
+            return config(hit), [], f, hit
 
     for key, opts in dimensions:
         cands = [dict(best, **{key: m}) for m in opts]
@@ -90,4 +158,6 @@ def tune_batch(scene, views, fps_of, frame0: int = 0, warmup: int = 2, frames: i
         best = dict(cands[win])
         table += [(cands[i], med[i], spread[i]) for i in med]
     _lib.set_tuning(best["atrous_block"], best["trace_tile_width"])
+    if cache is not None:
+        cache.put(cache_key, best, table)
     return config(best), table, f, best

@@ -23,6 +23,11 @@ to the host observing its batch's completion fence (vgi_fence).  As violet's
 frame fence (crates/violet-app/src/render_loop.rs:445-482), a batch still
 running `frame_timeout_ms` after its issue is counted as timed out and
 waited for further, and one finishing after 33.3 ms as slow.
+
+Metrics: with `metrics` (a path or an open text file) the server appends
+one JSON line per completed batch (slot, issue time, batch time, views,
+pixels, each request's latency, whether it timed out) and a summary line
+at the end of run() (the fields of ServeStats.summary()).
 """
 from __future__ import annotations
 
@@ -92,7 +97,7 @@ class Server:
     def __init__(self, scene: Scene, views: list[ViewSpec], hbm_budget_bytes: float,
                  config: renderer.RestirConfig | None = None, max_batch: int = 16, streams: int = 2,
                  batch_config: batch.BatchConfig | None = None, on_complete=None, max_batch_pixels: int = 0,
-                 frame_timeout_ms: int = gpu.Fence.TIMEOUT_MS):
+                 frame_timeout_ms: int = gpu.Fence.TIMEOUT_MS, metrics=None):
         import torch
 
         gpu.require_gpu()
@@ -105,6 +110,10 @@ class Server:
         self.max_batch_pixels = max_batch_pixels  # 0: no limit
         self.frame_timeout_ms = frame_timeout_ms
         self.on_complete = on_complete
+        # JSON-lines metrics stream: a path or a writable text file, or None
+        self.metrics = open(metrics, "a") if isinstance(metrics, (str, bytes)) or hasattr(metrics, "__fspath__") \
+            else metrics
+        self._own_metrics = self.metrics is not metrics
         bc = batch_config or batch.BatchConfig()
         bc.max_views = max(bc.max_views, max_batch)
         self.slots = [(batch.Batch(bc), torch.cuda.Stream()) for _ in range(streams)]
@@ -165,8 +174,15 @@ class Server:
                     stats.timeouts += 1
                 continue
             t_done = time.perf_counter()
-            if (t_done - self.t0 - b[0]) * 1e3 > 33.3:
+            batch_ms = (t_done - self.t0 - b[0]) * 1e3
+            if batch_ms > 33.3:
                 stats.slow_batches += 1
+            if self.metrics is not None:
+                self._emit({"event": "batch", "slot": k, "t_issue_s": round(b[0], 6), "batch_ms": round(batch_ms, 4),
+                            "views": [req.view for req, _ in b[1]],
+                            "pixels": sum(self.views[req.view].width * self.views[req.view].height for req, _ in b[1]),
+                            "latency_ms": [round((t_done - self.t0 - req.arrival_s) * 1e3, 4) for req, _ in b[1]],
+                            "timed_out": bool(b[2])})
             for req, outs in b[1]:
                 stats.latencies_ms.append((t_done - self.t0 - req.arrival_s) * 1e3)
                 spec = self.views[req.view]
@@ -246,9 +262,20 @@ class Server:
                 time.sleep(50e-6)
         self.torch.cuda.synchronize()
         stats.wall_s = time.perf_counter() - self.t0
+        if self.metrics is not None:
+            self._emit(dict({"event": "summary"}, **stats.summary()))
+            self.metrics.flush()
         return stats
 
+    def _emit(self, rec: dict) -> None:
+        import json
+
+        self.metrics.write(json.dumps(rec) + "\n")
+
     def close(self) -> None:
+        if self.metrics is not None and self._own_metrics:
+            self.metrics.close()
+        self.metrics = None
         for r in self.resident.values():
             r.close()
         self.resident.clear()
