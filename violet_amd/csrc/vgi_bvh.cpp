@@ -62,8 +62,9 @@ struct Prim {
 
 constexpr int kBins = 32;
 // Largest vertex coordinate magnitude a scene may have: keeps every node's
-// grid scale a float and the traversal's plane distances finite
-// (scale * |1/d| and q * scale * |1/d| stay below 2^120 for |1/d| <= 1e20).
+// grid step a float and the traversal's plane distances finite (the step
+// is below 2^34, so the stored step * 2^24 times |1/d| stays below 2^125
+// and q * step * |1/d| below 2^109 for |1/d| <= 1e20).
 constexpr float kMaxCoord = 1099511627776.0f;  // 2^40
 constexpr int kMaxLeaf = 15;
 constexpr int kMedianDepth = 36;
@@ -261,7 +262,7 @@ struct Collapser {
             if (!(lo <= hi)) lo = hi = 0.0f;  // no used child (cannot happen for a built tree)
             const double s = pow2_at_least(((double)hi - (double)lo) / 255.0);
             nd.org[a] = lo;
-            nd.scale[a] = (float)s;
+            nd.scale[a] = (float)std::ldexp(s, 24);  // see BvhNode::scale
             uint32_t qlo = 0, qhi = 0;
             for (int k = 0; k < 4; ++k) {
                 uint32_t l = 255u, h = 0u;  // unused: an empty interval

@@ -7,8 +7,10 @@
  * Layout (built on the host by vgi_bvh.cpp, uploaded once per scene):
  *   nodes : 4-wide BVH, one 64-byte node (half an L2 line, four 16-byte
  *           loads) holds its four children's boxes quantised to 8 bits per
- *           plane on a per-node grid, plane = org + q * scale with scale a
- *           power of two per axis, and four child links.  A child link >= 0
+ *           plane on a per-node grid, plane = org + q * step with step a
+ *           power of two per axis (stored as step * 2^24, the factor that
+ *           turns the traversal's binary16 grid lines q * 2^-24 back into
+ *           q), and four child links.  A child link >= 0
  *           is an inner node index; < 0 encodes a leaf ~((first << 4) | n).
  *           A child's quantised box contains its padded box (lo rounded
  *           down, hi rounded up).  An unused child has lo = 255, hi = 0 on
@@ -33,7 +35,7 @@ namespace vgi {
 
 struct BvhNode {
     float org[3];    // grid origin (at or below every used child's padded box)
-    float scale[3];  // grid step per axis, a power of two (255 steps span the node)
+    float scale[3];  // grid step per axis times 2^24, a power of two (255 steps span the node)
     uint32_t q[6];   // planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z: byte k = child k
     int32_t link[4];
 };

@@ -8,9 +8,13 @@
  * children are ordered by entry distance; the nearest is visited next and
  * the others are pushed farthest first.  Boxes are tested with the slab
  * method on the node's 8-bit grid: per axis the ray's distance to grid line
- * q is q * (scale / d) + (org - o) / d, one byte-to-float conversion and one
- * fused multiply-add per plane (conservative within the boxes' padding, see
- * slab_q).
+ * q is q * (step / d) + (org - o) / d.  One byte permute turns two planes'
+ * bytes into binary16 subnormals q * 2^-24 (exact), and one mixed-precision
+ * fused multiply-add per plane (v_fma_mix_f32, a binary16 operand widened
+ * exactly) multiplies them by the node's stored step * 2^24: the same
+ * q * step * |1/d| product as a byte-to-float conversion followed by an FMA,
+ * in 1.5 instructions per plane instead of 2 (conservative within the
+ * boxes' padding, see slab_q).
  */
 #pragma once
 #include "vgi_bvh.h"
@@ -64,27 +68,60 @@ __device__ __forceinline__ RayInv make_ray(f3 o, f3 d) {
     return r;
 }
 
-// Byte k of a plane word as a float (one v_cvt_f32_ubyte<k>).
+// Bytes 2j and 2j + 1 of a plane word (children 2j and 2j + 1) as the low
+// and high halves of a binary16 pair: byte q becomes the subnormal
+// q * 2^-24, exactly (one v_perm_b32 placing a zero byte above each).
+// Kernels keep the default binary16 denormal mode (inputs and outputs not
+// flushed), which the mixed FMA below relies on.
+// The permute selectors: held in scalar registers (an opaque scalar move,
+// so the compiler does not rematerialise the literals into two vector
+// registers kept live across the traversal).
+struct PairSel {
+    uint32_t s01, s23;
+};
+__device__ __forceinline__ PairSel pair_selectors() {
+    PairSel p;
+    asm volatile("s_mov_b32 %0, 0x0C010C00" : "=s"(p.s01));
+    asm volatile("s_mov_b32 %0, 0x0C030C02" : "=s"(p.s23));
+    return p;
+}
+template <int j>
+__device__ __forceinline__ uint32_t qpair(uint32_t w, const PairSel& sel) {
+    return __builtin_amdgcn_perm(0u, w, j ? sel.s23 : sel.s01);
+}
+// Child k's grid line from the pairs of children 0-1 (p01) and 2-3 (p23),
+// as the binary16 q * 2^-24 widened to float (folded into v_fma_mix_f32).
 template <int k>
-__device__ __forceinline__ float qbyte(uint32_t w) {
-    return (float)((w >> (8 * k)) & 0xFFu);
+__device__ __forceinline__ float qline(uint32_t p01, uint32_t p23) {
+    const uint32_t p = k < 2 ? p01 : p23;
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)((k & 1) ? p >> 16 : p & 0xFFFFu));
 }
 
+// A ray's near and far plane pairs of one node (children 0-1 and 2-3 per
+// axis and side).
+struct NodePlanes {
+    uint32_t nx01, nx23, fx01, fx23, ny01, ny23, fy01, fy23, nz01, nz23, fz01, fz23;
+};
+
 // Entry distance of child k of a quantised node, or +inf if missed.  Per
-// axis, qn / qf are the plane words of the near and far side for this ray
-// (near = lo where the direction component is >= 0, else hi) and the
-// distance to grid line q is q * a + b with a = scale * idir (exact: scale
-// is a power of two) and b = (org - o) * idir.  Against the exact distance
-// (org + q * scale - o) * idir this errs by about an ulp of |org - o| in
-// world units plus the final rounding, inside the 2^-16 relative padding of
-// every child box, so the test stays conservative.  An unused child (lo =
-// 255, hi = 0) has its near plane beyond its far one on every axis.
+// axis, the near and far planes are those of the lo or hi word (near = lo
+// where the direction component is >= 0, else hi) and the distance to grid
+// line q is (q * 2^-24) * a + b with a = (step * 2^24) * idir and b =
+// (org - o) * idir; both scalings are by powers of two, so the product is
+// q * step * idir as exactly as for step * idir.  Against the exact
+// distance (org + q * step - o) * idir this errs by about an ulp of
+// |org - o| in world units plus the final rounding, inside the 2^-16
+// relative padding of every child box, so the test stays conservative.  An
+// unused child (lo = 255, hi = 0) has its near plane beyond its far one on
+// every axis.
 template <int k>
-__device__ __forceinline__ float slab_q(f3 a, f3 b, uint32_t qnx, uint32_t qfx, uint32_t qny, uint32_t qfy,
-                                        uint32_t qnz, uint32_t qfz, float tmin, float tmax) {
-    const float tnx = __builtin_fmaf(qbyte<k>(qnx), a.x, b.x), tfx = __builtin_fmaf(qbyte<k>(qfx), a.x, b.x);
-    const float tny = __builtin_fmaf(qbyte<k>(qny), a.y, b.y), tfy = __builtin_fmaf(qbyte<k>(qfy), a.y, b.y);
-    const float tnz = __builtin_fmaf(qbyte<k>(qnz), a.z, b.z), tfz = __builtin_fmaf(qbyte<k>(qfz), a.z, b.z);
+__device__ __forceinline__ float slab_q(f3 a, f3 b, const NodePlanes& p, float tmin, float tmax) {
+    const float tnx = __builtin_fmaf(qline<k>(p.nx01, p.nx23), a.x, b.x);
+    const float tfx = __builtin_fmaf(qline<k>(p.fx01, p.fx23), a.x, b.x);
+    const float tny = __builtin_fmaf(qline<k>(p.ny01, p.ny23), a.y, b.y);
+    const float tfy = __builtin_fmaf(qline<k>(p.fy01, p.fy23), a.y, b.y);
+    const float tnz = __builtin_fmaf(qline<k>(p.nz01, p.nz23), a.z, b.z);
+    const float tfz = __builtin_fmaf(qline<k>(p.fz01, p.fz23), a.z, b.z);
     const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
     const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tmax);
     return tn <= tf ? tn : INFINITY;
@@ -131,6 +168,7 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
     // which plane word of each axis is the near one (lo where the direction
     // component is >= 0, else hi)
     const bool px = r.idir.x >= 0.0f, py = r.idir.y >= 0.0f, pz = r.idir.z >= 0.0f;
+    const PairSel sel = pair_selectors();
     const BufRsrc nodes = scene_rsrc(s.nodes, s.num_nodes * (uint32_t)sizeof(BvhNode));
     const BufRsrc tris = scene_rsrc(s.tris, s.num_tris * (uint32_t)sizeof(BvhTri));
     Hit h;
@@ -168,10 +206,14 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             }
             if (kCount) ++n_nodes;
             const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(BvhNode);
-            const Vf4 g0 = ld_buf16(nodes, nb);        // org.xyz, scale.x
-            const Vf4 g1 = ld_buf16(nodes, nb + 16u);  // scale.yz, q lo.x, q hi.x
+            const Vf4 g0 = ld_buf16(nodes, nb);        // org.xyz, step.x * 2^24
+            const Vf4 g1 = ld_buf16(nodes, nb + 16u);  // step.yz * 2^24, q lo.x, q hi.x
             const Vf4 g2 = ld_buf16(nodes, nb + 32u);  // q lo.y, hi.y, lo.z, hi.z
             const Vf4 lk = ld_buf16(nodes, nb + 48u);
+            // The links are first needed after the box tests, and left alone
+            // the compiler sinks their load to that point: a second round
+            // trip per node.  Using them here issues all four loads together.
+            asm volatile("" ::"v"(lk));
             const f3 a = mk3(g0.w * r.idir.x, g1.x * r.idir.y, g1.y * r.idir.z);
             const f3 b = mk3((g0.x - r.o.x) * r.idir.x, (g0.y - r.o.y) * r.idir.y, (g0.z - r.o.z) * r.idir.z);
             const uint32_t qlx = __float_as_uint(g1.z), qhx = __float_as_uint(g1.w);
@@ -180,10 +222,13 @@ __device__ __forceinline__ Hit traverse(const SceneView& s, f3 o, f3 d, float tm
             const uint32_t qnx = px ? qlx : qhx, qfx = px ? qhx : qlx;
             const uint32_t qny = py ? qly : qhy, qfy = py ? qhy : qly;
             const uint32_t qnz = pz ? qlz : qhz, qfz = pz ? qhz : qlz;
-            float t0 = slab_q<0>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
-            float t1 = slab_q<1>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
-            float t2 = slab_q<2>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
-            float t3 = slab_q<3>(a, b, qnx, qfx, qny, qfy, qnz, qfz, tmin, h.t);
+            const NodePlanes pl{qpair<0>(qnx, sel), qpair<1>(qnx, sel), qpair<0>(qfx, sel), qpair<1>(qfx, sel),
+                                qpair<0>(qny, sel), qpair<1>(qny, sel), qpair<0>(qfy, sel), qpair<1>(qfy, sel),
+                                qpair<0>(qnz, sel), qpair<1>(qnz, sel), qpair<0>(qfz, sel), qpair<1>(qfz, sel)};
+            float t0 = slab_q<0>(a, b, pl, tmin, h.t);
+            float t1 = slab_q<1>(a, b, pl, tmin, h.t);
+            float t2 = slab_q<2>(a, b, pl, tmin, h.t);
+            float t3 = slab_q<3>(a, b, pl, tmin, h.t);
             int32_t l0 = __float_as_int(lk.x), l1 = __float_as_int(lk.y);
             int32_t l2 = __float_as_int(lk.z), l3 = __float_as_int(lk.w);
             if (kAnyHit) {
