@@ -51,7 +51,7 @@ RESAMPLE_DESC = {
 # Pass name -> kernel symbol (for counter summaries of that kernel).
 PASS_KERNEL = {
     "denoise_atrous": "k_atrous", "denoise_temporal": "k_temporal", "denoise_disocclusion_fix": "k_disocclusion",
-    "denoise_spatial_fallback": "k_fallback", "ind_diff_spatial_resample": "k_diff_spatial",
+    "denoise_spatial_fallback": "k_fallback", "ind_diff_spatial_resample": "k_diff_spatial_taps",
     "ind_diff_temporal_resample": "k_diff_temporal", "ind_spec_temporal_resample": "k_spec_temporal",
     "ind_spec_spatial_resample": "k_spec_spatial", "raytraced_ao_gen": "k_ao_gen", "raytraced_ao_filter": "k_ao_filter",
     "hash_grid_query_compact": "k_hg_compact", "hash_grid_decay": "k_hg_decay", "hash_grid_query_select": "k_hg_select",

@@ -575,27 +575,35 @@ __global__ void __launch_bounds__(64) k_fast_clamp(vgi_denoise_fast_clamp_io io)
 // Lattice block shapes: a workgroup of B x B threads filters a B x B block
 // of lattice points (one residue class of the step) from its (B + 4)^2
 // footprint staged in LDS.  B is a launch shape (vgi_tuning): 8, 16 (the
-// default) or 32.  Each wave filters an 8x8 part of the block and footprint
-// rows are P dwords apart, so the 8 rows of a wave's reads start at P r mod
-// 64: for P = 24, 0, 24, 48, 8, 32, 56, 16, 40; for P = 40, 0, 40, 16, 56,
-// 32, 8, 48, 24.  The 64 lanes of one LDS read hit 64 distinct banks (and any
-// 32 consecutive lanes 32 distinct ones).
+// default) or 32.  A footprint entry is three 16-byte records (diffuse
+// colour and variance, specular colour and variance, normal and depth) and
+// one 8-byte record (the two luminances), each record kind in its own array
+// with rows P entries apart, so a tap is four LDS reads (ds_read_b128 x 3,
+// ds_read_b64) instead of fourteen dword reads.  Each wave filters an 8x8
+// part of the block; lanes 8r..8r+7 read 8 consecutive entries of row r.
+// With P = 24 or 40, one row starts 4P = 32 (mod 64) banks after the one
+// above for the 16-byte records (16 lanes, 64 distinct banks) and 2P = 48
+// or 16 (mod 64) banks after it for the 8-byte records (32 lanes, 64
+// distinct banks).
 template <int B>
 struct AtrousShape {
     static constexpr int F = B + 4;                // footprint edge (2 each side)
     static constexpr int N = F * F;                // footprint entries
     static constexpr int P = B == 32 ? 40 : 24;    // footprint row pitch
-    static constexpr int Row = F * P;              // dwords per field
+    static constexpr int Row = F * P;              // entries per record array
     static constexpr int T = B * B;                // threads per workgroup
     static_assert(P >= F, "pitch holds a footprint row");
 };
-enum AtrousField { kFDepth, kFNx, kFNy, kFNz, kFLd, kFLs, kFDr, kFDg, kFDb, kFSr, kFSg, kFSb, kFVx, kFVy, kFCount };
+typedef float AtrousV2 __attribute__((ext_vector_type(2)));
+typedef float AtrousV4 __attribute__((ext_vector_type(4)));
+enum AtrousRecord { kRDiff, kRSpec, kRNormal, kRCount };  // .xyz colour or normal, .w variance or depth
 
 template <class S, int B>
 __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, int step, uint32_t nbx, uint32_t nby, uint32_t bx_, uint32_t by_) {
     typedef AtrousShape<B> Sh;
     constexpr int kAtrousF = Sh::F, kAtrousN = Sh::N, kAtrousP = Sh::P, kAtrousRow = Sh::Row;
-    __shared__ float lds[kFCount][kAtrousRow];
+    __shared__ AtrousV4 lq[kRCount][kAtrousRow];
+    __shared__ AtrousV2 llum[kAtrousRow];
     const int w = (int)io.width, h = (int)io.height;
     const size_t plane = (size_t)w * h;
     // residue innermost: the s x s residue classes of one block region are
@@ -604,29 +612,40 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     const int ry = (int)(by_ % (uint32_t)step), by = (int)(by_ / (uint32_t)step);
     const int t = (int)threadIdx.x;
 
-#pragma unroll 1
-    for (int e = t; e < kAtrousN; e += Sh::T) {
-        int gx = bx * B + e % kAtrousF - 2, gy = by * B + e / kAtrousF - 2;
-        int cx = rx + step * gx, cy = ry + step * gy;
-        float v[kFCount];
+    // Staging: a thread's footprint entries (at most kIter) are all loaded
+    // before any is stored, so their memory round trips overlap.  The loads
+    // are unconditional (an entry outside the image or past the footprint
+    // reads pixel 0 and is then replaced by zeros): a load under a branch
+    // would have to complete before the branch joins.
+    constexpr int kIter = (kAtrousN + Sh::T - 1) / Sh::T;
+    AtrousV4 rd[kIter], rs[kIter], rn[kIter];
 #pragma unroll
-        for (int k = 0; k < kFCount; ++k) v[k] = 0.0f;
-        if (gx >= 0 && gy >= 0 && cx < w && cy < h) {
-            size_t j = (size_t)cy * w + cx;
-            f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
-            f3 dv = S::ld3(io.diff_texture, j);
-            f3 sv = S::ld3(io.spec_texture, j);
-            f2 vv = S::ld2(io.variance_texture, j);
-            v[kFDepth] = G(io.depth_texture)[j];
-            v[kFNx] = n.x, v[kFNy] = n.y, v[kFNz] = n.z;
-            v[kFLd] = luminance(dv), v[kFLs] = luminance(sv);
-            v[kFDr] = dv.x, v[kFDg] = dv.y, v[kFDb] = dv.z;
-            v[kFSr] = sv.x, v[kFSg] = sv.y, v[kFSb] = sv.z;
-            v[kFVx] = vv.x, v[kFVy] = vv.y;
+    for (int it = 0; it < kIter; ++it) {
+        const int e = t + it * Sh::T;
+        const int gx = bx * B + e % kAtrousF - 2, gy = by * B + e / kAtrousF - 2;
+        const int cx = rx + step * gx, cy = ry + step * gy;
+        const bool inside = e < kAtrousN && gx >= 0 && gy >= 0 && cx < w && cy < h;
+        const size_t j = inside ? (size_t)cy * w + cx : 0;
+        const f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
+        const f3 dv = S::ld3(io.diff_texture, j);
+        const f3 sv = S::ld3(io.spec_texture, j);
+        const f2 vv = S::ld2(io.variance_texture, j);
+        const float z = G(io.depth_texture)[j];
+        const AtrousV4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        rd[it] = inside ? AtrousV4{dv.x, dv.y, dv.z, vv.x} : zero;
+        rs[it] = inside ? AtrousV4{sv.x, sv.y, sv.z, vv.y} : zero;
+        rn[it] = inside ? AtrousV4{n.x, n.y, n.z, z} : zero;
+    }
+#pragma unroll
+    for (int it = 0; it < kIter; ++it) {
+        const int e = t + it * Sh::T;
+        if (e < kAtrousN) {
+            const int le = (e / kAtrousF) * kAtrousP + e % kAtrousF;
+            lq[kRDiff][le] = rd[it];
+            lq[kRSpec][le] = rs[it];
+            lq[kRNormal][le] = rn[it];
+            llum[le] = AtrousV2{luminance(mk3(rd[it].x, rd[it].y, rd[it].z)), luminance(mk3(rs[it].x, rs[it].y, rs[it].z))};
         }
-        const int le = (e / kAtrousF) * kAtrousP + e % kAtrousF;
-#pragma unroll
-        for (int k = 0; k < kFCount; ++k) lds[k][le] = v[k];
     }
     __syncthreads();
 
@@ -638,12 +657,14 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     const size_t i = (size_t)py * w + px;
     const int c = (lj + 2) * kAtrousP + (li + 2);
 
-    const float depth_center = lds[kFDepth][c];
-    const f3 nc = mk3(lds[kFNx][c], lds[kFNy][c], lds[kFNz][c]);
-    const f2 lc = mk2(lds[kFLd][c], lds[kFLs][c]);
-    f3 dsum = mk3(lds[kFDr][c], lds[kFDg][c], lds[kFDb][c]);
-    f3 ssum = mk3(lds[kFSr][c], lds[kFSg][c], lds[kFSb][c]);
-    f2 vsum = mk2(lds[kFVx][c], lds[kFVy][c]);
+    const AtrousV4 cd = lq[kRDiff][c], cs = lq[kRSpec][c], cn = lq[kRNormal][c];
+    const AtrousV2 cl = llum[c];
+    const float depth_center = cn.w;
+    const f3 nc = mk3(cn.x, cn.y, cn.z);
+    const f2 lc = mk2(cl.x, cl.y);
+    f3 dsum = mk3(cd.x, cd.y, cd.z);
+    f3 ssum = mk3(cs.x, cs.y, cs.z);
+    f2 vsum = mk2(cd.w, cs.w);
     const float fwz = decode_gbuffer_fwidth(G(io.gbuffer_texture)[2 * plane + i]).x;
     // exp(-|dl| / (sigma_l*std + eps) - dz / (fwidth_z*dist + eps)) as exp2 of
     // pre-scaled terms (sigma_l = 4, sigma_z = 1, eps = 1e-8)
@@ -663,18 +684,20 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
     const float lkw[3] = {0.0f, -0.5849625007211562f, -2.584962500721156f};  // log2 of 1, 2/3, 1/6
     f2 wsum = mk2(1.0f, 1.0f);
     // every tap read is at a constant offset from the footprint corner of
-    // this pixel's 5x5 window, so all LDS reads share one address register
-    const float* L = &lds[0][0] + (c - 2 * kAtrousP - 2);
-    // one base register per field, kept opaque so the compiler addresses the
-    // tap pairs of a field as immediate offsets from it instead of adding a
-    // field offset per pair (the empty asm emits nothing)
-    typedef __attribute__((address_space(3))) const float* LdsF;
-    LdsF F[kFCount];
+    // this pixel's 5x5 window: one base register per record array, kept
+    // opaque so the compiler addresses the taps as immediate offsets from it
+    // instead of adding an array offset per tap (the empty asm emits nothing)
+    typedef __attribute__((address_space(3))) const AtrousV4* Lds4;
+    typedef __attribute__((address_space(3))) const AtrousV2* Lds2;
+    const int corner = c - 2 * kAtrousP - 2;
+    Lds4 Q[kRCount];
 #pragma unroll
-    for (int k = 0; k < kFCount; ++k) {
-        F[k] = (LdsF)(L + k * kAtrousRow);
-        asm volatile("" : "+v"(F[k]));
+    for (int k = 0; k < kRCount; ++k) {
+        Q[k] = (Lds4)(&lq[k][0] + corner);
+        asm volatile("" : "+v"(Q[k]));
     }
+    Lds2 QL = (Lds2)(&llum[0] + corner);
+    asm volatile("" : "+v"(QL));
 #pragma unroll
     for (int y = -2; y <= 2; y++) {
 #pragma unroll
@@ -682,21 +705,21 @@ __device__ __forceinline__ void k_atrous_body(const vgi_denoise_atrous_io& io, i
             if (x == 0 || y == 0) continue;
             const int ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
             const int j = (y + 2) * kAtrousP + (x + 2);
-#define AT(f) F[f][j]
-            const float dz = fabsf(AT(kFDepth) - depth_center);
-            const float ndot = saturate(__builtin_fmaf(nc.z, AT(kFNz), __builtin_fmaf(nc.y, AT(kFNy), nc.x * AT(kFNx))));
+            const AtrousV4 td = Q[kRDiff][j], ts = Q[kRSpec][j], tn = Q[kRNormal][j];
+            const AtrousV2 tl = QL[j];
+            const float dz = fabsf(tn.w - depth_center);
+            const float ndot = saturate(__builtin_fmaf(nc.z, tn.z, __builtin_fmaf(nc.y, tn.y, nc.x * tn.x)));
             // sigma_n = 128
             const float e = __builtin_fmaf(__builtin_amdgcn_logf(ndot), 128.0f,
                                            __builtin_fmaf(dz, -kz[ax + ay - 2], lkw[ay] + lkw[ax]));
-            const float wx = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(AT(kFLd) - lc.x), kl_d, e));
-            const float wy = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(AT(kFLs) - lc.y), kl_s, e));
-            dsum = mk3(__builtin_fmaf(AT(kFDr), wx, dsum.x), __builtin_fmaf(AT(kFDg), wx, dsum.y),
-                       __builtin_fmaf(AT(kFDb), wx, dsum.z));
-            ssum = mk3(__builtin_fmaf(AT(kFSr), wy, ssum.x), __builtin_fmaf(AT(kFSg), wy, ssum.y),
-                       __builtin_fmaf(AT(kFSb), wy, ssum.z));
-            vsum = mk2(__builtin_fmaf(wx * wx, AT(kFVx), vsum.x), __builtin_fmaf(wy * wy, AT(kFVy), vsum.y));
+            const float wx = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(tl.x - lc.x), kl_d, e));
+            const float wy = __builtin_amdgcn_exp2f(__builtin_fmaf(-fabsf(tl.y - lc.y), kl_s, e));
+            dsum = mk3(__builtin_fmaf(td.x, wx, dsum.x), __builtin_fmaf(td.y, wx, dsum.y),
+                       __builtin_fmaf(td.z, wx, dsum.z));
+            ssum = mk3(__builtin_fmaf(ts.x, wy, ssum.x), __builtin_fmaf(ts.y, wy, ssum.y),
+                       __builtin_fmaf(ts.z, wy, ssum.z));
+            vsum = mk2(__builtin_fmaf(wx * wx, td.w, vsum.x), __builtin_fmaf(wy * wy, ts.w, vsum.y));
             wsum = mk2(wsum.x + wx, wsum.y + wy);
-#undef AT
         }
     }
     // the output addresses are formed after the taps: redefining the pixel
