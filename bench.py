@@ -210,10 +210,14 @@ def profile_passes(step, first: int, n: int) -> dict:
     return _lib.profiler_read()
 
 
-def dominant_pass(prof: dict, trace_passes: set) -> str:
-    """The roofline kernel: the HBM-bound pixel pass with the largest summed
-    time (trace passes are latency/divergence bound and reported apart)."""
-    return max(((k, v) for k, v in prof.items() if k not in trace_passes), key=lambda kv: kv[1]["ms"])[0]
+def dominant_pass(prof: dict, exclude: set) -> str:
+    """The roofline kernel: the pixel pass with the largest summed time among
+    those not in `exclude` (the trace passes, which are latency/divergence
+    bound and reported apart, and, when the frame runs two concurrent
+    streams, the passes of those streams: a kernel sharing the device with
+    another stream's kernels has no launch duration of its own to divide
+    its bytes by)."""
+    return max(((k, v) for k, v in prof.items() if k not in exclude), key=lambda kv: kv[1]["ms"])[0]
 
 
 def timed_region(step, first: int, steps: int, only: str) -> tuple[float, dict]:
@@ -300,7 +304,13 @@ def run_frame(args):
     counts = _lib.profiler_read()
     base += args.count_frames
     trace_passes = {k for k, v in counts.items() if v["rays_closest"] + v["rays_any"] > 0}
-    dname = dominant_pass(prof_iso, trace_passes)
+    # with concurrent lanes every pass before the denoiser shares the device
+    # with the other lane; the denoiser passes run alone unless composition
+    # traces the sun shadow mask beside them (then only the final lighting
+    # and TAA passes do)
+    alone = ("final_lighting", "temporal_aa") + (() if cfg.compose else ("denoise_",))
+    shared = {k for k in prof_iso if not k.startswith(alone)} if cfg.concurrent_lanes else set()
+    dname = dominant_pass(prof_iso, trace_passes | shared)
     # the dependent-fetch rate over a buffer of the BVH's size (the trace passes' ceiling)
     import ctypes as C
 
@@ -570,10 +580,11 @@ def main():
     prof = r["prof"]
     counts = r.get("counts", {})
     trace_passes = r.get("trace_passes", set())
-    # the roofline is the dominant HBM-bound pixel pass (chosen from the
-    # per-pass run), timed live inside the timed region; trace passes are
-    # latency/divergence bound and are reported in Mrays/s with their
-    # counted traversal bytes instead
+    # the roofline is the dominant pixel pass among those that run alone on
+    # the device (chosen from the per-pass run, see dominant_pass), timed
+    # live inside the timed region; trace passes are latency/divergence
+    # bound and are reported in Mrays/s with their counted traversal bytes
+    # and the dependent-fetch ceiling instead
     dname = r["dname"]
     d = r["timed"][dname]
     avg_ms = d["ms"] / max(d["count"], 1)
@@ -664,8 +675,9 @@ def main():
         "passes": passes,
         "passes_note": ("per-pass HIP-event times from a separate run of the same frames on one stream (the "
                         "frame itself runs the hash-grid update beside the G-buffer and specular beside diffuse "
-                        "on two streams); the roofline pass is the HBM-bound pixel pass with the largest share "
-                        "here, timed live in the timed region" if frame else
+                        "on two streams); the roofline pass is the pixel pass with the largest share here among "
+                        "those that run alone on the device in the frame (the denoiser passes: the others share it "
+                        "with the other stream), timed live in the timed region" if frame else
                         "per-pass HIP-event times from a separate run (the timed region times the roofline "
                         "pass only), one stream"),
         "trace": trace,
