@@ -328,28 +328,60 @@ __device__ __forceinline__ void k_disocclusion_body(const vgi_denoise_spatial_io
         f2 msum = mk2(0.0f, 0.0f);
         const int stride = (int)((14.0f * kHistoryLenUnit) / len);
         const float fs = (float)stride;
+        // A row of 5 taps at a time: the row's depth and normal loads are
+        // issued together, then its colour and moment loads, so a pixel's
+        // 25 taps cost 10 dependent round trips instead of up to 50.  The
+        // loads are unconditional (a tap outside the image reads this
+        // pixel and gets weight 0); the accumulation keeps the taps' order
+        // and adds a tap only when its weight is positive, so a zero-weight
+        // tap (whose colours may be anything) adds exactly nothing.
 #pragma unroll
         for (int y = -2; y <= 2; y++) {
+            size_t jt[5];
+            bool in[5];
+            float zt[5];
+            uint32_t nt[5];
 #pragma unroll
             for (int x = -2; x <= 2; x++) {
-                int cx = (int)px + x * stride, cy = (int)py + y * stride;
-                if (cx < 0 || cy < 0 || cx > (int)w - 1 || cy > (int)h - 1) continue;
-                size_t j = (size_t)cy * w + cx;
+                const int cx = (int)px + x * stride, cy = (int)py + y * stride;
+                in[x + 2] = cx >= 0 && cy >= 0 && cx <= (int)w - 1 && cy <= (int)h - 1;
+                jt[x + 2] = in[x + 2] ? (size_t)cy * w + cx : i;
+                zt[x + 2] = G(io.depth_texture)[jt[x + 2]];
+                nt[x + 2] = G(io.gbuffer_texture)[plane + jt[x + 2]];
+            }
+            float wt[5];
+#pragma unroll
+            for (int x = -2; x <= 2; x++) {
                 const float dist = fs * sqrtf((float)(x * x + y * y));  // folded per tap
-                float dz = fabsf(G(io.depth_texture)[j] - depth_center);
-                f3 n = decode_normal_fast(G(io.gbuffer_texture)[plane + j]);
-                float wn = pow_pow2(saturate(dot3(nc, n)), 5);
-                float wt = wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
-                // a zero-weight tap adds exactly nothing to the (finite)
-                // sums: its colours and moments are not fetched
-                if (wt > 0.0f) {
-                    f3 dv = S::ld3(io.diff_texture, j);
-                    f3 sv = S::ld3(io.spec_texture, j);
-                    f2 mv = ld_moments<S>(io.moments_texture, io.moments_comp_texture, j);
-                    wsum = mk2(wsum.x + wt, wsum.y + wt);
-                    dsum = dsum + dv * wt;
-                    ssum = ssum + sv * wt;
-                    msum = mk2(msum.x + mv.x * wt, msum.y + mv.y * wt);
+                const float dz = fabsf(zt[x + 2] - depth_center);
+                const f3 n = decode_normal_fast(nt[x + 2]);
+                const float wn = pow_pow2(saturate(dot3(nc, n)), 5);
+                const float wk =
+                    wn * __builtin_amdgcn_exp2f(-(dz * (kLog2e * __builtin_amdgcn_rcpf(fwz * dist + 1e-8f))));
+                wt[x + 2] = in[x + 2] ? wk : 0.0f;
+            }
+            f3 dv[5], sv[5];
+            f2 mv[5];
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                dv[x] = S::ld3(io.diff_texture, jt[x]);
+                sv[x] = S::ld3(io.spec_texture, jt[x]);
+                mv[x] = ld_moments<S>(io.moments_texture, io.moments_comp_texture, jt[x]);
+            }
+            // left alone the compiler sinks each tap's loads into its
+            // weight test below, one round trip per tap; using the values
+            // here (the empty asm emits nothing) keeps the row's loads together
+#pragma unroll
+            for (int x = 0; x < 5; x++)
+                asm volatile("" ::"v"(dv[x].x), "v"(dv[x].y), "v"(dv[x].z), "v"(sv[x].x), "v"(sv[x].y), "v"(sv[x].z),
+                             "v"(mv[x].x), "v"(mv[x].y));
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                if (wt[x] > 0.0f) {
+                    wsum = mk2(wsum.x + wt[x], wsum.y + wt[x]);
+                    dsum = dsum + dv[x] * wt[x];
+                    ssum = ssum + sv[x] * wt[x];
+                    msum = mk2(msum.x + mv[x].x * wt[x], msum.y + mv[x].y * wt[x]);
                 }
             }
         }
