@@ -389,11 +389,32 @@ VGI_HD f4 mul44(const float* m, f4 v) {
 }
 VGI_HD f3 xyz(const float* v4) { return mk3(v4[0], v4[1], v4[2]); }
 
+// a / b rounded to nearest, given y = 1 / b rounded to nearest (Markstein's
+// division: q = RN(a y); the fused a - b q is exact; RN(q + (a - b q) y) is
+// RN(a / b) when nothing overflows or underflows).  Three instructions
+// instead of a full IEEE division when y is shared by many quotients; the
+// pixel-centre quotients (x + 0.5) / w below are checked against IEEE
+// division for every x < w <= 16384.
+VGI_HD float div_by_recip(float a, float b, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
 // cs_depth_to_position (frame_bindings.hlsl:57-61); `m` = inv_view_proj or
 // prev_inv_view_proj; depth_bias reproduces the +2^-24 of the sample passes.
 VGI_HD f3 depth_to_position(const float* m, uint32_t px, uint32_t py, uint32_t w, uint32_t h, float depth) {
     float su = ((float)px + 0.5f) / (float)w;
     float sv = ((float)py + 0.5f) / (float)h;
+    f4 p = mul44(m, mk4(su * 2.0f - 1.0f, sv * 2.0f - 1.0f, depth, 1.0f));
+    return mk3(p.x / p.w, p.y / p.w, p.z / p.w);
+}
+// The same with rw = 1 / w and rh = 1 / h (rounded to nearest) given, for
+// loops over many pixels of one image: bit-identical results.
+VGI_HD f3 depth_to_position_r(const float* m, uint32_t px, uint32_t py, uint32_t w, uint32_t h, float rw, float rh,
+                              float depth) {
+    float su = div_by_recip((float)px + 0.5f, (float)w, rw);
+    float sv = div_by_recip((float)py + 0.5f, (float)h, rh);
     f4 p = mul44(m, mk4(su * 2.0f - 1.0f, sv * 2.0f - 1.0f, depth, 1.0f));
     return mk3(p.x / p.w, p.y / p.w, p.z / p.w);
 }

@@ -335,7 +335,8 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
         S::st3(io.rw_lighting_texture, i, mk3s(0.0f));
         return;
     }
-    const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
+    const float rw = 1.0f / (float)w, rh = 1.0f / (float)h;
+    const f3 position = depth_to_position_r(fp.inv_view_proj, px, py, w, h, rw, rh, depth);
     f3 nrm = decode_gbuffer_normal(kTaps ? ct.a.y : G(io.gbuffer_color)[plane + i]).normal;
     float ao = S::ld2(io.ao_texture, i).x;
     DiffWR res = kTaps ? diff_make_tap(ct, position, nrm) : diff_load<S>(io, i, position, nrm);
@@ -356,7 +357,7 @@ __device__ __forceinline__ void k_diff_spatial_body(const FrameParams& fp, const
         f3 nn = decode_gbuffer_normal(kTaps ? tt.a.y : G(io.gbuffer_color)[plane + j]).normal;
         float depth_n = kTaps ? __uint_as_float(tt.a.x) : G(io.gbuffer_depth)[j];
         bool geometrical_diff = dot3(nrm, nn) < tab.cos_tol;
-        f3 position_n = depth_to_position(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, depth_n);
+        f3 position_n = depth_to_position_r(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, rw, rh, depth_n);
         float dist_to_plane = dot3(position_n - position, nrm);
         geometrical_diff = geometrical_diff || (fabsf(dist_to_plane) > 0.05f);
         if (geometrical_diff) {
@@ -580,7 +581,8 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
         S::st1(io.rw_ray_len_texture, i, 0.0f);
         return;
     }
-    const f3 position = depth_to_position(fp.inv_view_proj, px, py, w, h, depth);
+    const float rw = 1.0f / (float)w, rh = 1.0f / (float)h;
+    const f3 position = depth_to_position_r(fp.inv_view_proj, px, py, w, h, rw, rh, depth);
     GB g = load_gb(io.gbuffer_color, plane, i);
     uint32_t M = G(io.reservoir_texture)[2 * i];
     float W = asfloat(G(io.reservoir_texture)[2 * i + 1]);
@@ -608,7 +610,7 @@ __device__ __forceinline__ void k_spec_spatial_body(const FrameParams& fp, const
         GB gn = load_gb(io.gbuffer_color, plane, j);
         float depth_n = G(io.gbuffer_depth)[j];
         bool diff = dot3(g.normal, gn.normal) < cos_tol;
-        f3 position_n = depth_to_position(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, depth_n);
+        f3 position_n = depth_to_position_r(fp.inv_view_proj, (uint32_t)cx, (uint32_t)cy, w, h, rw, rh, depth_n);
         float dist_to_plane = dot3(position_n - position, g.normal);
         diff = diff || (fabsf(dist_to_plane) > 0.05f);
         if (diff) {
