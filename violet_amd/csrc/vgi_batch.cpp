@@ -146,9 +146,11 @@ extern "C" vgi_status vgi_batch_create(const vgi_batch_config* cfg, vgi_batch** 
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
     if (e == hipSuccess && c.use_graph) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e == hipSuccess && c.concurrent_lanes) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
-    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
-    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
-    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_cross, hipEventDisableTiming);
+    // lane events order two streams of this device only (no system-scope release)
+    const unsigned lane_ev = hipEventDisableTiming | hipEventDisableSystemFence;
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_fork, lane_ev);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_join, lane_ev);
+    if (e == hipSuccess && c.concurrent_lanes) e = hipEventCreateWithFlags(&b->ev_cross, lane_ev);
     if (e != hipSuccess) {
         delete b;
         return check_hip(e, "vgi_batch_create");
