@@ -433,18 +433,13 @@ __device__ __forceinline__ void k_fallback_body(const vgi_denoise_spatial_io& io
     const bool inside = px < w && py < h;
     const size_t i = (size_t)py * w + px;
     // this pixel's history length and the max over its 2x2 quad
-    // (QuadReadAcrossX/Y; a partner outside the image counts 0)
-    float len = 0.0f, qmax = 0.0f;
-    if (inside) {
-        len = history_len_load(G(io.history_len_texture)[i]);
-        const int qx = px & ~1, qy = py & ~1;
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx)
-                if (qx + dx < w && qy + dy < h)
-                    qmax = fmaxf(qmax, history_len_load(G(io.history_len_texture)[(size_t)(qy + dy) * w + qx + dx]));
-    }
+    // (QuadReadAcrossX/Y; a partner outside the image counts 0).  A wave's
+    // 8x8 pixels are row-major in its lanes, so the x partner is lane ^ 1
+    // and the y partner lane ^ 8: two lane swaps instead of three loads.
+    const float len0 = inside ? history_len_load(G(io.history_len_texture)[i]) : 0.0f;
+    const float mx = fmaxf(len0, __shfl_xor(len0, 1));
+    float len = len0;
+    const float qmax = inside ? fmaxf(mx, __shfl_xor(mx, 8)) : 0.0f;
     const bool filtered = inside && len != 0.0f && qmax < 4.0f * kHistoryLenUnit;
     if (__syncthreads_or(filtered)) {
 #pragma unroll 1
