@@ -116,20 +116,30 @@ __device__ __forceinline__ void k_ao_filter_body(const vgi_ao_filter_io& io, uin
         return;
     }
     f2 center = S::ld2(io.ao_texture, i);
+    // the nine taps' depths and AO values are loaded together (every tap's
+    // AO, then a select), one round trip instead of one or two per tap
+    float sd[9];
+    f2 av[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        // the reference reads min(id + (x,y), size) as unsigned (one texel out of
+        // bounds at the far edges, wrapping at the near edges); clamped here
+        const int x = k % 3 - 1, y = k / 3 - 1;
+        const int cx = clampi((int)px + x, 0, (int)w - 1), cy = clampi((int)py + y, 0, (int)h - 1);
+        const size_t j = (size_t)cy * w + cx;
+        sd[k] = G(io.depth_buffer)[j];
+        av[k] = S::ld2(io.ao_texture, j);
+    }
+    // using the values here (the empty asm emits nothing) keeps the compiler
+    // from sinking each AO load into its depth test
+#pragma unroll
+    for (int k = 0; k < 9; k++) asm volatile("" ::"v"(av[k].x), "v"(av[k].y));
     f2 col_med[3], col_min[3], col_max[3];
     for (int y = -1; y <= 1; y++) {
         f2 row[3];
         for (int x = -1; x <= 1; x++) {
-            // the reference reads min(id + (x,y), size) as unsigned (one texel out of
-            // bounds at the far edges, wrapping at the near edges); clamped here
-            int cx = clampi((int)px + x, 0, (int)w - 1), cy = clampi((int)py + y, 0, (int)h - 1);
-            size_t j = (size_t)cy * w + cx;
-            float sd = G(io.depth_buffer)[j];
-            if (fabsf(depth - sd) > 0.0005f) {
-                row[x + 1] = center;
-                continue;
-            }
-            row[x + 1] = S::ld2(io.ao_texture, j);
+            const int k = (y + 1) * 3 + (x + 1);
+            row[x + 1] = fabsf(depth - sd[k]) > 0.0005f ? center : av[k];
         }
         // the reference's "median" is clamp(a, b, c) = min(max(a, b), c)
         col_med[y + 1] = clamp2v(row[0], row[1], row[2]);
