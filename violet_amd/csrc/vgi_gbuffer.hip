@@ -86,8 +86,13 @@ __device__ __forceinline__ void k_gbuffer_body(const FrameParams& fp, const Scen
     G(io.rw_gbuffer)[3 * plane + i] = 1u;
 }
 
+// 6 waves per SIMD (80 VGPRs) rather than the sample passes' 8: the pass
+// alone is 1.5% slower, but it runs beside the hash-grid cache update, and
+// with fewer resident G-buffer waves the update's kernels get the room to
+// overlap it (the frame is 0.4% faster).
+constexpr int kGBufferWaves = 6;
 template <bool kCount>
-__global__ void __launch_bounds__(64, kTraceWaves) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
+__global__ void __launch_bounds__(64, kGBufferWaves) k_gbuffer(FrameParams fp, SceneView s, vgi_gbuffer_io io) {
     k_gbuffer_body<kCount>(fp, s, io, blockIdx.x, blockIdx.y);
 }
 
